@@ -1,0 +1,221 @@
+/*
+ * jump_mpm.h -- C ABI of the MI355X explicit MPM step (libjumphip.so).
+ *
+ * This is the drop-in boundary for jump's explicit updated-Lagrangian MPM
+ * solvers:
+ *   solve_explicit_dynamics_2D(grid,solids,basis,alg::USL,output,fixes,data)
+ *       SolveExplicitSolidMechanics2D.jl:335-585
+ *   solve_explicit_dynamics_3D(grid,solids,basis,alg::MUSL,output,fixes,data)
+ *       SolveExplicitSolidMechanics3D.jl:14-223
+ * both exported by module Problem (Problem.jl:163-166).  A host binding keeps
+ * the reference's Grid/Solid/Material/Basis objects and calls these entry
+ * points instead of running the Julia time loop.
+ *
+ * Conventions
+ *  - Every function returns 0 (JUMP_OK) or a negative jump_status; no C++
+ *    exception crosses the ABI.  jump_last_error() returns the message of the
+ *    last failure on a context.
+ *  - All floating point data are IEEE binary64.
+ *  - Per-particle vectors are passed in the reference's AoS layout, i.e. the
+ *    memory of Vector{SVector{d,Float64}} (pos[d*i+k]) and of
+ *    Vector{SMatrix{d,d,Float64}} (column-major: M[d*d*i + a + d*b] = M[a,b]).
+ *    Host buffers are owned by the caller and copied; the context owns all
+ *    device memory.
+ *  - Node and particle indices are 0-based (the Julia binding subtracts 1).
+ *  - One HIP stream per context; a context is not thread-safe.  Calls that
+ *    return host data block until the stream has drained.
+ */
+#ifndef JUMP_MPM_H
+#define JUMP_MPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JUMP_ABI_VERSION 1
+
+typedef enum {
+  JUMP_OK = 0,
+  JUMP_ERR_ARG = -1,         /* invalid argument                           */
+  JUMP_ERR_STATE = -2,       /* call not valid in the current state        */
+  JUMP_ERR_DEVICE = -3,      /* HIP runtime failure                        */
+  JUMP_ERR_NOMEM = -4,       /* allocation failure                         */
+  JUMP_ERR_UNSUPPORTED = -5, /* combination the reference does not define  */
+  JUMP_ERR_NUMERIC = -6      /* step finished with a particle error flag   */
+} jump_status;
+
+/* Particle error flags raised during a step (OR-ed; jump_flags()).
+ * The reference only logs these (SolveExplicitSolidMechanics3D.jl:201-206,
+ * Basis.jl:71-74,113-126) and continues; the step here also continues, skips
+ * the offending particle's grid access, and reports the flag. */
+#define JUMP_FLAG_OUT_OF_GRID 1u  /* stencil leaves the grid: particle skipped */
+#define JUMP_FLAG_NEGATIVE_J 2u   /* det(F) < 0 after the update              */
+#define JUMP_FLAG_NONFINITE 4u    /* NaN/Inf in updated particle state        */
+
+/* Basis types: Basis.jl:37-48 (LinearBasis, QuadBsplineBasis,
+ * CubicBsplineBasis, CPDIQ4Basis) plus uGIMP, which the reference lacks. */
+typedef enum {
+  JUMP_BASIS_LINEAR = 0,
+  JUMP_BASIS_QUAD_BSPLINE = 1,
+  JUMP_BASIS_CUBIC_BSPLINE = 2,
+  JUMP_BASIS_CPDI_Q4 = 3, /* 2D only */
+  JUMP_BASIS_GIMP = 4
+} jump_basis;
+
+/* Algorithm tags: Algorithm.jl:16-18 (USL(tolerance)), :37-39 (MUSL(alpha)).
+ * The reference implements USL in 2D and MUSL in 3D on this path. */
+typedef enum { JUMP_ALG_USL = 0, JUMP_ALG_MUSL = 1 } jump_algorithm;
+
+/* Material kinds: Material.jl:29-31 (Rigid), :37-81 (Elastic/Hooke),
+ * :100-122 (NeoHookean), :128-275 (ElastoPlastic J2), :280-441 (JohnsonCook). */
+typedef enum {
+  JUMP_MAT_RIGID = 0,
+  JUMP_MAT_ELASTIC = 1,
+  JUMP_MAT_NEOHOOKEAN = 2,
+  JUMP_MAT_ELASTOPLASTIC = 3,
+  JUMP_MAT_JOHNSONCOOK = 4
+} jump_material_kind;
+
+/* Step parameters: the Grid2D/Grid3D constructor arguments (Grid.jl:104,198),
+ * the basis and algorithm objects, and the keys of the reference's `data`
+ * Dict (SolveExplicitSolidMechanics2D.jl:337-344: "dt", "bodyforce",
+ * "ghostcell"). */
+typedef struct {
+  int32_t dim;       /* 2 or 3                                            */
+  int32_t basis;     /* jump_basis                                        */
+  int32_t algorithm; /* jump_algorithm                                    */
+  int32_t ghostcell; /* data["ghostcell"] present                         */
+  double tolerance;  /* USL(tolerance): nodes with m <= tolerance skipped */
+  double alpha;      /* MUSL(alpha): FLIP/PIC blend                       */
+  double dt;         /* data["dt"]                                        */
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+  int64_t nx, ny, nz; /* node counts iN_x, iN_y, iN_z (nz ignored in 2D)   */
+  double body[3];     /* ConstantBodyForce2D/3D g (BodyForce.jl:56-91)     */
+} jump_params;
+
+/* Material parameters.  Lame constants are derived exactly as the reference
+ * constructors do (Material.jl:54-57,112-113,159-161,313-325). */
+typedef struct {
+  int32_t kind;  /* jump_material_kind                                  */
+  int32_t fixed; /* rigid only: zero the grid momentum under the body   */
+  double E, nu, density;
+  double fy, k1; /* ElastoPlastic: yield stress, linear hardening          */
+  double A, B, C, n, epsdot0, m, chi, Cp, cellsize; /* JohnsonCook          */
+  double vx, vy; /* rigid only: prescribed velocity (RigidMaterial(vx,vy,rho)) */
+} jump_material;
+
+/* One solid (Solid2D/Solid3D, Solid.jl:67-297).  Arrays are AoS as above;
+ * pos, vel: d*n; mass, volume, volume0: n; F, strain, stress: d*d*n.
+ * F/strain/stress may be NULL (identity / zero).  For CPDI-Q4 (2D), corners
+ * (2*ncorners, the solid's `nodes`) and elems (4*n, 0-based, the solid's
+ * `elems` rows) describe the particle domains.  For GIMP, lp[k] is the
+ * particle half-width along axis k (must be <= dx_k/2). */
+typedef struct {
+  int64_t n;
+  const double* pos;
+  const double* vel;
+  const double* mass;
+  const double* volume;
+  const double* volume0;
+  const double* F;
+  const double* strain;
+  const double* stress;
+  int64_t ncorners;
+  const double* corners;
+  const int64_t* elems;
+  double lp[3];
+} jump_solid_desc;
+
+typedef struct jump_ctx jump_ctx;
+
+/* Create a context for one grid/basis/algorithm.  Replaces the set-up part of
+ * solve_explicit_dynamics_2D/3D (SolveExplicitSolidMechanics2D.jl:335-371,
+ * SolveExplicitSolidMechanics3D.jl:14-39) and Grid2D/Grid3D allocation
+ * (Grid.jl:104-158, 198-267; the 2D dense mass matrix M is not allocated,
+ * USL never reads it).  Selects device `device` (HIP ordinal). */
+int jump_ctx_create(const jump_params* params, int device, jump_ctx** out);
+int jump_ctx_destroy(jump_ctx* ctx);
+const char* jump_last_error(const jump_ctx* ctx);
+
+/* Append one solid (Solid2D(coords,mat) Solid.jl:104-125, Solid2D(file,mat)
+ * for CPDI Solid.jl:162-221, Solid3D Solid.jl:283-296) with its material.
+ * Solids are processed in the order added, as the reference's `solids`
+ * vector.  Writes the solid index to *solid_index. */
+int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* desc,
+                   const jump_material* mat, int32_t* solid_index);
+
+/* Dirichlet mask, the reference's grid.fixedNodes (dim x nodeCount, Int64,
+ * Grid.jl:130-131): mask[dim*node + k] != 0 fixes direction k of node.
+ * Set by fix_Dirichlet_grid (Dirichlet.jl:28-82) on the host side. */
+int jump_set_fixed(jump_ctx* ctx, const uint8_t* mask);
+
+/* Run nsteps iterations of the reference's `while t < Tf` body
+ * (SolveExplicitSolidMechanics2D.jl:373-581 for USL,
+ *  SolveExplicitSolidMechanics3D.jl:41-220 for MUSL) without output, each
+ * ending with t += dt.  *t_inout holds t (data["time"]) and is advanced.
+ * Returns JUMP_ERR_NUMERIC (after finishing all steps) if a particle error
+ * flag was raised; the flags are then available from jump_flags(). */
+int jump_step(jump_ctx* ctx, int64_t nsteps, double* t_inout);
+
+/* OR of JUMP_FLAG_* raised since the last call; clears them. */
+int jump_flags(jump_ctx* ctx, uint32_t* flags);
+
+/* Copy a solid's particle state back (any pointer may be NULL), in the
+ * order the particles were added, AoS as for jump_add_solid. */
+int jump_get_solid(jump_ctx* ctx, int32_t solid, double* pos, double* vel,
+                   double* mass, double* volume, double* F, double* strain,
+                   double* stress);
+
+/* Material history held per particle (the reference keeps it in the
+ * material, indexed by particle: Material.jl:140-155, 301-303).
+ * alpha: equivalent plastic strain (J2, JC); alphadot (JC); vm: von Mises
+ * stress (3D elastic, J2, JC); pstrain: plastic strain, 3 Voigt components
+ * [e11, e22, 2*e12] per particle in 2D (Material.jl:140,191) or a full 3x3
+ * column-major matrix in 3D (Material.jl:151).  Any pointer may be NULL. */
+int jump_get_solid_state(jump_ctx* ctx, int32_t solid, double* alpha,
+                         double* alphadot, double* vm, double* pstrain);
+
+/* Copy the grid arrays of the last step (Grid.jl:78-90, 181-186), AoS
+ * d*nodeCount for the vector fields.  Any pointer may be NULL. */
+int jump_get_grid(jump_ctx* ctx, double* mass, double* momentum0,
+                  double* momentum, double* momentum2, double* force);
+
+/* Kinetic and strain energy over all solids, compute(::EnergiesFix)
+ * (Fix.jl:248-270). */
+int jump_energies(jump_ctx* ctx, double* ke, double* se);
+
+/* Diagnostics (no counterpart in the reference, which recomputes the support
+ * inside every shape-function call, Basis.jl:63-78,390-425).
+ * jump_get_bins: runs the binning of the current particle positions and
+ * returns, per particle in insertion order, the 0-based linear index of the
+ * first node of its stencil (-1: rigid in 2D or outside the grid).  For
+ * tests of the bit-exact support rule. */
+int jump_get_bins(jump_ctx* ctx, int32_t* keys);
+int jump_abi_version(void);
+
+/* Per-phase GPU time (diagnostic, no counterpart in the reference, whose only
+ * timing hook is `@time main()`, Main_TwoDisks.jl:142).  While enabled, every
+ * phase of jump_step is bracketed by HIP events recorded on the context's
+ * stream; jump_profile(ctx, on) enables/disables and resets the totals,
+ * jump_profile_read returns the summed milliseconds and the number of
+ * intervals of one phase. */
+typedef enum {
+  JUMP_PHASE_SORT = 0, /* binning + counting sort                    */
+  JUMP_PHASE_P2G = 1,  /* P2G fused with the grid update             */
+  JUMP_PHASE_RIGID = 2,/* 2D rigid-body grid override and motion     */
+  JUMP_PHASE_G2P = 3,  /* USL G2P + stress                           */
+  JUMP_PHASE_G2P1 = 4, /* MUSL particle velocity                     */
+  JUMP_PHASE_P2G2 = 5, /* MUSL momentum re-projection                */
+  JUMP_PHASE_G2P2 = 6, /* MUSL position, deformation, stress         */
+  JUMP_PHASE_COUNT = 7
+} jump_phase;
+int jump_profile(jump_ctx* ctx, int32_t enable);
+int jump_profile_read(jump_ctx* ctx, int32_t phase, double* total_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JUMP_MPM_H */

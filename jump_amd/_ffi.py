@@ -1,0 +1,157 @@
+"""ctypes binding of include/jump_mpm.h (libjumphip.so).
+
+The structs below are the ABI's data formats.  `load()` returns the HIP
+library; it raises if the library has not been built, so no caller can fall
+back to a CPU path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjumphip.so")
+
+JUMP_OK = 0
+JUMP_ERR_ARG = -1
+JUMP_ERR_STATE = -2
+JUMP_ERR_DEVICE = -3
+JUMP_ERR_NOMEM = -4
+JUMP_ERR_UNSUPPORTED = -5
+JUMP_ERR_NUMERIC = -6
+
+JUMP_FLAG_OUT_OF_GRID = 1
+JUMP_FLAG_NEGATIVE_J = 2
+JUMP_FLAG_NONFINITE = 4
+
+BASIS_LINEAR = 0
+BASIS_QUAD_BSPLINE = 1
+BASIS_CUBIC_BSPLINE = 2
+BASIS_CPDI_Q4 = 3
+BASIS_GIMP = 4
+
+ALG_USL = 0
+ALG_MUSL = 1
+
+MAT_RIGID = 0
+MAT_ELASTIC = 1
+MAT_NEOHOOKEAN = 2
+MAT_ELASTOPLASTIC = 3
+MAT_JOHNSONCOOK = 4
+
+_d = ctypes.c_double
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_pd = ctypes.POINTER(ctypes.c_double)
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+
+
+class JumpParams(ctypes.Structure):
+    _fields_ = [
+        ("dim", _i32), ("basis", _i32), ("algorithm", _i32), ("ghostcell", _i32),
+        ("tolerance", _d), ("alpha", _d), ("dt", _d),
+        ("xmin", _d), ("xmax", _d), ("ymin", _d), ("ymax", _d), ("zmin", _d), ("zmax", _d),
+        ("nx", _i64), ("ny", _i64), ("nz", _i64),
+        ("body", _d * 3),
+    ]
+
+
+class JumpMaterial(ctypes.Structure):
+    _fields_ = [
+        ("kind", _i32), ("fixed", _i32),
+        ("E", _d), ("nu", _d), ("density", _d),
+        ("fy", _d), ("k1", _d),
+        ("A", _d), ("B", _d), ("C", _d), ("n", _d), ("epsdot0", _d), ("m", _d), ("chi", _d), ("Cp", _d),
+        ("cellsize", _d),
+        ("vx", _d), ("vy", _d),
+    ]
+
+
+class JumpSolidDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", _i64),
+        ("pos", _pd), ("vel", _pd), ("mass", _pd), ("volume", _pd), ("volume0", _pd),
+        ("F", _pd), ("strain", _pd), ("stress", _pd),
+        ("ncorners", _i64), ("corners", _pd), ("elems", _pi64),
+        ("lp", _d * 3),
+    ]
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 array (or NULL for None)."""
+    if a is None:
+        return _pd()
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "expected C-contiguous float64"
+    return a.ctypes.data_as(_pd)
+
+
+def i64ptr(a):
+    if a is None:
+        return _pi64()
+    assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_pi64)
+
+
+class _Handle(ctypes.c_void_p):
+    pass
+
+
+def _declare(lib, prefix):
+    """Declare the entry points shared by the HIP library (prefix 'jump_')
+    and the CPU restatement (prefix 'orc_')."""
+    H = ctypes.c_void_p
+    sig = {
+        "add_solid": (ctypes.c_int, [H, ctypes.POINTER(JumpSolidDesc), ctypes.POINTER(JumpMaterial),
+                                     ctypes.POINTER(_i32)]),
+        "ctx_destroy": (ctypes.c_int, [H]),
+        "last_error": (ctypes.c_char_p, [H]),
+        "set_fixed": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_uint8)]),
+        "step": (ctypes.c_int, [H, _i64, _pd]),
+        "flags": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_uint32)]),
+        "get_solid": (ctypes.c_int, [H, _i32, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+        "get_solid_state": (ctypes.c_int, [H, _i32, _pd, _pd, _pd, _pd]),
+        "get_grid": (ctypes.c_int, [H, _pd, _pd, _pd, _pd, _pd]),
+        "energies": (ctypes.c_int, [H, _pd, _pd]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, prefix + name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def load():
+    """Load libjumphip.so; raises RuntimeError if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m jump_amd.build` "
+            "(the MPM step has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    _declare(lib, "jump_")
+    lib.jump_ctx_create.restype = ctypes.c_int
+    lib.jump_ctx_create.argtypes = [ctypes.POINTER(JumpParams), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.jump_get_bins.restype = ctypes.c_int
+    lib.jump_get_bins.argtypes = [ctypes.c_void_p, ctypes.POINTER(_i32)]
+    lib.jump_abi_version.restype = ctypes.c_int
+    lib.jump_abi_version.argtypes = []
+    lib.jump_profile.restype = ctypes.c_int
+    lib.jump_profile.argtypes = [ctypes.c_void_p, _i32]
+    lib.jump_profile_read.restype = ctypes.c_int
+    lib.jump_profile_read.argtypes = [ctypes.c_void_p, _i32, _pd, ctypes.POINTER(_i64)]
+    _LIB = lib
+    return lib
+
+
+PHASES = ["sort", "p2g", "rigid", "g2p", "g2p1", "p2g2", "g2p2"]
+
+EXPORTED = [
+    "jump_ctx_create", "jump_ctx_destroy", "jump_last_error", "jump_add_solid", "jump_set_fixed", "jump_step",
+    "jump_flags", "jump_get_solid", "jump_get_solid_state", "jump_get_grid", "jump_energies", "jump_get_bins",
+    "jump_abi_version", "jump_profile", "jump_profile_read",
+]

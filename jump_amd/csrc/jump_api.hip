@@ -1,0 +1,764 @@
+// jump_api.hip -- context, memory management and the exported C ABI of
+// libjumphip.so (declared in include/jump_mpm.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "mpm_kernels.hpp"
+
+using namespace jh;
+
+namespace {
+
+struct HostSolid {
+  int64_t n = 0;
+  jump_material mat{};
+  SolidDev dev{};
+  int64_t offset = 0;  // first particle in the concatenated arrays
+};
+
+inline int blocks_for(int64_t n) { return int((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+struct jump_ctx {
+  jump_params prm{};
+  GridDesc G{};
+  int D = 2;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  std::vector<HostSolid> solids;
+  std::vector<double> corners;  // CPDI corner coordinates of all solids
+
+  // Host staging in device layout (SoA, symmetric tensors compressed);
+  // authoritative while !on_device.
+  int64_t np = 0;
+  std::vector<double> hx[3], hv[3], hm, hV, hV0, hF[9], heps[6], hsig[6], halpha, halphadot, hvm, hep[6];
+  std::vector<int> hsid, helem[4];
+  std::vector<uint8_t> hfixed;
+  bool on_device = false;  // device holds the current particle state
+  int64_t dev_capacity = -1;
+
+  // device memory
+  Particles P{};
+  GridArrays g{};
+  std::vector<void*> part_allocs;
+  std::vector<void*> grid_allocs;
+  double* d_corners = nullptr;
+  SolidDev* d_solids = nullptr;
+  int *d_key = nullptr, *d_perm = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_cursor = nullptr,
+      *d_sums = nullptr;
+  unsigned* d_flags = nullptr;
+  double* d_red = nullptr;
+  int* d_wmax = nullptr;
+  uint8_t* d_fixed = nullptr;
+  int cpdi_w = 0;
+  bool grid_valid = false;
+  uint32_t flags = 0;
+
+  // per-phase timing with HIP events on the context stream (jump_profile)
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_recs;
+  double prof_ms[JUMP_PHASE_COUNT] = {};
+  int64_t prof_cnt[JUMP_PHASE_COUNT] = {};
+  hipEvent_t prof_open = nullptr;
+};
+
+// ------------------------------------------------------------------ utilities
+static int set_err(jump_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return set_err(ctx, JUMP_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static int nsym(int D) { return D == 2 ? 3 : 6; }
+
+template <class T> static int dev_alloc(jump_ctx* ctx, std::vector<void*>& list, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return set_err(ctx, JUMP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  list.push_back(q);
+  *p = static_cast<T*>(q);
+  return JUMP_OK;
+}
+
+static void free_list(std::vector<void*>& list) {
+  for (void* q : list) (void)hipFree(q);
+  list.clear();
+}
+
+// Lame constants as the reference constructors compute them.
+static SolidDev make_solid_dev(const jump_material& m, const double lp[3]) {
+  SolidDev s{};
+  s.kind = m.kind;
+  s.rigid = m.kind == JUMP_MAT_RIGID;
+  s.fixed = m.fixed != 0;
+  s.lambda = m.E * m.nu / (1.0 + m.nu) / (1.0 - 2.0 * m.nu);
+  s.mu = m.E / 2.0 / (1.0 + m.nu);
+  s.kappa = s.lambda + s.mu;
+  s.fy = m.fy;
+  s.k1 = m.k1;
+  s.A = m.A; s.B = m.B; s.C = m.C; s.n = m.n; s.epsdot0 = m.epsdot0; s.cellsize = m.cellsize;
+  s.velo = m.kind == JUMP_MAT_JOHNSONCOOK ? sqrt((s.lambda + 2.0 * s.mu) / m.density) : 0.0;
+  s.vx = m.vx;
+  s.vy = m.vy;
+  for (int k = 0; k < 3; k++) s.lp[k] = lp[k];
+  return s;
+}
+
+// ------------------------------------------------------------------ transfers
+static int alloc_particles(jump_ctx* ctx) {
+  free_list(ctx->part_allocs);
+  const int D = ctx->D, S = nsym(D);
+  const size_t n = size_t(ctx->np);
+  Particles& P = ctx->P;
+  P = Particles{};
+  int r = JUMP_OK;
+  for (int k = 0; k < D && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.x[k], n);
+  for (int k = 0; k < D && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.v[k], n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.m, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.V, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.V0, n);
+  for (int k = 0; k < D * D && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.F[k], n);
+  for (int k = 0; k < S && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.eps[k], n);
+  for (int k = 0; k < S && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.sig[k], n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.alpha, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.alphadot, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.vm, n);
+  for (int k = 0; k < S && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.ep[k], n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.sid, n);
+  if (ctx->prm.basis == JUMP_BASIS_CPDI_Q4)
+    for (int k = 0; k < 4 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.elem[k], n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_key, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_perm, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_red, 2 * size_t(blocks_for(ctx->np) + 1));
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_solids, ctx->solids.size());
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_corners, ctx->corners.size());
+  P.n = int(ctx->np);
+  ctx->dev_capacity = ctx->np;
+  return r;
+}
+
+template <class T> static int h2d(jump_ctx* ctx, T* dst, const std::vector<T>& src) {
+  if (src.empty()) return JUMP_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return JUMP_OK;
+}
+template <class T> static int d2h(jump_ctx* ctx, std::vector<T>& dst, const T* src) {
+  if (dst.empty()) return JUMP_OK;
+  HIP_TRY(hipMemcpyAsync(dst.data(), src, dst.size() * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  return JUMP_OK;
+}
+
+// Host staging -> device (all particle arrays).
+static int upload(jump_ctx* ctx) {
+  const int D = ctx->D, S = nsym(D);
+  int r = alloc_particles(ctx);
+  if (r) return r;
+  Particles& P = ctx->P;
+  for (int k = 0; k < D && !r; k++) r = h2d(ctx, P.x[k], ctx->hx[k]);
+  for (int k = 0; k < D && !r; k++) r = h2d(ctx, P.v[k], ctx->hv[k]);
+  if (!r) r = h2d(ctx, P.m, ctx->hm);
+  if (!r) r = h2d(ctx, P.V, ctx->hV);
+  if (!r) r = h2d(ctx, P.V0, ctx->hV0);
+  for (int k = 0; k < D * D && !r; k++) r = h2d(ctx, P.F[k], ctx->hF[k]);
+  for (int k = 0; k < S && !r; k++) r = h2d(ctx, P.eps[k], ctx->heps[k]);
+  for (int k = 0; k < S && !r; k++) r = h2d(ctx, P.sig[k], ctx->hsig[k]);
+  if (!r) r = h2d(ctx, P.alpha, ctx->halpha);
+  if (!r) r = h2d(ctx, P.alphadot, ctx->halphadot);
+  if (!r) r = h2d(ctx, P.vm, ctx->hvm);
+  for (int k = 0; k < S && !r; k++) r = h2d(ctx, P.ep[k], ctx->hep[k]);
+  if (!r) r = h2d(ctx, P.sid, ctx->hsid);
+  if (ctx->prm.basis == JUMP_BASIS_CPDI_Q4)
+    for (int k = 0; k < 4 && !r; k++) r = h2d(ctx, P.elem[k], ctx->helem[k]);
+  std::vector<SolidDev> sd;
+  for (auto& s : ctx->solids) sd.push_back(s.dev);
+  if (!r) r = h2d(ctx, ctx->d_solids, sd);
+  if (!r) r = h2d(ctx, ctx->d_corners, ctx->corners);
+  if (r) return r;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->on_device = true;
+  ctx->cpdi_w = 0;
+  return JUMP_OK;
+}
+
+// Device -> host staging (all particle arrays).
+static int download(jump_ctx* ctx) {
+  if (!ctx->on_device) return JUMP_OK;
+  const int D = ctx->D, S = nsym(D);
+  Particles& P = ctx->P;
+  int r = JUMP_OK;
+  for (int k = 0; k < D && !r; k++) r = d2h(ctx, ctx->hx[k], P.x[k]);
+  for (int k = 0; k < D && !r; k++) r = d2h(ctx, ctx->hv[k], P.v[k]);
+  if (!r) r = d2h(ctx, ctx->hm, P.m);
+  if (!r) r = d2h(ctx, ctx->hV, P.V);
+  if (!r) r = d2h(ctx, ctx->hV0, P.V0);
+  for (int k = 0; k < D * D && !r; k++) r = d2h(ctx, ctx->hF[k], P.F[k]);
+  for (int k = 0; k < S && !r; k++) r = d2h(ctx, ctx->heps[k], P.eps[k]);
+  for (int k = 0; k < S && !r; k++) r = d2h(ctx, ctx->hsig[k], P.sig[k]);
+  if (!r) r = d2h(ctx, ctx->halpha, P.alpha);
+  if (!r) r = d2h(ctx, ctx->halphadot, P.alphadot);
+  if (!r) r = d2h(ctx, ctx->hvm, P.vm);
+  for (int k = 0; k < S && !r; k++) r = d2h(ctx, ctx->hep[k], P.ep[k]);
+  if (r) return r;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return JUMP_OK;
+}
+
+static int ensure_device(jump_ctx* ctx) {
+  if (ctx->on_device) return JUMP_OK;
+  return upload(ctx);
+}
+
+static int alloc_grid(jump_ctx* ctx) {
+  const size_t nn = size_t(ctx->G.nn);
+  const int D = ctx->D;
+  GridArrays& g = ctx->g;
+  int r = dev_alloc(ctx, ctx->grid_allocs, &g.m, nn);
+  for (int k = 0; k < D && !r; k++) r = dev_alloc(ctx, ctx->grid_allocs, &g.p0[k], nn);
+  for (int k = 0; k < D && !r; k++) r = dev_alloc(ctx, ctx->grid_allocs, &g.p[k], nn);
+  for (int k = 0; k < D && !r; k++) r = dev_alloc(ctx, ctx->grid_allocs, &g.p2[k], nn);
+  for (int k = 0; k < D && !r; k++) r = dev_alloc(ctx, ctx->grid_allocs, &g.f[k], nn);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_fixed, nn);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_cnt, nn + 1);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_start, nn + 1);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_cursor, nn + 1);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_sums, (nn + 1) / kScanItems + 2);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_flags, 1);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_wmax, 1);
+  if (r) return r;
+  g.fixed = ctx->d_fixed;
+  HIP_TRY(hipMemsetAsync(ctx->d_fixed, 0, nn, ctx->stream));
+  HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return JUMP_OK;
+}
+
+// ------------------------------------------------------------------ profiling
+static hipEvent_t prof_event(jump_ctx* ctx) {
+  if (ctx->ev_used == ctx->ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->ev_pool.push_back(e);
+  }
+  return ctx->ev_pool[ctx->ev_used++];
+}
+static void prof_begin(jump_ctx* ctx) {
+  if (!ctx->prof) return;
+  ctx->prof_open = prof_event(ctx);
+  if (ctx->prof_open) (void)hipEventRecord(ctx->prof_open, ctx->stream);
+}
+static void prof_end(jump_ctx* ctx, int phase) {
+  if (!ctx->prof || !ctx->prof_open) return;
+  hipEvent_t e = prof_event(ctx);
+  if (!e) return;
+  (void)hipEventRecord(e, ctx->stream);
+  ctx->prof_recs.push_back({phase, {ctx->prof_open, e}});
+  ctx->prof_open = nullptr;
+}
+// After the stream has drained: fold the recorded intervals into the totals.
+static void prof_collect(jump_ctx* ctx) {
+  for (auto& r : ctx->prof_recs) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, r.second.first, r.second.second) == hipSuccess) {
+      ctx->prof_ms[r.first] += ms;
+      ctx->prof_cnt[r.first] += 1;
+    }
+  }
+  ctx->prof_recs.clear();
+  ctx->ev_used = 0;
+}
+
+// ------------------------------------------------------------------ the step
+#define LAUNCH(kernel, n, ...)                                                                \
+  do {                                                                                        \
+    if ((n) > 0) hipLaunchKernelGGL((kernel), dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, __VA_ARGS__); \
+  } while (0)
+
+static int scan_bins(jump_ctx* ctx) {
+  const int n = ctx->G.nn + 1;
+  const int nb = (n + kScanItems - 1) / kScanItems;
+  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->d_cnt, ctx->d_start, ctx->d_sums, n);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_sums, nb);
+  LAUNCH(k_scan_add, n, ctx->d_start, ctx->d_sums, n);
+  return JUMP_OK;
+}
+
+template <int D, int B> static int bin_and_sort(jump_ctx* ctx) {
+  const int nn = ctx->G.nn;
+  const int np = int(ctx->np);
+  LAUNCH(k_fill_i32, nn + 1, ctx->d_cnt, 0, nn + 1);
+  LAUNCH(k_fill_i32, nn + 1, ctx->d_cursor, 0, nn + 1);
+  LAUNCH((k_bin<D, B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_cnt, ctx->d_flags);
+  scan_bins(ctx);
+  LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_cursor, ctx->d_perm, np);
+  LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_perm, nn);
+  return JUMP_OK;
+}
+
+template <int B> static void step_usl2d(jump_ctx* ctx) {
+  const int np = int(ctx->np);
+  const double dt = ctx->prm.dt;
+  prof_begin(ctx);
+  bin_and_sort<2, B>(ctx);
+  prof_end(ctx, JUMP_PHASE_SORT);
+  prof_begin(ctx);
+  LAUNCH((k_p2g<2, B, 0>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->g,
+         ctx->cpdi_w, dt);
+  prof_end(ctx, JUMP_PHASE_P2G);
+  prof_begin(ctx);
+  for (auto& s : ctx->solids)
+    if (s.dev.rigid)
+      LAUNCH(k_rigid_override, s.n, ctx->P, ctx->G, s.dev, int(s.offset), int(s.offset + s.n), ctx->g, ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_RIGID);
+  prof_begin(ctx);
+  LAUNCH((k_g2p_usl2d<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.tolerance, dt,
+         ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_G2P);
+  prof_begin(ctx);
+  for (auto& s : ctx->solids)
+    if (s.dev.rigid) LAUNCH(k_rigid_move, s.n, ctx->P, s.dev, int(s.offset), int(s.offset + s.n), dt);
+  prof_end(ctx, JUMP_PHASE_RIGID);
+}
+
+template <int B> static void step_musl3d(jump_ctx* ctx) {
+  const int np = int(ctx->np);
+  const double dt = ctx->prm.dt;
+  prof_begin(ctx);
+  bin_and_sort<3, B>(ctx);
+  prof_end(ctx, JUMP_PHASE_SORT);
+  prof_begin(ctx);
+  LAUNCH((k_p2g<3, B, 0>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->g,
+         0, dt);
+  prof_end(ctx, JUMP_PHASE_P2G);
+  prof_begin(ctx);
+  LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_G2P1);
+  prof_begin(ctx);
+  LAUNCH((k_p2g<3, B, 1>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->g,
+         0, dt);
+  prof_end(ctx, JUMP_PHASE_P2G2);
+  // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
+  // zeroes components of momentum0/momentum already zeroed by the grid update.
+  prof_begin(ctx);
+  LAUNCH((k_g2p2_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, dt, ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_G2P2);
+}
+
+static int prepare_cpdi(jump_ctx* ctx) {
+  if (ctx->prm.basis != JUMP_BASIS_CPDI_Q4 || ctx->cpdi_w > 0) return JUMP_OK;
+  int zero = 0;
+  HIP_TRY(hipMemcpyAsync(ctx->d_wmax, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  LAUNCH(k_cpdi_width, ctx->np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_wmax);
+  int w = 0;
+  HIP_TRY(hipMemcpyAsync(&w, ctx->d_wmax, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (w > MaxW<JUMP_BASIS_CPDI_Q4>::W)
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "CPDI particle domain spans more than 4 cells");
+  ctx->cpdi_w = std::max(w, 2);
+  return JUMP_OK;
+}
+
+static int run_steps(jump_ctx* ctx, int64_t nsteps) {
+  for (int64_t s = 0; s < nsteps; s++) {
+    if (ctx->D == 2) {
+      switch (ctx->prm.basis) {
+        case JUMP_BASIS_LINEAR: step_usl2d<JUMP_BASIS_LINEAR>(ctx); break;
+        case JUMP_BASIS_QUAD_BSPLINE: step_usl2d<JUMP_BASIS_QUAD_BSPLINE>(ctx); break;
+        case JUMP_BASIS_CUBIC_BSPLINE: step_usl2d<JUMP_BASIS_CUBIC_BSPLINE>(ctx); break;
+        case JUMP_BASIS_CPDI_Q4: step_usl2d<JUMP_BASIS_CPDI_Q4>(ctx); break;
+        case JUMP_BASIS_GIMP: step_usl2d<JUMP_BASIS_GIMP>(ctx); break;
+      }
+    } else {
+      switch (ctx->prm.basis) {
+        case JUMP_BASIS_LINEAR: step_musl3d<JUMP_BASIS_LINEAR>(ctx); break;
+        case JUMP_BASIS_QUAD_BSPLINE: step_musl3d<JUMP_BASIS_QUAD_BSPLINE>(ctx); break;
+        case JUMP_BASIS_CUBIC_BSPLINE: step_musl3d<JUMP_BASIS_CUBIC_BSPLINE>(ctx); break;
+        case JUMP_BASIS_GIMP: step_musl3d<JUMP_BASIS_GIMP>(ctx); break;
+      }
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return JUMP_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
+  if (!p || !out) return JUMP_ERR_ARG;
+  *out = nullptr;
+  if (p->dim != 2 && p->dim != 3) return JUMP_ERR_ARG;
+  if (p->basis < JUMP_BASIS_LINEAR || p->basis > JUMP_BASIS_GIMP) return JUMP_ERR_ARG;
+  if (p->dim == 3 && p->basis == JUMP_BASIS_CPDI_Q4) return JUMP_ERR_UNSUPPORTED;
+  // The reference implements USL in 2D and MUSL in 3D on this path (2D MUSL
+  // needs a consistent mass solve; 3D USL is marked unfinished and broken).
+  if (p->dim == 2 && p->algorithm != JUMP_ALG_USL) return JUMP_ERR_UNSUPPORTED;
+  if (p->dim == 3 && p->algorithm != JUMP_ALG_MUSL) return JUMP_ERR_UNSUPPORTED;
+  if (p->nx < 2 || p->ny < 2 || (p->dim == 3 && p->nz < 2)) return JUMP_ERR_ARG;
+  const int64_t nz = p->dim == 3 ? p->nz : 1;
+  if (p->nx * p->ny * nz >= (int64_t(1) << 31) - 2) return JUMP_ERR_UNSUPPORTED;
+  if (!(p->dt > 0.0)) return JUMP_ERR_ARG;
+  jump_ctx* ctx = new (std::nothrow) jump_ctx();
+  if (!ctx) return JUMP_ERR_NOMEM;
+  ctx->prm = *p;
+  ctx->D = p->dim;
+  ctx->device = device;
+  GridDesc& G = ctx->G;
+  G.dim = p->dim;
+  G.nx = int(p->nx);
+  G.ny = int(p->ny);
+  G.nz = int(nz);
+  G.nxy = G.nx * G.ny;
+  G.nn = G.nxy * G.nz;
+  // Grid.jl:105-108, 199-204
+  G.xmin = p->xmin;
+  G.ymin = p->ymin;
+  G.zmin = p->dim == 3 ? p->zmin : 0.0;
+  G.dx = (p->xmax - p->xmin) / double(double(p->nx) - 1.0);
+  G.dy = (p->ymax - p->ymin) / double(double(p->ny) - 1.0);
+  G.dz = p->dim == 3 ? (p->zmax - p->zmin) / double(double(p->nz) - 1.0) : 1.0;
+  G.dxI = 1.0 / G.dx;
+  G.dyI = 1.0 / G.dy;
+  G.dzI = 1.0 / G.dz;
+  for (int k = 0; k < 3; k++) G.body[k] = p->body[k];
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return JUMP_ERR_DEVICE;
+  }
+  int r = alloc_grid(ctx);
+  if (r) {
+    free_list(ctx->grid_allocs);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return r;
+  }
+  ctx->hfixed.assign(size_t(G.nn), 0);
+  *out = ctx;
+  return JUMP_OK;
+}
+
+int jump_ctx_destroy(jump_ctx* ctx) {
+  if (!ctx) return JUMP_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_list(ctx->part_allocs);
+  free_list(ctx->grid_allocs);
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return JUMP_OK;
+}
+
+const char* jump_last_error(const jump_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material* mat, int32_t* idx) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (!ds || !mat) return set_err(ctx, JUMP_ERR_ARG, "null solid or material");
+  const int D = ctx->D, S = nsym(D);
+  if (ds->n < 0 || (ds->n > 0 && (!ds->pos || !ds->vel || !ds->mass || !ds->volume || !ds->volume0)))
+    return set_err(ctx, JUMP_ERR_ARG, "missing particle arrays");
+  if (mat->kind < JUMP_MAT_RIGID || mat->kind > JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_ARG, "bad material");
+  if (D == 3 && mat->kind == JUMP_MAT_RIGID)
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid (Solid3D has no rigid flag)");
+  if (D == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");
+  if (ctx->np + ds->n >= (int64_t(1) << 31) - 1) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "too many particles");
+  const bool cpdi = ctx->prm.basis == JUMP_BASIS_CPDI_Q4;
+  if (cpdi && (!ds->corners || !ds->elems || ds->ncorners <= 0))
+    return set_err(ctx, JUMP_ERR_ARG, "CPDI solid needs corners and elems");
+  double lp[3] = {0.0, 0.0, 0.0};
+  if (ctx->prm.basis == JUMP_BASIS_GIMP && mat->kind != JUMP_MAT_RIGID) {
+    const double hs[3] = {ctx->G.dx, ctx->G.dy, ctx->G.dz};
+    for (int k = 0; k < D; k++) {
+      if (!(ds->lp[k] > 0.0) || ds->lp[k] > 0.5 * hs[k])
+        return set_err(ctx, JUMP_ERR_ARG, "GIMP lp must be in (0, dx/2]");
+      lp[k] = ds->lp[k];
+    }
+  }
+  const size_t n = size_t(ds->n);
+  // symmetric inputs only (the reference's strain/stress are symmetric)
+  for (size_t i = 0; i < n; i++)
+    for (int a = 0; a < D; a++)
+      for (int b = a + 1; b < D; b++) {
+        if (ds->strain && ds->strain[D * D * i + a + D * b] != ds->strain[D * D * i + b + D * a])
+          return set_err(ctx, JUMP_ERR_ARG, "strain must be symmetric");
+        if (ds->stress && ds->stress[D * D * i + a + D * b] != ds->stress[D * D * i + b + D * a])
+          return set_err(ctx, JUMP_ERR_ARG, "stress must be symmetric");
+      }
+  if (cpdi)
+    for (size_t i = 0; i < 4 * n; i++)
+      if (ds->elems[i] < 0 || ds->elems[i] >= ds->ncorners) return set_err(ctx, JUMP_ERR_ARG, "CPDI elems out of range");
+  // bring the current device state back before growing the arrays
+  int r = download(ctx);
+  if (r) return r;
+  ctx->on_device = false;
+  HostSolid hs;
+  hs.n = ds->n;
+  hs.mat = *mat;
+  hs.dev = make_solid_dev(*mat, lp);
+  hs.offset = ctx->np;
+  const int sid = int(ctx->solids.size());
+  const int coff = int(ctx->corners.size() / 2);
+  if (cpdi) ctx->corners.insert(ctx->corners.end(), ds->corners, ds->corners + 2 * ds->ncorners);
+  for (size_t i = 0; i < n; i++) {
+    for (int k = 0; k < D; k++) {
+      ctx->hx[k].push_back(ds->pos[D * i + k]);
+      ctx->hv[k].push_back(ds->vel[D * i + k]);
+    }
+    ctx->hm.push_back(ds->mass[i]);
+    ctx->hV.push_back(ds->volume[i]);
+    ctx->hV0.push_back(ds->volume0[i]);
+    for (int k = 0; k < D * D; k++)
+      ctx->hF[k].push_back(ds->F ? ds->F[D * D * i + k] : ((k % (D + 1)) == 0 ? 1.0 : 0.0));
+    // compressed symmetric components in storage order
+    const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+    const int ia2[3] = {0, 1, 0}, ib2[3] = {0, 1, 1};
+    for (int c = 0; c < S; c++) {
+      int a = D == 2 ? ia2[c] : ia[c], b = D == 2 ? ib2[c] : ib[c];
+      ctx->heps[c].push_back(ds->strain ? ds->strain[D * D * i + a + D * b] : 0.0);
+      ctx->hsig[c].push_back(ds->stress ? ds->stress[D * D * i + a + D * b] : 0.0);
+      ctx->hep[c].push_back(0.0);
+    }
+    ctx->halpha.push_back(0.0);
+    ctx->halphadot.push_back(0.0);
+    ctx->hvm.push_back(0.0);
+    ctx->hsid.push_back(sid);
+    if (cpdi)
+      for (int q = 0; q < 4; q++) ctx->helem[q].push_back(coff + int(ds->elems[4 * i + q]));
+  }
+  ctx->np += ds->n;
+  ctx->solids.push_back(hs);
+  if (idx) *idx = sid;
+  return JUMP_OK;
+}
+
+int jump_set_fixed(jump_ctx* ctx, const uint8_t* mask) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (!mask) return set_err(ctx, JUMP_ERR_ARG, "null mask");
+  const int D = ctx->D;
+  for (int64_t i = 0; i < ctx->G.nn; i++) {
+    uint8_t b = 0;
+    for (int k = 0; k < D; k++)
+      if (mask[D * i + k]) b |= uint8_t(1u << k);
+    ctx->hfixed[size_t(i)] = b;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMemcpyAsync(ctx->d_fixed, ctx->hfixed.data(), ctx->hfixed.size(), hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return JUMP_OK;
+}
+
+int jump_step(jump_ctx* ctx, int64_t nsteps, double* t_inout) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (!t_inout || nsteps < 0) return set_err(ctx, JUMP_ERR_ARG, "bad step arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int r = ensure_device(ctx);
+  if (r) return r;
+  if (ctx->np == 0) {
+    for (int64_t s = 0; s < nsteps; s++) *t_inout += ctx->prm.dt;
+    return JUMP_OK;
+  }
+  r = prepare_cpdi(ctx);
+  if (r) return r;
+  r = run_steps(ctx, nsteps);
+  if (r) return r;
+  unsigned fl = 0;
+  HIP_TRY(hipMemcpyAsync(&fl, ctx->d_flags, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  for (int64_t s = 0; s < nsteps; s++) *t_inout += ctx->prm.dt;
+  if (nsteps > 0) ctx->grid_valid = true;
+  ctx->flags |= fl;
+  if (fl) return set_err(ctx, JUMP_ERR_NUMERIC, "particle error flags raised during the step");
+  return JUMP_OK;
+}
+
+int jump_flags(jump_ctx* ctx, uint32_t* flags) {
+  if (!ctx || !flags) return JUMP_ERR_ARG;
+  *flags = ctx->flags;
+  ctx->flags = 0;
+  return JUMP_OK;
+}
+
+// Copy the particle range of one solid, converting SoA -> AoS.
+int jump_get_solid(jump_ctx* ctx, int32_t si, double* pos, double* vel, double* mass, double* vol, double* F,
+                   double* strain, double* stress) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (si < 0 || size_t(si) >= ctx->solids.size()) return set_err(ctx, JUMP_ERR_ARG, "bad solid index");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int r = download(ctx);
+  if (r) return r;
+  const HostSolid& s = ctx->solids[size_t(si)];
+  const int D = ctx->D;
+  const size_t o = size_t(s.offset);
+  for (size_t i = 0; i < size_t(s.n); i++) {
+    for (int k = 0; k < D; k++) {
+      if (pos) pos[D * i + k] = ctx->hx[k][o + i];
+      if (vel) vel[D * i + k] = ctx->hv[k][o + i];
+    }
+    if (mass) mass[i] = ctx->hm[o + i];
+    if (vol) vol[i] = ctx->hV[o + i];
+    for (int k = 0; k < D * D; k++)
+      if (F) F[D * D * i + k] = ctx->hF[k][o + i];
+    for (int a = 0; a < D; a++)
+      for (int b = 0; b < D; b++) {
+        int c = a == b ? a : (D == 2 ? 2 : (a + b == 1 ? 3 : (a + b == 2 ? 4 : 5)));
+        if (strain) strain[D * D * i + a + D * b] = ctx->heps[c][o + i];
+        if (stress) stress[D * D * i + a + D * b] = ctx->hsig[c][o + i];
+      }
+  }
+  return JUMP_OK;
+}
+
+int jump_get_solid_state(jump_ctx* ctx, int32_t si, double* alpha, double* alphadot, double* vm, double* pstrain) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (si < 0 || size_t(si) >= ctx->solids.size()) return set_err(ctx, JUMP_ERR_ARG, "bad solid index");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int r = download(ctx);
+  if (r) return r;
+  const HostSolid& s = ctx->solids[size_t(si)];
+  const int D = ctx->D;
+  const size_t o = size_t(s.offset);
+  for (size_t i = 0; i < size_t(s.n); i++) {
+    if (alpha) alpha[i] = ctx->halpha[o + i];
+    if (alphadot) alphadot[i] = ctx->halphadot[o + i];
+    if (vm) vm[i] = ctx->hvm[o + i];
+    if (pstrain) {
+      if (D == 2) {
+        for (int c = 0; c < 3; c++) pstrain[3 * i + c] = ctx->hep[c][o + i];
+      } else {
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) {
+            int c = a == b ? a : (a + b == 1 ? 3 : (a + b == 2 ? 4 : 5));
+            pstrain[9 * i + a + 3 * b] = ctx->hep[c][o + i];
+          }
+      }
+    }
+  }
+  return JUMP_OK;
+}
+
+int jump_get_grid(jump_ctx* ctx, double* mass, double* p0, double* p, double* p2, double* f) {
+  if (!ctx) return JUMP_ERR_ARG;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t nn = size_t(ctx->G.nn);
+  const int D = ctx->D;
+  if (!ctx->grid_valid) {
+    if (mass) std::fill(mass, mass + nn, 0.0);
+    double* vs[4] = {p0, p, p2, f};
+    for (double* v : vs)
+      if (v) std::fill(v, v + D * nn, 0.0);
+    return JUMP_OK;
+  }
+  std::vector<double> tmp(nn);
+  if (mass) HIP_TRY(hipMemcpyAsync(mass, ctx->g.m, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  double* outs[4] = {p0, p, p2, f};
+  double* const* srcs[4] = {ctx->g.p0, ctx->g.p, ctx->g.p2, ctx->g.f};
+  for (int a = 0; a < 4; a++) {
+    if (!outs[a]) continue;
+    for (int k = 0; k < D; k++) {
+      HIP_TRY(hipMemcpyAsync(tmp.data(), srcs[a][k], nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      for (size_t i = 0; i < nn; i++) outs[a][D * i + k] = tmp[i];
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return JUMP_OK;
+}
+
+int jump_energies(jump_ctx* ctx, double* ke, double* se) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (!ke || !se) return set_err(ctx, JUMP_ERR_ARG, "null output");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int r = ensure_device(ctx);
+  if (r) return r;
+  if (ctx->np == 0) {
+    *ke = 0.0;
+    *se = 0.0;
+    return JUMP_OK;
+  }
+  const int nb = blocks_for(ctx->np);
+  double* out = ctx->d_red + 2 * size_t(nb);
+  if (ctx->D == 2) LAUNCH(k_energy_partial<2>, ctx->np, ctx->P, ctx->d_red);
+  else LAUNCH(k_energy_partial<3>, ctx->np, ctx->P, ctx->d_red);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_red, nb, out);
+  double h[2];
+  HIP_TRY(hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *ke = h[0];
+  *se = h[1];
+  return JUMP_OK;
+}
+
+int jump_get_bins(jump_ctx* ctx, int32_t* keys) {
+  if (!ctx || !keys) return JUMP_ERR_ARG;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int r = ensure_device(ctx);
+  if (r) return r;
+  if (ctx->np == 0) return JUMP_OK;
+  r = prepare_cpdi(ctx);
+  if (r) return r;
+  if (ctx->D == 2) {
+    switch (ctx->prm.basis) {
+      case JUMP_BASIS_LINEAR: bin_and_sort<2, JUMP_BASIS_LINEAR>(ctx); break;
+      case JUMP_BASIS_QUAD_BSPLINE: bin_and_sort<2, JUMP_BASIS_QUAD_BSPLINE>(ctx); break;
+      case JUMP_BASIS_CUBIC_BSPLINE: bin_and_sort<2, JUMP_BASIS_CUBIC_BSPLINE>(ctx); break;
+      case JUMP_BASIS_CPDI_Q4: bin_and_sort<2, JUMP_BASIS_CPDI_Q4>(ctx); break;
+      case JUMP_BASIS_GIMP: bin_and_sort<2, JUMP_BASIS_GIMP>(ctx); break;
+    }
+  } else {
+    switch (ctx->prm.basis) {
+      case JUMP_BASIS_LINEAR: bin_and_sort<3, JUMP_BASIS_LINEAR>(ctx); break;
+      case JUMP_BASIS_QUAD_BSPLINE: bin_and_sort<3, JUMP_BASIS_QUAD_BSPLINE>(ctx); break;
+      case JUMP_BASIS_CUBIC_BSPLINE: bin_and_sort<3, JUMP_BASIS_CUBIC_BSPLINE>(ctx); break;
+      case JUMP_BASIS_GIMP: bin_and_sort<3, JUMP_BASIS_GIMP>(ctx); break;
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(keys, ctx->d_key, size_t(ctx->np) * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return JUMP_OK;
+}
+
+int jump_profile(jump_ctx* ctx, int32_t enable) {
+  if (!ctx) return JUMP_ERR_ARG;
+  ctx->prof = enable != 0;
+  for (int k = 0; k < JUMP_PHASE_COUNT; k++) {
+    ctx->prof_ms[k] = 0.0;
+    ctx->prof_cnt[k] = 0;
+  }
+  ctx->prof_recs.clear();
+  ctx->ev_used = 0;
+  return JUMP_OK;
+}
+
+int jump_profile_read(jump_ctx* ctx, int32_t phase, double* total_ms, int64_t* count) {
+  if (!ctx || phase < 0 || phase >= JUMP_PHASE_COUNT || !total_ms || !count) return JUMP_ERR_ARG;
+  *total_ms = ctx->prof_ms[phase];
+  *count = ctx->prof_cnt[phase];
+  return JUMP_OK;
+}
+
+int jump_abi_version(void) { return JUMP_ABI_VERSION; }
+
+}  // extern "C"

@@ -1,0 +1,694 @@
+// mpm_kernels.hpp -- HIP kernels of one explicit MPM step (gfx950, wave64).
+//
+// Step structure (USL 2D, SolveExplicitSolidMechanics2D.jl:373-581;
+// MUSL 3D, SolveExplicitSolidMechanics3D.jl:41-220):
+//   k_bin        per particle: stencil origin node (the reference's integer
+//                support rule, bit-exact) -> bin key, per-bin count
+//   k_scan_*     exclusive scan of the bin counts -> bin_start
+//   k_scatter    particle index into its bin segment
+//   k_binsort    each bin segment sorted by particle index (deterministic)
+//   k_p2g        per grid node: gather of every particle whose stencil covers
+//                the node, in bin order then particle order -- a segmented
+//                reduction with a fixed order, no floating-point atomics,
+//                bitwise reproducible run to run; fused with the grid update
+//                p = p0 + f dt and the Dirichlet mask
+//   k_rigid_*    2D rigid bodies: grid override and motion
+//   k_g2p_*      per particle: gather of grid velocity, position, F, J, V and
+//                the constitutive update, in the reference's node order
+//   k_energy_*   kinetic/strain energy, fixed-order tree reduction
+#pragma once
+
+#include "mpm_device.hpp"
+
+namespace jh {
+
+constexpr int kBlock = 256;
+
+__global__ void k_fill_i32(int* __restrict__ a, int v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// ------------------------------------------------------------------ binning
+// key[i] = linear index of the particle's first stencil node (0-based), or -1
+// for particles that take no part in P2G (2D rigid bodies, out of grid).
+template <int D, int B>
+__global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                const double* __restrict__ corners, int* __restrict__ key,
+                                                int* __restrict__ cnt, unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  int k = -1;
+  if (!(D == 2 && s.rigid)) {
+    bool ok = true;
+    int f[3] = {1, 1, 1};
+    if (B == JUMP_BASIS_CPDI_Q4) {
+      CpdiW c;
+      ok = cpdi_setup(corners, P.elem[0][i], P.elem[1][i], P.elem[2][i], P.elem[3][i], G, c);
+      if (ok) {
+        f[0] = min(min(c.ci[0], c.ci[1]), min(c.ci[2], c.ci[3]));
+        f[1] = min(min(c.cj[0], c.cj[1]), min(c.cj[2], c.cj[3]));
+      }
+    } else {
+      for (int a = 0; a < D; a++) ok = ok && first_node<B, D>(P.x[a][i], a, axis_of(G, a), s.lp[a], f[a]);
+    }
+    if (ok) {
+      k = (f[0] - 1) + G.nx * ((f[1] - 1) + (D == 3 ? G.ny * (f[2] - 1) : 0));
+      atomicAdd(&cnt[k], 1);
+    } else {
+      atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    }
+  }
+  key[i] = k;
+}
+
+// CPDI stencil extent (static corners): max bounding-box width in nodes.
+__global__ void __launch_bounds__(kBlock) k_cpdi_width(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                       const double* __restrict__ corners, int* __restrict__ wmax) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  if (S[P.sid[i]].rigid) return;
+  CpdiW c;
+  if (!cpdi_setup(corners, P.elem[0][i], P.elem[1][i], P.elem[2][i], P.elem[3][i], G, c)) return;
+  int w = max(max(max(c.ci[0], c.ci[1]), max(c.ci[2], c.ci[3])) - min(min(c.ci[0], c.ci[1]), min(c.ci[2], c.ci[3])),
+              max(max(c.cj[0], c.cj[1]), max(c.cj[2], c.cj[3])) - min(min(c.cj[0], c.cj[1]), min(c.cj[2], c.cj[3]))) +
+          2;
+  atomicMax(wmax, w);
+}
+
+// Exclusive scan, 1024 items per block (256 threads x 4), three passes.
+constexpr int kScanItems = 1024;
+
+__device__ __forceinline__ int block_excl_scan(int v, int* lds, int& total) {
+  // lds: kBlock ints
+  int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int add = t >= off ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  total = lds[kBlock - 1];
+  int incl = lds[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_block(const int* __restrict__ in, int* __restrict__ out,
+                                                       int* __restrict__ sums, int n) {
+  __shared__ int lds[kBlock];
+  int base = blockIdx.x * kScanItems + threadIdx.x * 4;
+  int v[4];
+  for (int j = 0; j < 4; j++) v[j] = (base + j < n) ? in[base + j] : 0;
+  int tsum = v[0] + v[1] + v[2] + v[3];
+  int total;
+  int off = block_excl_scan(tsum, lds, total);
+  for (int j = 0; j < 4; j++) {
+    if (base + j < n) out[base + j] = off;
+    off += v[j];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the block sums in place.
+__global__ void __launch_bounds__(kBlock) k_scan_sums(int* __restrict__ sums, int nb) {
+  __shared__ int lds[kBlock];
+  int carry = 0;
+  for (int base = 0; base < nb; base += kBlock) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? sums[i] : 0;
+    int total;
+    int ex = block_excl_scan(v, lds, total);
+    if (i < nb) sums[i] = ex + carry;
+    carry += total;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_add(int* __restrict__ out, const int* __restrict__ sums, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += sums[i / kScanItems];
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter(const int* __restrict__ key, const int* __restrict__ start,
+                                                    int* __restrict__ cursor, int* __restrict__ perm, int np) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int k = key[i];
+  if (k < 0) return;
+  int pos = start[k] + atomicAdd(&cursor[k], 1);
+  perm[pos] = i;
+}
+
+// Order each bin segment by particle index so the P2G sum order is fixed.
+__global__ void __launch_bounds__(kBlock) k_binsort(const int* __restrict__ start, int* __restrict__ perm, int nbins) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int s = start[b], e = start[b + 1];
+  for (int i = s + 1; i < e; i++) {
+    int v = perm[i];
+    int j = i - 1;
+    while (j >= s && perm[j] > v) {
+      perm[j + 1] = perm[j];
+      j--;
+    }
+    perm[j + 1] = v;
+  }
+}
+
+// ------------------------------------------------------------------ P2G
+// Weight N and gradient of node (ii,jj,kk) (1-based) for particle p.
+// QUIRK selects the getShapeFunctions variant (used by the MUSL
+// re-projection): QBS boundary typing, cubic z-axis type-4 test on jd.
+template <int D, int B, bool QUIRK, bool GRAD>
+__device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& G, const SolidDev& s,
+                                            const double* corners, int p, int ii, int jj, int kk, double& N,
+                                            double* dN, bool& member) {
+  member = true;
+  if (B == JUMP_BASIS_CPDI_Q4) {
+    CpdiW c;
+    cpdi_setup(corners, P.elem[0][p], P.elem[1][p], P.elem[2][p], P.elem[3][p], G, c);
+    member = cpdi_member(c, ii, jj);
+    if (member) cpdi_node(c, G, ii, jj, N, dN);
+    return;
+  }
+  Axis X = axis_of(G, 0), Y = axis_of(G, 1);
+  double wx, dwx, wy, dwy;
+  axis_weight<B, QUIRK>(P.x[0][p], 0, X, s.lp[0], ii, ii, wx, dwx);
+  axis_weight<B, QUIRK>(P.x[1][p], 1, Y, s.lp[1], jj, jj, wy, dwy);
+  if (D == 2) {
+    if (GRAD) combine2<B>(X, Y, wx, dwx, wy, dwy, N, dN);
+    else N = wx * wy;
+  } else {
+    Axis Z = axis_of(G, 2);
+    double wz, dwz;
+    int sel4 = (QUIRK && B == JUMP_BASIS_CUBIC_BSPLINE) ? jj : kk;
+    axis_weight<B, QUIRK>(P.x[2][p], 2, Z, s.lp[2], kk, sel4, wz, dwz);
+    if (GRAD) combine3<B>(X, Y, Z, wx, dwx, wy, dwy, wz, dwz, N, dN);
+    else N = wx * wy * wz;
+  }
+}
+
+// Largest stencil window per axis: the basis width, or for CPDI the widest
+// corner bounding box (checked on the host).
+template <int B> struct MaxW { static constexpr int W = Width<B>::W; };
+template <> struct MaxW<JUMP_BASIS_CPDI_Q4> { static constexpr int W = 5; };
+constexpr int kNoParticle = 0x7fffffff;
+
+// PASS 0: mass, momentum0, force, then p = p0 + f dt and the Dirichlet mask
+//         (SolveExplicitSolidMechanics2D.jl:391-442; 3D.jl:55-105).
+// PASS 1: MUSL re-projection momentum2 = sum N m v (3D.jl:135-138).
+//
+// Summation order.  The reference adds each particle's contribution to a
+// node in particle order (solids in order, then particle index: the loops at
+// SolveExplicitSolidMechanics2D.jl:391-401).  A node's contributors are the
+// particles of the W^d bins of its window, each bin sorted by particle
+// index; the kernel merges those sorted lists (a k-way merge that consumes a
+// whole run from one bin while its next index stays below the smallest head
+// of the other bins), so every node folds its contributions in exactly the
+// reference's order and the grid equals the reference's bit for bit.
+template <int D, int B, int PASS>
+__global__ void __launch_bounds__(kBlock) k_p2g(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                const double* __restrict__ corners, const int* __restrict__ start,
+                                                const int* __restrict__ perm, GridArrays g, int Wrt, double dt) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.nn) return;
+  constexpr int MW = MaxW<B>::W;
+  constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
+  const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
+  const int ni = n % G.nx;
+  const int nj = (n / G.nx) % G.ny;
+  const int nk = (D == 3) ? n / G.nxy : 0;
+  // window bins (bin = stencil origin node); head = next particle index
+  int head[NB], cur[NB], endb[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
+    const int bi = ni - W + 1 + a0, bj = nj - W + 1 + a1, bk = (D == 3) ? nk - W + 1 + a2 : 0;
+    head[b] = kNoParticle;
+    cur[b] = 0;
+    endb[b] = 0;
+    if (a0 < W && a1 < W && a2 < W && bi >= 0 && bj >= 0 && bk >= 0) {
+      const int bin = bi + G.nx * (bj + G.ny * bk);
+      const int s0 = start[bin], e0 = start[bin + 1];
+      if (s0 < e0) {
+        cur[b] = s0;
+        endb[b] = e0;
+        head[b] = perm[s0];
+      }
+    }
+  }
+  double am = 0.0, ap[D], af[D];
+  for (int k = 0; k < D; k++) { ap[k] = 0.0; af[k] = 0.0; }
+  while (true) {
+    int best = kNoParticle, second = kNoParticle, sel = 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int h = head[b];
+      if (h < best) { second = best; best = h; sel = b; }
+      else if (h < second) second = h;
+    }
+    if (best == kNoParticle) break;
+    int c = 0, e = 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+      if (b == sel) { c = cur[b]; e = endb[b]; }
+    int p = best;
+    while (true) {
+      const SolidDev& s = S[P.sid[p]];
+      double N, dN[3];
+      bool member;
+      node_weight<D, B, PASS == 1, PASS == 0>(P, G, s, corners, p, ni + 1, nj + 1, nk + 1, N, dN, member);
+      if (member) {
+        const double fM = P.m[p];
+        if (PASS == 0) {
+          const double fV = P.V[p];
+          const double Nim = N * fM;
+          am += Nim;
+          for (int k = 0; k < D; k++) ap[k] += Nim * P.v[k][p];
+          if (D == 2) {
+            const double s11 = P.sig[0][p], s22 = P.sig[1][p], s12 = P.sig[2][p];
+            af[0] += (-fV) * (s11 * dN[0] + s12 * dN[1]) + (fM * G.body[0]) * N;
+            af[1] += (-fV) * (s12 * dN[0] + s22 * dN[1]) + (fM * G.body[1]) * N;
+          } else {
+            const double s11 = P.sig[0][p], s22 = P.sig[1][p], s33 = P.sig[2][p];
+            const double s12 = P.sig[3][p], s13 = P.sig[4][p], s23 = P.sig[5][p];
+            af[0] -= fV * (s11 * dN[0] + s12 * dN[1] + s13 * dN[2]);
+            af[1] -= fV * (s12 * dN[0] + s22 * dN[1] + s23 * dN[2]);
+            af[2] -= fV * (s13 * dN[0] + s23 * dN[1] + s33 * dN[2]);
+          }
+        } else {
+          const double Nm = N * fM;
+          for (int k = 0; k < D; k++) ap[k] += Nm * P.v[k][p];
+        }
+      }
+      ++c;
+      if (c >= e) { p = kNoParticle; break; }
+      p = perm[c];
+      if (p > second) break;
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+      if (b == sel) { cur[b] = c; head[b] = p; }
+  }
+  if (PASS == 0) {
+    const uint8_t fx = g.fixed[n];
+    g.m[n] = am;
+    for (int k = 0; k < D; k++) {
+      double p0 = ap[k];
+      double pk = p0 + af[k] * dt;
+      if (fx & (1u << k)) { p0 = 0.0; pk = 0.0; }
+      g.p0[k][n] = p0;
+      g.p[k][n] = pk;
+      g.f[k][n] = af[k];
+    }
+  } else {
+    for (int k = 0; k < D; k++) g.p2[k][n] = ap[k];
+  }
+}
+
+// ------------------------------------------------------------------ rigid (2D)
+// SolveExplicitSolidMechanics2D.jl:448-479.  All particles of one rigid solid
+// write the same function of the node mass, so concurrent writes agree;
+// solids are launched one after another in solid order.
+__global__ void __launch_bounds__(kBlock) k_rigid_override(Particles P, GridDesc G, SolidDev s, int begin, int end,
+                                                           GridArrays g, unsigned* __restrict__ flags) {
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int f[2];
+  bool ok = first_node<JUMP_BASIS_LINEAR, 2>(P.x[0][i], 0, axis_of(G, 0), 0.0, f[0]) &&
+            first_node<JUMP_BASIS_LINEAR, 2>(P.x[1][i], 1, axis_of(G, 1), 0.0, f[1]);
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  int b = (f[0] - 1) + G.nx * (f[1] - 1);
+  int ids[4] = {b, b + 1, b + G.nx, b + 1 + G.nx};
+  for (int a = 0; a < 4; a++) {
+    int id = ids[a];
+    double mi = g.m[id];
+    if (s.fixed) {
+      g.p[0][id] = 0.0; g.p[1][id] = 0.0;
+      g.p0[0][id] = 0.0; g.p0[1][id] = 0.0;
+    } else {
+      if (s.vx != 0.0) { g.p[0][id] = mi * s.vx; g.p0[0][id] = mi * s.vx; }
+      if (s.vy != 0.0) { g.p[1][id] = mi * s.vy; g.p0[1][id] = mi * s.vy; }
+    }
+  }
+}
+
+// SolveExplicitSolidMechanics2D.jl:539-551
+__global__ void __launch_bounds__(kBlock) k_rigid_move(Particles P, SolidDev s, int begin, int end, double dt) {
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  P.x[0][i] = P.x[0][i] + dt * s.vx;
+  P.x[1][i] = P.x[1][i] + dt * s.vy;
+}
+
+// ------------------------------------------------------------------ stencil walk
+// Calls fn(node, N, dN) for every support node of particle p in the
+// reference's node order: linear -- y (z) outer, x inner (Basis.jl:77,132);
+// B-splines/GIMP -- x outer, z inner (Basis.jl:428-459); CPDI -- ascending
+// node index.  Returns false if the stencil leaves the grid.
+template <int D, int B, bool QUIRK, bool GRAD, class Fn>
+__device__ __forceinline__ bool walk_stencil(const Particles& P, const GridDesc& G, const SolidDev& s,
+                                             const double* corners, int p, Fn&& fn) {
+  if (B == JUMP_BASIS_CPDI_Q4) {
+    CpdiW c;
+    if (!cpdi_setup(corners, P.elem[0][p], P.elem[1][p], P.elem[2][p], P.elem[3][p], G, c)) return false;
+    int i0 = min(min(c.ci[0], c.ci[1]), min(c.ci[2], c.ci[3]));
+    int i1 = max(max(c.ci[0], c.ci[1]), max(c.ci[2], c.ci[3])) + 1;
+    int j0 = min(min(c.cj[0], c.cj[1]), min(c.cj[2], c.cj[3]));
+    int j1 = max(max(c.cj[0], c.cj[1]), max(c.cj[2], c.cj[3])) + 1;
+    for (int jj = j0; jj <= j1; jj++)
+      for (int ii = i0; ii <= i1; ii++) {
+        if (!cpdi_member(c, ii, jj)) continue;
+        double N, dN[3];
+        cpdi_node(c, G, ii, jj, N, dN);
+        fn((ii - 1) + G.nx * (jj - 1), N, dN);
+      }
+    return true;
+  }
+  constexpr int W = Width<B>::W;
+  Axis A[3] = {axis_of(G, 0), axis_of(G, 1), axis_of(G, 2)};
+  int f[3] = {1, 1, 1};
+  double w[3][W], dw[3][W];
+  for (int a = 0; a < D; a++)
+    if (!first_node<B, D>(P.x[a][p], a, A[a], s.lp[a], f[a])) return false;
+  for (int a = 0; a < D; a++) {
+    if (D == 3 && a == 2 && QUIRK && B == JUMP_BASIS_CUBIC_BSPLINE) break;  // z depends on jd, below
+#pragma unroll
+    for (int o = 0; o < W; o++) axis_weight<B, QUIRK>(P.x[a][p], a, A[a], s.lp[a], f[a] + o, f[a] + o, w[a][o], dw[a][o]);
+  }
+  if (D == 2) {
+    if (B == JUMP_BASIS_LINEAR) {
+#pragma unroll
+      for (int b = 0; b < W; b++)
+#pragma unroll
+        for (int a = 0; a < W; a++) {
+          double N, dN[3];
+          if (GRAD) combine2<B>(A[0], A[1], w[0][a], dw[0][a], w[1][b], dw[1][b], N, dN);
+          else N = w[0][a] * w[1][b];
+          fn((f[0] + a - 1) + G.nx * (f[1] + b - 1), N, dN);
+        }
+    } else {
+#pragma unroll
+      for (int a = 0; a < W; a++)
+#pragma unroll
+        for (int b = 0; b < W; b++) {
+          double N, dN[3];
+          if (GRAD) combine2<B>(A[0], A[1], w[0][a], dw[0][a], w[1][b], dw[1][b], N, dN);
+          else N = w[0][a] * w[1][b];
+          fn((f[0] + a - 1) + G.nx * (f[1] + b - 1), N, dN);
+        }
+    }
+  } else {
+    if (B == JUMP_BASIS_LINEAR) {
+#pragma unroll
+      for (int c = 0; c < W; c++)
+#pragma unroll
+        for (int b = 0; b < W; b++)
+#pragma unroll
+          for (int a = 0; a < W; a++) {
+            double N, dN[3];
+            if (GRAD) combine3<B>(A[0], A[1], A[2], w[0][a], dw[0][a], w[1][b], dw[1][b], w[2][c], dw[2][c], N, dN);
+            else N = w[0][a] * w[1][b] * w[2][c];
+            fn((f[0] + a - 1) + G.nx * (f[1] + b - 1) + G.nxy * (f[2] + c - 1), N, dN);
+          }
+    } else {
+#pragma unroll
+      for (int a = 0; a < W; a++)
+#pragma unroll
+        for (int b = 0; b < W; b++) {
+          if (QUIRK && B == JUMP_BASIS_CUBIC_BSPLINE) {
+#pragma unroll
+            for (int c = 0; c < W; c++)
+              axis_weight<B, QUIRK>(P.x[2][p], 2, A[2], s.lp[2], f[2] + c, f[1] + b, w[2][c], dw[2][c]);
+          }
+#pragma unroll
+          for (int c = 0; c < W; c++) {
+            double N, dN[3];
+            if (GRAD) combine3<B>(A[0], A[1], A[2], w[0][a], dw[0][a], w[1][b], dw[1][b], w[2][c], dw[2][c], N, dN);
+            else N = w[0][a] * w[1][b] * w[2][c];
+            fn((f[0] + a - 1) + G.nx * (f[1] + b - 1) + G.nxy * (f[2] + c - 1), N, dN);
+          }
+        }
+    }
+  }
+  return true;
+}
+
+__device__ __forceinline__ bool finite3(double a, double b, double c) { return isfinite(a) && isfinite(b) && isfinite(c); }
+
+// ------------------------------------------------------------------ G2P, USL 2D
+// SolveExplicitSolidMechanics2D.jl:485-534
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                      const double* __restrict__ corners, GridArrays g, double tol,
+                                                      double dt, unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  if (s.rigid) return;
+  double v0 = P.v[0][i], v1 = P.v[1][i];
+  double x0 = P.x[0][i], x1 = P.x[1][i];
+  double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  bool ok = walk_stencil<2, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
+    const double mI = g.m[n];
+    if (mI > tol) {
+      const double invM = 1.0 / mI;
+      const double pn0 = g.p[0][n], pn1 = g.p[1][n];
+      const double vI0 = pn0 * invM, vI1 = pn1 * invM;
+      v0 += Ni * (pn0 - g.p0[0][n]) * invM;
+      v1 += Ni * (pn1 - g.p0[1][n]) * invM;
+      x0 += Ni * vI0 * dt;
+      x1 += Ni * vI1 * dt;
+      L0 += dN[0] * vI0;
+      L1 += dN[0] * vI1;
+      L2 += dN[1] * vI0;
+      L3 += dN[1] * vI1;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  P.v[0][i] = v0; P.v[1][i] = v1;
+  P.x[0][i] = x0; P.x[1][i] = x1;
+  // D = (L + L')/2, strain += dt D, F *= (I + L dt), J = det F, V = J V0
+  const double D11 = 0.5 * (L0 + L0), D22 = 0.5 * (L3 + L3), D12 = 0.5 * (L2 + L1);
+  double e[3] = {P.eps[0][i] + dt * D11, P.eps[1][i] + dt * D22, P.eps[2][i] + dt * D12};
+  P.eps[0][i] = e[0]; P.eps[1][i] = e[1]; P.eps[2][i] = e[2];
+  const double A0 = 1.0 + L0 * dt, A1 = L1 * dt, A2 = L2 * dt, A3 = 1.0 + L3 * dt;
+  const double F0 = P.F[0][i], F1 = P.F[1][i], F2 = P.F[2][i], F3 = P.F[3][i];
+  double Fn[4];
+  Fn[0] = F0 * A0 + F2 * A1;
+  Fn[1] = F1 * A0 + F3 * A1;
+  Fn[2] = F0 * A2 + F2 * A3;
+  Fn[3] = F1 * A2 + F3 * A3;
+  for (int k = 0; k < 4; k++) P.F[k][i] = Fn[k];
+  const double J = Fn[0] * Fn[3] - Fn[2] * Fn[1];
+  P.V[i] = J * P.V0[i];
+  unsigned fl = 0;
+  if (J < 0.0) fl |= JUMP_FLAG_NEGATIVE_J;
+  double sg[3] = {P.sig[0][i], P.sig[1][i], P.sig[2][i]};
+  switch (s.kind) {
+    case JUMP_MAT_ELASTIC: hooke2(s, e, sg); break;
+    case JUMP_MAT_NEOHOOKEAN: neohooke<2>(s, Fn, J, sg); break;
+    case JUMP_MAT_ELASTOPLASTIC: {
+      double ep[3] = {P.ep[0][i], P.ep[1][i], P.ep[2][i]};
+      double al = P.alpha[i], vm;
+      j2_2d(s, e, sg, ep, al, vm);
+      P.ep[0][i] = ep[0]; P.ep[1][i] = ep[1]; P.ep[2][i] = ep[2];
+      P.alpha[i] = al;
+      P.vm[i] = vm;
+      break;
+    }
+    default: break;
+  }
+  P.sig[0][i] = sg[0]; P.sig[1][i] = sg[1]; P.sig[2][i] = sg[2];
+  if (!(isfinite(x0) && isfinite(x1) && isfinite(v0) && isfinite(v1))) fl |= JUMP_FLAG_NONFINITE;
+  if (fl) atomicOr(flags, fl);
+}
+
+// ------------------------------------------------------------------ G2P, MUSL 3D
+// G2P-1, SolveExplicitSolidMechanics3D.jl:109-134 (uses getShapeFunctions).
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p1_musl(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                      const double* __restrict__ corners, GridArrays g, double alpha,
+                                                      unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  double vx[3] = {0.0, 0.0, 0.0};
+  bool ok = walk_stencil<3, B, true, false>(P, G, s, corners, i, [&](int n, double Ni, const double*) {
+    const double mI = g.m[n];
+    if (mI > 0.0) {
+      const double mii = (1.0 / mI) * Ni;
+      for (int k = 0; k < 3; k++) vx[k] += (g.p[k][n] - alpha * g.p0[k][n]) * mii;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  for (int k = 0; k < 3; k++) P.v[k][i] = alpha * P.v[k][i] + vx[k];
+}
+
+// G2P-2, SolveExplicitSolidMechanics3D.jl:161-211
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                      const double* __restrict__ corners, GridArrays g, double dt,
+                                                      unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  double xx[3] = {P.x[0][i], P.x[1][i], P.x[2][i]};
+  double L[9];
+  for (int k = 0; k < 9; k++) L[k] = 0.0;
+  bool ok = walk_stencil<3, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
+    const double m = g.m[n];
+    if (m > 0.0) {
+      const double mii = 1.0 / m;
+      const double vIx = mii * g.p2[0][n], vIy = mii * g.p2[1][n], vIz = mii * g.p2[2][n];
+      const double c = Ni * mii * dt;
+      for (int k = 0; k < 3; k++) xx[k] += c * g.p[k][n];
+      // L[a,b] = dN_a v_b (Basis gradient index first, as SMatrix{3,3}(...) at :189-191)
+      L[0] += dN[0] * vIx; L[1] += dN[1] * vIx; L[2] += dN[2] * vIx;
+      L[3] += dN[0] * vIy; L[4] += dN[1] * vIy; L[5] += dN[2] * vIy;
+      L[6] += dN[0] * vIz; L[7] += dN[1] * vIz; L[8] += dN[2] * vIz;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  for (int k = 0; k < 3; k++) P.x[k][i] = xx[k];
+  // symmetric part: D[a,b] = 0.5 (L[a,b] + L[b,a])
+  const double D[6] = {0.5 * (L[0] + L[0]), 0.5 * (L[4] + L[4]), 0.5 * (L[8] + L[8]),
+                       0.5 * (L[3] + L[1]), 0.5 * (L[6] + L[2]), 0.5 * (L[7] + L[5])};
+  double dD[6], e[6];
+  for (int k = 0; k < 6; k++) {
+    dD[k] = dt * D[k];
+    e[k] = P.eps[k][i] + dD[k];
+    P.eps[k][i] = e[k];
+  }
+  double A[9];
+  for (int k = 0; k < 9; k++) A[k] = L[k] * dt;
+  A[0] = 1.0 + A[0]; A[4] = 1.0 + A[4]; A[8] = 1.0 + A[8];
+  double F[9], Fn[9];
+  for (int k = 0; k < 9; k++) F[k] = P.F[k][i];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) Fn[a + 3 * b] = F[a] * A[3 * b] + F[a + 3] * A[1 + 3 * b] + F[a + 6] * A[2 + 3 * b];
+  for (int k = 0; k < 9; k++) P.F[k][i] = Fn[k];
+  // det = dot(col1, cross(col2, col3))
+  const double c0 = Fn[4] * Fn[8] - Fn[5] * Fn[7];
+  const double c1 = Fn[5] * Fn[6] - Fn[3] * Fn[8];
+  const double c2 = Fn[3] * Fn[7] - Fn[4] * Fn[6];
+  const double J = Fn[0] * c0 + Fn[1] * c1 + Fn[2] * c2;
+  P.V[i] = J * P.V0[i];
+  unsigned fl = 0;
+  if (J < 0.0) fl |= JUMP_FLAG_NEGATIVE_J;
+  double sg[6];
+  for (int k = 0; k < 6; k++) sg[k] = P.sig[k][i];
+  switch (s.kind) {
+    case JUMP_MAT_ELASTIC: {
+      double vm;
+      hooke3(s, e, sg, vm);
+      P.vm[i] = vm;
+      break;
+    }
+    case JUMP_MAT_NEOHOOKEAN: neohooke<3>(s, Fn, J, sg); break;
+    case JUMP_MAT_ELASTOPLASTIC: {
+      double ep[6];
+      for (int k = 0; k < 6; k++) ep[k] = P.ep[k][i];
+      double al = P.alpha[i], vm;
+      j2_3d(s, e, sg, ep, al, vm);
+      for (int k = 0; k < 6; k++) P.ep[k][i] = ep[k];
+      P.alpha[i] = al;
+      P.vm[i] = vm;
+      break;
+    }
+    case JUMP_MAT_JOHNSONCOOK: {
+      double al = P.alpha[i], ad = P.alphadot[i], vm;
+      johnson_cook(s, e, dD, dt, sg, al, ad, vm);
+      P.alpha[i] = al;
+      P.alphadot[i] = ad;
+      P.vm[i] = vm;
+      break;
+    }
+    default: break;
+  }
+  for (int k = 0; k < 6; k++) P.sig[k][i] = sg[k];
+  if (!(finite3(xx[0], xx[1], xx[2]) && finite3(P.v[0][i], P.v[1][i], P.v[2][i]))) fl |= JUMP_FLAG_NONFINITE;
+  if (fl) atomicOr(flags, fl);
+}
+
+// ------------------------------------------------------------------ energies
+// Fix.jl:248-270.  Per-block sums in a fixed tree order, then one block sums
+// the partials in block order: deterministic.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_energy_partial(Particles P, double* __restrict__ part) {
+  __shared__ double ske[kBlock], sse[kBlock];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0, se = 0.0;
+  if (i < P.n) {
+    double vv = P.v[0][i] * P.v[0][i];
+    for (int k = 1; k < D; k++) vv = vv + P.v[k][i] * P.v[k][i];
+    ke = 0.5 * P.m[i] * vv;
+    double acc;
+    if (D == 2) {
+      // column-major sum(sigma .* eps): 11, 21, 12, 22
+      const double a = P.sig[0][i] * P.eps[0][i], b = P.sig[2][i] * P.eps[2][i], c = P.sig[1][i] * P.eps[1][i];
+      acc = a + b + b + c;
+    } else {
+      // 11 21 31 12 22 32 13 23 33
+      const int order[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+      acc = P.sig[0][i] * P.eps[0][i];
+      for (int k = 1; k < 9; k++) acc = acc + P.sig[order[k]][i] * P.eps[order[k]][i];
+    }
+    se = 0.5 * P.V[i] * acc;
+  }
+  ske[threadIdx.x] = ke;
+  sse[threadIdx.x] = se;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ske[threadIdx.x] += ske[threadIdx.x + off];
+      sse[threadIdx.x] += sse[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ske[0];
+    part[2 * blockIdx.x + 1] = sse[0];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_energy_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double ske[kBlock], sse[kBlock];
+  double ke = 0.0, se = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+    ke += part[2 * b];
+    se += part[2 * b + 1];
+  }
+  ske[threadIdx.x] = ke;
+  sse[threadIdx.x] = se;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ske[threadIdx.x] += ske[threadIdx.x + off];
+      sse[threadIdx.x] += sse[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = ske[0];
+    out[1] = sse[0];
+  }
+}
+
+}  // namespace jh

@@ -1,0 +1,347 @@
+"""Synthetic workloads of the benchmark configurations, built with the
+reference's own particle builders (Solid.jl:322-509) and driver set-ups.
+
+A Scene holds ABI parameters and per-solid arrays; `load(engine)` pushes them
+into any Engine (the HIP library, or the oracle in tests).
+
+  two_disks      Main_TwoDisks.jl:40-116 at 20x20 cells, linear basis
+  impact3d       3D high-velocity impact: elastic sphere on a J2 plate,
+                 quadratic B-splines, MUSL (Main_HighVelocity_Impact_3D.jl)
+  taylor_bar     3D Taylor bar (Main_TaylorBar.jl), J2 copper, GIMP
+  rings2d        2D rings between rigid platens (Main_TwoRings.jl /
+                 Main_LateralCompression_Cellular.jl), any 2D basis
+  cpdi_tubes     2D lateral compression of a tube array with CPDI-Q4 domains
+"""
+import math
+
+import numpy as np
+
+from . import _ffi
+from .engine import make_material, make_params
+from .solid import (buildParticleForBlock, buildParticleForCircle, buildParticleForCylinder, buildParticleForRing,
+                    buildParticleForRectangle, buildParticleForSphere)
+
+
+class Scene:
+    def __init__(self, name, params, solids, fixed=None, steps=1, meta=None):
+        self.name = name
+        self.params = params
+        self.solids = solids
+        self.fixed = fixed
+        self.steps = steps
+        self.meta = meta or {}
+
+    @property
+    def n_particles(self):
+        return int(sum(s["pos"].shape[0] for s in self.solids))
+
+    def load(self, eng):
+        for s in self.solids:
+            eng.add_solid(s["material"], s["pos"], s["vel"], s["mass"], s["volume"], s["volume0"],
+                          s.get("F"), s.get("strain"), s.get("stress"), corners=s.get("corners"),
+                          elems=s.get("elems"), lp=s.get("lp"))
+        if self.fixed is not None:
+            eng.set_fixed(self.fixed)
+        return eng
+
+
+def _solid(material, pos, vel, vol, density, lp=None, **kw):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n, d = pos.shape
+    v = np.zeros((n, d))
+    v[:] = np.asarray(vel, dtype=np.float64).reshape(1, d)
+    vol = np.broadcast_to(np.asarray(vol, dtype=np.float64), (n,)).copy()
+    out = dict(material=material, pos=pos, vel=v, mass=density * vol, volume=vol.copy(), volume0=vol.copy(), lp=lp)
+    out.update(kw)
+    return out
+
+
+def _fixed_faces(nodes, faces, dirs=None):
+    """Dirichlet mask (nn, d) for whole faces, Dirichlet.jl:28-82."""
+    d = len(nodes)
+    nn = int(np.prod(nodes))
+    m = np.zeros((nn, d), dtype=np.uint8)
+    idx = np.arange(nn)
+    i = idx % nodes[0]
+    j = (idx // nodes[0]) % nodes[1]
+    k = idx // (nodes[0] * nodes[1]) if d == 3 else None
+    sel = {"left": i == 0, "right": i == nodes[0] - 1, "bottom": j == 0, "top": j == nodes[1] - 1}
+    if d == 3:
+        sel["front"] = k == 0
+        sel["back"] = k == nodes[2] - 1
+    for f in faces:
+        fx = dirs[f] if dirs else (1,) * d
+        for a in range(d):
+            if fx[a]:
+                m[sel[f], a] = 1
+    return m
+
+
+def elastic(E, nu, rho):
+    return make_material(_ffi.MAT_ELASTIC, E=E, nu=nu, density=rho)
+
+
+def neohookean(E, nu, rho):
+    return make_material(_ffi.MAT_NEOHOOKEAN, E=E, nu=nu, density=rho)
+
+
+def j2(E, nu, rho, fy, k1):
+    return make_material(_ffi.MAT_ELASTOPLASTIC, E=E, nu=nu, density=rho, fy=fy, k1=k1)
+
+
+def johnson_cook(E, nu, rho, A, B, C, n, epsdot0, cellsize):
+    return make_material(_ffi.MAT_JOHNSONCOOK, E=E, nu=nu, density=rho, A=A, B=B, C=C, n=n, epsdot0=epsdot0,
+                         m=1.0, chi=0.9, Cp=385.0, cellsize=cellsize)
+
+
+def rigid(vx, vy, rho, fixed=False):
+    return make_material(_ffi.MAT_RIGID, density=rho, vx=vx, vy=vy, fixed=fixed)
+
+
+# ---------------------------------------------------------------- 2D
+def two_disks(cells=20, ppc=2, steps=3500, basis=_ffi.BASIS_LINEAR, material="elastic"):
+    """Main_TwoDisks.jl:40-116: unit square, disks r=0.2 at (0.2,0.2) and
+    (0.8,0.8) moving at +-(0.1,0.1), E=1000, nu=0.3, rho=1000, dt=1e-3."""
+    n = cells + 1
+    dx = 1.0 / cells
+    fOffset = dx / ppc
+    c1 = buildParticleForCircle([0.2, 0.2], 0.2, fOffset)
+    c2 = buildParticleForCircle([0.8, 0.8], 0.2, fOffset)
+    mat = {"elastic": elastic, "neohookean": neohookean}[material](1000.0, 0.3, 1000.0) \
+        if material != "j2" else j2(1000.0, 0.3, 1000.0, 5.0, 100.0)
+    vol = fOffset * fOffset
+    lp = 0.5 * fOffset if basis == _ffi.BASIS_GIMP else None
+    s1 = _solid(mat, c1, (0.1, 0.1), vol, 1000.0, lp=lp)
+    s2 = _solid(mat, c2, (-0.1, -0.1), vol, 1000.0, lp=lp)
+    p = make_params(2, basis, _ffi.ALG_USL, 1e-3, (0.0, 0.0), (1.0, 1.0), (n, n), tolerance=1e-9)
+    return Scene("two_disks", p, [s1, s2], steps=steps, meta=dict(interval=10, total_time=3.5))
+
+
+def rings2d(cells=40, basis=_ffi.BASIS_LINEAR, material="neohookean", platen=True, steps=50, v=2.0, seed=0):
+    """Two rings pressed by a moving rigid platen (Main_TwoRings.jl,
+    Main_LateralCompression_Cellular.jl:58-59), unit-free small geometry."""
+    L = 1.0
+    dx = L / cells
+    fOffset = dx / 2
+    rho, E, nu = 1000.0, 1.0e5, 0.3
+    mk = {"elastic": lambda: elastic(E, nu, rho), "neohookean": lambda: neohookean(E, nu, rho),
+          "j2": lambda: j2(E, nu, rho, 500.0, 1.0e3)}[material]
+    lp = 0.5 * fOffset if basis == _ffi.BASIS_GIMP else None
+    solids = []
+    for cx, cy, vx in ((0.3, 0.4, 0.0), (0.6, 0.4, 0.0)):
+        pts = buildParticleForRing([cx, cy], 0.08, 0.13, fOffset)
+        solids.append(_solid(mk(), pts, (vx, 0.0), fOffset * fOffset, rho, lp=lp))
+    if platen:
+        top = buildParticleForRectangle([0.45, 0.6], 0.5, 2 * fOffset, fOffset)
+        solids.append(_solid(rigid(0.0, -v, 7800.0), top, (0.0, 0.0), fOffset * fOffset, 7800.0))
+    c = math.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) / rho)
+    dt = 0.2 * dx / c
+    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (L, L), (cells + 1, cells + 1), tolerance=1e-12,
+                    body=(0.0, -9.81))
+    fixed = _fixed_faces((cells + 1, cells + 1), ["bottom"], {"bottom": (0, 1)})
+    return Scene("rings2d", p, solids, fixed=fixed, steps=steps)
+
+
+def _ring_q4(cx, cy, ri, ro, nr, nt, jitter, rng):
+    """Polar Q4 mesh of a ring: (nt*(nr+1), 2) corners, (nt*nr, 4) elements
+    (counter-clockwise)."""
+    r = np.linspace(ri, ro, nr + 1)
+    t = np.linspace(0.0, 2 * math.pi, nt, endpoint=False)
+    tt, rr = np.meshgrid(t, r, indexing="ij")
+    x = cx + rr * np.cos(tt)
+    y = cy + rr * np.sin(tt)
+    if jitter > 0.0:
+        x = x + rng.uniform(-jitter, jitter, x.shape)
+        y = y + rng.uniform(-jitter, jitter, y.shape)
+    corners = np.stack([x.ravel(), y.ravel()], axis=1)
+    elems = []
+    for a in range(nt):
+        b = (a + 1) % nt
+        for q in range(nr):
+            n00 = a * (nr + 1) + q
+            n01 = a * (nr + 1) + q + 1
+            n10 = b * (nr + 1) + q
+            n11 = b * (nr + 1) + q + 1
+            elems.append((n00, n01, n11, n10))
+    return corners, np.asarray(elems, dtype=np.int64)
+
+
+def cpdi_tubes(rings=(2, 2), cells=64, nr=4, nt=96, jitter_cells=0.1, steps=20, seed=20261202, platens=True):
+    """Lateral compression of a tube array with CPDI-Q4 domains
+    (Main_LateralCompressionTube_CPDI.jl): Neo-Hookean E=1 GPa, nu=0.3, outer
+    radius 5 mm, wall 0.5 mm, rigid platens at 1 m/s; corner jitter
+    +-0.1 cell (seeded)."""
+    rng = np.random.default_rng(seed)
+    ro, wall = 5e-3, 0.5e-3
+    nxr, nyr = rings
+    width = 2 * ro * nxr
+    height = 2 * ro * nyr
+    margin = 4 * ro
+    L = max(width, height) + 2 * margin
+    dx = L / cells
+    rho, E, nu = 1000.0, 1.0e9, 0.3
+    solids = []
+    jit = jitter_cells * dx
+    corners_all, elems_all = [], []
+    base = 0
+    for j in range(nyr):
+        for i in range(nxr):
+            cx = margin + ro + 2 * ro * i
+            cy = margin + ro + 2 * ro * j
+            c, e = _ring_q4(cx, cy, ro - wall, ro, nr, nt, jit, rng)
+            corners_all.append(c)
+            elems_all.append(e + base)
+            base += c.shape[0]
+    corners = np.concatenate(corners_all)
+    elems = np.concatenate(elems_all)
+    cc = corners[elems]
+    x, y = cc[:, :, 0], cc[:, :, 1]
+    area = 0.5 * (x[:, 0] * y[:, 1] - x[:, 1] * y[:, 0] + x[:, 1] * y[:, 2] - x[:, 2] * y[:, 1]
+                  + x[:, 2] * y[:, 3] - x[:, 3] * y[:, 2] + x[:, 3] * y[:, 0] - x[:, 0] * y[:, 3])
+    centre = cc.mean(axis=1)
+    s = dict(material=neohookean(E, nu, rho), pos=np.ascontiguousarray(centre), vel=np.zeros_like(centre),
+             mass=rho * area, volume=area.copy(), volume0=area.copy(), corners=corners, elems=elems)
+    solids.append(s)
+    if platens:
+        f = dx / 2
+        top_y = margin + height + 2 * f
+        bot_y = margin - 2 * f
+        for yy, vy in ((top_y, -1.0), (bot_y, 1.0)):
+            pts = buildParticleForRectangle([margin + width / 2, yy], width + 4 * dx, 2 * f, f)
+            # rigid platens use the linear-basis stencil only; give them unit
+            # CPDI domains so the solid is well formed
+            nc = pts.shape[0]
+            pc = np.concatenate([pts + [-f / 2, -f / 2], pts + [f / 2, -f / 2], pts + [f / 2, f / 2],
+                                 pts + [-f / 2, f / 2]])
+            pe = np.stack([np.arange(nc), np.arange(nc) + nc, np.arange(nc) + 2 * nc, np.arange(nc) + 3 * nc], axis=1)
+            solids.append(dict(material=rigid(0.0, vy, 7800.0), pos=pts, vel=np.zeros_like(pts),
+                               mass=np.full(nc, 7800.0 * f * f), volume=np.full(nc, f * f), volume0=np.full(nc, f * f),
+                               corners=pc, elems=pe))
+    c = math.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) / rho)
+    dt = 0.2 * dx / c
+    p = make_params(2, _ffi.BASIS_CPDI_Q4, _ffi.ALG_USL, dt, (0.0, 0.0), (L, L), (cells + 1, cells + 1),
+                    tolerance=1e-14)
+    return Scene("cpdi_tubes", p, solids, steps=steps)
+
+
+# ---------------------------------------------------------------- 3D
+def impact3d(nodes=(256, 256, 64), plate_mm=(100.0, 100.0, 20.0), sphere_r_mm=5.0, steps=2000, cfl=0.2,
+             basis=_ffi.BASIS_QUAD_BSPLINE, plate_material="j2", sphere_material="elastic", v0=300.0):
+    """3D high-velocity impact: sphere (r=5 mm, 300 m/s, stiff elastic: 3D
+    solids cannot be rigid) on a J2 plate 100x100x20 mm (E=200 GPa, nu=0.3,
+    fy=300 MPa, linear hardening 1 GPa, rho=7800); quadratic B-splines, MUSL(1),
+    2x2x2 particles per cell, dx=0.72 mm, all grid faces fixed."""
+    dx = 0.72e-3
+    nx, ny, nz = nodes
+    f = dx / 2
+    E, nu, rho = 200e9, 0.3, 7800.0
+    W, H, T = (v * 1e-3 for v in plate_mm)
+    # particle lattice at (k+1/4) and (k+3/4) of a cell
+    cx = 0.5 * (nx - 1) * dx
+    wfl = math.floor(W / f) * f
+    x0 = cx - 0.5 * wfl + 0.5 * f
+    shift = (math.floor(x0 / dx) + 0.25) * dx - x0 if (x0 / dx) % 0.5 != 0.25 else 0.0
+    cxs = cx + shift
+    hfl = math.floor(H / f) * f
+    y0 = cx - 0.5 * hfl + 0.5 * f
+    shy = (math.floor(y0 / dx) + 0.25) * dx - y0
+    cys = cx + shy
+    zbot = 2.25 * dx
+    czp = zbot + 0.5 * T - 0.5 * f
+    plate = buildParticleForBlock([cxs, cys, czp], W, H, T, f)
+    zt = plate[:, 2].max()
+    sr = math.floor(sphere_r_mm * 1e-3 / f) * f
+    czs = zt + f + sr - 0.5 * f
+    sph = buildParticleForSphere([cxs, cys, czs], sphere_r_mm * 1e-3, f)
+    vol = f ** 3
+    pm = {"j2": j2(E, nu, rho, 300e6, 1e9), "elastic": elastic(E, nu, rho),
+          "neohookean": neohookean(E, nu, rho),
+          "jc": johnson_cook(E, nu, rho, 300e6, 1e9, 0.0, 1.0, 1.0, dx)}[plate_material]
+    sm = {"elastic": elastic(E, nu, rho), "neohookean": neohookean(E, nu, rho)}[sphere_material]
+    lp = 0.5 * f if basis == _ffi.BASIS_GIMP else None
+    solids = [_solid(sm, sph, (0.0, 0.0, -v0), vol, rho, lp=lp), _solid(pm, plate, (0.0, 0.0, 0.0), vol, rho, lp=lp)]
+    lam = E * nu / (1 + nu) / (1 - 2 * nu)
+    mu = E / 2 / (1 + nu)
+    c = math.sqrt((lam + 2 * mu) / rho)
+    dt = cfl * dx / c
+    hi = ((nx - 1) * dx, (ny - 1) * dx, (nz - 1) * dx)
+    p = make_params(3, basis, _ffi.ALG_MUSL, dt, (0.0, 0.0, 0.0), hi, nodes, alpha=1.0)
+    fixed = _fixed_faces(nodes, ["left", "right", "bottom", "top", "front", "back"])
+    return Scene("impact3d", p, solids, fixed=fixed, steps=steps)
+
+
+def taylor_bar(scale=1.0, steps=400, basis=_ffi.BASIS_GIMP, material="j2", E=117e9, fy=400e6, H=100e6, v0=190.0):
+    """Taylor bar (Main_TaylorBar.jl:43-130): copper cylinder r=3.2 mm,
+    L=32.4 mm along y, v0=190 m/s onto the grid bottom (fixed in y), J2,
+    GIMP.  `scale` multiplies the particle spacing (1.0: ~0.128 mm)."""
+    r, L = 3.2e-3, 32.4e-3
+    f = 0.128e-3 * scale
+    dx = 2 * f
+    rho, nu = 8930.0, 0.35
+    pts = buildParticleForCylinder([0.0, 0.0, 0.0], r, 0.5 * L, f, f)
+    margin = 4 * dx
+    ext = (2 * r + 2 * margin, L * 1.2 + 2 * margin, 2 * r + 2 * margin)
+    nodes = tuple(int(math.ceil(e / dx)) + 1 for e in ext)
+    pts = pts + np.array([0.5 * (nodes[0] - 1) * dx + 0.25 * dx, 0.5 * L + 2.25 * dx, 0.5 * (nodes[2] - 1) * dx + 0.25 * dx])
+    mat = j2(E, nu, rho, fy, H) if material == "j2" else johnson_cook(E, nu, rho, fy, H, 0.0, 1.0, 1.0, dx)
+    lp = 0.5 * f if basis == _ffi.BASIS_GIMP else None
+    solids = [_solid(mat, pts, (0.0, -v0, 0.0), f ** 3, rho, lp=lp)]
+    lam = E * nu / (1 + nu) / (1 - 2 * nu)
+    mu = E / 2 / (1 + nu)
+    c = math.sqrt((lam + 2 * mu) / rho)
+    dt = 0.2 * dx / c
+    hi = tuple((n - 1) * dx for n in nodes)
+    p = make_params(3, basis, _ffi.ALG_MUSL, dt, (0.0, 0.0, 0.0), hi, nodes, alpha=1.0)
+    fixed = _fixed_faces(nodes, ["bottom"], {"bottom": (0, 1, 0)})
+    return Scene("taylor_bar", p, solids, fixed=fixed, steps=steps)
+
+
+def block3d(nodes=(12, 12, 12), basis=_ffi.BASIS_QUAD_BSPLINE, material="elastic", steps=10, v=(1.0, -2.0, 0.5),
+            seed=0):
+    """Small 3D block with a random velocity field and random initial stress,
+    for parity tests of one material/basis pair."""
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / (nodes[0] - 1)
+    f = dx / 2
+    ctr = [0.5 + 0.13 * dx, 0.5 + 0.07 * dx, 0.5 + 0.11 * dx]
+    pts = buildParticleForBlock(ctr, 0.45, 0.45, 0.45, f)
+    rho, E, nu = 1000.0, 1.0e6, 0.3
+    mat = {"elastic": elastic(E, nu, rho), "neohookean": neohookean(E, nu, rho),
+           "j2": j2(E, nu, rho, 2.0e3, 1.0e4), "jc": johnson_cook(E, nu, rho, 2.0e3, 1.0e4, 0.0, 1.0, 1.0, dx)}[material]
+    lp = 0.5 * f if basis == _ffi.BASIS_GIMP else None
+    s = _solid(mat, pts, v, f ** 3, rho, lp=lp)
+    s["vel"] = s["vel"] + rng.normal(scale=0.5, size=s["vel"].shape)
+    n = pts.shape[0]
+    sig = rng.normal(scale=1.0e3, size=(n, 3, 3))
+    sig = 0.5 * (sig + sig.transpose(0, 2, 1))
+    s["stress"] = np.ascontiguousarray(sig.transpose(0, 2, 1).reshape(n, 9))
+    c = math.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) / rho)
+    dt = 0.2 * dx / c
+    hi = tuple((k - 1) * dx for k in nodes)
+    p = make_params(3, basis, _ffi.ALG_MUSL, dt, (0.0, 0.0, 0.0), hi, nodes, alpha=0.95)
+    fixed = _fixed_faces(nodes, ["bottom"], {"bottom": (0, 1, 0)})
+    return Scene("block3d", p, [s], fixed=fixed, steps=steps)
+
+
+def block2d(cells=16, basis=_ffi.BASIS_LINEAR, material="elastic", steps=10, seed=0):
+    """Small 2D block with random velocity and stress (parity tests)."""
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / cells
+    f = dx / 2
+    pts = buildParticleForRectangle([0.5 + 0.13 * dx, 0.5 + 0.07 * dx], 0.45, 0.45, f)
+    rho, E, nu = 1000.0, 1.0e6, 0.3
+    mat = {"elastic": elastic(E, nu, rho), "neohookean": neohookean(E, nu, rho),
+           "j2": j2(E, nu, rho, 2.0e3, 1.0e4)}[material]
+    lp = 0.5 * f if basis == _ffi.BASIS_GIMP else None
+    s = _solid(mat, pts, (1.0, -0.5), f * f, rho, lp=lp)
+    s["vel"] = s["vel"] + rng.normal(scale=0.5, size=s["vel"].shape)
+    n = pts.shape[0]
+    sig = rng.normal(scale=1.0e3, size=(n, 2, 2))
+    sig = 0.5 * (sig + sig.transpose(0, 2, 1))
+    s["stress"] = np.ascontiguousarray(sig.transpose(0, 2, 1).reshape(n, 4))
+    c = math.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) / rho)
+    dt = 0.2 * dx / c
+    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (1.0, 1.0), (cells + 1, cells + 1), tolerance=1e-12,
+                    body=(0.0, -9.81))
+    fixed = _fixed_faces((cells + 1, cells + 1), ["bottom"], {"bottom": (0, 1)})
+    return Scene("block2d", p, [s], fixed=fixed, steps=steps)

@@ -1,0 +1,93 @@
+"""CPU checks of the boundary: the HIP library loads (no GPU call) and
+exports every function include/jump_mpm.h declares; the host mirror of the
+reference API (ranges, builders, Dirichlet masks) matches the reference."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from jump_amd import (Grid2D, Grid3D, Solid2D, buildParticleForBlock, buildParticleForCircle,
+                      buildParticleForSphere, ElasticMaterial, fix_Dirichlet_grid, julia_range, _ffi)
+from jump_amd.fix import julia_float
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "jump_mpm.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(jump_\w+)\s*\(", src, re.M)))
+
+
+def test_header_declares_the_boundary():
+    names = _declared()
+    for must in ("jump_ctx_create", "jump_ctx_destroy", "jump_add_solid", "jump_set_fixed", "jump_step",
+                 "jump_get_solid", "jump_get_solid_state", "jump_get_grid", "jump_energies", "jump_last_error",
+                 "jump_flags"):
+        assert must in names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = _ffi.load()  # raises if the HIP library is not built
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert lib.jump_abi_version() == 1
+    assert sorted(_ffi.EXPORTED) == _declared()
+
+
+def test_oracle_exports_the_same_entry_points(oracle_lib):
+    for name in _declared():
+        if name in ("jump_get_bins", "jump_abi_version", "jump_profile", "jump_profile_read"):
+            continue  # diagnostics of the GPU library only
+        assert hasattr(oracle_lib, name.replace("jump_", "orc_")), name
+
+
+def test_julia_ranges_match_fixture():
+    d = json.load(open(os.path.join(GOLD, "julia_ranges.json")))
+    for v in d.values():
+        r = julia_range(v["start"], v["step"], v["stop"])
+        assert len(r) == v["length"]
+        assert r[0] == v["first"] and r[-1] == v["last"]
+    assert len(julia_range(0.0, 0.1, 1.0)) == 11
+    assert len(julia_range(0.0125, 0.025, 1.0 - 0.0125)) == 40
+
+
+def test_builders_match_reference_counts():
+    grid = Grid2D(0.0, 1.0, 0.0, 1.0, 21, 21)
+    f = grid.dx / 2
+    c1 = buildParticleForCircle([0.2, 0.2], 0.2, f)
+    assert c1.shape == (208, 2)
+    # x fastest, as the reference's nested loops (Solid.jl:327-333)
+    assert c1[1, 1] == c1[0, 1] and c1[1, 0] > c1[0, 0]
+    b = buildParticleForBlock([0.0, 0.0, 0.0], 100e-3, 100e-3, 20e-3, 0.36e-3)
+    assert b.shape[0] == 277 * 277 * 55
+    s = buildParticleForSphere([0.0, 0.0, 0.0], 5e-3, 0.36e-3)
+    assert s.shape[0] == 9328
+    sol = Solid2D(c1, ElasticMaterial(1000.0, 0.3, 1000.0, 0, 0, len(c1)))
+    assert sol.deformationGradient.shape == (208, 2, 2)
+    sol.deformationGradient[3, 0, 1] = 2.5  # [a, b] view over column-major storage
+    assert sol._F[3, 2] == 2.5
+
+
+def test_dirichlet_faces():
+    g = Grid2D(0.0, 1.0, 0.0, 1.0, 5, 4)
+    fix_Dirichlet_grid(g, {"dirichlet_grid": [("bottom", (0, 1, 0)), ("left", (1, 0, 0))]})
+    assert g.fixedNodes[1, 1:5].all() and not g.fixedNodes[0, 1:5].any()
+    assert g.fixedNodes[0, [0, 5, 10, 15]].all()
+    # `.= fix` assigns every component: the later "left" entry resets the
+    # corner's y fix (Dirichlet.jl:58,70)
+    assert list(g.fixedNodes[:, 0]) == [1, 0]
+    with pytest.raises(ValueError):
+        fix_Dirichlet_grid(g, {"dirichlet_grid": [("front", (1, 1, 1))]})
+    g3 = Grid3D(0, 1, 0, 1, 0, 1, 3, 4, 5)
+    fix_Dirichlet_grid(g3, {"dirichlet_grid": [("back", (1, 1, 1))]})
+    assert g3.fixedNodes[:, -12:].all() and not g3.fixedNodes[:, :-12].any()
+
+
+def test_energy_record_format():
+    assert julia_float(1e-5) == "1.0e-5"
+    assert julia_float(2.6) == "2.6"
+    assert julia_float(0.0) == "0.0"

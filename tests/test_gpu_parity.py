@@ -1,0 +1,267 @@
+"""GPU parity: the HIP step (through the C ABI) against the CPU restatement
+on the same seeded inputs, for every basis x material combination the
+reference defines on this path, plus the Two Disks energy history fixture and
+size-independent properties at the benchmark size.
+
+Both sides evaluate every term in the reference's operation order (no FMA
+contraction) and the GPU's P2G folds each node's contributions in the
+reference's particle order, so particle and grid state are bit-identical for
+materials built from +,-,*,/,sqrt (Hooke, J2).  Neo-Hookean (log) and
+Johnson-Cook (pow) call the device math library, which may differ from the
+host's in the last ulp: those agree within 1e-10 relative (normwise per field:
+max|gpu - cpu| <= 1e-10 * max|cpu|).  Energies are reduced in a different
+order on the GPU: 1e-12 relative.  Binning is bit-exact."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from jump_amd import _ffi, scenes
+from jump_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _close(a, b, rtol=RTOL, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    scale = float(np.max(np.abs(b))) if b.size else 0.0
+    err = float(np.max(np.abs(a - b))) if b.size else 0.0
+    assert err <= rtol * scale + 1e-300, f"{what}: max err {err:.3e} vs scale {scale:.3e}"
+
+
+def _run(scene, steps, hip_lib):
+    from oracle.pyoracle import OracleEngine
+    g = Engine(scene.params)
+    scene.load(g)
+    o = OracleEngine(scene.params)
+    scene.load(o)
+    tg, rg = g.step(steps, 0.0)
+    to, ro = o.step(steps, 0.0)
+    assert tg == to
+    assert rg == ro
+    assert g.flags() == o.flags()
+    return g, o
+
+
+def _same(a, b, exact, what):
+    if exact:
+        a, b = np.asarray(a), np.asarray(b)
+        bad = ~((a == b) | (np.isnan(a) & np.isnan(b)))
+        assert not bad.any(), f"{what}: {int(bad.sum())} values differ, max {np.max(np.abs(a - b)[bad]):.3e}"
+    else:
+        _close(a, b, what=what)
+
+
+def _compare(g, o, scene, grid=True, exact=True):
+    for s in range(len(scene.solids)):
+        a, b = g.get_solid(s), o.get_solid(s)
+        for k in ("pos", "vel", "volume", "F", "strain", "stress", "mass"):
+            _same(a[k], b[k], exact, f"solid {s} {k}")
+        sa, sb = g.get_state(s), o.get_state(s)
+        for k in ("alpha", "alphadot", "vm", "pstrain"):
+            _same(sa[k], sb[k], exact, f"solid {s} {k}")
+    if grid:
+        ga, gb = g.get_grid(), o.get_grid()
+        keys = ("mass", "momentum0", "momentum", "force") + (("momentum2",) if g.dim == 3 else ())
+        for k in keys:
+            _same(ga[k], gb[k], exact, f"grid {k}")
+    ea, eb = g.energies(), o.energies()
+    _close(ea, eb, rtol=1e-12, what="energies")
+
+
+def _exact(material):
+    return material in ("elastic", "j2")
+
+
+BASES_2D = [_ffi.BASIS_LINEAR, _ffi.BASIS_QUAD_BSPLINE, _ffi.BASIS_CUBIC_BSPLINE, _ffi.BASIS_GIMP]
+BASES_3D = [_ffi.BASIS_LINEAR, _ffi.BASIS_QUAD_BSPLINE, _ffi.BASIS_CUBIC_BSPLINE, _ffi.BASIS_GIMP]
+
+
+@pytest.mark.parametrize("material", ["elastic", "neohookean", "j2"])
+@pytest.mark.parametrize("basis", BASES_2D)
+def test_usl2d_block(hip_lib, basis, material):
+    _say(f"usl2d block basis={basis} material={material}")
+    sc = scenes.block2d(cells=16, basis=basis, material=material, steps=12)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=_exact(material))
+
+
+@pytest.mark.parametrize("material", ["elastic", "neohookean", "j2", "jc"])
+@pytest.mark.parametrize("basis", BASES_3D)
+def test_musl3d_block(hip_lib, basis, material):
+    _say(f"musl3d block basis={basis} material={material}")
+    sc = scenes.block3d(nodes=(11, 11, 11), basis=basis, material=material, steps=6)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=_exact(material))
+
+
+@pytest.mark.parametrize("basis", [_ffi.BASIS_LINEAR, _ffi.BASIS_QUAD_BSPLINE, _ffi.BASIS_GIMP])
+def test_usl2d_rings_rigid_platen(hip_lib, basis):
+    _say(f"rings2d basis={basis}")
+    sc = scenes.rings2d(cells=40, basis=basis, steps=60)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=False)
+    sc = scenes.rings2d(cells=40, basis=basis, material="elastic", steps=60)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=True)
+
+
+def test_usl2d_cpdi_tubes(hip_lib):
+    _say("cpdi tubes")
+    sc = scenes.cpdi_tubes(rings=(2, 2), cells=48, nr=3, nt=48, steps=15)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=False)
+
+
+def test_musl3d_impact_small(hip_lib):
+    _say("impact3d small")
+    sc = scenes.impact3d(nodes=(36, 36, 22), plate_mm=(16.0, 16.0, 5.0), sphere_r_mm=2.0, steps=25)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=True)
+
+
+def test_two_disks_energy_history(hip_lib):
+    """Config c1 (Main_TwoDisks.jl at 20x20 cells): 3500 steps, energies every
+    10 steps against the committed record and the live restatement."""
+    _say("two disks 3500 steps")
+    from oracle.pyoracle import OracleEngine
+    rec = np.load(os.path.join(GOLD, "two_disks_energy.npz"))["rec"]
+    sc = scenes.two_disks()
+    g = Engine(sc.params)
+    sc.load(g)
+    out = []
+    t, _ = g.step(1, 0.0)
+    out.append((0.0,) + g.energies())
+    for c in range(1, 3500):
+        tc = t
+        t, _ = g.step(1, t)
+        if c % 10 == 0:
+            out.append((tc,) + g.energies())
+    out = np.array(out)
+    assert np.array_equal(out[:, 0], rec[:, 0])
+    # the state is bit-identical; the energy sums differ only in reduction order
+    assert np.max(np.abs(out[:, 1:] - rec[:, 1:])) <= 1e-12 * rec[:, 1].max()
+    o = OracleEngine(sc.params)
+    sc.load(o)
+    o.step(3500, 0.0)
+    _compare(g, o, sc, grid=True, exact=True)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_binning_is_bit_exact(hip_lib, dim):
+    """Stencil origin (the reference's integer support rule, floor(u + 1.0)
+    with no contraction) equals the CPU restatement bit for bit, including
+    particles placed one ulp below node and half-cell positions."""
+    from oracle.pyoracle import OracleEngine
+    bases = BASES_2D + [_ffi.BASIS_CPDI_Q4] if dim == 2 else BASES_3D
+    for basis in bases:
+        _say(f"binning dim={dim} basis={basis}")
+        if basis == _ffi.BASIS_CPDI_Q4:
+            sc = scenes.cpdi_tubes(rings=(2, 2), cells=48, nr=3, nt=48, platens=False)
+        else:
+            sc = (scenes.block2d(cells=16, basis=basis) if dim == 2 else scenes.block3d(nodes=(11, 11, 11), basis=basis))
+            s = sc.solids[0]
+            h = 1.0 / 16 if dim == 2 else 1.0 / 10
+            rng = np.random.default_rng(7)
+            n = s["pos"].shape[0]
+            k = rng.integers(4, 8, size=(n, dim)).astype(np.float64)
+            adv = k * h
+            sel = rng.integers(0, 4, size=n)
+            # exact nodes, one ulp below, half cells, one ulp below half cells
+            adv[sel == 1] = np.nextafter(adv[sel == 1], 0.0)
+            adv[sel == 2] = (k[sel == 2] + 0.5) * h
+            adv[sel == 3] = np.nextafter((k[sel == 3] + 0.5) * h, 0.0)
+            s["pos"] = np.ascontiguousarray(adv)
+        g = Engine(sc.params)
+        sc.load(g)
+        o = OracleEngine(sc.params)
+        sc.load(o)
+        keys = g.bins()
+        nx, ny = sc.params.nx, sc.params.ny
+        off = 0
+        for si, s in enumerate(sc.solids):
+            for p in range(s["pos"].shape[0]):
+                sup = o.support(si, p, True)
+                if sup is None:
+                    assert keys[off + p] == -1
+                    continue
+                near = sup[0]
+                i, j = near % nx, (near // nx) % ny
+                kk = near // (nx * ny)
+                exp = i.min() + nx * (j.min() + ny * kk.min())
+                assert keys[off + p] == exp, (basis, si, p)
+            off += s["pos"].shape[0]
+
+
+def test_run_to_run_bitwise_reproducible(hip_lib):
+    """No floating-point atomics: two runs give identical bits."""
+    sc = scenes.impact3d(nodes=(36, 36, 22), plate_mm=(16.0, 16.0, 5.0), sphere_r_mm=2.0, steps=10)
+    res = []
+    for _ in range(2):
+        g = Engine(sc.params)
+        sc.load(g)
+        g.step(sc.steps, 0.0)
+        res.append((g.get_solid(1), g.get_grid(), g.energies()))
+        g.close()
+    for k in res[0][0]:
+        assert np.array_equal(res[0][0][k], res[1][0][k]), k
+    for k in res[0][1]:
+        assert np.array_equal(res[0][1][k], res[1][1][k]), k
+    assert res[0][2] == res[1][2]
+
+
+def test_error_flags_match_restatement(hip_lib):
+    """A particle outside the grid, a NaN velocity: the step finishes, skips or
+    propagates as the restatement does, and reports the same flags."""
+    sc = scenes.block2d(cells=16, steps=3)
+    s = sc.solids[0]
+    s["pos"][0] = (1.5, 0.5)      # outside the grid
+    s["vel"][1] = (np.nan, 0.0)   # non-finite
+    from oracle.pyoracle import OracleEngine
+    g = Engine(sc.params)
+    sc.load(g)
+    o = OracleEngine(sc.params)
+    sc.load(o)
+    _, rg = g.step(3, 0.0)
+    _, ro = o.step(3, 0.0)
+    assert rg == ro == _ffi.JUMP_ERR_NUMERIC
+    fg, fo = g.flags(), o.flags()
+    assert fg == fo
+    assert fg & _ffi.JUMP_FLAG_OUT_OF_GRID
+    assert np.array_equal(g.get_solid(0)["pos"][0], [1.5, 0.5])
+
+
+def test_benchmark_size_properties(hip_lib):
+    """Config c2 at full size (4.23M particles, 256x256x64 nodes): mass is
+    conserved by P2G, the grid update conserves momentum where no node is
+    fixed, and the run is bitwise reproducible."""
+    _say("impact3d full size")
+    sc = scenes.impact3d(steps=3)
+    mtot = sum(float(s["mass"].sum()) for s in sc.solids)
+    ptot = sum((s["mass"][:, None] * s["vel"]).sum(0) for s in sc.solids)
+    outs = []
+    for rep in range(2):
+        g = Engine(sc.params)
+        sc.load(g)
+        g.step(1, 0.0)
+        gr = g.get_grid()
+        assert math.isclose(gr["mass"].sum(), mtot, rel_tol=1e-12)
+        # the plate and sphere lie away from the fixed faces after one step
+        assert np.allclose(gr["momentum0"].sum(0), ptot, rtol=1e-9, atol=1e-9 * np.abs(ptot).max())
+        g.step(2, 0.0)
+        outs.append((g.get_solid(1)["pos"], g.energies()))
+        g.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
