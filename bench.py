@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--order", choices=["reference", "binned"], default="reference",
+                    help="P2G summation order: the reference's particle order (bit-identical) or bin order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is a "
@@ -85,6 +87,7 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     sc = scenes.impact3d(steps=args.steps)
+    sc.params.order = 0 if args.order == "reference" else 1
     n_sphere = sc.solids[0]["pos"].shape[0]
     n_plate = sc.solids[1]["pos"].shape[0]
     eng = Engine(sc.params, device=0)
@@ -133,7 +136,7 @@ def main():
         "config": {"workload": "impact3d: elastic sphere r=5mm 300 m/s on J2 plate 100x100x20 mm, "
                                "quadratic B-spline, MUSL(1.0), 2x2x2 particles per cell",
                    "particles": n, "grid_nodes": [sc.params.nx, sc.params.ny, sc.params.nz],
-                   "dt": sc.params.dt, "active_nodes": n_active},
+                   "dt": sc.params.dt, "active_nodes": n_active, "p2g_order": args.order},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
         "phases": phases,

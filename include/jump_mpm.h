@@ -78,6 +78,14 @@ typedef enum {
   JUMP_MAT_JOHNSONCOOK = 4
 } jump_material_kind;
 
+/* P2G summation order.  The reference folds each node's contributions in
+ * particle order (SolveExplicitSolidMechanics2D.jl:391-423).
+ * JUMP_ORDER_REFERENCE reproduces that fold exactly (grid and particle state
+ * bit-identical to the reference's arithmetic); JUMP_ORDER_BINNED sums bin by
+ * bin in a fixed order: deterministic run to run, equal to the reference up
+ * to floating-point reassociation of the node sums. */
+typedef enum { JUMP_ORDER_REFERENCE = 0, JUMP_ORDER_BINNED = 1 } jump_order;
+
 /* Step parameters: the Grid2D/Grid3D constructor arguments (Grid.jl:104,198),
  * the basis and algorithm objects, and the keys of the reference's `data`
  * Dict (SolveExplicitSolidMechanics2D.jl:337-344: "dt", "bodyforce",
@@ -93,6 +101,8 @@ typedef struct {
   double xmin, xmax, ymin, ymax, zmin, zmax;
   int64_t nx, ny, nz; /* node counts iN_x, iN_y, iN_z (nz ignored in 2D)   */
   double body[3];     /* ConstantBodyForce2D/3D g (BodyForce.jl:56-91)     */
+  int32_t order;      /* jump_order (0: reference summation order)          */
+  int32_t reserved;   /* must be 0                                          */
 } jump_params;
 
 /* Material parameters.  Lame constants are derived exactly as the reference

@@ -33,6 +33,9 @@ BASIS_GIMP = 4
 ALG_USL = 0
 ALG_MUSL = 1
 
+ORDER_REFERENCE = 0
+ORDER_BINNED = 1
+
 MAT_RIGID = 0
 MAT_ELASTIC = 1
 MAT_NEOHOOKEAN = 2
@@ -53,6 +56,7 @@ class JumpParams(ctypes.Structure):
         ("xmin", _d), ("xmax", _d), ("ymin", _d), ("ymax", _d), ("zmin", _d), ("zmax", _d),
         ("nx", _i64), ("ny", _i64), ("nz", _i64),
         ("body", _d * 3),
+        ("order", _i32), ("reserved", _i32),
     ]
 
 

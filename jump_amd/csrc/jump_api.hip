@@ -56,6 +56,7 @@ struct jump_ctx {
   SolidDev* d_solids = nullptr;
   int *d_key = nullptr, *d_perm = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_cursor = nullptr,
       *d_sums = nullptr;
+  unsigned long long* d_bxyz = nullptr;
   unsigned* d_flags = nullptr;
   double* d_red = nullptr;
   int* d_wmax = nullptr;
@@ -147,6 +148,7 @@ static int alloc_particles(jump_ctx* ctx) {
     for (int k = 0; k < 4 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.elem[k], n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_key, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_perm, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_bxyz, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_red, 2 * size_t(blocks_for(ctx->np) + 1));
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_solids, ctx->solids.size());
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_corners, ctx->corners.size());
@@ -290,6 +292,13 @@ static void prof_collect(jump_ctx* ctx) {
   do {                                                                                        \
     if ((n) > 0) hipLaunchKernelGGL((kernel), dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, __VA_ARGS__); \
   } while (0)
+#define LAUNCH_P2G(kernel, n, ...)                                                                   \
+  do {                                                                                               \
+    if ((n) > 0)                                                                                     \
+      hipLaunchKernelGGL((kernel), dim3(((n) + kP2GBlock - 1) / kP2GBlock),                                    \
+                         dim3(kP2GBlock), 0, ctx->stream,                                           \
+                         __VA_ARGS__);                                                               \
+  } while (0)
 
 static int scan_bins(jump_ctx* ctx) {
   const int n = ctx->G.nn + 1;
@@ -305,21 +314,22 @@ template <int D, int B> static int bin_and_sort(jump_ctx* ctx) {
   const int np = int(ctx->np);
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cnt, 0, nn + 1);
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cursor, 0, nn + 1);
-  LAUNCH((k_bin<D, B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_cnt, ctx->d_flags);
+  LAUNCH((k_bin<D, B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_bxyz, ctx->d_cnt,
+         ctx->d_flags);
   scan_bins(ctx);
   LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_cursor, ctx->d_perm, np);
   LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_perm, nn);
   return JUMP_OK;
 }
 
-template <int B> static void step_usl2d(jump_ctx* ctx) {
+template <int B, int ORDER> static void step_usl2d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
   prof_begin(ctx);
   bin_and_sort<2, B>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH((k_p2g<2, B, 0>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->g,
+  LAUNCH_P2G((k_p2g<2, B, 0, ORDER>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->d_bxyz, ctx->g,
          ctx->cpdi_w, dt);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
@@ -337,21 +347,21 @@ template <int B> static void step_usl2d(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_RIGID);
 }
 
-template <int B> static void step_musl3d(jump_ctx* ctx) {
+template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
   prof_begin(ctx);
   bin_and_sort<3, B>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH((k_p2g<3, B, 0>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->g,
+  LAUNCH_P2G((k_p2g<3, B, 0, ORDER>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->d_bxyz, ctx->g,
          0, dt);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
   prof_end(ctx, JUMP_PHASE_G2P1);
   prof_begin(ctx);
-  LAUNCH((k_p2g<3, B, 1>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->g,
+  LAUNCH_P2G((k_p2g<3, B, 1, ORDER>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->d_bxyz, ctx->g,
          0, dt);
   prof_end(ctx, JUMP_PHASE_P2G2);
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
@@ -376,21 +386,34 @@ static int prepare_cpdi(jump_ctx* ctx) {
 }
 
 static int run_steps(jump_ctx* ctx, int64_t nsteps) {
+  const bool ref = ctx->prm.order == JUMP_ORDER_REFERENCE;
   for (int64_t s = 0; s < nsteps; s++) {
     if (ctx->D == 2) {
       switch (ctx->prm.basis) {
-        case JUMP_BASIS_LINEAR: step_usl2d<JUMP_BASIS_LINEAR>(ctx); break;
-        case JUMP_BASIS_QUAD_BSPLINE: step_usl2d<JUMP_BASIS_QUAD_BSPLINE>(ctx); break;
-        case JUMP_BASIS_CUBIC_BSPLINE: step_usl2d<JUMP_BASIS_CUBIC_BSPLINE>(ctx); break;
-        case JUMP_BASIS_CPDI_Q4: step_usl2d<JUMP_BASIS_CPDI_Q4>(ctx); break;
-        case JUMP_BASIS_GIMP: step_usl2d<JUMP_BASIS_GIMP>(ctx); break;
+#define USL_CASE(BB)                                                     \
+  case BB:                                                               \
+    if (ref) step_usl2d<BB, JUMP_ORDER_REFERENCE>(ctx);                  \
+    else step_usl2d<BB, JUMP_ORDER_BINNED>(ctx);                         \
+    break;
+        USL_CASE(JUMP_BASIS_LINEAR)
+        USL_CASE(JUMP_BASIS_QUAD_BSPLINE)
+        USL_CASE(JUMP_BASIS_CUBIC_BSPLINE)
+        USL_CASE(JUMP_BASIS_CPDI_Q4)
+        USL_CASE(JUMP_BASIS_GIMP)
+#undef USL_CASE
       }
     } else {
       switch (ctx->prm.basis) {
-        case JUMP_BASIS_LINEAR: step_musl3d<JUMP_BASIS_LINEAR>(ctx); break;
-        case JUMP_BASIS_QUAD_BSPLINE: step_musl3d<JUMP_BASIS_QUAD_BSPLINE>(ctx); break;
-        case JUMP_BASIS_CUBIC_BSPLINE: step_musl3d<JUMP_BASIS_CUBIC_BSPLINE>(ctx); break;
-        case JUMP_BASIS_GIMP: step_musl3d<JUMP_BASIS_GIMP>(ctx); break;
+#define MUSL_CASE(BB)                                                    \
+  case BB:                                                               \
+    if (ref) step_musl3d<BB, JUMP_ORDER_REFERENCE>(ctx);                 \
+    else step_musl3d<BB, JUMP_ORDER_BINNED>(ctx);                        \
+    break;
+        MUSL_CASE(JUMP_BASIS_LINEAR)
+        MUSL_CASE(JUMP_BASIS_QUAD_BSPLINE)
+        MUSL_CASE(JUMP_BASIS_CUBIC_BSPLINE)
+        MUSL_CASE(JUMP_BASIS_GIMP)
+#undef MUSL_CASE
       }
     }
   }
@@ -415,6 +438,8 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   const int64_t nz = p->dim == 3 ? p->nz : 1;
   if (p->nx * p->ny * nz >= (int64_t(1) << 31) - 2) return JUMP_ERR_UNSUPPORTED;
   if (!(p->dt > 0.0)) return JUMP_ERR_ARG;
+  if (p->order != JUMP_ORDER_REFERENCE && p->order != JUMP_ORDER_BINNED) return JUMP_ERR_ARG;
+  if (p->nx >= 65536 || p->ny >= 65536 || nz >= 65536) return JUMP_ERR_UNSUPPORTED;
   jump_ctx* ctx = new (std::nothrow) jump_ctx();
   if (!ctx) return JUMP_ERR_NOMEM;
   ctx->prm = *p;

@@ -24,6 +24,25 @@ namespace jh {
 
 constexpr int kBlock = 256;
 
+// The 8 XCDs of an MI355X each own an L2; workgroups are dispatched to them
+// round-robin.  Kernels whose neighbouring workgroups share data remap the
+// hardware block index so that each XCD receives runs of kXcdChunk
+// consecutive logical blocks (spatial locality in its L2), the runs dealt to
+// the XCDs in turn (balance).  The grid is rounded up to a multiple of
+// kXcds * kXcdChunk; surplus blocks exit.
+constexpr int kXcds = 8;
+constexpr int kXcdChunk = 16;
+__device__ __forceinline__ int xcd_block(int nlogical) {
+  const int b = int(blockIdx.x);
+  const int group = b / (kXcds * kXcdChunk);
+  const int r = b % (kXcds * kXcdChunk);
+  const int logical = group * (kXcds * kXcdChunk) + (r % kXcds) * kXcdChunk + r / kXcds;
+  return logical < nlogical ? logical : -1;
+}
+__host__ __device__ constexpr int xcd_grid(int nlogical) {
+  return (nlogical + kXcds * kXcdChunk - 1) / (kXcds * kXcdChunk) * (kXcds * kXcdChunk);
+}
+
 __global__ void k_fill_i32(int* __restrict__ a, int v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -32,10 +51,13 @@ __global__ void k_fill_i32(int* __restrict__ a, int v, int n) {
 // ------------------------------------------------------------------ binning
 // key[i] = linear index of the particle's first stencil node (0-based), or -1
 // for particles that take no part in P2G (2D rigid bodies, out of grid).
+// bxyz[i] packs the origin's 0-based coordinates, 21 bits per axis (all
+// ones for excluded particles).
 template <int D, int B>
 __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                 const double* __restrict__ corners, int* __restrict__ key,
-                                                int* __restrict__ cnt, unsigned* __restrict__ flags) {
+                                                unsigned long long* __restrict__ bxyz, int* __restrict__ cnt,
+                                                unsigned* __restrict__ flags) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
@@ -59,8 +81,15 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G, const S
     } else {
       atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
     }
+    if (ok) {
+      key[i] = k;
+      bxyz[i] = (unsigned long long)(f[0] - 1) | ((unsigned long long)(f[1] - 1) << 21) |
+                ((unsigned long long)(D == 3 ? f[2] - 1 : 0) << 42);
+      return;
+    }
   }
-  key[i] = k;
+  key[i] = -1;
+  bxyz[i] = ~0ull;
 }
 
 // CPDI stencil extent (static corners): max bounding-box width in nodes.
@@ -160,13 +189,13 @@ __global__ void __launch_bounds__(kBlock) k_binsort(const int* __restrict__ star
 }
 
 // ------------------------------------------------------------------ P2G
-// Weight N and gradient of node (ii,jj,kk) (1-based) for particle p.
+// Weight N and gradient of node (ii,jj,kk) (1-based) for a particle at xp.
 // QUIRK selects the getShapeFunctions variant (used by the MUSL
 // re-projection): QBS boundary typing, cubic z-axis type-4 test on jd.
 template <int D, int B, bool QUIRK, bool GRAD>
-__device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& G, const SolidDev& s,
-                                            const double* corners, int p, int ii, int jj, int kk, double& N,
-                                            double* dN, bool& member) {
+__device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& G, const double* lp,
+                                            const double* corners, int p, const double* xp, int ii, int jj, int kk,
+                                            double& N, double* dN, bool& member) {
   member = true;
   if (B == JUMP_BASIS_CPDI_Q4) {
     CpdiW c;
@@ -177,8 +206,8 @@ __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& 
   }
   Axis X = axis_of(G, 0), Y = axis_of(G, 1);
   double wx, dwx, wy, dwy;
-  axis_weight<B, QUIRK>(P.x[0][p], 0, X, s.lp[0], ii, ii, wx, dwx);
-  axis_weight<B, QUIRK>(P.x[1][p], 1, Y, s.lp[1], jj, jj, wy, dwy);
+  axis_weight<B, QUIRK>(xp[0], 0, X, lp[0], ii, ii, wx, dwx);
+  axis_weight<B, QUIRK>(xp[1], 1, Y, lp[1], jj, jj, wy, dwy);
   if (D == 2) {
     if (GRAD) combine2<B>(X, Y, wx, dwx, wy, dwy, N, dN);
     else N = wx * wy;
@@ -186,7 +215,7 @@ __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& 
     Axis Z = axis_of(G, 2);
     double wz, dwz;
     int sel4 = (QUIRK && B == JUMP_BASIS_CUBIC_BSPLINE) ? jj : kk;
-    axis_weight<B, QUIRK>(P.x[2][p], 2, Z, s.lp[2], kk, sel4, wz, dwz);
+    axis_weight<B, QUIRK>(xp[2], 2, Z, lp[2], kk, sel4, wz, dwz);
     if (GRAD) combine3<B>(X, Y, Z, wx, dwx, wy, dwy, wz, dwz, N, dN);
     else N = wx * wy * wz;
   }
@@ -198,101 +227,173 @@ template <int B> struct MaxW { static constexpr int W = Width<B>::W; };
 template <> struct MaxW<JUMP_BASIS_CPDI_Q4> { static constexpr int W = 5; };
 constexpr int kNoParticle = 0x7fffffff;
 
+// Particle fields one P2G pass reads.  PASS 0: x, m, V, v, sigma; PASS 1
+// (re-projection): x, m, v.
+template <int D, int PASS> struct P2GFields {
+  static constexpr int S = D == 2 ? 3 : 6;
+  static constexpr int NF = PASS == 0 ? D + 2 + D + S : D + 1 + D;
+  __device__ static __forceinline__ double get(const Particles& P, int f, int q) {
+    if (f < D) return P.x[f][q];
+    if (f == D) return P.m[q];
+    if (PASS == 0) {
+      if (f == D + 1) return P.V[q];
+      if (f < D + 2 + D) return P.v[f - D - 2][q];
+      return P.sig[f - 2 * D - 2][q];
+    }
+    return P.v[f - D - 1][q];
+  }
+};
+
+// One particle's contribution to the node accumulators, in the reference's
+// operation order (SolveExplicitSolidMechanics2D.jl:411-421;
+// SolveExplicitSolidMechanics3D.jl:69-82 and :135-138).
+template <int D, int B, int PASS>
+__device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, const SolidDev* S,
+                                        const double* corners, int p, int ni, int nj, int nk, double& am,
+                                        double* ap, double* af) {
+  constexpr int NF = P2GFields<D, PASS>::NF;
+  double v[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) v[f] = P2GFields<D, PASS>::get(P, f, p);
+  double lp[3] = {0.0, 0.0, 0.0};
+  if (B == JUMP_BASIS_GIMP) {
+    const SolidDev& s = S[P.sid[p]];
+    for (int k = 0; k < D; k++) lp[k] = s.lp[k];
+  }
+  double N, dN[3];
+  bool member;
+  node_weight<D, B, PASS == 1, PASS == 0>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
+  if (!member) return;
+  const double fM = v[D];
+  if (PASS == 0) {
+    const double fV = v[D + 1];
+    const double* vp = v + D + 2;
+    const double* sg = v + 2 * D + 2;
+    const double Nim = N * fM;
+    am += Nim;
+    for (int k = 0; k < D; k++) ap[k] += Nim * vp[k];
+    if (D == 2) {
+      af[0] += (-fV) * (sg[0] * dN[0] + sg[2] * dN[1]) + (fM * G.body[0]) * N;
+      af[1] += (-fV) * (sg[2] * dN[0] + sg[1] * dN[1]) + (fM * G.body[1]) * N;
+    } else {
+      af[0] -= fV * (sg[0] * dN[0] + sg[3] * dN[1] + sg[4] * dN[2]);
+      af[1] -= fV * (sg[3] * dN[0] + sg[1] * dN[1] + sg[5] * dN[2]);
+      af[2] -= fV * (sg[4] * dN[0] + sg[5] * dN[1] + sg[2] * dN[2]);
+    }
+  } else {
+    const double Nm = N * fM;
+    for (int k = 0; k < D; k++) ap[k] += Nm * v[D + 1 + k];
+  }
+}
+
 // PASS 0: mass, momentum0, force, then p = p0 + f dt and the Dirichlet mask
 //         (SolveExplicitSolidMechanics2D.jl:391-442; 3D.jl:55-105).
 // PASS 1: MUSL re-projection momentum2 = sum N m v (3D.jl:135-138).
+// One thread per grid node gathers every particle whose stencil covers the
+// node: the particles of the W^d bins of its window (bin = stencil origin
+// node; each bin's list is sorted by particle index).  No floating-point
+// atomics: every node's sum has a fixed order, so runs are bitwise
+// reproducible.
 //
-// Summation order.  The reference adds each particle's contribution to a
-// node in particle order (solids in order, then particle index: the loops at
-// SolveExplicitSolidMechanics2D.jl:391-401).  A node's contributors are the
-// particles of the W^d bins of its window, each bin sorted by particle
-// index; the kernel merges those sorted lists (a k-way merge that consumes a
-// whole run from one bin while its next index stays below the smallest head
-// of the other bins), so every node folds its contributions in exactly the
-// reference's order and the grid equals the reference's bit for bit.
-template <int D, int B, int PASS>
-__global__ void __launch_bounds__(kBlock) k_p2g(Particles P, GridDesc G, const SolidDev* __restrict__ S,
-                                                const double* __restrict__ corners, const int* __restrict__ start,
-                                                const int* __restrict__ perm, GridArrays g, int Wrt, double dt) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= G.nn) return;
+// ORDER = JUMP_ORDER_REFERENCE.  The reference adds each particle's
+// contribution to a node in particle order (solids in order, then particle
+// index: SolveExplicitSolidMechanics2D.jl:391-401).  The kernel enumerates
+// the union of the window's bins in ascending index, so every node folds its
+// contributions in exactly the reference's order and the grid equals the
+// reference's bit for bit.  Enumeration: after particle p, if p+1 lies in
+// the window (its packed bin coordinates) it is next; otherwise every bin
+// head at or below p is advanced past p and the smallest head is taken.
+// Lattice-built solids give runs of consecutive indices, so the full scan
+// runs once per run.  Heads stay in registers (static indices only); the
+// per-bin cursors are 16-bit offsets in LDS.
+// ORDER = JUMP_ORDER_BINNED.  The window's bins in a fixed order, each bin's
+// particles in index order: deterministic, no merge, a different rounding
+// than the reference's fold.
+constexpr int kP2GBlock = 128;
+
+template <int D, int B, int PASS, int ORDER>
+__global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                   const double* __restrict__ corners, const int* __restrict__ start,
+                                                   const int* __restrict__ perm,
+                                                   const unsigned long long* __restrict__ bxyz, GridArrays g,
+                                                   int Wrt, double dt) {
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
+  __shared__ unsigned short s_off[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  const int t = threadIdx.x;
+  const int n = blockIdx.x * kP2GBlock + t;
+  if (n >= G.nn) return;
   const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
   const int ni = n % G.nx;
   const int nj = (n / G.nx) % G.ny;
   const int nk = (D == 3) ? n / G.nxy : 0;
-  // window bins (bin = stencil origin node); head = next particle index
-  int head[NB], cur[NB], endb[NB];
-#pragma unroll
-  for (int b = 0; b < NB; b++) {
-    const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
-    const int bi = ni - W + 1 + a0, bj = nj - W + 1 + a1, bk = (D == 3) ? nk - W + 1 + a2 : 0;
-    head[b] = kNoParticle;
-    cur[b] = 0;
-    endb[b] = 0;
-    if (a0 < W && a1 < W && a2 < W && bi >= 0 && bj >= 0 && bk >= 0) {
-      const int bin = bi + G.nx * (bj + G.ny * bk);
-      const int s0 = start[bin], e0 = start[bin + 1];
-      if (s0 < e0) {
-        cur[b] = s0;
-        endb[b] = e0;
-        head[b] = perm[s0];
-      }
-    }
-  }
   double am = 0.0, ap[D], af[D];
   for (int k = 0; k < D; k++) { ap[k] = 0.0; af[k] = 0.0; }
-  while (true) {
-    int best = kNoParticle, second = kNoParticle, sel = 0;
+  if (ORDER == JUMP_ORDER_BINNED) {
+    const int k0 = (D == 3) ? max(nk - W + 1, 0) : 0;
+    for (int bk = k0; bk <= nk; bk++)
+      for (int bj = max(nj - W + 1, 0); bj <= nj; bj++)
+        for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
+          const int bin = bi + G.nx * (bj + G.ny * bk);
+          const int e = start[bin + 1];
+          for (int q = start[bin]; q < e; q++)
+            p2g_add<D, B, PASS>(P, G, S, corners, perm[q], ni, nj, nk, am, ap, af);
+        }
+  } else {
+    // window bins; head[b] = next unconsumed particle index of bin b
+    // (kNoParticle when exhausted or outside the grid)
+    const int base = (ni - W + 1) + G.nx * ((nj - W + 1) + (D == 3 ? G.ny * (nk - W + 1) : 0));
+    int head[NB];
+    bool any = false;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-      const int h = head[b];
-      if (h < best) { second = best; best = h; sel = b; }
-      else if (h < second) second = h;
-    }
-    if (best == kNoParticle) break;
-    int c = 0, e = 0;
-#pragma unroll
-    for (int b = 0; b < NB; b++)
-      if (b == sel) { c = cur[b]; e = endb[b]; }
-    int p = best;
-    while (true) {
-      const SolidDev& s = S[P.sid[p]];
-      double N, dN[3];
-      bool member;
-      node_weight<D, B, PASS == 1, PASS == 0>(P, G, s, corners, p, ni + 1, nj + 1, nk + 1, N, dN, member);
-      if (member) {
-        const double fM = P.m[p];
-        if (PASS == 0) {
-          const double fV = P.V[p];
-          const double Nim = N * fM;
-          am += Nim;
-          for (int k = 0; k < D; k++) ap[k] += Nim * P.v[k][p];
-          if (D == 2) {
-            const double s11 = P.sig[0][p], s22 = P.sig[1][p], s12 = P.sig[2][p];
-            af[0] += (-fV) * (s11 * dN[0] + s12 * dN[1]) + (fM * G.body[0]) * N;
-            af[1] += (-fV) * (s12 * dN[0] + s22 * dN[1]) + (fM * G.body[1]) * N;
-          } else {
-            const double s11 = P.sig[0][p], s22 = P.sig[1][p], s33 = P.sig[2][p];
-            const double s12 = P.sig[3][p], s13 = P.sig[4][p], s23 = P.sig[5][p];
-            af[0] -= fV * (s11 * dN[0] + s12 * dN[1] + s13 * dN[2]);
-            af[1] -= fV * (s12 * dN[0] + s22 * dN[1] + s23 * dN[2]);
-            af[2] -= fV * (s13 * dN[0] + s23 * dN[1] + s33 * dN[2]);
-          }
-        } else {
-          const double Nm = N * fM;
-          for (int k = 0; k < D; k++) ap[k] += Nm * P.v[k][p];
-        }
+      const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
+      const int bi = ni - W + 1 + a0, bj = nj - W + 1 + a1, bk = (D == 3) ? nk - W + 1 + a2 : 0;
+      head[b] = kNoParticle;
+      s_off[b][t] = 0;
+      if (a0 < W && a1 < W && a2 < W && bi >= 0 && bj >= 0 && bk >= 0) {
+        const int bin = base + a0 + G.nx * (a1 + G.ny * a2);
+        const int c0 = start[bin];
+        if (c0 < start[bin + 1]) head[b] = perm[c0];
       }
-      ++c;
-      if (c >= e) { p = kNoParticle; break; }
-      p = perm[c];
-      if (p > second) break;
+      any = any || head[b] != kNoParticle;
     }
+    int p = -1;
+    while (any) {
+      int best = kNoParticle;
 #pragma unroll
-    for (int b = 0; b < NB; b++)
-      if (b == sel) { cur[b] = c; head[b] = p; }
+      for (int b = 0; b < NB; b++) {
+        int h = head[b];
+        if (h <= p) {
+          const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
+          const int bin = base + a0 + G.nx * (a1 + G.ny * a2);
+          const int s0 = start[bin], e0 = start[bin + 1];
+          int c = s0 + s_off[b][t];
+          do {
+            ++c;
+            h = c < e0 ? perm[c] : kNoParticle;
+          } while (h <= p);
+          s_off[b][t] = (unsigned short)(c - s0);
+          head[b] = h;
+        }
+        best = min(best, h);
+      }
+      if (best == kNoParticle) break;
+      p = best;
+      while (true) {
+        p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af);
+        // is p+1 in the window?  (bin coordinates packed 21 bits per axis;
+        // excluded particles hold all ones)
+        const int q = p + 1;
+        if (q >= P.n) break;
+        const unsigned long long c = bxyz[q];
+        const unsigned ux = unsigned(ni - int(c & 0x1fffffu));
+        const unsigned uy = unsigned(nj - int((c >> 21) & 0x1fffffu));
+        const unsigned uz = unsigned(nk - int((c >> 42) & 0x1fffffu));
+        if (!(ux < unsigned(W) && uy < unsigned(W) && (D == 2 || uz < unsigned(W))) || c == ~0ull) break;
+        p = q;
+      }
+    }
   }
   if (PASS == 0) {
     const uint8_t fx = g.fixed[n];
