@@ -12,7 +12,9 @@ HBM), the HBM roofline of the dominant kernel (algorithmic bytes per launch /
 its mean duration, from HIP events on the library's stream over the timed
 region), and the CPU restatement timed on a bounded sample on this host.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--order reference|binned]
+    python bench.py --copy-probe MIB   # device-to-device copy of MIB MiB, to
+                                       # calibrate HBM byte counters
 """
 import argparse
 import json
@@ -25,6 +27,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s + HBM GB/s (P2G+G2P+stress); batched & adjoint vs plain"
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per launch of each phase's kernel on this workload (reference-order
+# P2G), measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE and
+# scaled by what the same counters read for a 1 GiB device copy (FETCH x2.0,
+# WRITE x1.0); table and commands in profiles/hbm_traffic_impact3d.md.
+# Counters cannot be read from inside a timed run, so the figure is carried
+# here and must be re-measured when a kernel changes.
+MEASURED_TRAFFIC = {"p2g": 6.047e9, "p2g2": 3.452e9, "g2p1": 1.779e9, "g2p2": 4.478e9}
 
 
 def algorithmic_bytes(phase, n_plate, n_sphere, n_active):
@@ -62,6 +72,22 @@ def cpu_baseline(steps):
                       f"serial C++ restatement of the reference loops ({dt:.1f} s)"}
 
 
+def copy_probe(mib, reps=5):
+    """A plain device-to-device copy of known size (reads and writes `mib`
+    MiB per launch): the yardstick that scales FETCH_SIZE / WRITE_SIZE
+    counter readings to bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst, src)
+    print(json.dumps({"copy_probe_MiB": mib, "launches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +97,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", choices=["reference", "binned"], default="reference",
                     help="P2G summation order: the reference's particle order (bit-identical) or bin order")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is a "
               "design, not a measured path", file=sys.stderr)
@@ -138,7 +168,8 @@ def main():
                    "particles": n, "grid_nodes": [sc.params.nx, sc.params.ny, sc.params.nz],
                    "dt": sc.params.dt, "active_nodes": n_active, "p2g_order": args.order},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get(dom) if args.order == "reference" else None},
         "phases": phases,
         "step_flags": flags,
         "energies": [ke, se],
