@@ -123,7 +123,11 @@ def main():
     eng = Engine(sc.params, device=0)
     sc.load(eng)
     t = 0.0
-    t, rc = eng.step(args.warmup, t)  # first call uploads the particles
+    u0 = time.perf_counter()
+    t, rc = eng.step(0, t)  # zero steps: hands the host buffers over (AoS -> SoA, host -> HBM)
+    torch.cuda.synchronize()
+    upload_s = time.perf_counter() - u0
+    t, rc = eng.step(args.warmup, t)
     torch.cuda.synchronize()
     eng.profile(True)
     torch.cuda.synchronize()
@@ -136,6 +140,10 @@ def main():
     flags = eng.flags()
     elapsed = t1 - t0
     n = sc.n_particles
+    d0 = time.perf_counter()
+    for si in range(len(sc.solids)):
+        eng.get_solid(si)  # HBM -> host, SoA -> AoS
+    download_s = time.perf_counter() - d0
     g = eng.get_grid()
     n_active = int(np.count_nonzero(g["mass"] > 0.0))
     phases = {}
@@ -171,6 +179,8 @@ def main():
                      "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get(dom) if args.order == "reference" else None},
         "phases": phases,
+        "host_transfer": {"upload_s": round(upload_s, 4), "download_s": round(download_s, 4),
+                          "pcie_inclusive_value": n * args.steps / (elapsed + upload_s + download_s)},
         "step_flags": flags,
         "energies": [ke, se],
     }
