@@ -54,9 +54,10 @@ struct jump_ctx {
   std::vector<void*> grid_allocs;
   double* d_corners = nullptr;
   SolidDev* d_solids = nullptr;
-  int *d_key = nullptr, *d_perm = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_cursor = nullptr,
+  int *d_key = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_cursor = nullptr,
       *d_sums = nullptr;
   unsigned long long* d_bxyz = nullptr;
+  unsigned long long* d_sorted = nullptr;  // (bin << 32) | particle, by bin then index; [np] = sentinel
   unsigned* d_flags = nullptr;
   double* d_red = nullptr;
   int* d_wmax = nullptr;
@@ -147,11 +148,14 @@ static int alloc_particles(jump_ctx* ctx) {
   if (ctx->prm.basis == JUMP_BASIS_CPDI_Q4)
     for (int k = 0; k < 4 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.elem[k], n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_key, n);
-  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_perm, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_sorted, n + 1);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_bxyz, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_red, 2 * size_t(blocks_for(ctx->np) + 1));
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_solids, ctx->solids.size());
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_corners, ctx->corners.size());
+  if (r) return r;
+  // sentinel after the last entry: its bin half (all ones) matches no bin
+  HIP_TRY(hipMemsetAsync(ctx->d_sorted + n, 0xff, sizeof(unsigned long long), ctx->stream));
   P.n = int(ctx->np);
   ctx->dev_capacity = ctx->np;
   return r;
@@ -292,12 +296,10 @@ static void prof_collect(jump_ctx* ctx) {
   do {                                                                                        \
     if ((n) > 0) hipLaunchKernelGGL((kernel), dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, __VA_ARGS__); \
   } while (0)
-#define LAUNCH_P2G(kernel, n, ...)                                                                   \
+#define LAUNCH_P2G(kernel, D, ...)                                                                   \
   do {                                                                                               \
-    if ((n) > 0)                                                                                     \
-      hipLaunchKernelGGL((kernel), dim3(((n) + kP2GBlock - 1) / kP2GBlock),                                    \
-                         dim3(kP2GBlock), 0, ctx->stream,                                           \
-                         __VA_ARGS__);                                                               \
+    hipLaunchKernelGGL((kernel), dim3(p2g_blocks<D>(ctx->G.nx, ctx->G.ny, ctx->G.nz)), dim3(kP2GBlock), \
+                       0, ctx->stream, __VA_ARGS__);                                                 \
   } while (0)
 
 static int scan_bins(jump_ctx* ctx) {
@@ -317,8 +319,8 @@ template <int D, int B> static int bin_and_sort(jump_ctx* ctx) {
   LAUNCH((k_bin<D, B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_bxyz, ctx->d_cnt,
          ctx->d_flags);
   scan_bins(ctx);
-  LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_cursor, ctx->d_perm, np);
-  LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_perm, nn);
+  LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_cursor, ctx->d_sorted, np, nn);
+  LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_sorted, nn);
   return JUMP_OK;
 }
 
@@ -329,7 +331,7 @@ template <int B, int ORDER> static void step_usl2d(jump_ctx* ctx) {
   bin_and_sort<2, B>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<2, B, 0, ORDER>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->d_bxyz, ctx->g,
+  LAUNCH_P2G((k_p2g<2, B, 0, ORDER>), 2, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->g,
          ctx->cpdi_w, dt);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
@@ -354,14 +356,14 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   bin_and_sort<3, B>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<3, B, 0, ORDER>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->d_bxyz, ctx->g,
+  LAUNCH_P2G((k_p2g<3, B, 0, ORDER>), 3, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->g,
          0, dt);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
   prof_end(ctx, JUMP_PHASE_G2P1);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<3, B, 1, ORDER>), ctx->G.nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_perm, ctx->d_bxyz, ctx->g,
+  LAUNCH_P2G((k_p2g<3, B, 1, ORDER>), 3, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->g,
          0, dt);
   prof_end(ctx, JUMP_PHASE_P2G2);
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
@@ -505,7 +507,7 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
   if (D == 3 && mat->kind == JUMP_MAT_RIGID)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid (Solid3D has no rigid flag)");
   if (D == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");
-  if (ctx->np + ds->n >= (int64_t(1) << 31) - 1) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "too many particles");
+  if (ctx->np + ds->n > kMaxParticles) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "too many particles (at most 2^29 per context)");
   const bool cpdi = ctx->prm.basis == JUMP_BASIS_CPDI_Q4;
   if (cpdi && (!ds->corners || !ds->elems || ds->ncorners <= 0))
     return set_err(ctx, JUMP_ERR_ARG, "CPDI solid needs corners and elems");

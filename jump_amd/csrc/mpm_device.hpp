@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "../../include/jump_mpm.h"
+#include "mpm_basis.hpp"
 
 namespace jh {
 
@@ -67,118 +68,20 @@ struct GridArrays {
 };
 
 // ------------------------------------------------------------------ helpers
-__device__ __forceinline__ double jsign(double d) { return d > 0.0 ? 1.0 : (d < 0.0 ? -1.0 : d); }
+// Element i of an array whose base is wave-uniform (a kernel argument): the
+// address is the base plus a 32-bit byte offset, which gfx950 global loads
+// take as SGPR base + VGPR offset, so a gather of many fields at one index
+// costs one shift instead of a 64-bit add per field.  Requires i * sizeof(T)
+// < 2^32 (the library caps a context at kMaxParticles particles).
+constexpr long long kMaxParticles = 1ll << 29;
+template <class T> __device__ __forceinline__ T ldu(const T* __restrict__ base, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * unsigned(sizeof(T)));
+}
 
 // Index of symmetric component (a,b) in the compressed storage.
 template <int D> __device__ __forceinline__ constexpr int sym(int a, int b) {
   if (D == 2) return a == b ? a : 2;
   return a == b ? a : (a + b == 1 ? 3 : (a + b == 2 ? 4 : 5));
-}
-
-// ------------------------------------------------------------------ B-splines
-// Bsplines.jl:18-83
-__device__ __forceinline__ void qbs_t1(double r, double& N, double& dN) {
-  if (r < -1.5 || r > 1.5) { N = 0.0; dN = 0.0; return; }
-  if (r >= -1.5 && r <= -0.5) { N = 0.5 * r * r + 1.5 * r + 1.125; dN = r + 1.5; }
-  else if (r <= 0.0) { N = 1.0 + r; dN = 1.0; }
-  else if (r <= 0.5) { N = 1.0 - r; dN = -1.0; }
-  else { N = 0.5 * r * r - 1.5 * r + 1.125; dN = r - 1.5; }
-}
-__device__ __forceinline__ void qbs_t2(double r, double& N, double& dN) {
-  if (r < -1.0 || r > 1.5) { N = 0.0; dN = 0.0; return; }
-  if (r >= -1.0 && r <= -0.5) { N = 1.0 + r; dN = 1.0; }
-  else if (r <= 0.5) { N = (-r) * r + 0.75; dN = -2.0 * r; }
-  else { N = 0.5 * r * r - 1.5 * r + 1.125; dN = r - 1.5; }
-}
-__device__ __forceinline__ void qbs_t3(double r, double& N, double& dN) {
-  if (r < -1.5 || r > 1.5) { N = 0.0; dN = 0.0; return; }
-  if (r >= -1.5 && r <= -0.5) { N = 0.5 * r * r + 1.5 * r + 1.125; dN = r + 1.5; }
-  else if (r <= 0.5) { N = (-r) * r + 0.75; dN = -2.0 * r; }
-  else { N = 0.5 * r * r - 1.5 * r + 1.125; dN = r - 1.5; }
-}
-__device__ __forceinline__ void qbs_t4(double r, double& N, double& dN) {
-  if (r < -1.5 || r > 1.0) { N = 0.0; dN = 0.0; return; }
-  if (r >= -1.5 && r <= -0.5) { N = 0.5 * r * r + 1.5 * r + 1.125; dN = r + 1.5; }
-  else if (r <= 0.5) { N = (-r) * r + 0.75; dN = -2.0 * r; }
-  else { N = 1.0 - r; dN = -1.0; }
-}
-// Bsplines.jl:89-171
-__device__ __forceinline__ void cbs_t1(double r, double& N, double& dN) {
-  if (r < -2.0 || r > 2.0) { N = 0.0; dN = 0.0; return; }
-  double r2 = r * r;
-  if (r >= -2.0 && r <= -1.0) { N = 0.16666666666666666 * r * r2 + r2 + 2.0 * r + 1.3333333333333333; dN = 0.5 * r2 + 2.0 * r + 2.0; }
-  else if (-1.0 <= r && r <= 0.0) { N = -0.16666666666666666 * r * r2 + r + 1.0; dN = -0.5 * r2 + 1.0; }
-  else if (0.0 <= r && r <= 1.0) { N = 0.16666666666666666 * r * r2 - r + 1.0; dN = 0.5 * r2 - 1.0; }
-  else { N = -0.16666666666666666 * r * r2 + r2 - 2.0 * r + 1.3333333333333333; dN = -0.5 * r2 + 2.0 * r - 2.0; }
-}
-__device__ __forceinline__ void cbs_t2(double r, double& N, double& dN) {
-  if (r < -1.0 || r > 2.0) { N = 0.0; dN = 0.0; return; }
-  double r2 = r * r;
-  if (r >= -1.0 && r <= 0.0) { N = -0.3333333333333333 * r * r2 - r2 + 0.6666666666666666; dN = -r2 - 2.0 * r; }
-  else if (0.0 <= r && r <= 1.0) { N = 0.5 * r * r2 - r2 + 0.6666666666666666; dN = 1.5 * r2 - 2.0 * r; }
-  else { N = -0.16666666666666666 * r * r2 + r2 - 2.0 * r + 1.3333333333333333; dN = -0.5 * r2 + 2.0 * r - 2.0; }
-}
-__device__ __forceinline__ void cbs_t3(double r, double& N, double& dN) {
-  if (r < -2.0 || r > 2.0) { N = 0.0; dN = 0.0; return; }
-  double r2 = r * r;
-  if (r >= -2.0 && r <= -1.0) { N = 0.16666666666666666 * r * r2 + r2 + 2.0 * r + 1.3333333333333333; dN = 0.5 * r2 + 2.0 * r + 2.0; }
-  else if (-1.0 <= r && r <= 0.0) { N = -0.5 * r * r2 - r2 + 0.6666666666666666; dN = -1.5 * r2 - 2.0 * r; }
-  else if (0.0 <= r && r <= 1.0) { N = 0.5 * r * r2 - r2 + 0.6666666666666666; dN = 1.5 * r2 - 2.0 * r; }
-  else { N = -0.16666666666666666 * r * r2 + r2 - 2.0 * r + 1.3333333333333333; dN = -0.5 * r2 + 2.0 * r - 2.0; }
-}
-__device__ __forceinline__ void cbs_t4(double r, double& N, double& dN) {
-  if (r < -2.0 || r > 1.0) { N = 0.0; dN = 0.0; return; }
-  double r2 = r * r;
-  if (r >= -2.0 && r <= -1.0) { N = 0.16666666666666666 * r * r2 + r2 + 2.0 * r + 1.3333333333333333; dN = 0.5 * r2 + 2.0 * r + 2.0; }
-  else if (-1.0 <= r && r <= 0.0) { N = -0.5 * r * r2 - r2 + 0.6666666666666666; dN = -1.5 * r2 - 2.0 * r; }
-  else { N = 0.3333333333333333 * r * r2 - r2 + 0.6666666667; dN = r2 - 2.0 * r; }
-}
-
-// Boundary type of a node on one axis (1-based id, n nodes).  `quirk`: the
-// getShapeFunctions test `id == 1 | id == n`, which Julia evaluates as
-// `id == (1|id) == n` (Basis.jl:1079,1188).
-__device__ __forceinline__ int qbs_type(int id, int n, bool quirk) {
-  bool t1 = quirk ? (id == (1 | id) && (1 | id) == n) : (id == 1 || id == n);
-  return t1 ? 1 : (id == 2 ? 2 : (id == n - 1 ? 4 : 3));
-}
-__device__ __forceinline__ void qbs_eval(int type, double r, double& N, double& dN) {
-  switch (type) {
-    case 1: qbs_t1(r, N, dN); break;
-    case 2: qbs_t2(r, N, dN); break;
-    case 4: qbs_t4(r, N, dN); break;
-    default: qbs_t3(r, N, dN); break;
-  }
-}
-// `sel4`: index compared with n-1 for type 4 (the 3D getShapeFunctions z axis
-// compares the y index, Basis.jl:1434).
-__device__ __forceinline__ int cbs_type(int id, int n, int sel4) {
-  return (id == 1 || id == n) ? 1 : (id == 2 ? 2 : (sel4 == n - 1 ? 4 : 3));
-}
-__device__ __forceinline__ void cbs_eval(int type, double r, double& N, double& dN) {
-  switch (type) {
-    case 1: cbs_t1(r, N, dN); break;
-    case 2: cbs_t2(r, N, dN); break;
-    case 4: cbs_t4(r, N, dN); break;
-    default: cbs_t3(r, N, dN); break;
-  }
-}
-// uGIMP 1D weight and derivative (Bardenhagen & Kober 2004), lp <= h/2.
-__device__ __forceinline__ void gimp_eval(double xi, double h, double hI, double lp, double& N, double& dN) {
-  double a = fabs(xi);
-  if (a < lp) {
-    N = 1.0 - (xi * xi + lp * lp) / (2.0 * h * lp);
-    dN = -xi / (h * lp);
-  } else if (a <= h - lp) {
-    N = 1.0 - a * hI;
-    dN = -jsign(xi) * hI;
-  } else if (a < h + lp) {
-    double q = h + lp - a;
-    N = q * q / (4.0 * h * lp);
-    dN = -jsign(xi) * q / (2.0 * h * lp);
-  } else {
-    N = 0.0;
-    dN = 0.0;
-  }
 }
 
 // ------------------------------------------------------------------ stencils

@@ -90,6 +90,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G, const S
   }
   key[i] = -1;
   bxyz[i] = ~0ull;
+  atomicAdd(&cnt[G.nn], 1);
 }
 
 // CPDI stencil extent (static corners): max bounding-box width in nodes.
@@ -162,29 +163,36 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(int* __restrict__ out, cons
   if (i < n) out[i] += sums[i / kScanItems];
 }
 
+// Counting-sort scatter.  sorted[pos] = (bin << 32) | particle: carrying the
+// bin lets a reader that advances through a bin's segment see where it ends
+// from the entry itself.  Particles outside the grid go to the extra bin nn,
+// so every entry of sorted[0, np) is written.
 __global__ void __launch_bounds__(kBlock) k_scatter(const int* __restrict__ key, const int* __restrict__ start,
-                                                    int* __restrict__ cursor, int* __restrict__ perm, int np) {
+                                                    int* __restrict__ cursor, unsigned long long* __restrict__ sorted,
+                                                    int np, int nn) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int k = key[i];
-  if (k < 0) return;
+  if (k < 0) k = nn;
   int pos = start[k] + atomicAdd(&cursor[k], 1);
-  perm[pos] = i;
+  sorted[pos] = ((unsigned long long)unsigned(k) << 32) | unsigned(i);
 }
 
-// Order each bin segment by particle index so the P2G sum order is fixed.
-__global__ void __launch_bounds__(kBlock) k_binsort(const int* __restrict__ start, int* __restrict__ perm, int nbins) {
+// Order each bin segment by particle index so the P2G sum order is fixed
+// (the bin half of every entry in a segment is equal).
+__global__ void __launch_bounds__(kBlock) k_binsort(const int* __restrict__ start,
+                                                    unsigned long long* __restrict__ sorted, int nbins) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
   int s = start[b], e = start[b + 1];
   for (int i = s + 1; i < e; i++) {
-    int v = perm[i];
+    unsigned long long v = sorted[i];
     int j = i - 1;
-    while (j >= s && perm[j] > v) {
-      perm[j + 1] = perm[j];
+    while (j >= s && sorted[j] > v) {
+      sorted[j + 1] = sorted[j];
       j--;
     }
-    perm[j + 1] = v;
+    sorted[j + 1] = v;
   }
 }
 
@@ -232,15 +240,15 @@ constexpr int kNoParticle = 0x7fffffff;
 template <int D, int PASS> struct P2GFields {
   static constexpr int S = D == 2 ? 3 : 6;
   static constexpr int NF = PASS == 0 ? D + 2 + D + S : D + 1 + D;
-  __device__ static __forceinline__ double get(const Particles& P, int f, int q) {
-    if (f < D) return P.x[f][q];
-    if (f == D) return P.m[q];
+  __device__ static __forceinline__ double get(const Particles& P, int f, unsigned q) {
+    if (f < D) return ldu(P.x[f], q);
+    if (f == D) return ldu(P.m, q);
     if (PASS == 0) {
-      if (f == D + 1) return P.V[q];
-      if (f < D + 2 + D) return P.v[f - D - 2][q];
-      return P.sig[f - 2 * D - 2][q];
+      if (f == D + 1) return ldu(P.V, q);
+      if (f < D + 2 + D) return ldu(P.v[f - D - 2], q);
+      return ldu(P.sig[f - 2 * D - 2], q);
     }
-    return P.v[f - D - 1][q];
+    return ldu(P.v[f - D - 1], q);
   }
 };
 
@@ -254,7 +262,7 @@ __device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, c
   constexpr int NF = P2GFields<D, PASS>::NF;
   double v[NF];
 #pragma unroll
-  for (int f = 0; f < NF; f++) v[f] = P2GFields<D, PASS>::get(P, f, p);
+  for (int f = 0; f < NF; f++) v[f] = P2GFields<D, PASS>::get(P, f, unsigned(p));
   double lp[3] = {0.0, 0.0, 0.0};
   if (B == JUMP_BASIS_GIMP) {
     const SolidDev& s = S[P.sid[p]];
@@ -300,33 +308,55 @@ __device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, c
 // index: SolveExplicitSolidMechanics2D.jl:391-401).  The kernel enumerates
 // the union of the window's bins in ascending index, so every node folds its
 // contributions in exactly the reference's order and the grid equals the
-// reference's bit for bit.  Enumeration: after particle p, if p+1 lies in
-// the window (its packed bin coordinates) it is next; otherwise every bin
-// head at or below p is advanced past p and the smallest head is taken.
-// Lattice-built solids give runs of consecutive indices, so the full scan
-// runs once per run.  Heads stay in registers (static indices only); the
-// per-bin cursors are 16-bit offsets in LDS.
+// reference's bit for bit.  Enumeration: every window bin keeps, in LDS, its
+// next unconsumed particle (its head) and that particle's position in
+// `sorted`.  The smallest head is the next particle; consuming it advances
+// only its own bin (one load of the following entry, whose bin half says
+// whether the segment goes on).  After particle p, if p+1 lies in the window
+// (its packed bin coordinates) it is next and its bin is known, so the scan
+// over all heads runs once per run of consecutive indices, which
+// lattice-built solids make long.
 // ORDER = JUMP_ORDER_BINNED.  The window's bins in a fixed order, each bin's
 // particles in index order: deterministic, no merge, a different rounding
 // than the reference's fold.
 constexpr int kP2GBlock = 128;
+// Node tile of one P2G block (X*Y*Z = kP2GBlock).  A wave is 64 nodes
+// consecutive in x: neighbouring lanes gather particles a couple of indices
+// apart, so each field load of a wave touches few cache lines.  Measured on
+// the 4.2M-particle impact, rows beat every 3D tile (8x4x4 took 1.7x as
+// long): the loss of coalescing outweighs the smaller window per block.
+// Blocks go to the XCDs round robin; 128-node row segments keep every XCD
+// on both halves of each row, where 64x2 tiles pinned each XCD to one
+// quarter of x (and left the XCDs on empty quarters idle).
+template <int D> struct P2GTile { static constexpr int X = 128, Y = 1, Z = 1; };
+template <> struct P2GTile<3> { static constexpr int X = 128, Y = 1, Z = 1; };
+template <int D> inline int p2g_blocks(int nx, int ny, int nz) {
+  using T = P2GTile<D>;
+  return ((nx + T::X - 1) / T::X) * ((ny + T::Y - 1) / T::Y) * (D == 3 ? (nz + T::Z - 1) / T::Z : 1);
+}
 
 template <int D, int B, int PASS, int ORDER>
 __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                    const double* __restrict__ corners, const int* __restrict__ start,
-                                                   const int* __restrict__ perm,
+                                                   const unsigned long long* __restrict__ sorted,
                                                    const unsigned long long* __restrict__ bxyz, GridArrays g,
                                                    int Wrt, double dt) {
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
-  __shared__ unsigned short s_off[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  // Reference order: per window bin, the position in `sorted` of its next
+  // unconsumed particle and that particle (kNoParticle when exhausted).
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  using T = P2GTile<D>;
   const int t = threadIdx.x;
-  const int n = blockIdx.x * kP2GBlock + t;
-  if (n >= G.nn) return;
+  const int ntx = (G.nx + T::X - 1) / T::X, nty = (G.ny + T::Y - 1) / T::Y;
+  const int bxy = blockIdx.x % (ntx * nty);
+  const int ni = (bxy % ntx) * T::X + t % T::X;
+  const int nj = (bxy / ntx) * T::Y + (t / T::X) % T::Y;
+  const int nk = (D == 3) ? int(blockIdx.x / (ntx * nty)) * T::Z + t / (T::X * T::Y) : 0;
+  if (ni >= G.nx || nj >= G.ny || (D == 3 && nk >= G.nz)) return;
+  const int n = ni + G.nx * (nj + G.ny * nk);
   const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
-  const int ni = n % G.nx;
-  const int nj = (n / G.nx) % G.ny;
-  const int nk = (D == 3) ? n / G.nxy : 0;
   double am = 0.0, ap[D], af[D];
   for (int k = 0; k < D; k++) { ap[k] = 0.0; af[k] = 0.0; }
   if (ORDER == JUMP_ORDER_BINNED) {
@@ -335,62 +365,54 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G, cons
       for (int bj = max(nj - W + 1, 0); bj <= nj; bj++)
         for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
           const int bin = bi + G.nx * (bj + G.ny * bk);
-          const int e = start[bin + 1];
-          for (int q = start[bin]; q < e; q++)
-            p2g_add<D, B, PASS>(P, G, S, corners, perm[q], ni, nj, nk, am, ap, af);
+          const int e = ldu(start, unsigned(bin + 1));
+          for (int q = ldu(start, unsigned(bin)); q < e; q++)
+            p2g_add<D, B, PASS>(P, G, S, corners, int(unsigned(ldu(sorted, unsigned(q)))), ni, nj, nk, am, ap, af);
         }
   } else {
-    // window bins; head[b] = next unconsumed particle index of bin b
-    // (kNoParticle when exhausted or outside the grid)
-    const int base = (ni - W + 1) + G.nx * ((nj - W + 1) + (D == 3 ? G.ny * (nk - W + 1) : 0));
-    int head[NB];
-    bool any = false;
+    const int wx0 = ni - W + 1, wy0 = nj - W + 1, wz0 = (D == 3) ? nk - W + 1 : 0;  // window origin bin
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
-      const int bi = ni - W + 1 + a0, bj = nj - W + 1 + a1, bk = (D == 3) ? nk - W + 1 + a2 : 0;
-      head[b] = kNoParticle;
-      s_off[b][t] = 0;
-      if (a0 < W && a1 < W && a2 < W && bi >= 0 && bj >= 0 && bk >= 0) {
-        const int bin = base + a0 + G.nx * (a1 + G.ny * a2);
-        const int c0 = start[bin];
-        if (c0 < start[bin + 1]) head[b] = perm[c0];
+      int c = 0, h = kNoParticle;
+      if (a0 < W && a1 < W && a2 < W && wx0 + a0 >= 0 && wy0 + a1 >= 0 && wz0 + a2 >= 0) {
+        const int bin = (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2));
+        c = ldu(start, unsigned(bin));
+        if (c < ldu(start, unsigned(bin + 1))) h = int(unsigned(ldu(sorted, unsigned(c))));
       }
-      any = any || head[b] != kNoParticle;
+      s_cur[b][t] = c;
+      s_head[b][t] = h;
     }
-    int p = -1;
-    while (any) {
+    while (true) {
+      // the smallest head is the window's next particle in index order
       int best = kNoParticle;
 #pragma unroll
-      for (int b = 0; b < NB; b++) {
-        int h = head[b];
-        if (h <= p) {
-          const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
-          const int bin = base + a0 + G.nx * (a1 + G.ny * a2);
-          const int s0 = start[bin], e0 = start[bin + 1];
-          int c = s0 + s_off[b][t];
-          do {
-            ++c;
-            h = c < e0 ? perm[c] : kNoParticle;
-          } while (h <= p);
-          s_off[b][t] = (unsigned short)(c - s0);
-          head[b] = h;
-        }
-        best = min(best, h);
-      }
+      for (int b = 0; b < NB; b++) best = min(best, s_head[b][t]);
       if (best == kNoParticle) break;
-      p = best;
+      int b = 0;
+#pragma unroll
+      for (int k = NB - 1; k >= 0; k--)
+        if (s_head[k][t] == best) b = k;
+      int p = best;
       while (true) {
+        // p is the head of window bin b: consume it and advance the bin
+        const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
+        const unsigned bin = unsigned((wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2)));
+        const int c = s_cur[b][t] + 1;
+        const unsigned long long e = ldu(sorted, unsigned(c));  // sorted[np] is a sentinel
+        s_cur[b][t] = c;
         p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af);
+        s_head[b][t] = unsigned(e >> 32) == bin ? int(unsigned(e)) : kNoParticle;
         // is p+1 in the window?  (bin coordinates packed 21 bits per axis;
         // excluded particles hold all ones)
         const int q = p + 1;
         if (q >= P.n) break;
-        const unsigned long long c = bxyz[q];
-        const unsigned ux = unsigned(ni - int(c & 0x1fffffu));
-        const unsigned uy = unsigned(nj - int((c >> 21) & 0x1fffffu));
-        const unsigned uz = unsigned(nk - int((c >> 42) & 0x1fffffu));
-        if (!(ux < unsigned(W) && uy < unsigned(W) && (D == 2 || uz < unsigned(W))) || c == ~0ull) break;
+        const unsigned long long cc = ldu(bxyz, unsigned(q));
+        const unsigned ux = unsigned(int(cc & 0x1fffffu) - wx0);
+        const unsigned uy = unsigned(int((cc >> 21) & 0x1fffffu) - wy0);
+        const unsigned uz = unsigned(int((cc >> 42) & 0x1fffffu) - wz0);
+        if (!(ux < unsigned(W) && uy < unsigned(W) && (D == 2 || uz < unsigned(W))) || cc == ~0ull) break;
+        b = int(ux) + MW * (int(uy) + (D == 3 ? MW * int(uz) : 0));
         p = q;
       }
     }
