@@ -57,6 +57,7 @@ struct jump_ctx {
   int *d_key = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_cursor = nullptr,
       *d_sums = nullptr;
   unsigned long long* d_bxyz = nullptr;
+  int *d_nodepos = nullptr, *d_order = nullptr;  // P2G work order: scan of active flags, node per slot
   unsigned long long* d_sorted = nullptr;  // (bin << 32) | particle, by bin then index; [np] = sentinel
   unsigned* d_flags = nullptr;
   double* d_red = nullptr;
@@ -245,6 +246,8 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_cnt, nn + 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_start, nn + 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_cursor, nn + 1);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_nodepos, nn + 1);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_order, nn);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_sums, (nn + 1) / kScanItems + 2);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_flags, 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_wmax, 1);
@@ -296,19 +299,19 @@ static void prof_collect(jump_ctx* ctx) {
   do {                                                                                        \
     if ((n) > 0) hipLaunchKernelGGL((kernel), dim3(blocks_for(n)), dim3(kBlock), 0, ctx->stream, __VA_ARGS__); \
   } while (0)
-#define LAUNCH_P2G(kernel, D, ...)                                                                   \
+#define LAUNCH_P2G(kernel, ...)                                                                      \
   do {                                                                                               \
-    hipLaunchKernelGGL((kernel), dim3(p2g_blocks<D>(ctx->G.nx, ctx->G.ny, ctx->G.nz)), dim3(kP2GBlock), \
-                       0, ctx->stream, __VA_ARGS__);                                                 \
+    hipLaunchKernelGGL((kernel), dim3((ctx->G.nn + kP2GBlock - 1) / kP2GBlock), dim3(kP2GBlock), 0,   \
+                       ctx->stream, __VA_ARGS__);                                                    \
   } while (0)
 
-static int scan_bins(jump_ctx* ctx) {
+// Exclusive prefix sum of in[0, nn+1) into out (three passes).
+static void scan_nodes(jump_ctx* ctx, const int* in, int* out) {
   const int n = ctx->G.nn + 1;
   const int nb = (n + kScanItems - 1) / kScanItems;
-  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->d_cnt, ctx->d_start, ctx->d_sums, n);
+  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kBlock), 0, ctx->stream, in, out, ctx->d_sums, n);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_sums, nb);
-  LAUNCH(k_scan_add, n, ctx->d_start, ctx->d_sums, n);
-  return JUMP_OK;
+  LAUNCH(k_scan_add, n, out, ctx->d_sums, n);
 }
 
 template <int D, int B> static int bin_and_sort(jump_ctx* ctx) {
@@ -318,9 +321,14 @@ template <int D, int B> static int bin_and_sort(jump_ctx* ctx) {
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cursor, 0, nn + 1);
   LAUNCH((k_bin<D, B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_bxyz, ctx->d_cnt,
          ctx->d_flags);
-  scan_bins(ctx);
+  scan_nodes(ctx, ctx->d_cnt, ctx->d_start);
   LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_cursor, ctx->d_sorted, np, nn);
   LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_sorted, nn);
+  // P2G work order: active nodes first (d_cnt is free again as the flags)
+  const int W = (B == JUMP_BASIS_CPDI_Q4) ? ctx->cpdi_w : Width<B>::W;
+  LAUNCH(k_node_flags<D>, nn + 1, ctx->G, ctx->d_start, W, ctx->d_cnt);
+  scan_nodes(ctx, ctx->d_cnt, ctx->d_nodepos);
+  LAUNCH(k_node_order, nn, ctx->d_nodepos, nn, ctx->d_order);
   return JUMP_OK;
 }
 
@@ -331,7 +339,7 @@ template <int B, int ORDER> static void step_usl2d(jump_ctx* ctx) {
   bin_and_sort<2, B>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<2, B, 0, ORDER>), 2, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->g,
+  LAUNCH_P2G((k_p2g<2, B, 0, ORDER, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g,
          ctx->cpdi_w, dt);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
@@ -356,14 +364,14 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   bin_and_sort<3, B>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<3, B, 0, ORDER>), 3, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->g,
+  LAUNCH_P2G((k_p2g<3, B, 0, ORDER, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g,
          0, dt);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
   prof_end(ctx, JUMP_PHASE_G2P1);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<3, B, 1, ORDER>), 3, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->g,
+  LAUNCH_P2G((k_p2g<3, B, 1, ORDER, true>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g,
          0, dt);
   prof_end(ctx, JUMP_PHASE_P2G2);
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
