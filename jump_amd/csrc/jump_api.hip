@@ -42,6 +42,7 @@ struct jump_ctx {
   // authoritative while !on_device.
   int64_t np = 0;
   std::vector<double> hx[3], hv[3], hm, hV, hV0, hF[9], heps[6], hsig[6], halpha, halphadot, hvm, hep[6];
+  std::vector<std::vector<double2>> hpairs;  // transfer buffers of the interleaved gather fields
   std::vector<int> hsid, helem[4];
   std::vector<uint8_t> hfixed;
   bool on_device = false;  // device holds the current particle state
@@ -126,6 +127,26 @@ static SolidDev make_solid_dev(const jump_material& m, const double lp[3]) {
 }
 
 // ------------------------------------------------------------------ transfers
+// The gather fields (x, m, V, v, sigma) live two per 16-byte slot: field f
+// of that order is half f%2 of pair array f/2.
+static SPtr pair_field(const Particles& P, int f) { return SPtr{reinterpret_cast<double*>(P.pr[f / 2]) + (f % 2)}; }
+static SPtr* gather_field(Particles& P, int D, int f) {
+  if (f < D) return &P.x[f];
+  if (f == D) return &P.m;
+  if (f == D + 1) return &P.V;
+  if (f < 2 * D + 2) return &P.v[f - D - 2];
+  return &P.sig[f - 2 * D - 2];
+}
+static std::vector<double>* gather_host(jump_ctx* ctx, int f) {
+  const int D = ctx->D;
+  if (f < D) return &ctx->hx[f];
+  if (f == D) return &ctx->hm;
+  if (f == D + 1) return &ctx->hV;
+  if (f < 2 * D + 2) return &ctx->hv[f - D - 2];
+  return &ctx->hsig[f - 2 * D - 2];
+}
+static int nfields(int D) { return D == 3 ? PairLayout<3>::NF : PairLayout<2>::NF; }
+
 static int alloc_particles(jump_ctx* ctx) {
   free_list(ctx->part_allocs);
   const int D = ctx->D, S = nsym(D);
@@ -133,14 +154,13 @@ static int alloc_particles(jump_ctx* ctx) {
   Particles& P = ctx->P;
   P = Particles{};
   int r = JUMP_OK;
-  for (int k = 0; k < D && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.x[k], n);
-  for (int k = 0; k < D && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.v[k], n);
-  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.m, n);
-  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.V, n);
+  const int npair = D == 3 ? PairLayout<3>::NP : PairLayout<2>::NP;
+  for (int j = 0; j < npair && !r; j++) r = dev_alloc(ctx, ctx->part_allocs, &P.pr[j], n);
+  if (r) return r;
+  for (int f = 0; f < (D == 3 ? PairLayout<3>::NF : PairLayout<2>::NF); f++) *gather_field(P, D, f) = pair_field(P, f);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.V0, n);
   for (int k = 0; k < D * D && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.F[k], n);
   for (int k = 0; k < S && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.eps[k], n);
-  for (int k = 0; k < S && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.sig[k], n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.alpha, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.alphadot, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.vm, n);
@@ -179,14 +199,20 @@ static int upload(jump_ctx* ctx) {
   int r = alloc_particles(ctx);
   if (r) return r;
   Particles& P = ctx->P;
-  for (int k = 0; k < D && !r; k++) r = h2d(ctx, P.x[k], ctx->hx[k]);
-  for (int k = 0; k < D && !r; k++) r = h2d(ctx, P.v[k], ctx->hv[k]);
-  if (!r) r = h2d(ctx, P.m, ctx->hm);
-  if (!r) r = h2d(ctx, P.V, ctx->hV);
+  {  // interleave the gather fields into their pair slots
+    const size_t n = size_t(ctx->np);
+    const int nf = nfields(D);
+    ctx->hpairs.assign((nf + 1) / 2, std::vector<double2>(n, double2{0.0, 0.0}));
+    for (int f = 0; f < nf; f++) {
+      const std::vector<double>& src = *gather_host(ctx, f);
+      std::vector<double2>& dst = ctx->hpairs[f / 2];
+      for (size_t i = 0; i < n; i++) (f % 2 ? dst[i].y : dst[i].x) = src[i];
+    }
+    for (int j = 0; j < (nf + 1) / 2 && !r; j++) r = h2d(ctx, P.pr[j], ctx->hpairs[j]);
+  }
   if (!r) r = h2d(ctx, P.V0, ctx->hV0);
   for (int k = 0; k < D * D && !r; k++) r = h2d(ctx, P.F[k], ctx->hF[k]);
   for (int k = 0; k < S && !r; k++) r = h2d(ctx, P.eps[k], ctx->heps[k]);
-  for (int k = 0; k < S && !r; k++) r = h2d(ctx, P.sig[k], ctx->hsig[k]);
   if (!r) r = h2d(ctx, P.alpha, ctx->halpha);
   if (!r) r = h2d(ctx, P.alphadot, ctx->halphadot);
   if (!r) r = h2d(ctx, P.vm, ctx->hvm);
@@ -211,20 +237,26 @@ static int download(jump_ctx* ctx) {
   const int D = ctx->D, S = nsym(D);
   Particles& P = ctx->P;
   int r = JUMP_OK;
-  for (int k = 0; k < D && !r; k++) r = d2h(ctx, ctx->hx[k], P.x[k]);
-  for (int k = 0; k < D && !r; k++) r = d2h(ctx, ctx->hv[k], P.v[k]);
-  if (!r) r = d2h(ctx, ctx->hm, P.m);
-  if (!r) r = d2h(ctx, ctx->hV, P.V);
+  const int nf = nfields(D);
+  ctx->hpairs.resize((nf + 1) / 2);
+  for (int j = 0; j < (nf + 1) / 2 && !r; j++) {
+    ctx->hpairs[j].resize(size_t(ctx->np));
+    r = d2h(ctx, ctx->hpairs[j], P.pr[j]);
+  }
   if (!r) r = d2h(ctx, ctx->hV0, P.V0);
   for (int k = 0; k < D * D && !r; k++) r = d2h(ctx, ctx->hF[k], P.F[k]);
   for (int k = 0; k < S && !r; k++) r = d2h(ctx, ctx->heps[k], P.eps[k]);
-  for (int k = 0; k < S && !r; k++) r = d2h(ctx, ctx->hsig[k], P.sig[k]);
   if (!r) r = d2h(ctx, ctx->halpha, P.alpha);
   if (!r) r = d2h(ctx, ctx->halphadot, P.alphadot);
   if (!r) r = d2h(ctx, ctx->hvm, P.vm);
   for (int k = 0; k < S && !r; k++) r = d2h(ctx, ctx->hep[k], P.ep[k]);
   if (r) return r;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int f = 0; f < nf; f++) {
+    std::vector<double>& dst = *gather_host(ctx, f);
+    const std::vector<double2>& src = ctx->hpairs[f / 2];
+    for (size_t i = 0; i < dst.size(); i++) dst[i] = f % 2 ? src[i].y : src[i].x;
+  }
   return JUMP_OK;
 }
 

@@ -40,21 +40,42 @@ struct SolidDev {
   double lp[3];
 };
 
+// One field of an interleaved pair array: element i is p[2*i].  The fields
+// the P2G gather reads (x, m, V, v, sigma, in that order) are stored two per
+// 16-byte slot, so a lane fetches two of them with one load instruction.
+// On gfx950 a wave's load instruction costs the vector memory pipeline about
+// the same whether each lane reads 8 or 16 bytes, and the gather is bound by
+// that instruction count.  Byte offsets are 32-bit: i < 2^28.
+struct SPtr {
+  double* p;
+  __device__ __forceinline__ double& operator[](int i) const {
+    return *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + unsigned(i) * 16u);
+  }
+};
+// Slot of gather field f (0..D-1 x, D m, D+1 V, D+2..2D+1 v, then sigma):
+// pair f/2, half f%2.  3D: (x,y)(z,m)(V,vx)(vy,vz)(s11,s22)(s33,s12)(s13,s23);
+// 2D: (x,y)(m,V)(vx,vy)(s11,s22)(s12,-).
+template <int D> struct PairLayout {
+  static constexpr int NF = D == 3 ? 14 : 9;
+  static constexpr int NP = (NF + 1) / 2;
+};
+
 struct Particles {
-  double* x[3];
-  double* v[3];
-  double* m;
-  double* V;
+  SPtr x[3];
+  SPtr v[3];
+  SPtr m;
+  SPtr V;
   double* V0;
   double* F[9];
   double* eps[6];
-  double* sig[6];
+  SPtr sig[6];
   double* alpha;
   double* alphadot;
   double* vm;
   double* ep[6];
   int* sid;      // solid index of each particle
   int* elem[4];  // CPDI: global corner index of each domain corner
+  double2* pr[7];  // the pair arrays x, m, V, v and sigma live in
   int n;
 };
 
@@ -72,8 +93,9 @@ struct GridArrays {
 // address is the base plus a 32-bit byte offset, which gfx950 global loads
 // take as SGPR base + VGPR offset, so a gather of many fields at one index
 // costs one shift instead of a 64-bit add per field.  Requires i * sizeof(T)
-// < 2^32 (the library caps a context at kMaxParticles particles).
-constexpr long long kMaxParticles = 1ll << 29;
+// < 2^32 (the library caps a context at kMaxParticles particles, so that
+// also 16-byte elements stay in range).
+constexpr long long kMaxParticles = 1ll << 28;
 template <class T> __device__ __forceinline__ T ldu(const T* __restrict__ base, unsigned i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * unsigned(sizeof(T)));
 }

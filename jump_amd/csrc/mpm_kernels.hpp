@@ -235,20 +235,23 @@ template <int B> struct MaxW { static constexpr int W = Width<B>::W; };
 template <> struct MaxW<JUMP_BASIS_CPDI_Q4> { static constexpr int W = 5; };
 constexpr int kNoParticle = 0x7fffffff;
 
-// Particle fields one P2G pass reads.  PASS 0: x, m, V, v, sigma; PASS 1
-// (re-projection): x, m, v.
+// Particle fields one P2G pass reads, fetched as whole 16-byte slots.
+// PASS 0: x, m, V, v, sigma (every slot); PASS 1 (re-projection): x, m, v
+// (the first four slots in 3D, of the third only vx).
 template <int D, int PASS> struct P2GFields {
-  static constexpr int S = D == 2 ? 3 : 6;
-  static constexpr int NF = PASS == 0 ? D + 2 + D + S : D + 1 + D;
-  __device__ static __forceinline__ double get(const Particles& P, int f, unsigned q) {
-    if (f < D) return ldu(P.x[f], q);
-    if (f == D) return ldu(P.m, q);
-    if (PASS == 0) {
-      if (f == D + 1) return ldu(P.V, q);
-      if (f < D + 2 + D) return ldu(P.v[f - D - 2], q);
-      return ldu(P.sig[f - 2 * D - 2], q);
+  static constexpr int NF = PairLayout<D>::NF;
+  static constexpr int NPAIR = PASS == 0 ? PairLayout<D>::NP : (D == 3 ? 4 : 3);
+  __device__ static __forceinline__ void load(const Particles& P, unsigned q, double* v) {
+#pragma unroll
+    for (int j = 0; j < NPAIR; j++) {
+      if (PASS == 1 && D == 3 && j == 2) {  // only vx of (V, vx)
+        v[5] = ldu(reinterpret_cast<const double*>(P.pr[2]), 2 * q + 1);
+        continue;
+      }
+      const double2 w = ldu(P.pr[j], q);
+      v[2 * j] = w.x;
+      if (2 * j + 1 < NF) v[2 * j + 1] = w.y;
     }
-    return ldu(P.v[f - D - 1], q);
   }
 };
 
@@ -260,9 +263,8 @@ __device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, c
                                         const double* corners, int p, int ni, int nj, int nk, double& am,
                                         double* ap, double* af) {
   constexpr int NF = P2GFields<D, PASS>::NF;
-  double v[NF];
-#pragma unroll
-  for (int f = 0; f < NF; f++) v[f] = P2GFields<D, PASS>::get(P, f, unsigned(p));
+  double v[NF + 1];
+  P2GFields<D, PASS>::load(P, unsigned(p), v);
   double lp[3] = {0.0, 0.0, 0.0};
   if (B == JUMP_BASIS_GIMP) {
     const SolidDev& s = S[P.sid[p]];
@@ -290,7 +292,7 @@ __device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, c
     }
   } else {
     const double Nm = N * fM;
-    for (int k = 0; k < D; k++) ap[k] += Nm * v[D + 1 + k];
+    for (int k = 0; k < D; k++) ap[k] += Nm * v[D + 2 + k];
   }
 }
 
