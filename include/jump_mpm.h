@@ -196,6 +196,18 @@ int jump_get_grid(jump_ctx* ctx, double* mass, double* momentum0,
  * (Fix.jl:248-270). */
 int jump_energies(jump_ctx* ctx, double* ke, double* se);
 
+/* Checkpoint / restart of the particle state (the reference has none; its
+ * dumps, Output.jl:146-290, are write-only).  A checkpoint holds every
+ * per-particle array: positions, velocities, mass, volumes, F, strain,
+ * stress and the material history; the grid is rebuilt by the next step.
+ * The caller keeps t.  jump_checkpoint_bytes gives the buffer size;
+ * jump_checkpoint_load accepts only a checkpoint of a context with the same
+ * dimension and particle count (the same solids) and rejects anything else
+ * with JUMP_ERR_ARG. */
+int jump_checkpoint_bytes(jump_ctx* ctx, int64_t* bytes);
+int jump_checkpoint_save(jump_ctx* ctx, void* buf, int64_t bytes);
+int jump_checkpoint_load(jump_ctx* ctx, const void* buf, int64_t bytes);
+
 /* Diagnostics (no counterpart in the reference, which recomputes the support
  * inside every shape-function call, Basis.jl:63-78,390-425).
  * jump_get_bins: runs the binning of the current particle positions and

@@ -148,6 +148,12 @@ def load():
     lib.jump_profile.argtypes = [ctypes.c_void_p, _i32]
     lib.jump_profile_read.restype = ctypes.c_int
     lib.jump_profile_read.argtypes = [ctypes.c_void_p, _i32, _pd, ctypes.POINTER(_i64)]
+    lib.jump_checkpoint_bytes.restype = ctypes.c_int
+    lib.jump_checkpoint_bytes.argtypes = [ctypes.c_void_p, ctypes.POINTER(_i64)]
+    lib.jump_checkpoint_save.restype = ctypes.c_int
+    lib.jump_checkpoint_save.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _i64]
+    lib.jump_checkpoint_load.restype = ctypes.c_int
+    lib.jump_checkpoint_load.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _i64]
     _LIB = lib
     return lib
 
@@ -158,4 +164,5 @@ EXPORTED = [
     "jump_ctx_create", "jump_ctx_destroy", "jump_last_error", "jump_add_solid", "jump_set_fixed", "jump_step",
     "jump_flags", "jump_get_solid", "jump_get_solid_state", "jump_get_grid", "jump_energies", "jump_get_bins",
     "jump_abi_version", "jump_profile", "jump_profile_read",
+    "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
 ]

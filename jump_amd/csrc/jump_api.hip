@@ -659,6 +659,96 @@ int jump_step(jump_ctx* ctx, int64_t nsteps, double* t_inout) {
   return JUMP_OK;
 }
 
+// ------------------------------------------------------------------ checkpoints
+// A checkpoint is every per-particle array of the state (positions,
+// velocities, mass, volumes, F, strain, stress, material history) in the
+// staging layout, after a fixed header.  The grid is not part of it: the
+// step rebuilds every grid field from the particles.
+namespace {
+struct CkptHeader {
+  char magic[8];
+  int32_t version, dim;
+  int64_t np, narrays;
+};
+constexpr char kCkptMagic[8] = {'J', 'U', 'M', 'P', 'C', 'K', 'P', 'T'};
+}  // namespace
+
+static std::vector<std::vector<double>*> state_arrays(jump_ctx* ctx) {
+  const int D = ctx->D, S = nsym(D);
+  std::vector<std::vector<double>*> a;
+  for (int k = 0; k < D; k++) a.push_back(&ctx->hx[k]);
+  for (int k = 0; k < D; k++) a.push_back(&ctx->hv[k]);
+  a.push_back(&ctx->hm);
+  a.push_back(&ctx->hV);
+  a.push_back(&ctx->hV0);
+  for (int k = 0; k < D * D; k++) a.push_back(&ctx->hF[k]);
+  for (int k = 0; k < S; k++) a.push_back(&ctx->heps[k]);
+  for (int k = 0; k < S; k++) a.push_back(&ctx->hsig[k]);
+  a.push_back(&ctx->halpha);
+  a.push_back(&ctx->halphadot);
+  a.push_back(&ctx->hvm);
+  for (int k = 0; k < S; k++) a.push_back(&ctx->hep[k]);
+  return a;
+}
+
+int jump_checkpoint_bytes(jump_ctx* ctx, int64_t* bytes) {
+  if (!ctx || !bytes) return JUMP_ERR_ARG;
+  *bytes = int64_t(sizeof(CkptHeader)) + int64_t(state_arrays(ctx).size()) * ctx->np * int64_t(sizeof(double));
+  return JUMP_OK;
+}
+
+int jump_checkpoint_save(jump_ctx* ctx, void* buf, int64_t bytes) {
+  if (!ctx || !buf) return JUMP_ERR_ARG;
+  int64_t need = 0;
+  jump_checkpoint_bytes(ctx, &need);
+  if (bytes < need) return set_err(ctx, JUMP_ERR_ARG, "checkpoint buffer too small");
+  if (ctx->on_device) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    int r = download(ctx);
+    if (r) return r;
+  }
+  auto arrays = state_arrays(ctx);
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = 1;
+  h.dim = ctx->D;
+  h.np = ctx->np;
+  h.narrays = int64_t(arrays.size());
+  char* out = static_cast<char*>(buf);
+  std::memcpy(out, &h, sizeof h);
+  out += sizeof h;
+  for (auto* a : arrays) {
+    std::memcpy(out, a->data(), size_t(ctx->np) * sizeof(double));
+    out += size_t(ctx->np) * sizeof(double);
+  }
+  return JUMP_OK;
+}
+
+int jump_checkpoint_load(jump_ctx* ctx, const void* buf, int64_t bytes) {
+  if (!ctx || !buf) return JUMP_ERR_ARG;
+  if (bytes < int64_t(sizeof(CkptHeader))) return set_err(ctx, JUMP_ERR_ARG, "checkpoint too short");
+  CkptHeader h;
+  std::memcpy(&h, buf, sizeof h);
+  auto arrays = state_arrays(ctx);
+  if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1)
+    return set_err(ctx, JUMP_ERR_ARG, "not a checkpoint of this library");
+  if (h.dim != ctx->D || h.np != ctx->np || h.narrays != int64_t(arrays.size()))
+    return set_err(ctx, JUMP_ERR_ARG, "checkpoint does not match this context's solids");
+  int64_t need = 0;
+  jump_checkpoint_bytes(ctx, &need);
+  if (bytes < need) return set_err(ctx, JUMP_ERR_ARG, "checkpoint truncated");
+  const char* in = static_cast<const char*>(buf) + sizeof h;
+  for (auto* a : arrays) {
+    a->resize(size_t(ctx->np));
+    std::memcpy(a->data(), in, size_t(ctx->np) * sizeof(double));
+    in += size_t(ctx->np) * sizeof(double);
+  }
+  ctx->on_device = false;  // the next call that needs the device uploads the restored state
+  ctx->grid_valid = false;
+  ctx->flags = 0;
+  return JUMP_OK;
+}
+
 int jump_flags(jump_ctx* ctx, uint32_t* flags) {
   if (!ctx || !flags) return JUMP_ERR_ARG;
   *flags = ctx->flags;

@@ -185,6 +185,18 @@ class Engine:
             out[name] = (ms.value, cnt.value)
         return out
 
+    def checkpoint(self):
+        """Snapshot of the particle state (bytes); see restore()."""
+        n = ctypes.c_int64()
+        self._check(self.lib.jump_checkpoint_bytes(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(self.lib.jump_checkpoint_save(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, blob):
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        self._check(self.lib.jump_checkpoint_load(self.h, buf, len(blob)))
+
     def bins(self):
         """Diagnostic: per-particle stencil origin (HIP library only)."""
         n = sum(self.counts)
