@@ -208,6 +208,25 @@ int jump_checkpoint_bytes(jump_ctx* ctx, int64_t* bytes);
 int jump_checkpoint_save(jump_ctx* ctx, void* buf, int64_t bytes);
 int jump_checkpoint_load(jump_ctx* ctx, const void* buf, int64_t bytes);
 
+/* Batches: many independent 2D USL problems stepped with shared launches
+ * (the reference runs one problem per solve_explicit_dynamics_2D call,
+ * SolveExplicitSolidMechanics2D.jl:335).  jump_batch_create takes n
+ * configured contexts (2D USL, solids added, Dirichlet mask set; any basis
+ * but CPDI, any grid size, dt, tolerance and materials) on one device and
+ * copies their state; jump_batch_step advances every member nsteps steps
+ * (t_inout: n times, each advanced by its member's dt); jump_batch_sync
+ * copies the state back into the member contexts, whose getters then read
+ * it; jump_batch_energies writes n kinetic and strain energies.  Each member
+ * steps bit for bit as it would alone.  The member contexts are not stepped
+ * while in a batch; destroy the batch before them. */
+typedef struct jump_batch jump_batch;
+int jump_batch_create(jump_ctx* const* ctxs, int32_t n, int device, jump_batch** out);
+int jump_batch_step(jump_batch* batch, int64_t nsteps, double* t_inout);
+int jump_batch_sync(jump_batch* batch);
+int jump_batch_energies(jump_batch* batch, double* ke, double* se);
+int jump_batch_destroy(jump_batch* batch);
+const char* jump_batch_last_error(const jump_batch* batch);
+
 /* Diagnostics (no counterpart in the reference, which recomputes the support
  * inside every shape-function call, Basis.jl:63-78,390-425).
  * jump_get_bins: runs the binning of the current particle positions and

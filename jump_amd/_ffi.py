@@ -154,6 +154,16 @@ def load():
     lib.jump_checkpoint_save.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _i64]
     lib.jump_checkpoint_load.restype = ctypes.c_int
     lib.jump_checkpoint_load.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _i64]
+    H = ctypes.c_void_p
+    for name, res, args in (("jump_batch_create", ctypes.c_int, [ctypes.POINTER(H), _i32, ctypes.c_int, ctypes.POINTER(H)]),
+                            ("jump_batch_step", ctypes.c_int, [H, _i64, _pd]),
+                            ("jump_batch_sync", ctypes.c_int, [H]),
+                            ("jump_batch_energies", ctypes.c_int, [H, _pd, _pd]),
+                            ("jump_batch_destroy", ctypes.c_int, [H]),
+                            ("jump_batch_last_error", ctypes.c_char_p, [H])):
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
     _LIB = lib
     return lib
 
@@ -165,4 +175,6 @@ EXPORTED = [
     "jump_flags", "jump_get_solid", "jump_get_solid_state", "jump_get_grid", "jump_energies", "jump_get_bins",
     "jump_abi_version", "jump_profile", "jump_profile_read",
     "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
+    "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies", "jump_batch_destroy",
+    "jump_batch_last_error",
 ]

@@ -66,6 +66,18 @@ struct jump_ctx {
   uint8_t* d_fixed = nullptr;
   int cpdi_w = 0;
   bool grid_valid = false;
+
+  // Batch instances (an internal context of jump_batch holds several 2D
+  // problems; an ordinary context is one instance with grid G).  NN: nodes
+  // of all instances together.
+  int ninst = 1;
+  int NN = 0;
+  std::vector<InstDev> hinst;
+  std::vector<int> hnode_off;
+  InstDev* d_inst = nullptr;
+  int* d_node_off = nullptr;
+  int max_rigid_rank = -1;  // highest rigid_rank of any solid (-1: none)
+  BatchDev bt() const { return BatchDev{d_inst, d_node_off, ninst, NN}; }
   uint32_t flags = 0;
 
   // per-phase timing with HIP events on the context stream (jump_profile)
@@ -89,6 +101,12 @@ static int set_err(jump_ctx* c, int code, const std::string& msg) {
     hipError_t e_ = (expr);                                                                 \
     if (e_ != hipSuccess)                                                                   \
       return set_err(ctx, JUMP_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+#define HIP_TRY_B(expr)                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return batch_err(b, JUMP_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));   \
   } while (0)
 
 static int nsym(int D) { return D == 2 ? 3 : 6; }
@@ -266,7 +284,7 @@ static int ensure_device(jump_ctx* ctx) {
 }
 
 static int alloc_grid(jump_ctx* ctx) {
-  const size_t nn = size_t(ctx->G.nn);
+  const size_t nn = size_t(ctx->NN);
   const int D = ctx->D;
   GridArrays& g = ctx->g;
   int r = dev_alloc(ctx, ctx->grid_allocs, &g.m, nn);
@@ -283,6 +301,16 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_sums, (nn + 1) / kScanItems + 2);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_flags, 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_wmax, 1);
+  if (!r && ctx->ninst > 1) {
+    r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_inst, ctx->hinst.size());
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_node_off, ctx->hnode_off.size());
+    if (!r) {
+      HIP_TRY(hipMemcpyAsync(ctx->d_inst, ctx->hinst.data(), ctx->hinst.size() * sizeof(InstDev),
+                             hipMemcpyHostToDevice, ctx->stream));
+      HIP_TRY(hipMemcpyAsync(ctx->d_node_off, ctx->hnode_off.data(), ctx->hnode_off.size() * sizeof(int),
+                             hipMemcpyHostToDevice, ctx->stream));
+    }
+  }
   if (r) return r;
   g.fixed = ctx->d_fixed;
   HIP_TRY(hipMemsetAsync(ctx->d_fixed, 0, nn, ctx->stream));
@@ -333,59 +361,59 @@ static void prof_collect(jump_ctx* ctx) {
   } while (0)
 #define LAUNCH_P2G(kernel, ...)                                                                      \
   do {                                                                                               \
-    hipLaunchKernelGGL((kernel), dim3((ctx->G.nn + kP2GBlock - 1) / kP2GBlock), dim3(kP2GBlock), 0,   \
+    hipLaunchKernelGGL((kernel), dim3((ctx->NN + kP2GBlock - 1) / kP2GBlock), dim3(kP2GBlock), 0,     \
                        ctx->stream, __VA_ARGS__);                                                    \
   } while (0)
 
 // Exclusive prefix sum of in[0, nn+1) into out (three passes).
 static void scan_nodes(jump_ctx* ctx, const int* in, int* out) {
-  const int n = ctx->G.nn + 1;
+  const int n = ctx->NN + 1;
   const int nb = (n + kScanItems - 1) / kScanItems;
   hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kBlock), 0, ctx->stream, in, out, ctx->d_sums, n);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_sums, nb);
   LAUNCH(k_scan_add, n, out, ctx->d_sums, n);
 }
 
-template <int D, int B> static int bin_and_sort(jump_ctx* ctx) {
-  const int nn = ctx->G.nn;
+template <int D, int B, bool MULTI> static int bin_and_sort(jump_ctx* ctx) {
+  const int nn = ctx->NN;
   const int np = int(ctx->np);
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cnt, 0, nn + 1);
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cursor, 0, nn + 1);
-  LAUNCH((k_bin<D, B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_bxyz, ctx->d_cnt,
-         ctx->d_flags);
+  LAUNCH((k_bin<D, B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_bxyz,
+         ctx->d_cnt, ctx->d_flags, ctx->bt());
   scan_nodes(ctx, ctx->d_cnt, ctx->d_start);
   LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_cursor, ctx->d_sorted, np, nn);
   LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_sorted, nn);
   // P2G work order: active nodes first (d_cnt is free again as the flags)
   const int W = (B == JUMP_BASIS_CPDI_Q4) ? ctx->cpdi_w : Width<B>::W;
-  LAUNCH(k_node_flags<D>, nn + 1, ctx->G, ctx->d_start, W, ctx->d_cnt);
+  LAUNCH((k_node_flags<D, MULTI>), nn + 1, ctx->G, ctx->d_start, W, ctx->d_cnt, ctx->bt());
   scan_nodes(ctx, ctx->d_cnt, ctx->d_nodepos);
   LAUNCH(k_node_order, nn, ctx->d_nodepos, nn, ctx->d_order);
   return JUMP_OK;
 }
 
-template <int B, int ORDER> static void step_usl2d(jump_ctx* ctx) {
+// MULTI (a batch of 2D instances): the gather runs over the compacted active
+// nodes, since the instances' grids are mostly larger than their solids.
+template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
   prof_begin(ctx);
-  bin_and_sort<2, B>(ctx);
+  bin_and_sort<2, B, MULTI>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<2, B, 0, ORDER, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g,
-         ctx->cpdi_w, dt);
+  LAUNCH_P2G((k_p2g<2, B, 0, ORDER, MULTI, MULTI>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt());
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
-  for (auto& s : ctx->solids)
-    if (s.dev.rigid)
-      LAUNCH(k_rigid_override, s.n, ctx->P, ctx->G, s.dev, int(s.offset), int(s.offset + s.n), ctx->g, ctx->d_flags);
+  for (int r = 0; r <= ctx->max_rigid_rank; r++)
+    LAUNCH((k_rigid_override<MULTI>), np, ctx->P, ctx->G, ctx->d_solids, r, ctx->g, ctx->d_flags, ctx->bt());
   prof_end(ctx, JUMP_PHASE_RIGID);
   prof_begin(ctx);
-  LAUNCH((k_g2p_usl2d<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.tolerance, dt,
-         ctx->d_flags);
+  LAUNCH((k_g2p_usl2d<B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.tolerance, dt,
+         ctx->d_flags, ctx->bt());
   prof_end(ctx, JUMP_PHASE_G2P);
   prof_begin(ctx);
-  for (auto& s : ctx->solids)
-    if (s.dev.rigid) LAUNCH(k_rigid_move, s.n, ctx->P, s.dev, int(s.offset), int(s.offset + s.n), dt);
+  if (ctx->max_rigid_rank >= 0) LAUNCH((k_rigid_move<MULTI>), np, ctx->P, ctx->d_solids, dt, ctx->bt());
   prof_end(ctx, JUMP_PHASE_RIGID);
 }
 
@@ -393,18 +421,18 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
   prof_begin(ctx);
-  bin_and_sort<3, B>(ctx);
+  bin_and_sort<3, B, false>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<3, B, 0, ORDER, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g,
-         0, dt);
+  LAUNCH_P2G((k_p2g<3, B, 0, ORDER, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt());
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
   prof_end(ctx, JUMP_PHASE_G2P1);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<3, B, 1, ORDER, true>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g,
-         0, dt);
+  LAUNCH_P2G((k_p2g<3, B, 1, ORDER, true, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt());
   prof_end(ctx, JUMP_PHASE_P2G2);
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
   // zeroes components of momentum0/momentum already zeroed by the grid update.
@@ -434,8 +462,13 @@ static int run_steps(jump_ctx* ctx, int64_t nsteps) {
       switch (ctx->prm.basis) {
 #define USL_CASE(BB)                                                     \
   case BB:                                                               \
-    if (ref) step_usl2d<BB, JUMP_ORDER_REFERENCE>(ctx);                  \
-    else step_usl2d<BB, JUMP_ORDER_BINNED>(ctx);                         \
+    if (ctx->ninst > 1) {                                                \
+      if (ref) step_usl2d<BB, JUMP_ORDER_REFERENCE, true>(ctx);          \
+      else step_usl2d<BB, JUMP_ORDER_BINNED, true>(ctx);                 \
+    } else {                                                             \
+      if (ref) step_usl2d<BB, JUMP_ORDER_REFERENCE, false>(ctx);         \
+      else step_usl2d<BB, JUMP_ORDER_BINNED, false>(ctx);                \
+    }                                                                    \
     break;
         USL_CASE(JUMP_BASIS_LINEAR)
         USL_CASE(JUMP_BASIS_QUAD_BSPLINE)
@@ -505,6 +538,7 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   G.dyI = 1.0 / G.dy;
   G.dzI = 1.0 / G.dz;
   for (int k = 0; k < 3; k++) G.body[k] = p->body[k];
+  ctx->NN = G.nn;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -547,7 +581,7 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
   if (D == 3 && mat->kind == JUMP_MAT_RIGID)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid (Solid3D has no rigid flag)");
   if (D == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");
-  if (ctx->np + ds->n > kMaxParticles) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "too many particles (at most 2^29 per context)");
+  if (ctx->np + ds->n > kMaxParticles) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "too many particles (at most 2^28 per context)");
   const bool cpdi = ctx->prm.basis == JUMP_BASIS_CPDI_Q4;
   if (cpdi && (!ds->corners || !ds->elems || ds->ncorners <= 0))
     return set_err(ctx, JUMP_ERR_ARG, "CPDI solid needs corners and elems");
@@ -581,6 +615,10 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
   hs.n = ds->n;
   hs.mat = *mat;
   hs.dev = make_solid_dev(*mat, lp);
+  if (hs.dev.rigid) {
+    hs.dev.rigid_rank = ctx->max_rigid_rank + 1;  // rigid solids are overridden in solid order
+    ctx->max_rigid_rank = hs.dev.rigid_rank;
+  }
   hs.offset = ctx->np;
   const int sid = int(ctx->solids.size());
   const int coff = int(ctx->corners.size() / 2);
@@ -842,6 +880,27 @@ int jump_get_grid(jump_ctx* ctx, double* mass, double* p0, double* p, double* p2
   return JUMP_OK;
 }
 
+// KE and SE of the particles [begin, end), reduced over blocks of kBlock
+// particles aligned at begin, then over the block sums.
+static int range_energies(jump_ctx* ctx, int64_t begin, int64_t end, double* ke, double* se) {
+  if (end <= begin) {
+    *ke = 0.0;
+    *se = 0.0;
+    return JUMP_OK;
+  }
+  const int nb = blocks_for(end - begin);
+  double* out = ctx->d_red + 2 * size_t(nb);
+  if (ctx->D == 2) LAUNCH(k_energy_partial<2>, end - begin, ctx->P, int(begin), int(end), ctx->d_red);
+  else LAUNCH(k_energy_partial<3>, end - begin, ctx->P, int(begin), int(end), ctx->d_red);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_red, nb, out);
+  double h[2];
+  HIP_TRY(hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *ke = h[0];
+  *se = h[1];
+  return JUMP_OK;
+}
+
 int jump_energies(jump_ctx* ctx, double* ke, double* se) {
   if (!ctx) return JUMP_ERR_ARG;
   if (!ke || !se) return set_err(ctx, JUMP_ERR_ARG, "null output");
@@ -853,17 +912,8 @@ int jump_energies(jump_ctx* ctx, double* ke, double* se) {
     *se = 0.0;
     return JUMP_OK;
   }
-  const int nb = blocks_for(ctx->np);
-  double* out = ctx->d_red + 2 * size_t(nb);
-  if (ctx->D == 2) LAUNCH(k_energy_partial<2>, ctx->np, ctx->P, ctx->d_red);
-  else LAUNCH(k_energy_partial<3>, ctx->np, ctx->P, ctx->d_red);
-  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_red, nb, out);
-  double h[2];
-  HIP_TRY(hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  *ke = h[0];
-  *se = h[1];
-  return JUMP_OK;
+  r = range_energies(ctx, 0, ctx->np, ke, se);
+  return r;
 }
 
 int jump_get_bins(jump_ctx* ctx, int32_t* keys) {
@@ -876,18 +926,18 @@ int jump_get_bins(jump_ctx* ctx, int32_t* keys) {
   if (r) return r;
   if (ctx->D == 2) {
     switch (ctx->prm.basis) {
-      case JUMP_BASIS_LINEAR: bin_and_sort<2, JUMP_BASIS_LINEAR>(ctx); break;
-      case JUMP_BASIS_QUAD_BSPLINE: bin_and_sort<2, JUMP_BASIS_QUAD_BSPLINE>(ctx); break;
-      case JUMP_BASIS_CUBIC_BSPLINE: bin_and_sort<2, JUMP_BASIS_CUBIC_BSPLINE>(ctx); break;
-      case JUMP_BASIS_CPDI_Q4: bin_and_sort<2, JUMP_BASIS_CPDI_Q4>(ctx); break;
-      case JUMP_BASIS_GIMP: bin_and_sort<2, JUMP_BASIS_GIMP>(ctx); break;
+      case JUMP_BASIS_LINEAR: bin_and_sort<2, JUMP_BASIS_LINEAR, false>(ctx); break;
+      case JUMP_BASIS_QUAD_BSPLINE: bin_and_sort<2, JUMP_BASIS_QUAD_BSPLINE, false>(ctx); break;
+      case JUMP_BASIS_CUBIC_BSPLINE: bin_and_sort<2, JUMP_BASIS_CUBIC_BSPLINE, false>(ctx); break;
+      case JUMP_BASIS_CPDI_Q4: bin_and_sort<2, JUMP_BASIS_CPDI_Q4, false>(ctx); break;
+      case JUMP_BASIS_GIMP: bin_and_sort<2, JUMP_BASIS_GIMP, false>(ctx); break;
     }
   } else {
     switch (ctx->prm.basis) {
-      case JUMP_BASIS_LINEAR: bin_and_sort<3, JUMP_BASIS_LINEAR>(ctx); break;
-      case JUMP_BASIS_QUAD_BSPLINE: bin_and_sort<3, JUMP_BASIS_QUAD_BSPLINE>(ctx); break;
-      case JUMP_BASIS_CUBIC_BSPLINE: bin_and_sort<3, JUMP_BASIS_CUBIC_BSPLINE>(ctx); break;
-      case JUMP_BASIS_GIMP: bin_and_sort<3, JUMP_BASIS_GIMP>(ctx); break;
+      case JUMP_BASIS_LINEAR: bin_and_sort<3, JUMP_BASIS_LINEAR, false>(ctx); break;
+      case JUMP_BASIS_QUAD_BSPLINE: bin_and_sort<3, JUMP_BASIS_QUAD_BSPLINE, false>(ctx); break;
+      case JUMP_BASIS_CUBIC_BSPLINE: bin_and_sort<3, JUMP_BASIS_CUBIC_BSPLINE, false>(ctx); break;
+      case JUMP_BASIS_GIMP: bin_and_sort<3, JUMP_BASIS_GIMP, false>(ctx); break;
     }
   }
   HIP_TRY(hipGetLastError());
@@ -919,3 +969,221 @@ int jump_profile_read(jump_ctx* ctx, int32_t phase, double* total_ms, int64_t* c
 int jump_abi_version(void) { return JUMP_ABI_VERSION; }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ batches
+// A batch steps several independent 2D USL contexts with shared launches.
+// Members with the same basis and summation order form one group, held by an
+// internal context whose particles are the members' particles back to back
+// and whose nodes are the members' grids back to back (instance i at
+// node_off[i]); every kernel finds the instance of its particle or node and
+// works in that instance's grid.  Each member's particles keep their order
+// and each member's reductions their tree, so a member steps bit for bit as
+// it would alone.
+struct jump_batch {
+  int device = 0;
+  std::string err;
+  std::vector<jump_ctx*> members;
+  struct Group {
+    jump_ctx* inner = nullptr;
+    std::vector<int> members;        // member index of each instance
+    std::vector<int64_t> part_off;   // first particle of each instance in the group
+  };
+  std::vector<Group> groups;
+};
+
+static int batch_err(jump_batch* b, int code, const std::string& m) {
+  b->err = m;
+  return code;
+}
+
+const char* jump_batch_last_error(const jump_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int jump_batch_destroy(jump_batch* b) {
+  if (!b) return JUMP_OK;
+  for (auto& g : b->groups) jump_ctx_destroy(g.inner);
+  delete b;
+  return JUMP_OK;
+}
+
+int jump_batch_create(jump_ctx* const* ctxs, int32_t n, int device, jump_batch** out) {
+  if (!ctxs || n <= 0 || !out) return JUMP_ERR_ARG;
+  *out = nullptr;
+  for (int i = 0; i < n; i++) {
+    const jump_ctx* c = ctxs[i];
+    if (!c) return JUMP_ERR_ARG;
+    if (c->D != 2 || c->prm.algorithm != JUMP_ALG_USL) return JUMP_ERR_UNSUPPORTED;  // batches are 2D USL
+    if (c->prm.basis == JUMP_BASIS_CPDI_Q4) return JUMP_ERR_UNSUPPORTED;
+    if (c->device != device) return JUMP_ERR_ARG;
+  }
+  jump_batch* b = new (std::nothrow) jump_batch();
+  if (!b) return JUMP_ERR_NOMEM;
+  b->device = device;
+  b->members.assign(ctxs, ctxs + n);
+  // group by (basis, order), members in the given order
+  std::vector<std::pair<int, int>> keys;
+  std::vector<int> gof(n);
+  for (int i = 0; i < n; i++) {
+    const std::pair<int, int> k{ctxs[i]->prm.basis, ctxs[i]->prm.order};
+    auto it = std::find(keys.begin(), keys.end(), k);
+    gof[i] = int(it - keys.begin());
+    if (it == keys.end()) keys.push_back(k);
+  }
+  b->groups.resize(keys.size());
+  for (size_t gi = 0; gi < keys.size(); gi++) {
+    jump_batch::Group& grp = b->groups[gi];
+    for (int i = 0; i < n; i++)
+      if (gof[i] == int(gi)) grp.members.push_back(i);
+    jump_ctx* first = ctxs[grp.members[0]];
+    jump_ctx* in = nullptr;
+    int r = jump_ctx_create(&first->prm, device, &in);
+    if (r) {
+      jump_batch_destroy(b);
+      return r;
+    }
+    grp.inner = in;
+    // instances: grids back to back
+    in->ninst = int(grp.members.size());
+    in->hinst.clear();
+    in->hnode_off.assign(1, 0);
+    int64_t nn = 0;
+    for (int m : grp.members) {
+      const jump_ctx* c = ctxs[m];
+      in->hinst.push_back(InstDev{c->G, c->prm.dt, c->prm.tolerance});
+      nn += c->G.nn;
+      if (nn >= (int64_t(1) << 31) - 2) {
+        jump_batch_destroy(b);
+        return JUMP_ERR_UNSUPPORTED;
+      }
+      in->hnode_off.push_back(int(nn));
+    }
+    in->NN = int(nn);
+    free_list(in->grid_allocs);
+    r = alloc_grid(in);
+    if (r) {
+      jump_batch_destroy(b);
+      return r;
+    }
+    in->hfixed.clear();
+    // particles, solids and Dirichlet masks, member after member
+    for (size_t ii = 0; ii < grp.members.size(); ii++) {
+      jump_ctx* c = ctxs[grp.members[ii]];
+      if (c->on_device) {
+        hipSetDevice(c->device);
+        r = download(c);
+        if (r) {
+          jump_batch_destroy(b);
+          return r;
+        }
+      }
+      grp.part_off.push_back(in->np);
+      const int soff = int(in->solids.size());
+      for (const HostSolid& hs0 : c->solids) {
+        HostSolid hs = hs0;
+        hs.offset += in->np;
+        hs.dev.inst = int(ii);
+        in->solids.push_back(hs);
+        in->max_rigid_rank = std::max(in->max_rigid_rank, hs.dev.rigid ? hs.dev.rigid_rank : -1);
+      }
+      auto cat = [](std::vector<double>& d, const std::vector<double>& s2) { d.insert(d.end(), s2.begin(), s2.end()); };
+      for (int k = 0; k < 2; k++) { cat(in->hx[k], c->hx[k]); cat(in->hv[k], c->hv[k]); }
+      cat(in->hm, c->hm);
+      cat(in->hV, c->hV);
+      cat(in->hV0, c->hV0);
+      for (int k = 0; k < 4; k++) cat(in->hF[k], c->hF[k]);
+      for (int k = 0; k < 3; k++) { cat(in->heps[k], c->heps[k]); cat(in->hsig[k], c->hsig[k]); cat(in->hep[k], c->hep[k]); }
+      cat(in->halpha, c->halpha);
+      cat(in->halphadot, c->halphadot);
+      cat(in->hvm, c->hvm);
+      for (int sid : c->hsid) in->hsid.push_back(sid + soff);
+      in->np += c->np;
+      in->hfixed.insert(in->hfixed.end(), c->hfixed.begin(), c->hfixed.end());
+    }
+    if (in->np > kMaxParticles) {
+      jump_batch_destroy(b);
+      return JUMP_ERR_UNSUPPORTED;
+    }
+    hipSetDevice(device);
+    if (hipMemcpyAsync(in->d_fixed, in->hfixed.data(), in->hfixed.size(), hipMemcpyHostToDevice, in->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(in->stream) != hipSuccess) {
+      jump_batch_destroy(b);
+      return JUMP_ERR_DEVICE;
+    }
+    in->on_device = false;
+  }
+  *out = b;
+  return JUMP_OK;
+}
+
+int jump_batch_step(jump_batch* b, int64_t nsteps, double* t_inout) {
+  if (!b) return JUMP_ERR_ARG;
+  if (!t_inout || nsteps < 0) return batch_err(b, JUMP_ERR_ARG, "bad step arguments");
+  HIP_TRY_B(hipSetDevice(b->device));
+  unsigned fl = 0;
+  for (auto& grp : b->groups) {  // every group's launches are queued before any wait
+    jump_ctx* in = grp.inner;
+    int r = ensure_device(in);
+    if (!r) r = run_steps(in, nsteps);
+    if (r) return batch_err(b, r, in->err);
+  }
+  for (auto& grp : b->groups) {
+    jump_ctx* in = grp.inner;
+    unsigned f = 0;
+    HIP_TRY_B(hipMemcpyAsync(&f, in->d_flags, sizeof(unsigned), hipMemcpyDeviceToHost, in->stream));
+    HIP_TRY_B(hipMemsetAsync(in->d_flags, 0, sizeof(unsigned), in->stream));
+    HIP_TRY_B(hipStreamSynchronize(in->stream));
+    if (nsteps > 0) in->grid_valid = true;
+    in->flags |= f;
+    fl |= f;
+  }
+  for (size_t m = 0; m < b->members.size(); m++)
+    for (int64_t s = 0; s < nsteps; s++) t_inout[m] += b->members[m]->prm.dt;
+  if (fl) return batch_err(b, JUMP_ERR_NUMERIC, "particle error flags raised during the step");
+  return JUMP_OK;
+}
+
+int jump_batch_sync(jump_batch* b) {
+  if (!b) return JUMP_ERR_ARG;
+  HIP_TRY_B(hipSetDevice(b->device));
+  for (auto& grp : b->groups) {
+    jump_ctx* in = grp.inner;
+    int r = download(in);
+    if (r) return batch_err(b, r, in->err);
+    for (size_t ii = 0; ii < grp.members.size(); ii++) {
+      jump_ctx* c = b->members[grp.members[ii]];
+      const int64_t o = grp.part_off[ii], n = c->np;
+      auto put = [&](std::vector<double>& d, const std::vector<double>& s2) {
+        std::copy(s2.begin() + o, s2.begin() + o + n, d.begin());
+      };
+      for (int k = 0; k < 2; k++) { put(c->hx[k], in->hx[k]); put(c->hv[k], in->hv[k]); }
+      put(c->hm, in->hm);
+      put(c->hV, in->hV);
+      put(c->hV0, in->hV0);
+      for (int k = 0; k < 4; k++) put(c->hF[k], in->hF[k]);
+      for (int k = 0; k < 3; k++) { put(c->heps[k], in->heps[k]); put(c->hsig[k], in->hsig[k]); put(c->hep[k], in->hep[k]); }
+      put(c->halpha, in->halpha);
+      put(c->halphadot, in->halphadot);
+      put(c->hvm, in->hvm);
+      c->on_device = false;  // the member's own device copy (if any) is stale now
+      c->grid_valid = false;
+    }
+  }
+  return JUMP_OK;
+}
+
+int jump_batch_energies(jump_batch* b, double* ke, double* se) {
+  if (!b || !ke || !se) return JUMP_ERR_ARG;
+  HIP_TRY_B(hipSetDevice(b->device));
+  for (auto& grp : b->groups) {
+    jump_ctx* in = grp.inner;
+    int r = ensure_device(in);
+    if (r) return batch_err(b, r, in->err);
+    for (size_t ii = 0; ii < grp.members.size(); ii++) {
+      const int m = grp.members[ii];
+      const int64_t o = grp.part_off[ii];
+      r = range_energies(in, o, o + b->members[m]->np, &ke[m], &se[m]);
+      if (r) return batch_err(b, r, in->err);
+    }
+  }
+  return JUMP_OK;
+}

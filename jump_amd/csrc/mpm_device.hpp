@@ -32,8 +32,11 @@ struct GridDesc {
 };
 
 // Per-solid constants (material, rigid prescription, GIMP half-widths).
+// inst: the batch instance the solid belongs to (0 outside batches);
+// rigid_rank: order of a rigid solid among the rigid solids of its instance.
 struct SolidDev {
-  int kind, rigid, fixed, pad;
+  int kind, rigid, fixed, inst;
+  int rigid_rank, pad;
   double lambda, mu, kappa, fy, k1;
   double A, B, C, n, epsdot0, cellsize, velo;
   double vx, vy;
@@ -87,6 +90,42 @@ struct GridArrays {
   double* f[3];
   const uint8_t* fixed;  // bit k set: direction k fixed
 };
+
+// A batch steps several independent 2D problems with the same launches.
+// Instance i has its own grid, whose nodes (and bins) occupy
+// [node_off[i], node_off[i+1]) of the concatenated node arrays, and its own
+// dt and USL tolerance.  Particles find their instance through their solid,
+// nodes through node_off.
+struct InstDev {
+  GridDesc G;
+  double dt, tol;
+};
+struct BatchDev {
+  const InstDev* inst;
+  const int* node_off;  // ninst + 1 entries
+  int ninst;
+  int NN;  // total nodes
+};
+__device__ __forceinline__ int inst_of_node(const BatchDev& Bt, int n) {
+  int lo = 0, hi = Bt.ninst - 1;  // largest i with node_off[i] <= n
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (Bt.node_off[mid] <= n) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+__device__ __forceinline__ GridArrays grid_view(GridArrays g, int off) {
+  g.m += off;
+  for (int k = 0; k < 3; k++) {
+    if (g.p0[k]) g.p0[k] += off;
+    if (g.p[k]) g.p[k] += off;
+    if (g.p2[k]) g.p2[k] += off;
+    if (g.f[k]) g.f[k] += off;
+  }
+  g.fixed += off;
+  return g;
+}
 
 // ------------------------------------------------------------------ helpers
 // Element i of an array whose base is wave-uniform (a kernel argument): the

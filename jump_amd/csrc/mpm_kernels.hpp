@@ -53,14 +53,17 @@ __global__ void k_fill_i32(int* __restrict__ a, int v, int n) {
 // for particles that take no part in P2G (2D rigid bodies, out of grid).
 // bxyz[i] packs the origin's 0-based coordinates, 21 bits per axis (all
 // ones for excluded particles).
-template <int D, int B>
-__global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+template <int D, int B, bool MULTI>
+__global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
                                                 const double* __restrict__ corners, int* __restrict__ key,
                                                 unsigned long long* __restrict__ bxyz, int* __restrict__ cnt,
-                                                unsigned* __restrict__ flags) {
+                                                unsigned* __restrict__ flags, BatchDev Bt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
+  const GridDesc& G = MULTI ? Bt.inst[s.inst].G : G0;
+  const int off = MULTI ? Bt.node_off[s.inst] : 0;
+  const int NN = MULTI ? Bt.NN : G0.nn;
   int k = -1;
   if (!(D == 2 && s.rigid)) {
     bool ok = true;
@@ -76,21 +79,23 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G, const S
       for (int a = 0; a < D; a++) ok = ok && first_node<B, D>(P.x[a][i], a, axis_of(G, a), s.lp[a], f[a]);
     }
     if (ok) {
-      k = (f[0] - 1) + G.nx * ((f[1] - 1) + (D == 3 ? G.ny * (f[2] - 1) : 0));
+      k = off + (f[0] - 1) + G.nx * ((f[1] - 1) + (D == 3 ? G.ny * (f[2] - 1) : 0));
       atomicAdd(&cnt[k], 1);
     } else {
       atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
     }
     if (ok) {
       key[i] = k;
+      // bin coordinates 21 bits per axis; a 2D batch keeps the instance in
+      // the third field, so a window of one instance never admits another's
       bxyz[i] = (unsigned long long)(f[0] - 1) | ((unsigned long long)(f[1] - 1) << 21) |
-                ((unsigned long long)(D == 3 ? f[2] - 1 : 0) << 42);
+                ((unsigned long long)(D == 3 ? f[2] - 1 : (MULTI ? s.inst : 0)) << 42);
       return;
     }
   }
   key[i] = -1;
   bxyz[i] = ~0ull;
-  atomicAdd(&cnt[G.nn], 1);
+  atomicAdd(&cnt[NN], 1);
 }
 
 // CPDI stencil extent (static corners): max bounding-box width in nodes.
@@ -335,12 +340,18 @@ constexpr int kP2GBlock = 128;
 //   k_node_flags: flag[n] = 1 when the W^d window of node n has a particle
 //                 (each window row of bins is one contiguous range of start)
 //   k_node_order: order[pos] = n, actives first, both parts ascending
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_node_flags(GridDesc G, const int* __restrict__ start, int W,
-                                                       int* __restrict__ flag) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n > G.nn) return;
-  if (n == G.nn) { flag[n] = 0; return; }
+template <int D, bool MULTI>
+__global__ void __launch_bounds__(kBlock) k_node_flags(GridDesc G0, const int* __restrict__ start0, int W,
+                                                       int* __restrict__ flag, BatchDev Bt) {
+  const int n0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int NN = MULTI ? Bt.NN : G0.nn;
+  if (n0 > NN) return;
+  if (n0 == NN) { flag[n0] = 0; return; }
+  const int inst = MULTI ? inst_of_node(Bt, n0) : 0;
+  const GridDesc& G = MULTI ? Bt.inst[inst].G : G0;
+  const int off = MULTI ? Bt.node_off[inst] : 0;
+  const int* start = start0 + off;
+  const int n = n0 - off;
   const int ni = n % G.nx, nj = (n / G.nx) % G.ny, nk = (D == 3) ? n / G.nxy : 0;
   const int lo = max(ni - W + 1, 0);
   int any = 0;
@@ -349,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_node_flags(GridDesc G, const int* __
       const int row = G.nx * (bj + G.ny * bk);
       any |= ldu(start, unsigned(row + ni + 1)) - ldu(start, unsigned(row + lo));
     }
-  flag[n] = any != 0;
+  flag[n0] = any != 0;
 }
 __global__ void __launch_bounds__(kBlock) k_node_order(const int* __restrict__ pos, int nn, int* __restrict__ order) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -359,13 +370,13 @@ __global__ void __launch_bounds__(kBlock) k_node_order(const int* __restrict__ p
   order[(pos[n + 1] - p) ? p : nact + n - p] = n;
 }
 
-template <int D, int B, int PASS, int ORDER, bool COMPACT>
-__global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G, const SolidDev* __restrict__ S,
-                                                   const double* __restrict__ corners, const int* __restrict__ start,
+template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI>
+__global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
+                                                   const double* __restrict__ corners, const int* __restrict__ start0,
                                                    const unsigned long long* __restrict__ sorted,
                                                    const unsigned long long* __restrict__ bxyz,
                                                    const int* __restrict__ order, const int* __restrict__ nodepos,
-                                                   GridArrays g, int Wrt, double dt) {
+                                                   GridArrays g0, int Wrt, double dt0, BatchDev Bt) {
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
   // Reference order: per window bin, the position in `sorted` of its next
@@ -374,17 +385,26 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G, cons
   __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
   const int t = threadIdx.x;
   const int i = blockIdx.x * kP2GBlock + t;
-  if (i >= G.nn) return;
-  const int n = COMPACT ? ldu(order, unsigned(i)) : i;
-  if (COMPACT && i >= ldu(nodepos, unsigned(G.nn))) {  // inactive node: no particle reaches it
+  const int NN = MULTI ? Bt.NN : G0.nn;
+  if (i >= NN) return;
+  const int ng = COMPACT ? ldu(order, unsigned(i)) : i;  // node in the (concatenated) node arrays
+  if (COMPACT && i >= ldu(nodepos, unsigned(NN))) {  // inactive node: no particle reaches it
     if (PASS == 0) {
-      g.m[n] = 0.0;
-      for (int k = 0; k < D; k++) { g.p0[k][n] = 0.0; g.p[k][n] = 0.0; g.f[k][n] = 0.0; }
+      g0.m[ng] = 0.0;
+      for (int k = 0; k < D; k++) { g0.p0[k][ng] = 0.0; g0.p[k][ng] = 0.0; g0.f[k][ng] = 0.0; }
     } else {
-      for (int k = 0; k < D; k++) g.p2[k][n] = 0.0;
+      for (int k = 0; k < D; k++) g0.p2[k][ng] = 0.0;
     }
     return;
   }
+  // the node's own grid: the single grid, or its batch instance's
+  const int inst = MULTI ? inst_of_node(Bt, ng) : 0;
+  const GridDesc& G = MULTI ? Bt.inst[inst].G : G0;
+  const int off = MULTI ? Bt.node_off[inst] : 0;
+  const double dt = MULTI ? Bt.inst[inst].dt : dt0;
+  const int* start = start0 + off;
+  const GridArrays g = MULTI ? grid_view(g0, off) : g0;
+  const int n = ng - off;
   const int ni = n % G.nx;
   const int nj = (n / G.nx) % G.ny;
   const int nk = (D == 3) ? n / G.nxy : 0;
@@ -429,7 +449,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G, cons
       while (true) {
         // p is the head of window bin b: consume it and advance the bin
         const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
-        const unsigned bin = unsigned((wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2)));
+        const unsigned bin = unsigned(off + (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2)));
         const int c = s_cur[b][t] + 1;
         const unsigned long long e = ldu(sorted, unsigned(c));  // sorted[np] is a sentinel
         s_cur[b][t] = c;
@@ -443,7 +463,10 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G, cons
         const unsigned ux = unsigned(int(cc & 0x1fffffu) - wx0);
         const unsigned uy = unsigned(int((cc >> 21) & 0x1fffffu) - wy0);
         const unsigned uz = unsigned(int((cc >> 42) & 0x1fffffu) - wz0);
-        if (!(ux < unsigned(W) && uy < unsigned(W) && (D == 2 || uz < unsigned(W))) || cc == ~0ull) break;
+        // 2D: the third field is the batch instance and must be this node's
+        if (!(ux < unsigned(W) && uy < unsigned(W) && (D == 2 ? uz == unsigned(inst) : uz < unsigned(W))) ||
+            cc == ~0ull)
+          break;
         b = int(ux) + MW * (int(uy) + (D == 3 ? MW * int(uz) : 0));
         p = q;
       }
@@ -467,12 +490,19 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G, cons
 
 // ------------------------------------------------------------------ rigid (2D)
 // SolveExplicitSolidMechanics2D.jl:448-479.  All particles of one rigid solid
-// write the same function of the node mass, so concurrent writes agree;
-// solids are launched one after another in solid order.
-__global__ void __launch_bounds__(kBlock) k_rigid_override(Particles P, GridDesc G, SolidDev s, int begin, int end,
-                                                           GridArrays g, unsigned* __restrict__ flags) {
-  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= end) return;
+// write the same function of the node mass, so concurrent writes agree.  The
+// reference overrides solid after solid; a launch handles the rigid solids of
+// one rank (the r-th rigid solid of every instance), ranks in order.
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock) k_rigid_override(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
+                                                           int rank, GridArrays g0, unsigned* __restrict__ flags,
+                                                           BatchDev Bt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  if (!s.rigid || s.rigid_rank != rank) return;
+  const GridDesc& G = MULTI ? Bt.inst[s.inst].G : G0;
+  const GridArrays g = MULTI ? grid_view(g0, Bt.node_off[s.inst]) : g0;
   int f[2];
   bool ok = first_node<JUMP_BASIS_LINEAR, 2>(P.x[0][i], 0, axis_of(G, 0), 0.0, f[0]) &&
             first_node<JUMP_BASIS_LINEAR, 2>(P.x[1][i], 1, axis_of(G, 1), 0.0, f[1]);
@@ -496,9 +526,14 @@ __global__ void __launch_bounds__(kBlock) k_rigid_override(Particles P, GridDesc
 }
 
 // SolveExplicitSolidMechanics2D.jl:539-551
-__global__ void __launch_bounds__(kBlock) k_rigid_move(Particles P, SolidDev s, int begin, int end, double dt) {
-  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= end) return;
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock) k_rigid_move(Particles P, const SolidDev* __restrict__ S, double dt0,
+                                                       BatchDev Bt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  if (!s.rigid) return;
+  const double dt = MULTI ? Bt.inst[s.inst].dt : dt0;
   P.x[0][i] = P.x[0][i] + dt * s.vx;
   P.x[1][i] = P.x[1][i] + dt * s.vy;
 }
@@ -600,14 +635,18 @@ __device__ __forceinline__ bool finite3(double a, double b, double c) { return i
 
 // ------------------------------------------------------------------ G2P, USL 2D
 // SolveExplicitSolidMechanics2D.jl:485-534
-template <int B>
-__global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G, const SolidDev* __restrict__ S,
-                                                      const double* __restrict__ corners, GridArrays g, double tol,
-                                                      double dt, unsigned* __restrict__ flags) {
+template <int B, bool MULTI>
+__global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
+                                                      const double* __restrict__ corners, GridArrays g0, double tol0,
+                                                      double dt0, unsigned* __restrict__ flags, BatchDev Bt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
   if (s.rigid) return;
+  const GridDesc& G = MULTI ? Bt.inst[s.inst].G : G0;
+  const GridArrays g = MULTI ? grid_view(g0, Bt.node_off[s.inst]) : g0;
+  const double tol = MULTI ? Bt.inst[s.inst].tol : tol0;
+  const double dt = MULTI ? Bt.inst[s.inst].dt : dt0;
   double v0 = P.v[0][i], v1 = P.v[1][i];
   double x0 = P.x[0][i], x1 = P.x[1][i];
   double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
@@ -786,11 +825,13 @@ __global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, c
 // Fix.jl:248-270.  Per-block sums in a fixed tree order, then one block sums
 // the partials in block order: deterministic.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_energy_partial(Particles P, double* __restrict__ part) {
+// Partial sums over blocks of kBlock particles of [begin, end), the blocks
+// aligned at begin (a batch instance reduces exactly as a context of its own).
+__global__ void __launch_bounds__(kBlock) k_energy_partial(Particles P, int begin, int end, double* __restrict__ part) {
   __shared__ double ske[kBlock], sse[kBlock];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
   double ke = 0.0, se = 0.0;
-  if (i < P.n) {
+  if (i < end) {
     double vv = P.v[0][i] * P.v[0][i];
     for (int k = 1; k < D; k++) vv = vv + P.v[k][i] * P.v[k][i];
     ke = 0.5 * P.m[i] * vv;

@@ -203,3 +203,52 @@ class Engine:
         keys = np.empty(n, dtype=np.int32)
         self._check(self.lib.jump_get_bins(self.h, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return keys
+
+
+class Batch:
+    """Many 2D USL Engines stepped together (jump_batch_*).  The engines keep
+    their particle counts; after step() call sync() before reading them."""
+
+    def __init__(self, engines, device=0):
+        self.lib = engines[0].lib
+        self.engines = list(engines)
+        arr = (ctypes.c_void_p * len(engines))(*[e.h.value for e in engines])
+        h = ctypes.c_void_p()
+        rc = self.lib.jump_batch_create(arr, len(engines), device, ctypes.byref(h))
+        if rc != 0:
+            raise JumpError(rc, "batch creation failed")
+        self.h = h
+        self.t = np.zeros(len(engines))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.jump_batch_last_error(self.h)
+            raise JumpError(rc, msg.decode() if msg else "")
+        return rc
+
+    def step(self, nsteps):
+        t = np.ascontiguousarray(self.t)
+        rc = self.lib.jump_batch_step(self.h, int(nsteps), _ffi.dptr(t))
+        self.t = t
+        if rc not in (0, _ffi.JUMP_ERR_NUMERIC):
+            self._check(rc)
+        return rc
+
+    def sync(self):
+        self._check(self.lib.jump_batch_sync(self.h))
+
+    def energies(self):
+        ke, se = np.zeros(len(self.engines)), np.zeros(len(self.engines))
+        self._check(self.lib.jump_batch_energies(self.h, _ffi.dptr(ke), _ffi.dptr(se)))
+        return ke, se
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.jump_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

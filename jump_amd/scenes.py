@@ -11,6 +11,8 @@ into any Engine (the HIP library, or the oracle in tests).
   rings2d        2D rings between rigid platens (Main_TwoRings.jl /
                  Main_LateralCompression_Cellular.jl), any 2D basis
   cpdi_tubes     2D lateral compression of a tube array with CPDI-Q4 domains
+  two_rings_instance  one seeded instance of the batched two-rings impact
+                 (Main_TwoRings.jl:40-130 scaled to a drawn grid and size)
 """
 import math
 
@@ -140,6 +142,43 @@ def rings2d(cells=40, basis=_ffi.BASIS_LINEAR, material="neohookean", platen=Tru
                     body=(0.0, -9.81))
     fixed = _fixed_faces((cells + 1, cells + 1), ["bottom"], {"bottom": (0, 1)})
     return Scene("rings2d", p, solids, fixed=fixed, steps=steps)
+
+
+def two_rings_instance(seed, steps=1000):
+    """One instance of the batched workload: the two-rings impact of
+    Main_TwoRings.jl:40-130 (domain 200 x 100 mm, rings r 30..40 mm at
+    x = 50 and 150, Neo-Hookean, rho 1010e-9, USL(1e-18), dt 0.6 dx / c)
+    with drawn sizes.  rng = default_rng(seed):
+      grid side S ~ log-uniform [64, 512] nodes along x (odd, ny = (S+1)/2,
+        so dy = dx); particle count ~ log-uniform [5k, 80k], through the
+        particle spacing dx/k with k >= 1 per cell;
+      E ~ U[0.5, 5] MPa (nu 0.3); impact speed ~ U[10, 100] (mm/s, the
+      reference's units); basis linear or GIMP with probability 1/2."""
+    rng = np.random.default_rng(seed)
+    side = int(round(math.exp(rng.uniform(math.log(64), math.log(512))))) | 1
+    np_target = math.exp(rng.uniform(math.log(5000.0), math.log(80000.0)))
+    E = rng.uniform(0.5, 5.0)
+    vel = rng.uniform(10.0, 100.0)
+    basis = _ffi.BASIS_GIMP if rng.uniform() < 0.5 else _ffi.BASIS_LINEAR
+    l, w, rin, rout, rho, nu = 200.0, 100.0, 30.0, 40.0, 1010e-9, 0.3
+    nx, ny = side, (side + 1) // 2
+    dx = l / (nx - 1)
+    area = 2.0 * math.pi * (rout * rout - rin * rin)
+    k = max(1, int(round(math.sqrt(np_target / area) * dx)))
+    fOffset = dx / k
+    lp = 0.5 * fOffset if basis == _ffi.BASIS_GIMP else None
+    mat = lambda: neohookean(E, nu, rho)
+    solids = []
+    for cx, vx in ((0.25 * l, vel), (0.75 * l, -vel)):
+        pts = buildParticleForRing([cx, 0.5 * w], rin, rout, fOffset)
+        solids.append(_solid(mat(), pts, (vx, 0.0), fOffset * fOffset, rho, lp=lp))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    c = math.sqrt((lam + 2 * mu) / rho)
+    dt = 0.6 * dx / c
+    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (l, w), (nx, ny), tolerance=1e-18)
+    return Scene("two_rings_instance", p, solids, steps=steps,
+                 meta={"seed": seed, "side": side, "E": E, "vel": vel, "ppc": k})
 
 
 def _ring_q4(cx, cy, ri, ro, nr, nt, jitter, rng):

@@ -26,7 +26,9 @@ def test_header_declares_the_boundary():
     names = _declared()
     for must in ("jump_ctx_create", "jump_ctx_destroy", "jump_add_solid", "jump_set_fixed", "jump_step",
                  "jump_get_solid", "jump_get_solid_state", "jump_get_grid", "jump_energies", "jump_last_error",
-                 "jump_flags", "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load"):
+                 "jump_flags", "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
+                 "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies",
+                 "jump_batch_destroy"):
         assert must in names
 
 
@@ -41,8 +43,9 @@ def test_library_loads_and_exports_every_declared_symbol():
 def test_oracle_exports_the_same_entry_points(oracle_lib):
     for name in _declared():
         if name in ("jump_get_bins", "jump_abi_version", "jump_profile", "jump_profile_read",
-                    "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load"):
-            continue  # diagnostics and device-state transfer of the GPU library only
+                    "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load") or \
+                name.startswith("jump_batch_"):
+            continue  # diagnostics, device-state transfer and batching of the GPU library only
         assert hasattr(oracle_lib, name.replace("jump_", "orc_")), name
 
 

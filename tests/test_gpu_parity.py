@@ -326,3 +326,36 @@ def test_checkpoint_restart_is_bitwise(hip_lib, dim):
     other.load(c)
     with pytest.raises(JumpError):
         c.restore(blob)
+
+
+def test_batch_members_step_as_alone(hip_lib):
+    """A batch of 2D instances of different grids, bases (linear, GIMP,
+    quadratic B-spline), materials and dt, one with a rigid platen and a
+    Dirichlet face: after stepping together, every member equals the same
+    problem stepped alone, bit for bit, energies included."""
+    from jump_amd.engine import Batch
+    scs = [scenes.two_rings_instance(1000 + i, steps=15) for i in (1, 2, 4)]
+    scs.append(scenes.rings2d(cells=24, basis=_ffi.BASIS_LINEAR, material="j2", steps=15))
+    scs.append(scenes.block2d(cells=12, basis=_ffi.BASIS_QUAD_BSPLINE, material="elastic", steps=15))
+    alone, members = [], []
+    for sc in scs:
+        a = Engine(sc.params)
+        sc.load(a)
+        a.step(15, 0.0)
+        alone.append(a)
+        m = Engine(sc.params)
+        sc.load(m)
+        members.append(m)
+    b = Batch(members)
+    b.step(10)
+    b.step(5)
+    ke, se = b.energies()
+    b.sync()
+    for i, (a, m, sc) in enumerate(zip(alone, members, scs)):
+        assert b.t[i] == sum([sc.params.dt] * 15) or abs(b.t[i] - 15 * sc.params.dt) < 1e-12 * b.t[i]
+        for s in range(len(sc.solids)):
+            sa, sm = a.get_solid(s), m.get_solid(s)
+            for k in sa:
+                assert np.array_equal(sa[k], sm[k]), (i, s, k)
+        assert (ke[i], se[i]) == a.energies(), i
+    b.close()
