@@ -13,6 +13,7 @@ its mean duration, from HIP events on the library's stream over the timed
 region), and the CPU restatement timed on a bounded sample on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--order reference|binned]
+                    [--batch-instances N] [--batch-steps K]
     python bench.py --copy-probe MIB   # device-to-device copy of MIB MiB, to
                                        # calibrate HBM byte counters
 """
@@ -88,6 +89,58 @@ def copy_probe(mib, reps=5):
     print(json.dumps({"copy_probe_MiB": mib, "launches": reps}), flush=True)
 
 
+def batched_leg(n_inst, steps, warmup):
+    """The batched workload: n_inst seeded two-rings instances (2D USL,
+    linear or GIMP, grids 64..512 nodes wide, 5k..80k particles each).
+    plain: every instance in a context of its own, stepped one after the
+    other (one solve per problem, as the reference runs them); batched: all
+    of them in one jump_batch.  Same steps, inputs resident in HBM."""
+    import torch
+
+    from jump_amd import scenes
+    from jump_amd.engine import Batch, Engine
+
+    scs = [scenes.two_rings_instance(1000 + i, steps=steps) for i in range(n_inst)]
+    npart = sum(sc.n_particles for sc in scs)
+    plain = []
+    for sc in scs:
+        e = Engine(sc.params, device=0)
+        sc.load(e)
+        e.step(warmup, 0.0)
+        plain.append(e)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e in plain:
+        e.step(steps, 0.0)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    flags = 0
+    for e in plain:
+        flags |= e.flags()
+        e.close()
+    members = []
+    for sc in scs:
+        e = Engine(sc.params, device=0)
+        sc.load(e)
+        members.append(e)
+    b = Batch(members)
+    b.step(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    for e in members:
+        flags |= e.flags()
+        e.close()
+    return {"workload": f"{n_inst} two-rings instances (seeds 1000..{1000 + n_inst - 1}), 2D USL, linear/GIMP",
+            "particles": npart, "steps": steps,
+            "plain_sequential": {"value": npart * steps / t_plain, "ms_per_step": t_plain * 1e3 / steps},
+            "batched": {"value": npart * steps / t_batch, "ms_per_step": t_batch * 1e3 / steps},
+            "speedup": t_plain / t_batch, "unit": "particle-steps/s", "step_flags": flags}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +151,8 @@ def main():
     ap.add_argument("--order", choices=["reference", "binned"], default="reference",
                     help="P2G summation order: the reference's particle order (bit-identical) or bin order")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    ap.add_argument("--batch-instances", type=int, default=256, help="0 skips the batched leg")
+    ap.add_argument("--batch-steps", type=int, default=50)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -186,6 +241,8 @@ def main():
     }
     sys.stdout.flush()
     eng.close()
+    if args.batch_instances > 0:
+        out["batched"] = batched_leg(args.batch_instances, args.batch_steps, 2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

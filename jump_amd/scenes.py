@@ -147,11 +147,15 @@ def rings2d(cells=40, basis=_ffi.BASIS_LINEAR, material="neohookean", platen=Tru
 def two_rings_instance(seed, steps=1000):
     """One instance of the batched workload: the two-rings impact of
     Main_TwoRings.jl:40-130 (domain 200 x 100 mm, rings r 30..40 mm at
-    x = 50 and 150, Neo-Hookean, rho 1010e-9, USL(1e-18), dt 0.6 dx / c)
+    x = 50 and 150 (nearest grid nodes), Neo-Hookean, rho 1010e-9,
+    USL(1e-18); dt 0.3 dx / c where the reference takes 0.6, at which
+    hat-function instances go unstable at the free surface within 50 steps)
     with drawn sizes.  rng = default_rng(seed):
       grid side S ~ log-uniform [64, 512] nodes along x (odd, ny = (S+1)/2,
         so dy = dx); particle count ~ log-uniform [5k, 80k], through the
-        particle spacing dx/k with k >= 1 per cell;
+        particle spacing dx/k, k clamped to 1..4 particles per cell and
+        direction (denser packings are unstable at this dt, so small grids
+        get fewer particles than drawn);
       E ~ U[0.5, 5] MPa (nu 0.3); impact speed ~ U[10, 100] (mm/s, the
       reference's units); basis linear or GIMP with probability 1/2."""
     rng = np.random.default_rng(seed)
@@ -164,18 +168,24 @@ def two_rings_instance(seed, steps=1000):
     nx, ny = side, (side + 1) // 2
     dx = l / (nx - 1)
     area = 2.0 * math.pi * (rout * rout - rin * rin)
-    k = max(1, int(round(math.sqrt(np_target / area) * dx)))
+    k = min(4, max(1, int(round(math.sqrt(np_target / area) * dx))))
     fOffset = dx / k
     lp = 0.5 * fOffset if basis == _ffi.BASIS_GIMP else None
     mat = lambda: neohookean(E, nu, rho)
+    # Ring centres on grid nodes: the builder offsets particles by
+    # half-integer multiples of dx/k from the centre, so none lies on a grid
+    # line, where the hat gradient's sign(0) = 0 (Basis.jl:219-251) leaves
+    # sum(grad N) != 0 and strains a translating body.
+    dy = w / (ny - 1)
+    cy = round(0.5 * w / dy) * dy
     solids = []
-    for cx, vx in ((0.25 * l, vel), (0.75 * l, -vel)):
-        pts = buildParticleForRing([cx, 0.5 * w], rin, rout, fOffset)
+    for cx, vx in ((round(0.25 * l / dx) * dx, vel), (round(0.75 * l / dx) * dx, -vel)):
+        pts = buildParticleForRing([cx, cy], rin, rout, fOffset)
         solids.append(_solid(mat(), pts, (vx, 0.0), fOffset * fOffset, rho, lp=lp))
     lam = E * nu / ((1 + nu) * (1 - 2 * nu))
     mu = E / (2 * (1 + nu))
     c = math.sqrt((lam + 2 * mu) / rho)
-    dt = 0.6 * dx / c
+    dt = 0.3 * dx / c
     p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (l, w), (nx, ny), tolerance=1e-18)
     return Scene("two_rings_instance", p, solids, steps=steps,
                  meta={"seed": seed, "side": side, "E": E, "vel": vel, "ppc": k})
