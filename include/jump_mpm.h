@@ -227,6 +227,22 @@ int jump_batch_energies(jump_batch* batch, double* ke, double* se);
 int jump_batch_destroy(jump_batch* batch);
 const char* jump_batch_last_error(const jump_batch* batch);
 
+/* Reverse mode of the 2D USL step (the reference has no derivatives; this is
+ * the pullback a ChainRulesCore rrule around solve_explicit_dynamics_2D,
+ * SolveExplicitSolidMechanics2D.jl:335, would call).  Covers the linear and
+ * GIMP bases and Hooke / Neo-Hookean solids (else JUMP_ERR_UNSUPPORTED).
+ * jump_adjoint_forward runs nsteps steps like jump_step and records the
+ * particle state every K steps.  jump_adjoint_backward takes the adjoint of
+ * the final positions (pos_bar, AoS n x 2, required) and velocities
+ * (vel_bar, may be NULL) and writes, per solid, dL/dE, dL/dnu and dL/dv0
+ * (the derivative with respect to a uniform change of that solid's initial
+ * velocity, 2 per solid); outputs may be NULL.  Each step of the sweep
+ * recomputes its states from the nearest snapshot (checkpoint and
+ * recompute); the context is left in the recorded final state. */
+int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, double* t_inout);
+int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* vel_bar, double* dE,
+                          double* dnu, double* dvel0);
+
 /* Diagnostics (no counterpart in the reference, which recomputes the support
  * inside every shape-function call, Basis.jl:63-78,390-425).
  * jump_get_bins: runs the binning of the current particle positions and

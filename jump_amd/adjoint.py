@@ -1,0 +1,37 @@
+"""Reverse mode of the 2D USL step through the C ABI (jump_adjoint_*).
+
+`rrule(engine, nsteps, checkpoint_every)` is the shape of a ChainRulesCore
+rrule around solve_explicit_dynamics_2D (SolveExplicitSolidMechanics2D.jl:335):
+it runs the steps and returns the final particle positions and velocities
+with a pullback that maps their adjoints to gradients with respect to each
+solid's E, nu and initial velocity.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from .engine import JumpError
+
+
+def rrule(engine, nsteps, checkpoint_every=20, t=0.0):
+    lib = engine.lib
+    tt = ctypes.c_double(t)
+    rc = lib.jump_adjoint_forward(engine.h, int(nsteps), int(checkpoint_every), ctypes.byref(tt))
+    if rc not in (0, _ffi.JUMP_ERR_NUMERIC):
+        engine._check(rc)
+    pos = np.concatenate([engine.get_solid(s)["pos"] for s in range(len(engine.counts))])
+    vel = np.concatenate([engine.get_solid(s)["vel"] for s in range(len(engine.counts))])
+
+    def pullback(pos_bar, vel_bar=None):
+        ns = len(engine.counts)
+        pb = np.ascontiguousarray(pos_bar, dtype=np.float64).reshape(-1, 2)
+        vb = None if vel_bar is None else np.ascontiguousarray(vel_bar, dtype=np.float64).reshape(-1, 2)
+        dE, dnu, dv0 = np.zeros(ns), np.zeros(ns), np.zeros((ns, 2))
+        rc2 = lib.jump_adjoint_backward(engine.h, _ffi.dptr(pb), _ffi.dptr(vb), _ffi.dptr(dE), _ffi.dptr(dnu),
+                                        _ffi.dptr(dv0))
+        if rc2 != 0:
+            engine._check(rc2)
+        return {"E": dE, "nu": dnu, "v0": dv0}
+
+    return (pos, vel, tt.value, rc), pullback

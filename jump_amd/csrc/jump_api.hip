@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "mpm_kernels.hpp"
+#include "mpm_adjoint.hpp"
 
 using namespace jh;
 
@@ -78,6 +78,20 @@ struct jump_ctx {
   int* d_node_off = nullptr;
   int max_rigid_rank = -1;  // highest rigid_rank of any solid (-1: none)
   BatchDev bt() const { return BatchDev{d_inst, d_node_off, ninst, NN}; }
+
+  // reverse mode (jump_adjoint_*): state snapshots of the recorded run and
+  // the adjoint arrays
+  struct Adjoint {
+    int64_t nsteps = 0;
+    int K = 0;
+    bool ready = false;
+    std::vector<void*> allocs;
+    std::vector<double*> ckpt;    // state at steps 0, K, 2K, ...
+    std::vector<double*> segbuf;  // the states of one segment, recomputed
+    double* final_state = nullptr;
+    AdjParticles ap{};
+    AdjNodes an{};
+  } adj;
   uint32_t flags = 0;
 
   // per-phase timing with HIP events on the context stream (jump_profile)
@@ -561,6 +575,7 @@ int jump_ctx_destroy(jump_ctx* ctx) {
   if (!ctx) return JUMP_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_list(ctx->adj.allocs);
   free_list(ctx->part_allocs);
   free_list(ctx->grid_allocs);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -1185,5 +1200,201 @@ int jump_batch_energies(jump_batch* b, double* ke, double* se) {
       if (r) return batch_err(b, r, in->err);
     }
   }
+  return JUMP_OK;
+}
+
+// ------------------------------------------------------------------ reverse mode
+// The 2D USL step differentiated by hand (mpm_adjoint.hpp).  The recorded
+// run keeps the particle state every K steps; the backward sweep rebuilds
+// each segment's states from its snapshot, then walks the segment's steps
+// backwards, rebuilding each step's grid.  Memory: (nsteps/K + K + 1)
+// snapshots of 136 bytes per particle.
+static size_t snap_doubles(const jump_ctx* ctx) { return size_t(ctx->np) * (2 * PairLayout<2>::NP + 7); }
+
+static int snap_copy(jump_ctx* ctx, double* buf, bool save) {
+  const size_t n = size_t(ctx->np);
+  double* q = buf;
+  auto cp = [&](void* dev, size_t bytes) {
+    hipError_t e = save ? hipMemcpyAsync(q, dev, bytes, hipMemcpyDeviceToDevice, ctx->stream)
+                        : hipMemcpyAsync(dev, q, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    q += bytes / sizeof(double);
+    return e;
+  };
+  for (int j = 0; j < PairLayout<2>::NP; j++)
+    if (cp(ctx->P.pr[j], n * sizeof(double2)) != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "snapshot copy");
+  for (int k = 0; k < 4; k++)
+    if (cp(ctx->P.F[k], n * sizeof(double)) != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "snapshot copy");
+  for (int k = 0; k < 3; k++)
+    if (cp(ctx->P.eps[k], n * sizeof(double)) != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "snapshot copy");
+  return JUMP_OK;
+}
+
+static int adjoint_supported(jump_ctx* ctx) {
+  if (ctx->D != 2 || ctx->prm.algorithm != JUMP_ALG_USL || ctx->ninst != 1)
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the 2D USL step");
+  if (ctx->prm.basis != JUMP_BASIS_LINEAR && ctx->prm.basis != JUMP_BASIS_GIMP)
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the linear and GIMP bases");
+  for (auto& s : ctx->solids)
+    if (s.dev.kind != JUMP_MAT_ELASTIC && s.dev.kind != JUMP_MAT_NEOHOOKEAN)
+      return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers Hooke and Neo-Hookean solids");
+  return JUMP_OK;
+}
+
+template <int B> static void adj_grid_of_state(jump_ctx* ctx) {
+  bin_and_sort<2, B, false>(ctx);
+  if (ctx->prm.order == JUMP_ORDER_REFERENCE)
+    LAUNCH_P2G((k_p2g<2, B, 0, JUMP_ORDER_REFERENCE, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners,
+               ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w,
+               ctx->prm.dt, ctx->bt());
+  else
+    LAUNCH_P2G((k_p2g<2, B, 0, JUMP_ORDER_BINNED, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners,
+               ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w,
+               ctx->prm.dt, ctx->bt());
+}
+
+template <int B> static void adj_step_back(jump_ctx* ctx) {
+  const int np = int(ctx->np), nn = ctx->NN;
+  const double dt = ctx->prm.dt;
+  adj_grid_of_state<B>(ctx);
+  AdjNodes& an = ctx->adj.an;
+  LAUNCH(k_fill_f64, nn, an.mb, 0.0, nn);
+  for (int k = 0; k < 2; k++) {
+    LAUNCH(k_fill_f64, nn, an.Pb[k], 0.0, nn);
+    LAUNCH(k_fill_f64, nn, an.P0b[k], 0.0, nn);
+  }
+  LAUNCH((k_adj_g2p<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, ctx->prm.tolerance, dt, ctx->adj.ap, an);
+  LAUNCH(k_adj_grid, nn, nn, ctx->d_fixed, dt, an);
+  LAUNCH((k_adj_p2g<B>), np, ctx->P, ctx->G, ctx->d_solids, an, ctx->adj.ap);
+}
+
+int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K, double* t_inout) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (!t_inout || nsteps <= 0 || K <= 0) return set_err(ctx, JUMP_ERR_ARG, "bad adjoint run arguments");
+  int r = adjoint_supported(ctx);
+  if (r) return r;
+  HIP_TRY(hipSetDevice(ctx->device));
+  r = ensure_device(ctx);
+  if (r) return r;
+  auto& A = ctx->adj;
+  free_list(A.allocs);
+  A = jump_ctx::Adjoint{};
+  const size_t sd = snap_doubles(ctx);
+  const int64_t nck = (nsteps + K - 1) / K;
+  auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.allocs, p, count); };
+  A.ckpt.resize(size_t(nck));
+  for (auto& c : A.ckpt)
+    if ((r = alloc(&c, sd))) return r;
+  A.segbuf.resize(size_t(std::min<int64_t>(K, nsteps)));
+  for (auto& c : A.segbuf)
+    if ((r = alloc(&c, sd))) return r;
+  if ((r = alloc(&A.final_state, sd))) return r;
+  unsigned fl = 0;
+  for (int64_t s = 0; s < nsteps; s++) {
+    if (s % K == 0 && (r = snap_copy(ctx, A.ckpt[size_t(s / K)], true))) return r;
+    if ((r = run_steps(ctx, 1))) return r;
+  }
+  if ((r = snap_copy(ctx, A.final_state, true))) return r;
+  HIP_TRY(hipMemcpyAsync(&fl, ctx->d_flags, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int64_t s = 0; s < nsteps; s++) *t_inout += ctx->prm.dt;
+  ctx->grid_valid = true;
+  A.nsteps = nsteps;
+  A.K = K;
+  A.ready = true;
+  ctx->flags |= fl;
+  if (fl) return set_err(ctx, JUMP_ERR_NUMERIC, "particle error flags raised during the recorded run");
+  return JUMP_OK;
+}
+
+int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* vel_bar, double* dE, double* dnu,
+                          double* dvel0) {
+  if (!ctx) return JUMP_ERR_ARG;
+  auto& A = ctx->adj;
+  if (!A.ready) return set_err(ctx, JUMP_ERR_STATE, "no recorded run (call jump_adjoint_forward first)");
+  if (!pos_bar) return set_err(ctx, JUMP_ERR_ARG, "pos_bar is required");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t n = size_t(ctx->np), nn = size_t(ctx->NN);
+  int r = JUMP_OK;
+  AdjParticles& ap = A.ap;
+  AdjNodes& an = A.an;
+  if (!ap.Vb) {
+    auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.allocs, p, count); };
+    for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.xb[k], n);
+    for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.vb[k], n);
+    for (int k = 0; k < 4 && !r; k++) r = alloc(&ap.Fb[k], n);
+    for (int k = 0; k < 3 && !r; k++) r = alloc(&ap.eb[k], n);
+    for (int k = 0; k < 3 && !r; k++) r = alloc(&ap.sb[k], n);
+    if (!r) r = alloc(&ap.Vb, n);
+    if (!r) r = alloc(&ap.lam, n);
+    if (!r) r = alloc(&ap.mu, n);
+    if (!r) r = alloc(&an.mb, nn);
+    for (int k = 0; k < 2 && !r; k++) r = alloc(&an.Pb[k], nn);
+    for (int k = 0; k < 2 && !r; k++) r = alloc(&an.P0b[k], nn);
+    if (r) return r;
+  }
+  // adjoints of the final state: positions and velocities as given, the rest 0
+  {
+    std::vector<double> h(n);
+    for (int k = 0; k < 2; k++) {
+      for (size_t i = 0; i < n; i++) h[i] = pos_bar[2 * i + k];
+      HIP_TRY(hipMemcpyAsync(ap.xb[k], h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+      for (size_t i = 0; i < n; i++) h[i] = vel_bar ? vel_bar[2 * i + k] : 0.0;
+      HIP_TRY(hipMemcpyAsync(ap.vb[k], h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    double* zs[12] = {ap.Fb[0], ap.Fb[1], ap.Fb[2], ap.Fb[3], ap.eb[0], ap.eb[1], ap.eb[2],
+                      ap.sb[0], ap.sb[1], ap.sb[2], ap.Vb, ap.lam};
+    for (double* z : zs) LAUNCH(k_fill_f64, n, z, 0.0, int(n));
+    LAUNCH(k_fill_f64, n, ap.mu, 0.0, int(n));
+  }
+  const int64_t nseg = (A.nsteps + A.K - 1) / A.K;
+  for (int64_t seg = nseg - 1; seg >= 0 && !r; seg--) {
+    const int64_t s0 = seg * A.K, s1 = std::min<int64_t>(s0 + A.K, A.nsteps);
+    if ((r = snap_copy(ctx, A.ckpt[size_t(seg)], false))) break;
+    for (int64_t j = s0; j < s1 && !r; j++) {
+      if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
+      if (j + 1 < s1) r = run_steps(ctx, 1);
+    }
+    for (int64_t j = s1 - 1; j >= s0 && !r; j--) {
+      if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], false))) break;
+      if (ctx->prm.basis == JUMP_BASIS_LINEAR) adj_step_back<JUMP_BASIS_LINEAR>(ctx);
+      else adj_step_back<JUMP_BASIS_GIMP>(ctx);
+    }
+  }
+  if (r) return r;
+  HIP_TRY(hipGetLastError());
+  // leave the recorded final state in place
+  if ((r = snap_copy(ctx, A.final_state, false))) return r;
+  HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));
+  // per-solid reductions on the host
+  std::vector<double> lam(n), mu(n), v0(n), v1(n);
+  HIP_TRY(hipMemcpyAsync(lam.data(), ap.lam, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(mu.data(), ap.mu, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(v0.data(), ap.vb[0], n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(v1.data(), ap.vb[1], n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (size_t si = 0; si < ctx->solids.size(); si++) {
+    const HostSolid& hs = ctx->solids[si];
+    double lb = 0.0, mb = 0.0, vb0 = 0.0, vb1 = 0.0;
+    for (int64_t i = hs.offset; i < hs.offset + hs.n; i++) {
+      lb += lam[size_t(i)];
+      mb += mu[size_t(i)];
+      vb0 += v0[size_t(i)];
+      vb1 += v1[size_t(i)];
+    }
+    // lambda = E nu / ((1+nu)(1-2nu)),  mu = E / (2(1+nu))
+    const double E = hs.mat.E, nu = hs.mat.nu, den = (1.0 + nu) * (1.0 - 2.0 * nu);
+    const double dl_dE = nu / den, dm_dE = 1.0 / (2.0 * (1.0 + nu));
+    const double dl_dnu = E * (1.0 + 2.0 * nu * nu) / (den * den), dm_dnu = -E / (2.0 * (1.0 + nu) * (1.0 + nu));
+    if (dE) dE[si] = lb * dl_dE + mb * dm_dE;
+    if (dnu) dnu[si] = lb * dl_dnu + mb * dm_dnu;
+    if (dvel0) {
+      dvel0[2 * si] = vb0;
+      dvel0[2 * si + 1] = vb1;
+    }
+  }
+  ctx->grid_valid = false;
   return JUMP_OK;
 }

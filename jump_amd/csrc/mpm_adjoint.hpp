@@ -1,0 +1,254 @@
+// mpm_adjoint.hpp -- reverse mode of the 2D USL step (linear and GIMP bases,
+// Hooke and Neo-Hookean solids): given the adjoints of the particle state
+// after a step, the adjoints of the state before it and of the material
+// constants lambda, mu.
+//
+// Forward step (SolveExplicitSolidMechanics2D.jl:391-534), particle p, node I,
+// N = N_I(x_p), g = grad N_I(x_p):
+//   m_I = sum N m_p          P0_I = sum N m_p v_p
+//   f_I = sum -V_p sigma_p g + m_p b N          P_I = P0_I + dt f_I  (Dirichlet: 0)
+//   for nodes with m_I > tol:  u = P/m,  a = (P - P0)/m
+//   v' = v + sum N a     x' = x + dt sum N u     L = sum u (x) g   (L[a][b] = u_a g_b)
+//   eps' = eps + dt sym(L)   F' = F (I + dt L)   J = det F'   V' = J V0
+//   sigma' = Hooke(eps') | (mu (F'F'^T - I) + lambda ln J I) / J
+// The adjoint runs the three parts backwards:
+//   k_adj_g2p    per particle: stress, V, F, eps, then the G2P sums; the node
+//                adjoints of m, P, P0 are added atomically, x picks up
+//                dN/dx and d(grad N)/dx terms
+//   k_adj_grid   per node: P = P0 + dt f with the Dirichlet mask
+//   k_adj_p2g    per particle: the P2G sums gathered from the nodes
+// Nodes below the tolerance and the Dirichlet components are constants of
+// the step, as in the forward.  Node adjoints are summed with atomics, so
+// gradients are reproducible to rounding, not bit for bit.
+#pragma once
+
+#include "mpm_kernels.hpp"
+
+namespace jh {
+
+struct AdjParticles {
+  double* xb[2];
+  double* vb[2];
+  double* Fb[4];
+  double* eb[3];
+  double* sb[3];
+  double* Vb;
+  double* lam;  // accumulated adjoint of lambda, per particle
+  double* mu;   // accumulated adjoint of mu, per particle
+};
+struct AdjNodes {
+  double* mb;
+  double* Pb[2];
+  double* P0b[2];
+};
+
+// Weight, first and second derivative of node id (1-based) on one axis.
+template <int B>
+__device__ __forceinline__ void axis_w2(double x, const Axis& A, double lp, int id, double& w, double& d1, double& d2) {
+  const double xi = x - (A.mn + double(id - 1) * A.h);
+  if (B == JUMP_BASIS_LINEAR) {
+    w = 1.0 - fabs(xi) * A.hI;
+    d1 = -jsign(xi) * A.hI;
+    d2 = 0.0;
+  } else {  // uGIMP
+    double dn;
+    gimp_eval(xi, A.h, A.hI, lp, w, dn);
+    d1 = dn;
+    const double a = fabs(xi);
+    if (a < lp) d2 = -1.0 / (A.h * lp);
+    else if (a <= A.h - lp) d2 = 0.0;
+    else if (a < A.h + lp) d2 = 1.0 / (2.0 * A.h * lp);
+    else d2 = 0.0;
+  }
+}
+
+// Calls fn(node, N, g[2], H[3] = {Nxx, Nxy, Nyy}) for the support of p.
+template <int B, class Fn>
+__device__ __forceinline__ bool walk_stencil_h(const Particles& P, const GridDesc& G, const SolidDev& s, int p,
+                                               Fn&& fn) {
+  constexpr int W = Width<B>::W;
+  const Axis A[2] = {axis_of(G, 0), axis_of(G, 1)};
+  int f[2];
+  for (int a = 0; a < 2; a++)
+    if (!first_node<B, 2>(P.x[a][p], a, A[a], s.lp[a], f[a])) return false;
+  double w[2][W], d1[2][W], d2[2][W];
+  for (int a = 0; a < 2; a++)
+    for (int o = 0; o < W; o++) axis_w2<B>(P.x[a][p], A[a], s.lp[a], f[a] + o, w[a][o], d1[a][o], d2[a][o]);
+  for (int b = 0; b < W; b++)
+    for (int a = 0; a < W; a++) {
+      const double N = w[0][a] * w[1][b];
+      const double g[2] = {d1[0][a] * w[1][b], w[0][a] * d1[1][b]};
+      const double H[3] = {d2[0][a] * w[1][b], d1[0][a] * d1[1][b], w[0][a] * d2[1][b]};
+      fn((f[0] + a - 1) + G.nx * (f[1] + b - 1), N, g, H);
+    }
+  return true;
+}
+
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                    GridArrays g, double tol, double dt, AdjParticles ad, AdjNodes an) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  if (s.rigid) return;
+  // forward recomputation of L, eps', F', J
+  double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  const bool ok = walk_stencil_h<B>(P, G, s, i, [&](int n, double, const double* gr, const double*) {
+    const double mI = g.m[n];
+    if (mI > tol) {
+      const double u0 = g.p[0][n] / mI, u1 = g.p[1][n] / mI;
+      L0 += gr[0] * u0;
+      L1 += gr[0] * u1;
+      L2 += gr[1] * u0;
+      L3 += gr[1] * u1;
+    }
+  });
+  if (!ok) return;  // the forward left this particle unchanged: identity adjoint
+  const double e[3] = {P.eps[0][i] + dt * L0, P.eps[1][i] + dt * L3, P.eps[2][i] + dt * (0.5 * (L2 + L1))};
+  const double A[4] = {1.0 + L0 * dt, L1 * dt, L2 * dt, 1.0 + L3 * dt};
+  const double F[4] = {P.F[0][i], P.F[1][i], P.F[2][i], P.F[3][i]};
+  const double Fn[4] = {F[0] * A[0] + F[2] * A[1], F[1] * A[0] + F[3] * A[1], F[0] * A[2] + F[2] * A[3],
+                        F[1] * A[2] + F[3] * A[3]};
+  const double J = Fn[0] * Fn[3] - Fn[2] * Fn[1];
+  // incoming adjoints of the state after the step
+  const double sb[3] = {ad.sb[0][i], ad.sb[1][i], ad.sb[2][i]};
+  double eb[3] = {ad.eb[0][i], ad.eb[1][i], ad.eb[2][i]};
+  double Fnb[4] = {ad.Fb[0][i], ad.Fb[1][i], ad.Fb[2][i], ad.Fb[3][i]};
+  double Jb = ad.Vb[i] * P.V0[i];
+  double lamb = 0.0, mub = 0.0;
+  if (s.kind == JUMP_MAT_ELASTIC) {
+    const double tr = e[0] + e[1], trb = sb[0] + sb[1];
+    eb[0] += s.lambda * trb + 2.0 * s.mu * sb[0];
+    eb[1] += s.lambda * trb + 2.0 * s.mu * sb[1];
+    eb[2] += 2.0 * s.mu * sb[2];
+    lamb += tr * trb;
+    mub += 2.0 * (e[0] * sb[0] + e[1] * sb[1] + e[2] * sb[2]);
+  } else if (s.kind == JUMP_MAT_NEOHOOKEAN) {
+    const double B11 = Fn[0] * Fn[0] + Fn[2] * Fn[2], B22 = Fn[1] * Fn[1] + Fn[3] * Fn[3];
+    const double B12 = Fn[0] * Fn[1] + Fn[2] * Fn[3];
+    const double lnJ = log(J), iJ = 1.0 / J;
+    const double T11 = s.mu * (B11 - 1.0) + s.lambda * lnJ, T22 = s.mu * (B22 - 1.0) + s.lambda * lnJ,
+                 T12 = s.mu * B12;
+    const double Tb11 = iJ * sb[0], Tb22 = iJ * sb[1], Tb12 = iJ * sb[2];
+    Jb += -(T11 * sb[0] + T22 * sb[1] + T12 * sb[2]) * iJ * iJ;
+    Jb += s.lambda * (Tb11 + Tb22) * iJ;
+    lamb += lnJ * (Tb11 + Tb22);
+    mub += (B11 - 1.0) * Tb11 + (B22 - 1.0) * Tb22 + B12 * Tb12;
+    const double Bb11 = s.mu * Tb11, Bb22 = s.mu * Tb22, Bb12 = s.mu * Tb12;
+    Fnb[0] += 2.0 * Bb11 * Fn[0] + Bb12 * Fn[1];
+    Fnb[2] += 2.0 * Bb11 * Fn[2] + Bb12 * Fn[3];
+    Fnb[1] += 2.0 * Bb22 * Fn[1] + Bb12 * Fn[0];
+    Fnb[3] += 2.0 * Bb22 * Fn[3] + Bb12 * Fn[2];
+  }
+  // J = Fn0 Fn3 - Fn2 Fn1
+  Fnb[0] += Jb * Fn[3];
+  Fnb[3] += Jb * Fn[0];
+  Fnb[2] -= Jb * Fn[1];
+  Fnb[1] -= Jb * Fn[2];
+  // F' = F A
+  double Fb[4], Ab[4];
+  Fb[0] = Fnb[0] * A[0] + Fnb[2] * A[2];
+  Fb[1] = Fnb[1] * A[0] + Fnb[3] * A[2];
+  Fb[2] = Fnb[0] * A[1] + Fnb[2] * A[3];
+  Fb[3] = Fnb[1] * A[1] + Fnb[3] * A[3];
+  Ab[0] = Fnb[0] * F[0] + Fnb[1] * F[1];
+  Ab[1] = Fnb[0] * F[2] + Fnb[1] * F[3];
+  Ab[2] = Fnb[2] * F[0] + Fnb[3] * F[1];
+  Ab[3] = Fnb[2] * F[2] + Fnb[3] * F[3];
+  // A = I + dt L;  eps' = eps + dt sym(L)
+  const double Lb0 = dt * Ab[0] + dt * eb[0];
+  const double Lb3 = dt * Ab[3] + dt * eb[1];
+  const double Lb1 = dt * Ab[1] + 0.5 * dt * eb[2];
+  const double Lb2 = dt * Ab[2] + 0.5 * dt * eb[2];
+  const double xb1[2] = {ad.xb[0][i], ad.xb[1][i]};
+  const double vb1[2] = {ad.vb[0][i], ad.vb[1][i]};
+  double xb[2] = {xb1[0], xb1[1]};
+  walk_stencil_h<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
+    const double mI = g.m[n];
+    if (!(mI > tol)) return;
+    const double iM = 1.0 / mI;
+    const double u[2] = {g.p[0][n] * iM, g.p[1][n] * iM};
+    const double a[2] = {(g.p[0][n] - g.p0[0][n]) * iM, (g.p[1][n] - g.p0[1][n]) * iM};
+    const double ub[2] = {N * dt * xb1[0] + Lb0 * gr[0] + Lb2 * gr[1], N * dt * xb1[1] + Lb1 * gr[0] + Lb3 * gr[1]};
+    const double abar[2] = {N * vb1[0], N * vb1[1]};
+    const double Nb = dt * (u[0] * xb1[0] + u[1] * xb1[1]) + a[0] * vb1[0] + a[1] * vb1[1];
+    const double gb[2] = {Lb0 * u[0] + Lb1 * u[1], Lb2 * u[0] + Lb3 * u[1]};
+    atomicAdd(&an.Pb[0][n], (ub[0] + abar[0]) * iM);
+    atomicAdd(&an.Pb[1][n], (ub[1] + abar[1]) * iM);
+    atomicAdd(&an.P0b[0][n], -abar[0] * iM);
+    atomicAdd(&an.P0b[1][n], -abar[1] * iM);
+    atomicAdd(&an.mb[n], -(u[0] * ub[0] + u[1] * ub[1] + a[0] * abar[0] + a[1] * abar[1]) * iM);
+    xb[0] += Nb * gr[0] + H[0] * gb[0] + H[1] * gb[1];
+    xb[1] += Nb * gr[1] + H[1] * gb[0] + H[2] * gb[1];
+  });
+  ad.xb[0][i] = xb[0];
+  ad.xb[1][i] = xb[1];
+  // v' = v + ...: identity part of v stays in ad.vb
+  for (int k = 0; k < 4; k++) ad.Fb[k][i] = Fb[k];
+  for (int k = 0; k < 3; k++) {
+    ad.eb[k][i] = eb[k];
+    ad.sb[k][i] = 0.0;  // sigma is overwritten by the update; its P2G use is added by k_adj_p2g
+  }
+  ad.Vb[i] = 0.0;  // likewise V
+  ad.lam[i] += lamb;
+  ad.mu[i] += mub;
+}
+
+// P = P0 + dt f; Dirichlet components of P0 and P are constants.
+__global__ void __launch_bounds__(kBlock) k_adj_grid(int nn, const uint8_t* __restrict__ fixed, double dt,
+                                                     AdjNodes an) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const uint8_t fx = fixed[n];
+  for (int k = 0; k < 2; k++) {
+    if (fx & (1u << k)) {
+      an.P0b[k][n] = 0.0;
+      an.Pb[k][n] = 0.0;  // holds the force adjoint from here on
+    } else {
+      const double pb = an.Pb[k][n];
+      an.P0b[k][n] += pb;
+      an.Pb[k][n] = dt * pb;
+    }
+  }
+}
+
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_adj_p2g(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                    AdjNodes an, AdjParticles ad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  if (s.rigid) return;
+  const double m = P.m[i], V = P.V[i];
+  const double v[2] = {P.v[0][i], P.v[1][i]};
+  const double sg[3] = {P.sig[0][i], P.sig[1][i], P.sig[2][i]};
+  double xb[2] = {0.0, 0.0}, vb[2] = {0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0}, Vb = 0.0;
+  walk_stencil_h<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
+    const double mb = an.mb[n];
+    const double p0b[2] = {an.P0b[0][n], an.P0b[1][n]};
+    const double fb[2] = {an.Pb[0][n], an.Pb[1][n]};
+    const double Nb = mb * m + m * (v[0] * p0b[0] + v[1] * p0b[1]) + m * (G.body[0] * fb[0] + G.body[1] * fb[1]);
+    vb[0] += N * m * p0b[0];
+    vb[1] += N * m * p0b[1];
+    sb[0] -= V * gr[0] * fb[0];
+    sb[1] -= V * gr[1] * fb[1];
+    sb[2] -= V * (gr[1] * fb[0] + gr[0] * fb[1]);
+    Vb -= (sg[0] * gr[0] + sg[2] * gr[1]) * fb[0] + (sg[2] * gr[0] + sg[1] * gr[1]) * fb[1];
+    const double gb[2] = {-V * (sg[0] * fb[0] + sg[2] * fb[1]), -V * (sg[2] * fb[0] + sg[1] * fb[1])};
+    xb[0] += Nb * gr[0] + H[0] * gb[0] + H[1] * gb[1];
+    xb[1] += Nb * gr[1] + H[1] * gb[0] + H[2] * gb[1];
+  });
+  ad.xb[0][i] += xb[0];
+  ad.xb[1][i] += xb[1];
+  ad.vb[0][i] += vb[0];
+  ad.vb[1][i] += vb[1];
+  for (int k = 0; k < 3; k++) ad.sb[k][i] += sb[k];
+  ad.Vb[i] += Vb;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_f64(double* __restrict__ a, double v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+}  // namespace jh

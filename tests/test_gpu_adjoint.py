@@ -1,0 +1,75 @@
+"""Reverse mode of the 2D USL step (jump_adjoint_*): gradients of
+L = sum |x_T - x*|^2 with respect to E, nu and each solid's initial velocity
+against central finite differences of the forward step on the GPU (the
+forward step is itself bitwise equal to the CPU restatement)."""
+import numpy as np
+import pytest
+
+from jump_amd import _ffi, scenes
+from jump_amd.adjoint import rrule
+from jump_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 20
+
+
+def _scene(basis, material):
+    return scenes.block2d(cells=16, basis=basis, material=material, steps=NSTEPS, seed=3)
+
+
+def _loss(basis, material, target, dE=0.0, dnu=0.0, dv=(0.0, 0.0)):
+    sc = _scene(basis, material)
+    m = sc.solids[0]["material"]
+    m.E += dE
+    m.nu += dnu
+    sc.solids[0]["vel"] = sc.solids[0]["vel"] + np.asarray(dv)
+    e = Engine(sc.params)
+    sc.load(e)
+    e.step(NSTEPS, 0.0)
+    pos = e.get_solid(0)["pos"]
+    e.close()
+    return float(np.sum((pos - target) ** 2))
+
+
+@pytest.mark.parametrize("basis,material", [(_ffi.BASIS_LINEAR, "elastic"), (_ffi.BASIS_LINEAR, "neohookean"),
+                                            (_ffi.BASIS_GIMP, "elastic"), (_ffi.BASIS_GIMP, "neohookean")])
+def test_adjoint_gradients_match_central_differences(hip_lib, basis, material):
+    sc = _scene(basis, material)
+    target = sc.solids[0]["pos"] + np.array([0.004, -0.003])
+    e = Engine(sc.params)
+    sc.load(e)
+    (pos, vel, t, rc), pullback = rrule(e, NSTEPS, checkpoint_every=6)
+    assert rc == 0
+    g = pullback(2.0 * (pos - target))
+    E, nu = sc.solids[0]["material"].E, sc.solids[0]["material"].nu
+    fd = {}
+    h = 1e-5 * E
+    fd["E"] = (_loss(basis, material, target, dE=h) - _loss(basis, material, target, dE=-h)) / (2 * h)
+    h = 1e-5  # the central difference in nu is most accurate here (truncation above, rounding below)
+    fd["nu"] = (_loss(basis, material, target, dnu=h) - _loss(basis, material, target, dnu=-h)) / (2 * h)
+    for k in range(2):
+        dv = np.zeros(2)
+        dv[k] = 1e-6
+        fd[f"v{k}"] = (_loss(basis, material, target, dv=dv) - _loss(basis, material, target, dv=-dv)) / 2e-6
+    got = {"E": g["E"][0], "nu": g["nu"][0], "v0": g["v0"][0, 0], "v1": g["v0"][0, 1]}
+    print(basis, material, {k: (got[k], fd[k]) for k in fd})
+    for k in fd:
+        assert abs(got[k] - fd[k]) <= 1e-6 * abs(fd[k]) + 1e-12, (k, got[k], fd[k])
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(hip_lib):
+    sc = _scene(_ffi.BASIS_LINEAR, "neohookean")
+    target = sc.solids[0]["pos"]
+    res = []
+    for K in (1, 7, NSTEPS):
+        e = Engine(sc.params)
+        sc.load(e)
+        (pos, vel, t, rc), pullback = rrule(e, NSTEPS, checkpoint_every=K)
+        res.append(pullback(2.0 * (pos - target), vel_bar=np.ones_like(vel)))
+        after = e.get_solid(0)["pos"]
+        assert np.array_equal(after, pos)  # the context keeps the recorded final state
+    for r in res[1:]:
+        for k in ("E", "nu", "v0"):
+            print(k, r[k], res[0][k])
+            assert np.allclose(r[k], res[0][k], rtol=1e-9, atol=0.0), k
