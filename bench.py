@@ -13,11 +13,12 @@ its mean duration, from HIP events on the library's stream over the timed
 region), and the CPU restatement timed on a bounded sample on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--order reference|binned]
-                    [--batch-instances N] [--batch-steps K]
+                    [--batch-instances N] [--batch-steps K] [--adjoint-steps K]
     python bench.py --copy-probe MIB   # device-to-device copy of MIB MiB, to
                                        # calibrate HBM byte counters
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -141,6 +142,68 @@ def batched_leg(n_inst, steps, warmup):
             "speedup": t_plain / t_batch, "unit": "particle-steps/s", "step_flags": flags}
 
 
+def adjoint_leg(steps, K):
+    """Reverse mode against the plain step on the two-rings impact at 1025 x
+    513 nodes, 4 particles per cell and direction (1,844,952 particles,
+    2D USL, GIMP, Neo-Hookean E = 2 MPa, 30 mm/s), `steps` steps.  plain:
+    jump_step; adjoint: the recorded run (snapshots every K steps) plus the
+    backward sweep, for L = sum |x_T - x_0|^2."""
+    import numpy as np
+    import torch
+
+    from jump_amd import _ffi, scenes
+    from jump_amd.engine import Engine
+
+    sc = scenes.two_rings(1025, 4, _ffi.BASIS_GIMP, 2.0, 30.0)
+    e = Engine(sc.params, device=0)
+    sc.load(e)
+    e.step(2, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(steps, 0.0)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    e.close()
+    e = Engine(sc.params, device=0)
+    sc.load(e)
+    x0 = np.concatenate([s_["pos"] for s_ in sc.solids])
+    lib = e.lib
+    ns = len(sc.solids)
+    dE, dnu, dv0 = np.zeros(ns), np.zeros(ns), np.zeros((ns, 2))
+    e.step(0, 0.0)
+    start = e.checkpoint()
+    # a first recorded run and sweep on this context allocate its buffers,
+    # which are kept; the timed pair then starts again from the same state
+    tt = ctypes.c_double(0.0)
+    e._check(lib.jump_adjoint_forward(e.h, steps, K, ctypes.byref(tt)))
+    xb = np.zeros_like(x0)
+    e._check(lib.jump_adjoint_backward(e.h, _ffi.dptr(xb), None, None, None, None))
+    e.restore(start)
+    e.step(0, 0.0)
+    torch.cuda.synchronize()
+    tt = ctypes.c_double(0.0)
+    t0 = time.perf_counter()
+    rc = lib.jump_adjoint_forward(e.h, steps, K, ctypes.byref(tt))
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    pos = np.concatenate([e.get_solid(s_)["pos"] for s_ in range(ns)])
+    xb = np.ascontiguousarray(2.0 * (pos - x0))
+    t0 = time.perf_counter()
+    e._check(lib.jump_adjoint_backward(e.h, _ffi.dptr(xb), None, _ffi.dptr(dE), _ffi.dptr(dnu), _ffi.dptr(dv0)))
+    torch.cuda.synchronize()
+    t_back = time.perf_counter() - t0
+    g = {"E": dE, "nu": dnu}
+    e.close()
+    return {"workload": f"two-rings 1025x513 nodes (2D USL, GIMP, Neo-Hookean), {sc.n_particles} particles",
+            "steps": steps, "checkpoint_every": K,
+            "plain_ms_per_step": t_plain * 1e3 / steps,
+            "recorded_forward_ms_per_step": t_fwd * 1e3 / steps,
+            "backward_ms_per_step": t_back * 1e3 / steps,
+            "adjoint_over_plain": (t_fwd + t_back) / t_plain,
+            "backward_over_plain": t_back / t_plain,
+            "dL_dE": float(g["E"][0]), "dL_dnu": float(g["nu"][0]), "step_flags": rc}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +216,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--batch-instances", type=int, default=256, help="0 skips the batched leg")
     ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--adjoint-steps", type=int, default=50, help="0 skips the adjoint leg")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -243,6 +307,8 @@ def main():
     eng.close()
     if args.batch_instances > 0:
         out["batched"] = batched_leg(args.batch_instances, args.batch_steps, 2)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = adjoint_leg(args.adjoint_steps, 10)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -85,9 +85,13 @@ struct jump_ctx {
     int64_t nsteps = 0;
     int K = 0;
     bool ready = false;
-    std::vector<void*> allocs;
+    int64_t cached_seg = -1;   // segment whose states and grids segbuf/gridbuf hold
+    int64_t snap_np = -1, adj_np = -1;  // particle counts the buffers were sized for
+    std::vector<void*> allocs;      // snapshots and grids of the recorded run
+    std::vector<void*> adj_allocs;  // adjoint arrays
     std::vector<double*> ckpt;    // state at steps 0, K, 2K, ...
     std::vector<double*> segbuf;  // the states of one segment, recomputed
+    std::vector<double*> gridbuf; // and their grids (m, p0, p)
     double* final_state = nullptr;
     AdjParticles ap{};
     AdjNodes an{};
@@ -576,6 +580,7 @@ int jump_ctx_destroy(jump_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_list(ctx->adj.allocs);
+  free_list(ctx->adj.adj_allocs);
   free_list(ctx->part_allocs);
   free_list(ctx->grid_allocs);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -666,6 +671,7 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
   }
   ctx->np += ds->n;
   ctx->solids.push_back(hs);
+  ctx->adj.ready = false;  // a recorded run no longer matches the particles
   if (idx) *idx = sid;
   return JUMP_OK;
 }
@@ -1206,9 +1212,9 @@ int jump_batch_energies(jump_batch* b, double* ke, double* se) {
 // ------------------------------------------------------------------ reverse mode
 // The 2D USL step differentiated by hand (mpm_adjoint.hpp).  The recorded
 // run keeps the particle state every K steps; the backward sweep rebuilds
-// each segment's states from its snapshot, then walks the segment's steps
-// backwards, rebuilding each step's grid.  Memory: (nsteps/K + K + 1)
-// snapshots of 136 bytes per particle.
+// each segment's states from its snapshot (keeping the grid each step used),
+// then walks the segment's steps backwards.  Memory: (nsteps/K + K + 1)
+// snapshots of 136 bytes per particle and K grids of 40 bytes per node.
 static size_t snap_doubles(const jump_ctx* ctx) { return size_t(ctx->np) * (2 * PairLayout<2>::NP + 7); }
 
 static int snap_copy(jump_ctx* ctx, double* buf, bool save) {
@@ -1240,28 +1246,25 @@ static int adjoint_supported(jump_ctx* ctx) {
   return JUMP_OK;
 }
 
-template <int B> static void adj_grid_of_state(jump_ctx* ctx) {
-  bin_and_sort<2, B, false>(ctx);
-  if (ctx->prm.order == JUMP_ORDER_REFERENCE)
-    LAUNCH_P2G((k_p2g<2, B, 0, JUMP_ORDER_REFERENCE, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners,
-               ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w,
-               ctx->prm.dt, ctx->bt());
-  else
-    LAUNCH_P2G((k_p2g<2, B, 0, JUMP_ORDER_BINNED, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners,
-               ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w,
-               ctx->prm.dt, ctx->bt());
+// The grid a step's G2P used: nodal mass and momenta (5 arrays of NN).
+static int grid_copy(jump_ctx* ctx, double* buf, bool save) {
+  const size_t nn = size_t(ctx->NN);
+  double* arrs[5] = {ctx->g.m, ctx->g.p0[0], ctx->g.p0[1], ctx->g.p[0], ctx->g.p[1]};
+  for (int a = 0; a < 5; a++) {
+    hipError_t e = save ? hipMemcpyAsync(buf + a * nn, arrs[a], nn * sizeof(double), hipMemcpyDeviceToDevice,
+                                         ctx->stream)
+                        : hipMemcpyAsync(arrs[a], buf + a * nn, nn * sizeof(double), hipMemcpyDeviceToDevice,
+                                         ctx->stream);
+    if (e != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "grid copy");
+  }
+  return JUMP_OK;
 }
 
 template <int B> static void adj_step_back(jump_ctx* ctx) {
   const int np = int(ctx->np), nn = ctx->NN;
   const double dt = ctx->prm.dt;
-  adj_grid_of_state<B>(ctx);
   AdjNodes& an = ctx->adj.an;
-  LAUNCH(k_fill_f64, nn, an.mb, 0.0, nn);
-  for (int k = 0; k < 2; k++) {
-    LAUNCH(k_fill_f64, nn, an.Pb[k], 0.0, nn);
-    LAUNCH(k_fill_f64, nn, an.P0b[k], 0.0, nn);
-  }
+  LAUNCH(k_adj_zero_nodes, nn, an, nn);
   LAUNCH((k_adj_g2p<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, ctx->prm.tolerance, dt, ctx->adj.ap, an);
   LAUNCH(k_adj_grid, nn, nn, ctx->d_fixed, dt, an);
   LAUNCH((k_adj_p2g<B>), np, ctx->P, ctx->G, ctx->d_solids, an, ctx->adj.ap);
@@ -1276,23 +1279,39 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K, double* t_ino
   r = ensure_device(ctx);
   if (r) return r;
   auto& A = ctx->adj;
-  free_list(A.allocs);
-  A = jump_ctx::Adjoint{};
   const size_t sd = snap_doubles(ctx);
   const int64_t nck = (nsteps + K - 1) / K;
-  auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.allocs, p, count); };
-  A.ckpt.resize(size_t(nck));
-  for (auto& c : A.ckpt)
-    if ((r = alloc(&c, sd))) return r;
-  A.segbuf.resize(size_t(std::min<int64_t>(K, nsteps)));
-  for (auto& c : A.segbuf)
-    if ((r = alloc(&c, sd))) return r;
-  if ((r = alloc(&A.final_state, sd))) return r;
+  const size_t nseg = size_t(std::min<int64_t>(K, nsteps));
+  // buffers are kept from run to run while they are large enough
+  if (A.ckpt.size() < size_t(nck) || A.segbuf.size() < nseg || A.snap_np != ctx->np) {
+    free_list(A.allocs);
+    A.ckpt.clear();
+    A.segbuf.clear();
+    A.gridbuf.clear();
+    A.final_state = nullptr;
+    auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.allocs, p, count); };
+    A.ckpt.resize(size_t(nck));
+    for (auto& c : A.ckpt)
+      if ((r = alloc(&c, sd))) return r;
+    A.segbuf.resize(nseg);
+    for (auto& c : A.segbuf)
+      if ((r = alloc(&c, sd))) return r;
+    A.gridbuf.resize(nseg);
+    for (auto& c : A.gridbuf)
+      if ((r = alloc(&c, 5 * size_t(ctx->NN)))) return r;
+    if ((r = alloc(&A.final_state, sd))) return r;
+    A.snap_np = ctx->np;
+  }
+  A.ready = false;
   unsigned fl = 0;
+  const int64_t last0 = (nck - 1) * K;  // the last segment's states and grids are kept as they go by
   for (int64_t s = 0; s < nsteps; s++) {
     if (s % K == 0 && (r = snap_copy(ctx, A.ckpt[size_t(s / K)], true))) return r;
+    if (s >= last0 && (r = snap_copy(ctx, A.segbuf[size_t(s - last0)], true))) return r;
     if ((r = run_steps(ctx, 1))) return r;
+    if (s >= last0 && (r = grid_copy(ctx, A.gridbuf[size_t(s - last0)], true))) return r;
   }
+  A.cached_seg = nck - 1;
   if ((r = snap_copy(ctx, A.final_state, true))) return r;
   HIP_TRY(hipMemcpyAsync(&fl, ctx->d_flags, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));
@@ -1318,8 +1337,12 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
   int r = JUMP_OK;
   AdjParticles& ap = A.ap;
   AdjNodes& an = A.an;
-  if (!ap.Vb) {
-    auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.allocs, p, count); };
+  if (A.adj_np != ctx->np) {
+    free_list(A.adj_allocs);
+    ap = AdjParticles{};
+    an = AdjNodes{};
+    A.adj_np = ctx->np;
+    auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.adj_allocs, p, count); };
     for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.xb[k], n);
     for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.vb[k], n);
     for (int k = 0; k < 4 && !r; k++) r = alloc(&ap.Fb[k], n);
@@ -1352,13 +1375,18 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
   const int64_t nseg = (A.nsteps + A.K - 1) / A.K;
   for (int64_t seg = nseg - 1; seg >= 0 && !r; seg--) {
     const int64_t s0 = seg * A.K, s1 = std::min<int64_t>(s0 + A.K, A.nsteps);
-    if ((r = snap_copy(ctx, A.ckpt[size_t(seg)], false))) break;
-    for (int64_t j = s0; j < s1 && !r; j++) {
-      if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
-      if (j + 1 < s1) r = run_steps(ctx, 1);
+    if (seg != A.cached_seg) {
+      if ((r = snap_copy(ctx, A.ckpt[size_t(seg)], false))) break;
+      for (int64_t j = s0; j < s1 && !r; j++) {  // rebuild the segment: states and the grids their steps used
+        if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
+        if ((r = run_steps(ctx, 1))) break;
+        r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], true);
+      }
+      A.cached_seg = seg;
     }
     for (int64_t j = s1 - 1; j >= s0 && !r; j--) {
       if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], false))) break;
+      if ((r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], false))) break;
       if (ctx->prm.basis == JUMP_BASIS_LINEAR) adj_step_back<JUMP_BASIS_LINEAR>(ctx);
       else adj_step_back<JUMP_BASIS_GIMP>(ctx);
     }

@@ -18,8 +18,9 @@
 //   k_adj_grid   per node: P = P0 + dt f with the Dirichlet mask
 //   k_adj_p2g    per particle: the P2G sums gathered from the nodes
 // Nodes below the tolerance and the Dirichlet components are constants of
-// the step, as in the forward.  Node adjoints are summed with atomics, so
-// gradients are reproducible to rounding, not bit for bit.
+// the step, as in the forward.  Node adjoints are summed with the hardware
+// FP64 atomic add of device memory, so gradients are reproducible to
+// rounding, not bit for bit.
 #pragma once
 
 #include "mpm_kernels.hpp"
@@ -173,11 +174,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
     const double abar[2] = {N * vb1[0], N * vb1[1]};
     const double Nb = dt * (u[0] * xb1[0] + u[1] * xb1[1]) + a[0] * vb1[0] + a[1] * vb1[1];
     const double gb[2] = {Lb0 * u[0] + Lb1 * u[1], Lb2 * u[0] + Lb3 * u[1]};
-    atomicAdd(&an.Pb[0][n], (ub[0] + abar[0]) * iM);
-    atomicAdd(&an.Pb[1][n], (ub[1] + abar[1]) * iM);
-    atomicAdd(&an.P0b[0][n], -abar[0] * iM);
-    atomicAdd(&an.P0b[1][n], -abar[1] * iM);
-    atomicAdd(&an.mb[n], -(u[0] * ub[0] + u[1] * ub[1] + a[0] * abar[0] + a[1] * abar[1]) * iM);
+    unsafeAtomicAdd(&an.Pb[0][n], (ub[0] + abar[0]) * iM);
+    unsafeAtomicAdd(&an.Pb[1][n], (ub[1] + abar[1]) * iM);
+    unsafeAtomicAdd(&an.P0b[0][n], -abar[0] * iM);
+    unsafeAtomicAdd(&an.P0b[1][n], -abar[1] * iM);
+    unsafeAtomicAdd(&an.mb[n], -(u[0] * ub[0] + u[1] * ub[1] + a[0] * abar[0] + a[1] * abar[1]) * iM);
     xb[0] += Nb * gr[0] + H[0] * gb[0] + H[1] * gb[1];
     xb[1] += Nb * gr[1] + H[1] * gb[0] + H[2] * gb[1];
   });
@@ -249,6 +250,16 @@ __global__ void __launch_bounds__(kBlock) k_adj_p2g(Particles P, GridDesc G, con
 __global__ void __launch_bounds__(kBlock) k_fill_f64(double* __restrict__ a, double v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_zero_nodes(AdjNodes an, int nn) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  an.mb[n] = 0.0;
+  an.Pb[0][n] = 0.0;
+  an.Pb[1][n] = 0.0;
+  an.P0b[0][n] = 0.0;
+  an.P0b[1][n] = 0.0;
 }
 
 }  // namespace jh

@@ -11,8 +11,9 @@ into any Engine (the HIP library, or the oracle in tests).
   rings2d        2D rings between rigid platens (Main_TwoRings.jl /
                  Main_LateralCompression_Cellular.jl), any 2D basis
   cpdi_tubes     2D lateral compression of a tube array with CPDI-Q4 domains
-  two_rings_instance  one seeded instance of the batched two-rings impact
-                 (Main_TwoRings.jl:40-130 scaled to a drawn grid and size)
+  two_rings      2D two-rings impact (Main_TwoRings.jl:40-130) at a given
+                 resolution; two_rings_instance: one seeded instance of the
+                 batched workload (drawn grid, size, material, speed, basis)
 """
 import math
 
@@ -144,18 +145,43 @@ def rings2d(cells=40, basis=_ffi.BASIS_LINEAR, material="neohookean", platen=Tru
     return Scene("rings2d", p, solids, fixed=fixed, steps=steps)
 
 
+def two_rings(side, k, basis, E, vel, steps=1000, nu=0.3, meta=None):
+    """The two-rings impact of Main_TwoRings.jl:40-130 (domain 200 x 100 mm,
+    rings r 30..40 mm at x = 50 and 150, Neo-Hookean, rho 1010e-9,
+    USL(1e-18)) on `side` x (side+1)/2 nodes (side odd, so dy = dx) with k
+    particles per cell and direction.  The ring centres sit on the nearest
+    grid nodes: the builder offsets particles by half-integer multiples of
+    dx/k from the centre, so none lies on a grid line, where the hat
+    gradient's sign(0) = 0 (Basis.jl:219-251) leaves sum(grad N) != 0 and
+    strains a translating body.  dt = 0.3 dx / c where the reference takes
+    0.6, at which hat-function runs go unstable at the free surface within
+    50 steps."""
+    l, w, rin, rout, rho = 200.0, 100.0, 30.0, 40.0, 1010e-9
+    nx, ny = side, (side + 1) // 2
+    dx = l / (nx - 1)
+    dy = w / (ny - 1)
+    fOffset = dx / k
+    lp = 0.5 * fOffset if basis == _ffi.BASIS_GIMP else None
+    cy = round(0.5 * w / dy) * dy
+    solids = []
+    for cx, vx in ((round(0.25 * l / dx) * dx, vel), (round(0.75 * l / dx) * dx, -vel)):
+        pts = buildParticleForRing([cx, cy], rin, rout, fOffset)
+        solids.append(_solid(neohookean(E, nu, rho), pts, (vx, 0.0), fOffset * fOffset, rho, lp=lp))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    c = math.sqrt((lam + 2 * mu) / rho)
+    dt = 0.3 * dx / c
+    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (l, w), (nx, ny), tolerance=1e-18)
+    return Scene("two_rings", p, solids, steps=steps, meta=meta or {"side": side, "E": E, "vel": vel, "ppc": k})
+
+
 def two_rings_instance(seed, steps=1000):
-    """One instance of the batched workload: the two-rings impact of
-    Main_TwoRings.jl:40-130 (domain 200 x 100 mm, rings r 30..40 mm at
-    x = 50 and 150 (nearest grid nodes), Neo-Hookean, rho 1010e-9,
-    USL(1e-18); dt 0.3 dx / c where the reference takes 0.6, at which
-    hat-function instances go unstable at the free surface within 50 steps)
-    with drawn sizes.  rng = default_rng(seed):
-      grid side S ~ log-uniform [64, 512] nodes along x (odd, ny = (S+1)/2,
-        so dy = dx); particle count ~ log-uniform [5k, 80k], through the
-        particle spacing dx/k, k clamped to 1..4 particles per cell and
-        direction (denser packings are unstable at this dt, so small grids
-        get fewer particles than drawn);
+    """One instance of the batched workload: two_rings() with drawn sizes.
+    rng = default_rng(seed):
+      grid side S ~ log-uniform [64, 512] nodes along x (made odd);
+      particle count ~ log-uniform [5k, 80k], through k particles per cell
+        and direction, clamped to 1..4 (denser packings are unstable at this
+        dt, so small grids get fewer particles than drawn);
       E ~ U[0.5, 5] MPa (nu 0.3); impact speed ~ U[10, 100] (mm/s, the
       reference's units); basis linear or GIMP with probability 1/2."""
     rng = np.random.default_rng(seed)
@@ -164,31 +190,11 @@ def two_rings_instance(seed, steps=1000):
     E = rng.uniform(0.5, 5.0)
     vel = rng.uniform(10.0, 100.0)
     basis = _ffi.BASIS_GIMP if rng.uniform() < 0.5 else _ffi.BASIS_LINEAR
-    l, w, rin, rout, rho, nu = 200.0, 100.0, 30.0, 40.0, 1010e-9, 0.3
-    nx, ny = side, (side + 1) // 2
-    dx = l / (nx - 1)
-    area = 2.0 * math.pi * (rout * rout - rin * rin)
+    dx = 200.0 / (side - 1)
+    area = 2.0 * math.pi * (40.0 * 40.0 - 30.0 * 30.0)
     k = min(4, max(1, int(round(math.sqrt(np_target / area) * dx))))
-    fOffset = dx / k
-    lp = 0.5 * fOffset if basis == _ffi.BASIS_GIMP else None
-    mat = lambda: neohookean(E, nu, rho)
-    # Ring centres on grid nodes: the builder offsets particles by
-    # half-integer multiples of dx/k from the centre, so none lies on a grid
-    # line, where the hat gradient's sign(0) = 0 (Basis.jl:219-251) leaves
-    # sum(grad N) != 0 and strains a translating body.
-    dy = w / (ny - 1)
-    cy = round(0.5 * w / dy) * dy
-    solids = []
-    for cx, vx in ((round(0.25 * l / dx) * dx, vel), (round(0.75 * l / dx) * dx, -vel)):
-        pts = buildParticleForRing([cx, cy], rin, rout, fOffset)
-        solids.append(_solid(mat(), pts, (vx, 0.0), fOffset * fOffset, rho, lp=lp))
-    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
-    mu = E / (2 * (1 + nu))
-    c = math.sqrt((lam + 2 * mu) / rho)
-    dt = 0.3 * dx / c
-    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (l, w), (nx, ny), tolerance=1e-18)
-    return Scene("two_rings_instance", p, solids, steps=steps,
-                 meta={"seed": seed, "side": side, "E": E, "vel": vel, "ppc": k})
+    return two_rings(side, k, basis, E, vel, steps=steps,
+                     meta={"seed": seed, "side": side, "E": E, "vel": vel, "ppc": k})
 
 
 def _ring_q4(cx, cy, ri, ro, nr, nt, jitter, rng):
