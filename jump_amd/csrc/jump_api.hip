@@ -92,9 +92,11 @@ struct jump_ctx {
     std::vector<double*> ckpt;    // state at steps 0, K, 2K, ...
     std::vector<double*> segbuf;  // the states of one segment, recomputed
     std::vector<double*> gridbuf; // and their grids (m, p0, p)
+    std::vector<void*> binbuf;    // and their bins (start, sorted)
     double* final_state = nullptr;
     AdjParticles ap{};
     AdjNodes an{};
+    AdjSources src{};
   } adj;
   uint32_t flags = 0;
 
@@ -1246,6 +1248,23 @@ static int adjoint_supported(jump_ctx* ctx) {
   return JUMP_OK;
 }
 
+// The bins a step sorted its particles into: start (NN+1) and sorted (np+1).
+static size_t bins_bytes(const jump_ctx* ctx) {
+  return size_t(ctx->NN + 1) * sizeof(int) + size_t(ctx->np + 1) * sizeof(unsigned long long);
+}
+static int bins_copy(jump_ctx* ctx, void* buf, bool save) {
+  const size_t sb = size_t(ctx->NN + 1) * sizeof(int), qb = size_t(ctx->np + 1) * sizeof(unsigned long long);
+  char* q = static_cast<char*>(buf);
+  hipError_t e1 = save ? hipMemcpyAsync(q, ctx->d_start, sb, hipMemcpyDeviceToDevice, ctx->stream)
+                       : hipMemcpyAsync(ctx->d_start, q, sb, hipMemcpyDeviceToDevice, ctx->stream);
+  // the sorted array is kept 8-byte aligned after the start array
+  char* q2 = q + ((sb + 255) & ~size_t(255));
+  hipError_t e2 = save ? hipMemcpyAsync(q2, ctx->d_sorted, qb, hipMemcpyDeviceToDevice, ctx->stream)
+                       : hipMemcpyAsync(ctx->d_sorted, q2, qb, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e1 != hipSuccess || e2 != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "bins copy");
+  return JUMP_OK;
+}
+
 // The grid a step's G2P used: nodal mass and momenta (5 arrays of NN).
 static int grid_copy(jump_ctx* ctx, double* buf, bool save) {
   const size_t nn = size_t(ctx->NN);
@@ -1264,8 +1283,10 @@ template <int B> static void adj_step_back(jump_ctx* ctx) {
   const int np = int(ctx->np), nn = ctx->NN;
   const double dt = ctx->prm.dt;
   AdjNodes& an = ctx->adj.an;
-  LAUNCH(k_adj_zero_nodes, nn, an, nn);
-  LAUNCH((k_adj_g2p<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, ctx->prm.tolerance, dt, ctx->adj.ap, an);
+  LAUNCH((k_adj_g2p<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, ctx->prm.tolerance, dt, ctx->adj.ap,
+         ctx->adj.src);
+  LAUNCH((k_adj_nodes<B>), nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g,
+         ctx->prm.tolerance, dt, ctx->adj.src, an);
   LAUNCH(k_adj_grid, nn, nn, ctx->d_fixed, dt, an);
   LAUNCH((k_adj_p2g<B>), np, ctx->P, ctx->G, ctx->d_solids, an, ctx->adj.ap);
 }
@@ -1299,6 +1320,12 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K, double* t_ino
     A.gridbuf.resize(nseg);
     for (auto& c : A.gridbuf)
       if ((r = alloc(&c, 5 * size_t(ctx->NN)))) return r;
+    A.binbuf.resize(nseg);
+    for (auto& c : A.binbuf) {
+      char* q = nullptr;
+      if ((r = dev_alloc(ctx, A.allocs, &q, bins_bytes(ctx) + 256))) return r;
+      c = q;
+    }
     if ((r = alloc(&A.final_state, sd))) return r;
     A.snap_np = ctx->np;
   }
@@ -1310,6 +1337,7 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K, double* t_ino
     if (s >= last0 && (r = snap_copy(ctx, A.segbuf[size_t(s - last0)], true))) return r;
     if ((r = run_steps(ctx, 1))) return r;
     if (s >= last0 && (r = grid_copy(ctx, A.gridbuf[size_t(s - last0)], true))) return r;
+    if (s >= last0 && (r = bins_copy(ctx, A.binbuf[size_t(s - last0)], true))) return r;
   }
   A.cached_seg = nck - 1;
   if ((r = snap_copy(ctx, A.final_state, true))) return r;
@@ -1341,6 +1369,7 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
     free_list(A.adj_allocs);
     ap = AdjParticles{};
     an = AdjNodes{};
+    A.src = AdjSources{};
     A.adj_np = ctx->np;
     auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.adj_allocs, p, count); };
     for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.xb[k], n);
@@ -1351,6 +1380,10 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
     if (!r) r = alloc(&ap.Vb, n);
     if (!r) r = alloc(&ap.lam, n);
     if (!r) r = alloc(&ap.mu, n);
+    AdjSources& src = A.src;
+    for (int k = 0; k < 2 && !r; k++) r = alloc(&src.xb[k], n);
+    for (int k = 0; k < 2 && !r; k++) r = alloc(&src.vb[k], n);
+    for (int k = 0; k < 4 && !r; k++) r = alloc(&src.Lb[k], n);
     if (!r) r = alloc(&an.mb, nn);
     for (int k = 0; k < 2 && !r; k++) r = alloc(&an.Pb[k], nn);
     for (int k = 0; k < 2 && !r; k++) r = alloc(&an.P0b[k], nn);
@@ -1380,13 +1413,15 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
       for (int64_t j = s0; j < s1 && !r; j++) {  // rebuild the segment: states and the grids their steps used
         if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
         if ((r = run_steps(ctx, 1))) break;
-        r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], true);
+        if ((r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], true))) break;
+        r = bins_copy(ctx, A.binbuf[size_t(j - s0)], true);
       }
       A.cached_seg = seg;
     }
     for (int64_t j = s1 - 1; j >= s0 && !r; j--) {
       if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], false))) break;
       if ((r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], false))) break;
+      if ((r = bins_copy(ctx, A.binbuf[size_t(j - s0)], false))) break;
       if (ctx->prm.basis == JUMP_BASIS_LINEAR) adj_step_back<JUMP_BASIS_LINEAR>(ctx);
       else adj_step_back<JUMP_BASIS_GIMP>(ctx);
     }
@@ -1396,22 +1431,34 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
   // leave the recorded final state in place
   if ((r = snap_copy(ctx, A.final_state, false))) return r;
   HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));
-  // per-solid reductions on the host
-  std::vector<double> lam(n), mu(n), v0(n), v1(n);
-  HIP_TRY(hipMemcpyAsync(lam.data(), ap.lam, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(mu.data(), ap.mu, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(v0.data(), ap.vb[0], n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(v1.data(), ap.vb[1], n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
-  for (size_t si = 0; si < ctx->solids.size(); si++) {
+  // per-solid sums on the device
+  const size_t ns = ctx->solids.size();
+  std::vector<long long> range(2 * ns);
+  for (size_t si = 0; si < ns; si++) {
+    range[2 * si] = ctx->solids[si].offset;
+    range[2 * si + 1] = ctx->solids[si].offset + ctx->solids[si].n;
+  }
+  long long* d_range = nullptr;
+  double* d_sums = nullptr;
+  std::vector<void*> tmp;
+  if ((r = dev_alloc(ctx, tmp, &d_range, range.size()))) return r;
+  if ((r = dev_alloc(ctx, tmp, &d_sums, 4 * ns))) {
+    free_list(tmp);
+    return r;
+  }
+  std::vector<double> sums(4 * ns);
+  hipError_t he = hipMemcpyAsync(d_range, range.data(), range.size() * sizeof(long long), hipMemcpyHostToDevice,
+                                 ctx->stream);
+  if (he == hipSuccess) {
+    hipLaunchKernelGGL(k_adj_solid_sums, dim3(unsigned(ns)), dim3(kBlock), 0, ctx->stream, d_range, ap, d_sums);
+    he = hipMemcpyAsync(sums.data(), d_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  }
+  if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+  free_list(tmp);
+  if (he != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "adjoint sums");
+  for (size_t si = 0; si < ns; si++) {
     const HostSolid& hs = ctx->solids[si];
-    double lb = 0.0, mb = 0.0, vb0 = 0.0, vb1 = 0.0;
-    for (int64_t i = hs.offset; i < hs.offset + hs.n; i++) {
-      lb += lam[size_t(i)];
-      mb += mu[size_t(i)];
-      vb0 += v0[size_t(i)];
-      vb1 += v1[size_t(i)];
-    }
+    const double lb = sums[4 * si], mb = sums[4 * si + 1], vb0 = sums[4 * si + 2], vb1 = sums[4 * si + 3];
     // lambda = E nu / ((1+nu)(1-2nu)),  mu = E / (2(1+nu))
     const double E = hs.mat.E, nu = hs.mat.nu, den = (1.0 + nu) * (1.0 - 2.0 * nu);
     const double dl_dE = nu / den, dm_dE = 1.0 / (2.0 * (1.0 + nu));

@@ -12,15 +12,16 @@
 //   eps' = eps + dt sym(L)   F' = F (I + dt L)   J = det F'   V' = J V0
 //   sigma' = Hooke(eps') | (mu (F'F'^T - I) + lambda ln J I) / J
 // The adjoint runs the three parts backwards:
-//   k_adj_g2p    per particle: stress, V, F, eps, then the G2P sums; the node
-//                adjoints of m, P, P0 are added atomically, x picks up
-//                dN/dx and d(grad N)/dx terms
+//   k_adj_g2p    per particle: stress, V, F, eps, then the G2P sums; x picks
+//                up dN/dx and d(grad N)/dx terms; the particle's sources of
+//                the node adjoints are stored
+//   k_adj_nodes  per node: the node adjoints of m, P, P0, gathered from the
+//                particles of its window (no atomics)
 //   k_adj_grid   per node: P = P0 + dt f with the Dirichlet mask
 //   k_adj_p2g    per particle: the P2G sums gathered from the nodes
 // Nodes below the tolerance and the Dirichlet components are constants of
-// the step, as in the forward.  Node adjoints are summed with the hardware
-// FP64 atomic add of device memory, so gradients are reproducible to
-// rounding, not bit for bit.
+// the step, as in the forward.  Every sum has a fixed order, so gradients
+// are bitwise reproducible run to run.
 #pragma once
 
 #include "mpm_kernels.hpp"
@@ -41,6 +42,13 @@ struct AdjNodes {
   double* mb;
   double* Pb[2];
   double* P0b[2];
+};
+// Per-particle sources of the G2P node adjoints: the adjoints of x' and v'
+// and of L, written by k_adj_g2p and gathered per node by k_adj_nodes.
+struct AdjSources {
+  double* xb[2];
+  double* vb[2];
+  double* Lb[4];
 };
 
 // Weight, first and second derivative of node id (1-based) on one axis.
@@ -87,7 +95,8 @@ __device__ __forceinline__ bool walk_stencil_h(const Particles& P, const GridDes
 
 template <int B>
 __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, const SolidDev* __restrict__ S,
-                                                    GridArrays g, double tol, double dt, AdjParticles ad, AdjNodes an) {
+                                                    GridArrays g, double tol, double dt, AdjParticles ad,
+                                                    AdjSources src) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
@@ -104,7 +113,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
       L3 += gr[1] * u1;
     }
   });
-  if (!ok) return;  // the forward left this particle unchanged: identity adjoint
+  if (!ok) {  // the forward left this particle unchanged: identity adjoint, no node sources
+    for (int k = 0; k < 2; k++) { src.xb[k][i] = 0.0; src.vb[k][i] = 0.0; }
+    for (int k = 0; k < 4; k++) src.Lb[k][i] = 0.0;
+    return;
+  }
   const double e[3] = {P.eps[0][i] + dt * L0, P.eps[1][i] + dt * L3, P.eps[2][i] + dt * (0.5 * (L2 + L1))};
   const double A[4] = {1.0 + L0 * dt, L1 * dt, L2 * dt, 1.0 + L3 * dt};
   const double F[4] = {P.F[0][i], P.F[1][i], P.F[2][i], P.F[3][i]};
@@ -163,6 +176,14 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
   const double Lb2 = dt * Ab[2] + 0.5 * dt * eb[2];
   const double xb1[2] = {ad.xb[0][i], ad.xb[1][i]};
   const double vb1[2] = {ad.vb[0][i], ad.vb[1][i]};
+  src.xb[0][i] = xb1[0];
+  src.xb[1][i] = xb1[1];
+  src.vb[0][i] = vb1[0];
+  src.vb[1][i] = vb1[1];
+  src.Lb[0][i] = Lb0;
+  src.Lb[1][i] = Lb1;
+  src.Lb[2][i] = Lb2;
+  src.Lb[3][i] = Lb3;
   double xb[2] = {xb1[0], xb1[1]};
   walk_stencil_h<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
     const double mI = g.m[n];
@@ -170,15 +191,8 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
     const double iM = 1.0 / mI;
     const double u[2] = {g.p[0][n] * iM, g.p[1][n] * iM};
     const double a[2] = {(g.p[0][n] - g.p0[0][n]) * iM, (g.p[1][n] - g.p0[1][n]) * iM};
-    const double ub[2] = {N * dt * xb1[0] + Lb0 * gr[0] + Lb2 * gr[1], N * dt * xb1[1] + Lb1 * gr[0] + Lb3 * gr[1]};
-    const double abar[2] = {N * vb1[0], N * vb1[1]};
     const double Nb = dt * (u[0] * xb1[0] + u[1] * xb1[1]) + a[0] * vb1[0] + a[1] * vb1[1];
     const double gb[2] = {Lb0 * u[0] + Lb1 * u[1], Lb2 * u[0] + Lb3 * u[1]};
-    unsafeAtomicAdd(&an.Pb[0][n], (ub[0] + abar[0]) * iM);
-    unsafeAtomicAdd(&an.Pb[1][n], (ub[1] + abar[1]) * iM);
-    unsafeAtomicAdd(&an.P0b[0][n], -abar[0] * iM);
-    unsafeAtomicAdd(&an.P0b[1][n], -abar[1] * iM);
-    unsafeAtomicAdd(&an.mb[n], -(u[0] * ub[0] + u[1] * ub[1] + a[0] * abar[0] + a[1] * abar[1]) * iM);
     xb[0] += Nb * gr[0] + H[0] * gb[0] + H[1] * gb[1];
     xb[1] += Nb * gr[1] + H[1] * gb[0] + H[2] * gb[1];
   });
@@ -193,6 +207,58 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
   ad.Vb[i] = 0.0;  // likewise V
   ad.lam[i] += lamb;
   ad.mu[i] += mub;
+}
+
+// Node adjoints of the G2P sums, gathered per node over the particles of
+// its window (the step's bins, any order: the sum is not bit-matched to a
+// reference).  u = P/m, a = (P - P0)/m:
+//   Pb  += (ub + ab)/m    P0b -= ab/m    mb -= (u.ub + a.ab)/m
+//   ub = N dt xb' + Lb g (L[a][b] = u_a g_b),   ab = N vb'
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                      const int* __restrict__ start,
+                                                      const unsigned long long* __restrict__ sorted, GridArrays g,
+                                                      double tol, double dt, AdjSources src, AdjNodes an) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.nn) return;
+  const double mI = g.m[n];
+  double Pb0 = 0.0, Pb1 = 0.0, P0b0 = 0.0, P0b1 = 0.0, mb = 0.0;
+  if (mI > tol) {
+    constexpr int W = Width<B>::W;
+    const int ni = n % G.nx, nj = n / G.nx;
+    const double iM = 1.0 / mI;
+    const double u[2] = {g.p[0][n] * iM, g.p[1][n] * iM};
+    const double a[2] = {(g.p[0][n] - g.p0[0][n]) * iM, (g.p[1][n] - g.p0[1][n]) * iM};
+    const Axis X = axis_of(G, 0), Y = axis_of(G, 1);
+    for (int bj = max(nj - W + 1, 0); bj <= nj; bj++)
+      for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
+        const int bin = bi + G.nx * bj;
+        const int e = start[bin + 1];
+        for (int c = start[bin]; c < e; c++) {
+          const int q = int(unsigned(sorted[c]));
+          const SolidDev& s = S[P.sid[q]];
+          double wx, d1x, d2x, wy, d1y, d2y;
+          axis_w2<B>(P.x[0][q], X, s.lp[0], ni + 1, wx, d1x, d2x);
+          axis_w2<B>(P.x[1][q], Y, s.lp[1], nj + 1, wy, d1y, d2y);
+          const double N = wx * wy, gx = d1x * wy, gy = wx * d1y;
+          const double xb[2] = {src.xb[0][q], src.xb[1][q]}, vb[2] = {src.vb[0][q], src.vb[1][q]};
+          const double Lb[4] = {src.Lb[0][q], src.Lb[1][q], src.Lb[2][q], src.Lb[3][q]};
+          const double ub0 = N * dt * xb[0] + Lb[0] * gx + Lb[2] * gy;
+          const double ub1 = N * dt * xb[1] + Lb[1] * gx + Lb[3] * gy;
+          const double ab0 = N * vb[0], ab1 = N * vb[1];
+          Pb0 += (ub0 + ab0) * iM;
+          Pb1 += (ub1 + ab1) * iM;
+          P0b0 -= ab0 * iM;
+          P0b1 -= ab1 * iM;
+          mb -= (u[0] * ub0 + u[1] * ub1 + a[0] * ab0 + a[1] * ab1) * iM;
+        }
+      }
+  }
+  an.mb[n] = mb;
+  an.Pb[0][n] = Pb0;
+  an.Pb[1][n] = Pb1;
+  an.P0b[0][n] = P0b0;
+  an.P0b[1][n] = P0b1;
 }
 
 // P = P0 + dt f; Dirichlet components of P0 and P are constants.
@@ -252,14 +318,27 @@ __global__ void __launch_bounds__(kBlock) k_fill_f64(double* __restrict__ a, dou
   if (i < n) a[i] = v;
 }
 
-__global__ void __launch_bounds__(kBlock) k_adj_zero_nodes(AdjNodes an, int nn) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
-  an.mb[n] = 0.0;
-  an.Pb[0][n] = 0.0;
-  an.Pb[1][n] = 0.0;
-  an.P0b[0][n] = 0.0;
-  an.P0b[1][n] = 0.0;
+// Per-solid sums of the accumulated lambda and mu adjoints and of the
+// initial-velocity adjoints: one block per solid over its particle range.
+__global__ void __launch_bounds__(kBlock) k_adj_solid_sums(const long long* __restrict__ range, AdjParticles ad,
+                                                           double* __restrict__ out) {
+  __shared__ double sh[4][kBlock];
+  const long long b = range[2 * blockIdx.x], e = range[2 * blockIdx.x + 1];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long long i = b + threadIdx.x; i < e; i += kBlock) {
+    acc[0] += ad.lam[i];
+    acc[1] += ad.mu[i];
+    acc[2] += ad.vb[0][i];
+    acc[3] += ad.vb[1][i];
+  }
+  for (int k = 0; k < 4; k++) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int k = 0; k < 4; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 }  // namespace jh

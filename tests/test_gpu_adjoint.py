@@ -72,4 +72,4 @@ def test_checkpoint_interval_does_not_change_the_gradient(hip_lib):
     for r in res[1:]:
         for k in ("E", "nu", "v0"):
             print(k, r[k], res[0][k])
-            assert np.allclose(r[k], res[0][k], rtol=1e-9, atol=0.0), k
+            assert np.array_equal(r[k], res[0][k]), k  # every adjoint sum has a fixed order
