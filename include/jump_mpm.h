@@ -53,6 +53,7 @@ typedef enum {
 #define JUMP_FLAG_OUT_OF_GRID 1u  /* stencil leaves the grid: particle skipped */
 #define JUMP_FLAG_NEGATIVE_J 2u   /* det(F) < 0 after the update              */
 #define JUMP_FLAG_NONFINITE 4u    /* NaN/Inf in updated particle state        */
+#define JUMP_FLAG_CONTACT_OVERFLOW 8u /* contact: a node touched by > 2 solids  */
 
 /* Basis types: Basis.jl:37-48 (LinearBasis, QuadBsplineBasis,
  * CubicBsplineBasis, CPDIQ4Basis) plus uGIMP, which the reference lacks. */
@@ -102,8 +103,23 @@ typedef struct {
   int64_t nx, ny, nz; /* node counts iN_x, iN_y, iN_z (nz ignored in 2D)   */
   double body[3];     /* ConstantBodyForce2D/3D g (BodyForce.jl:56-91)     */
   int32_t order;      /* jump_order (0: reference summation order)          */
-  int32_t reserved;   /* must be 0                                          */
+  int32_t contact;    /* 2D USL: 1 = multi-body contact between solids     */
+  double friction;    /* Coulomb coefficient of that contact               */
 } jump_params;
+
+/* Multi-body contact (contact = 1).  The reference's ULMPM step has none:
+ * all solids share one velocity field.  This is the algorithm of its FEMP
+ * contact solver (SolveExplicitFEMP2DContact.jl:311-387) on the MPM grid:
+ * each node keeps the mass, momenta, force and normal sum(m_p grad N) of at
+ * most two solids (in particle order; a third raises
+ * JUMP_FLAG_CONTACT_OVERFLOW); where two solids meet, the common normal is
+ * n = (n1 - n2)/|n1 - n2| (-n for the second), and a solid whose velocity
+ * relative to the centre-of-mass velocity approaches along its normal
+ * (D = (v - v_cm).n > 0) gets v -= D (n + mu'/|C| (n_y C, -n_x C)),
+ * C = (v - v_cm) x n, mu' = min(friction, |C|/D) (the tangential term is 0
+ * when C = 0, where the reference divides 0 by 0).  G2P then reads each
+ * particle's own solid's nodal velocities.  Rigid solids, batches and the
+ * reverse mode do not combine with contact. */
 
 /* Material parameters.  Lame constants are derived exactly as the reference
  * constructors do (Material.jl:54-57,112-113,159-161,313-325). */

@@ -23,6 +23,7 @@ JUMP_ERR_NUMERIC = -6
 JUMP_FLAG_OUT_OF_GRID = 1
 JUMP_FLAG_NEGATIVE_J = 2
 JUMP_FLAG_NONFINITE = 4
+JUMP_FLAG_CONTACT_OVERFLOW = 8
 
 BASIS_LINEAR = 0
 BASIS_QUAD_BSPLINE = 1
@@ -56,7 +57,7 @@ class JumpParams(ctypes.Structure):
         ("xmin", _d), ("xmax", _d), ("ymin", _d), ("ymax", _d), ("zmin", _d), ("zmax", _d),
         ("nx", _i64), ("ny", _i64), ("nz", _i64),
         ("body", _d * 3),
-        ("order", _i32), ("reserved", _i32),
+        ("order", _i32), ("contact", _i32), ("friction", _d),
     ]
 
 

@@ -51,6 +51,7 @@ struct jump_ctx {
   // device memory
   Particles P{};
   GridArrays g{};
+  ContactArrays cg{};  // contact mode only
   std::vector<void*> part_allocs;
   std::vector<void*> grid_allocs;
   double* d_corners = nullptr;
@@ -321,6 +322,15 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_sums, (nn + 1) / kScanItems + 2);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_flags, 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_wmax, 1);
+  if (ctx->prm.contact) {
+    ContactArrays& c = ctx->cg;
+    for (int k = 0; k < 2 && !r; k++) {
+      r = dev_alloc(ctx, ctx->grid_allocs, &c.body[k], nn);
+      if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.m[k], nn);
+      for (int a = 0; a < 2 && !r; a++) r = dev_alloc(ctx, ctx->grid_allocs, &c.v0[k][a], nn);
+      for (int a = 0; a < 2 && !r; a++) r = dev_alloc(ctx, ctx->grid_allocs, &c.v[k][a], nn);
+    }
+  }
   if (!r && ctx->ninst > 1) {
     r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_inst, ctx->hinst.size());
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_node_off, ctx->hnode_off.size());
@@ -437,6 +447,24 @@ template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_RIGID);
 }
 
+// 2D USL with multi-body contact: P2G per solid slot, grid update and
+// contact correction in one gather; G2P from each particle's own slot.
+template <int B, int ORDER> static void step_usl2d_contact(jump_ctx* ctx) {
+  const int np = int(ctx->np);
+  const double dt = ctx->prm.dt;
+  prof_begin(ctx);
+  bin_and_sort<2, B, false>(ctx);
+  prof_end(ctx, JUMP_PHASE_SORT);
+  prof_begin(ctx);
+  LAUNCH_P2G((k_p2g_contact<B, ORDER>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
+             ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_P2G);
+  prof_begin(ctx);
+  LAUNCH((k_g2p_usl2d_contact<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->cg, ctx->prm.tolerance, dt,
+         ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_G2P);
+}
+
 template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
@@ -482,7 +510,10 @@ static int run_steps(jump_ctx* ctx, int64_t nsteps) {
       switch (ctx->prm.basis) {
 #define USL_CASE(BB)                                                     \
   case BB:                                                               \
-    if (ctx->ninst > 1) {                                                \
+    if (ctx->prm.contact) {                                              \
+      if (ref) step_usl2d_contact<BB, JUMP_ORDER_REFERENCE>(ctx);        \
+      else step_usl2d_contact<BB, JUMP_ORDER_BINNED>(ctx);               \
+    } else if (ctx->ninst > 1) {                                         \
       if (ref) step_usl2d<BB, JUMP_ORDER_REFERENCE, true>(ctx);          \
       else step_usl2d<BB, JUMP_ORDER_BINNED, true>(ctx);                 \
     } else {                                                             \
@@ -535,6 +566,9 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   if (!(p->dt > 0.0)) return JUMP_ERR_ARG;
   if (p->order != JUMP_ORDER_REFERENCE && p->order != JUMP_ORDER_BINNED) return JUMP_ERR_ARG;
   if (p->nx >= 65536 || p->ny >= 65536 || nz >= 65536) return JUMP_ERR_UNSUPPORTED;
+  if (p->contact != 0 && p->contact != 1) return JUMP_ERR_ARG;
+  if (p->contact && p->dim != 2) return JUMP_ERR_UNSUPPORTED;  // the 2D USL step only
+  if (p->contact && !(p->friction >= 0.0 && std::isfinite(p->friction))) return JUMP_ERR_ARG;
   jump_ctx* ctx = new (std::nothrow) jump_ctx();
   if (!ctx) return JUMP_ERR_NOMEM;
   ctx->prm = *p;
@@ -602,6 +636,8 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
   if (mat->kind < JUMP_MAT_RIGID || mat->kind > JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_ARG, "bad material");
   if (D == 3 && mat->kind == JUMP_MAT_RIGID)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid (Solid3D has no rigid flag)");
+  if (ctx->prm.contact && mat->kind == JUMP_MAT_RIGID)
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "rigid solids do not combine with contact");
   if (D == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");
   if (ctx->np + ds->n > kMaxParticles) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "too many particles (at most 2^28 per context)");
   const bool cpdi = ctx->prm.basis == JUMP_BASIS_CPDI_Q4;
@@ -1035,7 +1071,7 @@ int jump_batch_create(jump_ctx* const* ctxs, int32_t n, int device, jump_batch**
     const jump_ctx* c = ctxs[i];
     if (!c) return JUMP_ERR_ARG;
     if (c->D != 2 || c->prm.algorithm != JUMP_ALG_USL) return JUMP_ERR_UNSUPPORTED;  // batches are 2D USL
-    if (c->prm.basis == JUMP_BASIS_CPDI_Q4) return JUMP_ERR_UNSUPPORTED;
+    if (c->prm.basis == JUMP_BASIS_CPDI_Q4 || c->prm.contact) return JUMP_ERR_UNSUPPORTED;
     if (c->device != device) return JUMP_ERR_ARG;
   }
   jump_batch* b = new (std::nothrow) jump_batch();
@@ -1240,6 +1276,7 @@ static int snap_copy(jump_ctx* ctx, double* buf, bool save) {
 static int adjoint_supported(jump_ctx* ctx) {
   if (ctx->D != 2 || ctx->prm.algorithm != JUMP_ALG_USL || ctx->ninst != 1)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the 2D USL step");
+  if (ctx->prm.contact) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint does not cover contact");
   if (ctx->prm.basis != JUMP_BASIS_LINEAR && ctx->prm.basis != JUMP_BASIS_GIMP)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the linear and GIMP bases");
   for (auto& s : ctx->solids)

@@ -91,6 +91,17 @@ struct GridArrays {
   const uint8_t* fixed;  // bit k set: direction k fixed
 };
 
+// Multi-body contact: each node keeps up to two solids ("slots", filled in
+// particle order).  body[k][n]: solid index in slot k (-1: empty); m: the
+// slot's mass; v0/v: its velocity before / after the force update, v after
+// the contact correction.  [slot][component][node].
+struct ContactArrays {
+  int* body[2];
+  double* m[2];
+  double* v0[2][2];
+  double* v[2][2];
+};
+
 // A batch steps several independent 2D problems with the same launches.
 // Instance i has its own grid, whose nodes (and bins) occupy
 // [node_off[i], node_off[i+1]) of the concatenated node arrays, and its own

@@ -370,6 +370,84 @@ __global__ void __launch_bounds__(kBlock) k_node_order(const int* __restrict__ p
   order[(pos[n + 1] - p) ? p : nact + n - p] = n;
 }
 
+// Calls visit(p) for every particle whose stencil covers node (ni,nj,nk):
+// the particles of the window's W^d bins, in ascending index for
+// JUMP_ORDER_REFERENCE (k-way merge of the bins, heads and cursors in the
+// LDS arrays s_cur/s_head of this thread's column), bin after bin for
+// JUMP_ORDER_BINNED.  start: the node's own instance's bins; off: that
+// instance's first node; sorted/bxyz: all particles.
+template <int D, int B, int ORDER, class Visit>
+__device__ __forceinline__ void for_window_particles(const Particles& P, const GridDesc& G, const int* start,
+                                                     const unsigned long long* sorted,
+                                                     const unsigned long long* bxyz, int off, int inst, int W,
+                                                     int ni, int nj, int nk, int (*s_cur)[kP2GBlock],
+                                                     int (*s_head)[kP2GBlock], Visit&& visit) {
+  constexpr int MW = MaxW<B>::W;
+  constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
+  const int t = threadIdx.x;
+  if (ORDER == JUMP_ORDER_BINNED) {
+    const int k0 = (D == 3) ? max(nk - W + 1, 0) : 0;
+    for (int bk = k0; bk <= nk; bk++)
+      for (int bj = max(nj - W + 1, 0); bj <= nj; bj++)
+        for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
+          const int bin = bi + G.nx * (bj + G.ny * bk);
+          const int e = ldu(start, unsigned(bin + 1));
+          for (int q = ldu(start, unsigned(bin)); q < e; q++)
+            visit(int(unsigned(ldu(sorted, unsigned(q)))));
+        }
+  } else {
+    const int wx0 = ni - W + 1, wy0 = nj - W + 1, wz0 = (D == 3) ? nk - W + 1 : 0;  // window origin bin
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
+      int c = 0, h = kNoParticle;
+      if (a0 < W && a1 < W && a2 < W && wx0 + a0 >= 0 && wy0 + a1 >= 0 && wz0 + a2 >= 0) {
+        const int bin = (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2));
+        c = ldu(start, unsigned(bin));
+        if (c < ldu(start, unsigned(bin + 1))) h = int(unsigned(ldu(sorted, unsigned(c))));
+      }
+      s_cur[b][t] = c;
+      s_head[b][t] = h;
+    }
+    while (true) {
+      // the smallest head is the window's next particle in index order
+      int best = kNoParticle;
+#pragma unroll
+      for (int b = 0; b < NB; b++) best = min(best, s_head[b][t]);
+      if (best == kNoParticle) break;
+      int b = 0;
+#pragma unroll
+      for (int k = NB - 1; k >= 0; k--)
+        if (s_head[k][t] == best) b = k;
+      int p = best;
+      while (true) {
+        // p is the head of window bin b: consume it and advance the bin
+        const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
+        const unsigned bin = unsigned(off + (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2)));
+        const int c = s_cur[b][t] + 1;
+        const unsigned long long e = ldu(sorted, unsigned(c));  // sorted[np] is a sentinel
+        s_cur[b][t] = c;
+        visit(p);
+        s_head[b][t] = unsigned(e >> 32) == bin ? int(unsigned(e)) : kNoParticle;
+        // is p+1 in the window?  (bin coordinates packed 21 bits per axis;
+        // excluded particles hold all ones)
+        const int q = p + 1;
+        if (q >= P.n) break;
+        const unsigned long long cc = ldu(bxyz, unsigned(q));
+        const unsigned ux = unsigned(int(cc & 0x1fffffu) - wx0);
+        const unsigned uy = unsigned(int((cc >> 21) & 0x1fffffu) - wy0);
+        const unsigned uz = unsigned(int((cc >> 42) & 0x1fffffu) - wz0);
+        // 2D: the third field is the batch instance and must be this node's
+        if (!(ux < unsigned(W) && uy < unsigned(W) && (D == 2 ? uz == unsigned(inst) : uz < unsigned(W))) ||
+            cc == ~0ull)
+          break;
+        b = int(ux) + MW * (int(uy) + (D == 3 ? MW * int(uz) : 0));
+        p = q;
+      }
+    }
+  }
+}
+
 template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI>
 __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
                                                    const double* __restrict__ corners, const int* __restrict__ start0,
@@ -411,67 +489,8 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
   double am = 0.0, ap[D], af[D];
   for (int k = 0; k < D; k++) { ap[k] = 0.0; af[k] = 0.0; }
-  if (ORDER == JUMP_ORDER_BINNED) {
-    const int k0 = (D == 3) ? max(nk - W + 1, 0) : 0;
-    for (int bk = k0; bk <= nk; bk++)
-      for (int bj = max(nj - W + 1, 0); bj <= nj; bj++)
-        for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
-          const int bin = bi + G.nx * (bj + G.ny * bk);
-          const int e = ldu(start, unsigned(bin + 1));
-          for (int q = ldu(start, unsigned(bin)); q < e; q++)
-            p2g_add<D, B, PASS>(P, G, S, corners, int(unsigned(ldu(sorted, unsigned(q)))), ni, nj, nk, am, ap, af);
-        }
-  } else {
-    const int wx0 = ni - W + 1, wy0 = nj - W + 1, wz0 = (D == 3) ? nk - W + 1 : 0;  // window origin bin
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-      const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
-      int c = 0, h = kNoParticle;
-      if (a0 < W && a1 < W && a2 < W && wx0 + a0 >= 0 && wy0 + a1 >= 0 && wz0 + a2 >= 0) {
-        const int bin = (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2));
-        c = ldu(start, unsigned(bin));
-        if (c < ldu(start, unsigned(bin + 1))) h = int(unsigned(ldu(sorted, unsigned(c))));
-      }
-      s_cur[b][t] = c;
-      s_head[b][t] = h;
-    }
-    while (true) {
-      // the smallest head is the window's next particle in index order
-      int best = kNoParticle;
-#pragma unroll
-      for (int b = 0; b < NB; b++) best = min(best, s_head[b][t]);
-      if (best == kNoParticle) break;
-      int b = 0;
-#pragma unroll
-      for (int k = NB - 1; k >= 0; k--)
-        if (s_head[k][t] == best) b = k;
-      int p = best;
-      while (true) {
-        // p is the head of window bin b: consume it and advance the bin
-        const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
-        const unsigned bin = unsigned(off + (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2)));
-        const int c = s_cur[b][t] + 1;
-        const unsigned long long e = ldu(sorted, unsigned(c));  // sorted[np] is a sentinel
-        s_cur[b][t] = c;
-        p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af);
-        s_head[b][t] = unsigned(e >> 32) == bin ? int(unsigned(e)) : kNoParticle;
-        // is p+1 in the window?  (bin coordinates packed 21 bits per axis;
-        // excluded particles hold all ones)
-        const int q = p + 1;
-        if (q >= P.n) break;
-        const unsigned long long cc = ldu(bxyz, unsigned(q));
-        const unsigned ux = unsigned(int(cc & 0x1fffffu) - wx0);
-        const unsigned uy = unsigned(int((cc >> 21) & 0x1fffffu) - wy0);
-        const unsigned uz = unsigned(int((cc >> 42) & 0x1fffffu) - wz0);
-        // 2D: the third field is the batch instance and must be this node's
-        if (!(ux < unsigned(W) && uy < unsigned(W) && (D == 2 ? uz == unsigned(inst) : uz < unsigned(W))) ||
-            cc == ~0ull)
-          break;
-        b = int(ux) + MW * (int(uy) + (D == 3 ? MW * int(uz) : 0));
-        p = q;
-      }
-    }
-  }
+  for_window_particles<D, B, ORDER>(P, G, start, sorted, bxyz, off, inst, W, ni, nj, nk, s_cur, s_head,
+                                   [&](int p) { p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af); });
   if (PASS == 0) {
     const uint8_t fx = g.fixed[n];
     g.m[n] = am;
@@ -485,6 +504,142 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
     }
   } else {
     for (int k = 0; k < D; k++) g.p2[k][n] = ap[k];
+  }
+}
+
+// ------------------------------------------------------------------ contact (2D)
+// P2G and grid update with multi-body contact, the algorithm of the FEMP
+// contact solver (SolveExplicitFEMP2DContact.jl:189-268 per-solid P2G with
+// normals sum(m_p grad N) at :237; :311-387 the correction) on the USL grid.
+// One thread per node gathers its window's particles in index order (the
+// solids' order); the first solid reaching the node takes slot 0, the next
+// slot 1, a third is dropped and flagged.  Per slot: mass, momentum0, force
+// (as the single-field pass) and normal; then p = p0 + f dt, the Dirichlet
+// mask, v0 = p0/m, v = p/m.  At a node with two solids the common normal is
+// n = (n0 - n1)/|n0 - n1| (slot 1 uses -n) and each slot whose velocity
+// approaches the centre-of-mass velocity v_cm = (p0 + p1)/(m0 + m1) along
+// its normal (D = (v - v_cm).n > 0) gets v -= D (n + mu'/|C| (n_y C, -n_x C)),
+// C = (v - v_cm) x n, mu' = min(mu, |C|/D) (:366-381; no tangential term
+// when C = 0).  The single-field arrays receive the slot totals.
+struct ContactSlot {
+  int body = -1;
+  double m = 0.0, p0[2] = {0.0, 0.0}, f[2] = {0.0, 0.0}, nr[2] = {0.0, 0.0};
+};
+
+template <int B, int ORDER>
+__global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                           const double* __restrict__ corners,
+                                                           const int* __restrict__ start,
+                                                           const unsigned long long* __restrict__ sorted,
+                                                           const unsigned long long* __restrict__ bxyz, GridArrays g,
+                                                           ContactArrays c, int Wrt, double dt, double fric,
+                                                           unsigned* __restrict__ flags) {
+  constexpr int MW = MaxW<B>::W;
+  constexpr int NB = MW * MW;
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  const int n = blockIdx.x * kP2GBlock + threadIdx.x;
+  if (n >= G.nn) return;
+  const int ni = n % G.nx, nj = n / G.nx;
+  const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
+  ContactSlot s0, s1;
+  bool overflow = false;
+  auto add = [&](ContactSlot& s, int sid, double N, const double* dN, const double* v) {
+    const double fM = v[2], fV = v[3];
+    const double* vp = v + 4;
+    const double* sg = v + 6;
+    const double Nim = N * fM;
+    s.body = sid;
+    s.m += Nim;
+    s.p0[0] += Nim * vp[0];
+    s.p0[1] += Nim * vp[1];
+    s.f[0] += (-fV) * (sg[0] * dN[0] + sg[2] * dN[1]) + (fM * G.body[0]) * N;
+    s.f[1] += (-fV) * (sg[2] * dN[0] + sg[1] * dN[1]) + (fM * G.body[1]) * N;
+    s.nr[0] += fM * dN[0];
+    s.nr[1] += fM * dN[1];
+  };
+  for_window_particles<2, B, ORDER>(P, G, start, sorted, bxyz, 0, 0, W, ni, nj, 0, s_cur, s_head, [&](int p) {
+    double v[PairLayout<2>::NF + 1];
+    P2GFields<2, 0>::load(P, unsigned(p), v);
+    double lp[3] = {0.0, 0.0, 0.0};
+    const int sid = ldu(P.sid, unsigned(p));
+    if (B == JUMP_BASIS_GIMP) {
+      lp[0] = S[sid].lp[0];
+      lp[1] = S[sid].lp[1];
+    }
+    double N, dN[3];
+    bool member;
+    node_weight<2, B, false, true>(P, G, lp, corners, p, v, ni + 1, nj + 1, 1, N, dN, member);
+    if (!member) return;
+    if (s0.body < 0 || s0.body == sid) add(s0, sid, N, dN, v);
+    else if (s1.body < 0 || s1.body == sid) add(s1, sid, N, dN, v);
+    else overflow = true;
+  });
+  if (overflow) atomicOr(flags, JUMP_FLAG_CONTACT_OVERFLOW);
+  const uint8_t fx = g.fixed[n];
+  double pk[2][2], v[2][2];
+  ContactSlot* sl[2] = {&s0, &s1};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    ContactSlot& s = *sl[k];
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      pk[k][a] = s.p0[a] + s.f[a] * dt;
+      if (fx & (1u << a)) { s.p0[a] = 0.0; pk[k][a] = 0.0; }
+    }
+    double v0[2] = {0.0, 0.0};
+    v[k][0] = 0.0;
+    v[k][1] = 0.0;
+    if (s.body >= 0) {
+      const double invM = 1.0 / s.m;
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        v0[a] = s.p0[a] * invM;
+        v[k][a] = pk[k][a] * invM;
+      }
+    }
+    c.body[k][n] = s.body;
+    c.m[k][n] = s.m;
+    c.v0[k][0][n] = v0[0];
+    c.v0[k][1][n] = v0[1];
+  }
+  g.m[n] = s0.m + s1.m;
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    g.p0[a][n] = s0.p0[a] + s1.p0[a];
+    g.p[a][n] = pk[0][a] + pk[1][a];
+    g.f[a][n] = s0.f[a] + s1.f[a];
+  }
+  if (s0.body >= 0 && s1.body >= 0) {
+    const double mS = s0.m + s1.m;
+    const double vcm[2] = {(pk[0][0] + pk[1][0]) / mS, (pk[0][1] + pk[1][1]) / mS};
+    double nc[2] = {s0.nr[0] - s1.nr[0], s0.nr[1] - s1.nr[1]};
+    const double nrm = sqrt(nc[0] * nc[0] + nc[1] * nc[1]);
+    nc[0] = nc[0] / nrm;
+    nc[1] = nc[1] / nrm;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const double nk[2] = {k == 0 ? nc[0] : -nc[0], k == 0 ? nc[1] : -nc[1]};
+      const double dv[2] = {v[k][0] - vcm[0], v[k][1] - vcm[1]};
+      const double Dn = dv[0] * nk[0] + dv[1] * nk[1];
+      if (!(Dn > 0.0)) continue;
+      const double C = dv[0] * nk[1] - dv[1] * nk[0];
+      const double absC = C < 0.0 ? -C : C;
+      const double mup = fric < absC / Dn ? fric : absC / Dn;
+      double tx = 0.0, ty = 0.0;
+      if (absC > 0.0) {
+        const double cc = mup / absC;
+        tx = cc * (nk[1] * C);
+        ty = cc * (-(nk[0] * C));
+      }
+      v[k][0] = v[k][0] - Dn * (nk[0] + tx);
+      v[k][1] = v[k][1] - Dn * (nk[1] + ty);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    c.v[k][0][n] = v[k][0];
+    c.v[k][1][n] = v[k][1];
   }
 }
 
@@ -634,42 +789,12 @@ __device__ __forceinline__ bool walk_stencil(const Particles& P, const GridDesc&
 __device__ __forceinline__ bool finite3(double a, double b, double c) { return isfinite(a) && isfinite(b) && isfinite(c); }
 
 // ------------------------------------------------------------------ G2P, USL 2D
-// SolveExplicitSolidMechanics2D.jl:485-534
-template <int B, bool MULTI>
-__global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
-                                                      const double* __restrict__ corners, GridArrays g0, double tol0,
-                                                      double dt0, unsigned* __restrict__ flags, BatchDev Bt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const SolidDev& s = S[P.sid[i]];
-  if (s.rigid) return;
-  const GridDesc& G = MULTI ? Bt.inst[s.inst].G : G0;
-  const GridArrays g = MULTI ? grid_view(g0, Bt.node_off[s.inst]) : g0;
-  const double tol = MULTI ? Bt.inst[s.inst].tol : tol0;
-  const double dt = MULTI ? Bt.inst[s.inst].dt : dt0;
-  double v0 = P.v[0][i], v1 = P.v[1][i];
-  double x0 = P.x[0][i], x1 = P.x[1][i];
-  double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
-  bool ok = walk_stencil<2, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
-    const double mI = g.m[n];
-    if (mI > tol) {
-      const double invM = 1.0 / mI;
-      const double pn0 = g.p[0][n], pn1 = g.p[1][n];
-      const double vI0 = pn0 * invM, vI1 = pn1 * invM;
-      v0 += Ni * (pn0 - g.p0[0][n]) * invM;
-      v1 += Ni * (pn1 - g.p0[1][n]) * invM;
-      x0 += Ni * vI0 * dt;
-      x1 += Ni * vI1 * dt;
-      L0 += dN[0] * vI0;
-      L1 += dN[0] * vI1;
-      L2 += dN[1] * vI0;
-      L3 += dN[1] * vI1;
-    }
-  });
-  if (!ok) {
-    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
-    return;
-  }
+// The particle update after the USL 2D gather (SolveExplicitSolidMechanics2D.jl:
+// 506-532): new v and x, strain += dt D, F = F (I + L dt), V = det(F) V0,
+// stress.  L = (L0, L1, L2, L3) = (dN_x v_x, dN_x v_y, dN_y v_x, dN_y v_y).
+__device__ __forceinline__ void usl2d_particle_update(const Particles& P, const SolidDev& s, int i, double dt,
+                                                      double v0, double v1, double x0, double x1, double L0,
+                                                      double L1, double L2, double L3, unsigned* flags) {
   P.v[0][i] = v0; P.v[1][i] = v1;
   P.x[0][i] = x0; P.x[1][i] = x1;
   // D = (L + L')/2, strain += dt D, F *= (I + L dt), J = det F, V = J V0
@@ -706,6 +831,81 @@ __global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, 
   P.sig[0][i] = sg[0]; P.sig[1][i] = sg[1]; P.sig[2][i] = sg[2];
   if (!(isfinite(x0) && isfinite(x1) && isfinite(v0) && isfinite(v1))) fl |= JUMP_FLAG_NONFINITE;
   if (fl) atomicOr(flags, fl);
+}
+
+// SolveExplicitSolidMechanics2D.jl:485-534
+template <int B, bool MULTI>
+__global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
+                                                      const double* __restrict__ corners, GridArrays g0, double tol0,
+                                                      double dt0, unsigned* __restrict__ flags, BatchDev Bt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  if (s.rigid) return;
+  const GridDesc& G = MULTI ? Bt.inst[s.inst].G : G0;
+  const GridArrays g = MULTI ? grid_view(g0, Bt.node_off[s.inst]) : g0;
+  const double tol = MULTI ? Bt.inst[s.inst].tol : tol0;
+  const double dt = MULTI ? Bt.inst[s.inst].dt : dt0;
+  double v0 = P.v[0][i], v1 = P.v[1][i];
+  double x0 = P.x[0][i], x1 = P.x[1][i];
+  double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  bool ok = walk_stencil<2, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
+    const double mI = g.m[n];
+    if (mI > tol) {
+      const double invM = 1.0 / mI;
+      const double pn0 = g.p[0][n], pn1 = g.p[1][n];
+      const double vI0 = pn0 * invM, vI1 = pn1 * invM;
+      v0 += Ni * (pn0 - g.p0[0][n]) * invM;
+      v1 += Ni * (pn1 - g.p0[1][n]) * invM;
+      x0 += Ni * vI0 * dt;
+      x1 += Ni * vI1 * dt;
+      L0 += dN[0] * vI0;
+      L1 += dN[0] * vI1;
+      L2 += dN[1] * vI0;
+      L3 += dN[1] * vI1;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  usl2d_particle_update(P, s, i, dt, v0, v1, x0, x1, L0, L1, L2, L3, flags);
+}
+
+// G2P with contact: each particle reads the slot of its own solid at every
+// support node (SolveExplicitFEMP2DContact.jl:393-426 form: v += N (v_I -
+// v0_I), x += N v_I dt), then the same particle update.
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p_usl2d_contact(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                              const double* __restrict__ corners, ContactArrays c,
+                                                              double tol, double dt, unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int sid = P.sid[i];
+  const SolidDev& s = S[sid];
+  double v0 = P.v[0][i], v1 = P.v[1][i];
+  double x0 = P.x[0][i], x1 = P.x[1][i];
+  double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  bool ok = walk_stencil<2, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
+    const int k = c.body[0][n] == sid ? 0 : (c.body[1][n] == sid ? 1 : -1);
+    if (k < 0) return;
+    if (c.m[k][n] > tol) {
+      const double vI0 = c.v[k][0][n], vI1 = c.v[k][1][n];
+      v0 += Ni * (vI0 - c.v0[k][0][n]);
+      v1 += Ni * (vI1 - c.v0[k][1][n]);
+      x0 += Ni * vI0 * dt;
+      x1 += Ni * vI1 * dt;
+      L0 += dN[0] * vI0;
+      L1 += dN[0] * vI1;
+      L2 += dN[1] * vI0;
+      L3 += dN[1] * vI1;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  usl2d_particle_update(P, s, i, dt, v0, v1, x0, x1, L0, L1, L2, L3, flags);
 }
 
 // ------------------------------------------------------------------ G2P, MUSL 3D

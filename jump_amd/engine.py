@@ -18,7 +18,7 @@ class JumpError(RuntimeError):
 
 
 def make_params(dim, basis, algorithm, dt, lo, hi, nodes, tolerance=0.0, alpha=1.0, body=(0.0, 0.0, 0.0),
-                ghostcell=False, order=_ffi.ORDER_REFERENCE):
+                ghostcell=False, order=_ffi.ORDER_REFERENCE, contact=False, friction=0.0):
     p = _ffi.JumpParams()
     p.dim = dim
     p.basis = basis
@@ -36,7 +36,8 @@ def make_params(dim, basis, algorithm, dt, lo, hi, nodes, tolerance=0.0, alpha=1
     for k in range(3):
         p.body[k] = body[k] if k < len(body) else 0.0
     p.order = order
-    p.reserved = 0
+    p.contact = 1 if contact else 0
+    p.friction = friction
     return p
 
 

@@ -5,6 +5,8 @@ A Scene holds ABI parameters and per-solid arrays; `load(engine)` pushes them
 into any Engine (the HIP library, or the oracle in tests).
 
   two_disks      Main_TwoDisks.jl:40-116 at 20x20 cells, linear basis
+  disks_contact  two disks colliding head-on or obliquely, with or without
+                 multi-body contact
   impact3d       3D high-velocity impact: elastic sphere on a J2 plate,
                  quadratic B-splines, MUSL (Main_HighVelocity_Impact_3D.jl)
   taylor_bar     3D Taylor bar (Main_TaylorBar.jl), J2 copper, GIMP
@@ -118,6 +120,34 @@ def two_disks(cells=20, ppc=2, steps=3500, basis=_ffi.BASIS_LINEAR, material="el
     s2 = _solid(mat, c2, (-0.1, -0.1), vol, 1000.0, lp=lp)
     p = make_params(2, basis, _ffi.ALG_USL, 1e-3, (0.0, 0.0), (1.0, 1.0), (n, n), tolerance=1e-9)
     return Scene("two_disks", p, [s1, s2], steps=steps, meta=dict(interval=10, total_time=3.5))
+
+
+def disks_contact(cells=32, ppc=2, r=0.15, gap_cells=2, v=0.1, offset=0.0, basis=_ffi.BASIS_LINEAR,
+                  material="elastic", contact=True, friction=0.0, steps=1000, dt=1e-3):
+    """Two Two-Disks disks (Main_TwoDisks.jl:40-116 material: E=1000,
+    nu=0.3, rho=1000) on a horizontal line of the unit square, `gap_cells`
+    cells apart, moving towards each other at +-v; `offset` shifts the left
+    disk down and the right disk up (an oblique impact).  Centres sit on grid
+    nodes so that no particle lies on a grid line.  With contact on, the
+    disks interact only through the contact correction
+    (SolveExplicitFEMP2DContact.jl:311-387)."""
+    n = cells + 1
+    dx = 1.0 / cells
+    fOffset = dx / ppc
+    rr = math.floor(r / fOffset) * fOffset
+    half = round((rr + 0.5 * gap_cells * dx) / dx) * dx
+    dyc = round(0.5 * offset / dx) * dx
+    c1 = buildParticleForCircle([0.5 - half, 0.5 - dyc], r, fOffset)
+    c2 = buildParticleForCircle([0.5 + half, 0.5 + dyc], r, fOffset)
+    mat = {"elastic": elastic, "neohookean": neohookean}[material](1000.0, 0.3, 1000.0) \
+        if material != "j2" else j2(1000.0, 0.3, 1000.0, 5.0, 100.0)
+    vol = fOffset * fOffset
+    lp = 0.5 * fOffset if basis == _ffi.BASIS_GIMP else None
+    s1 = _solid(mat, c1, (v, 0.0), vol, 1000.0, lp=lp)
+    s2 = _solid(mat, c2, (-v, 0.0), vol, 1000.0, lp=lp)
+    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (1.0, 1.0), (n, n), tolerance=1e-9, contact=contact,
+                    friction=friction)
+    return Scene("disks_contact", p, [s1, s2], steps=steps)
 
 
 def rings2d(cells=40, basis=_ffi.BASIS_LINEAR, material="neohookean", platen=True, steps=50, v=2.0, seed=0):

@@ -107,6 +107,11 @@ template <class T> struct Ctx {
   std::vector<uint8_t> fixed;              // dim*nn
   std::vector<Solid<T>> solids;
   uint32_t flags = 0;
+  bool contact = false;  // multi-body contact (2D USL)
+  T fric;
+  // contact: per node two solid slots
+  std::vector<int> cbody;                    // 2*nn, -1 empty
+  std::vector<T> cm, cp0, cf, cn, cv0, cv;   // 2*nn, 4*nn (slot, component)
   std::string err;
   // scratch for the support (Basis.jl initialise, :1658-1727)
   std::vector<i64> near;
@@ -862,6 +867,161 @@ template <class T> void step_usl_2d(Ctx<T>& g, const T& t) {
     }
   }
   // CPDI corner update (:556-572) only moves a local copy: no effect.
+  (void)t;
+}
+
+// Multi-body contact in the 2D USL step (optional; the reference's ULMPM
+// step has none).  The algorithm of SolveExplicitFEMP2DContact.jl:311-387 on
+// the MPM grid: per node the fields of at most two solids, taken in particle
+// order; where both are present the common normal and the approach
+// correction; G2P reads each particle's own solid's velocities.
+// Floating-point order (the GPU follows it term by term):
+//   slot sums as the P2G sums above, plus normal += (fM * dN)
+//   p = p0 + f dt; Dirichlet zeroes p0, p; invM = 1/m; v0 = p0 invM; v = p invM
+//   mS = m_a + m_b; pS = p_a + p_b; vcm = pS / mS
+//   n = n_a - n_b; n /= sqrt(n.n)   (slot b uses -n)
+//   dv = v - vcm; D = dv.n; C = dv_x n_y - dv_y n_x; mu' = min(fric, |C|/D)
+//   v = v - D (n + (mu'/|C|) (n_y C, -(n_x C)))   (tangential 0 if C == 0)
+template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
+  const T dt = g.dt;
+  const i64 nn = g.nn;
+  g.cbody.assign(size_t(2 * nn), -1);
+  g.cm.assign(size_t(2 * nn), T(0.0));
+  g.cp0.assign(size_t(4 * nn), T(0.0));
+  g.cf.assign(size_t(4 * nn), T(0.0));
+  g.cn.assign(size_t(4 * nn), T(0.0));
+  g.cv0.assign(size_t(4 * nn), T(0.0));
+  g.cv.assign(size_t(4 * nn), T(0.0));
+  grid_reset(g, false);
+  for (size_t si = 0; si < g.solids.size(); si++) {
+    auto& s = g.solids[si];
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, true);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T fV = s.vol[ip], fM = s.mass[ip];
+      const T* vp = &s.vel[2 * ip];
+      const T* sg = &s.stress[4 * ip];
+      T b0 = fM * g.body[0], b1 = fM * g.body[1];
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        int k;
+        if (g.cbody[2 * in] < 0 || g.cbody[2 * in] == int(si)) k = 0;
+        else if (g.cbody[2 * in + 1] < 0 || g.cbody[2 * in + 1] == int(si)) k = 1;
+        else { g.flags |= JUMP_FLAG_CONTACT_OVERFLOW; continue; }
+        g.cbody[2 * in + k] = int(si);
+        T Ni = g.funcs[i];
+        T d1 = g.ders[2 * i], d2 = g.ders[2 * i + 1];
+        T Nim = Ni * fM;
+        const i64 c = 2 * in + k, c2 = 4 * in + 2 * k;
+        g.cm[c] += Nim;
+        g.cp0[c2] += Nim * vp[0];
+        g.cp0[c2 + 1] += Nim * vp[1];
+        g.cf[c2] += (-fV) * (sg[0] * d1 + sg[2] * d2) + b0 * Ni;
+        g.cf[c2 + 1] += (-fV) * (sg[1] * d1 + sg[3] * d2) + b1 * Ni;
+        g.cn[c2] += fM * d1;
+        g.cn[c2 + 1] += fM * d2;
+      }
+    }
+  }
+  for (i64 i = 0; i < nn; i++) {
+    T p[2][2];
+    for (int k = 0; k < 2; k++) {
+      const i64 c2 = 4 * i + 2 * k;
+      for (int a = 0; a < 2; a++) {
+        p[k][a] = g.cp0[c2 + a] + g.cf[c2 + a] * dt;
+        if (g.fixed[2 * i + a]) { g.cp0[c2 + a] = 0.0; p[k][a] = 0.0; }
+      }
+      if (g.cbody[2 * i + k] >= 0) {
+        T invM = 1.0 / g.cm[2 * i + k];
+        for (int a = 0; a < 2; a++) {
+          g.cv0[c2 + a] = g.cp0[c2 + a] * invM;
+          g.cv[c2 + a] = p[k][a] * invM;
+        }
+      }
+    }
+    // totals (the grid fields a caller reads back)
+    g.gmass[i] = g.cm[2 * i] + g.cm[2 * i + 1];
+    for (int a = 0; a < 2; a++) {
+      g.gp0[2 * i + a] = g.cp0[4 * i + a] + g.cp0[4 * i + 2 + a];
+      g.gp[2 * i + a] = p[0][a] + p[1][a];
+      g.gf[2 * i + a] = g.cf[4 * i + a] + g.cf[4 * i + 2 + a];
+    }
+    if (g.cbody[2 * i] < 0 || g.cbody[2 * i + 1] < 0) continue;
+    // contact node
+    const T mS = g.cm[2 * i] + g.cm[2 * i + 1];
+    const T vcm[2] = {(p[0][0] + p[1][0]) / mS, (p[0][1] + p[1][1]) / mS};
+    T nc[2] = {g.cn[4 * i] - g.cn[4 * i + 2], g.cn[4 * i + 1] - g.cn[4 * i + 3]};
+    const T nrm = sqrt(nc[0] * nc[0] + nc[1] * nc[1]);
+    nc[0] = nc[0] / nrm;
+    nc[1] = nc[1] / nrm;
+    for (int k = 0; k < 2; k++) {
+      const T nk[2] = {k == 0 ? nc[0] : -nc[0], k == 0 ? nc[1] : -nc[1]};
+      T* v = &g.cv[4 * i + 2 * k];
+      const T dv[2] = {v[0] - vcm[0], v[1] - vcm[1]};
+      const T D = dv[0] * nk[0] + dv[1] * nk[1];
+      if (!(val(D) > 0.0)) continue;
+      const T C = dv[0] * nk[1] - dv[1] * nk[0];
+      const T absC = C < 0.0 ? -C : C;
+      const T mup = val(g.fric) < val(absC / D) ? g.fric : absC / D;
+      T tx = 0.0, ty = 0.0;
+      if (val(absC) > 0.0) {
+        const T cc = mup / absC;
+        tx = cc * (nk[1] * C);
+        ty = cc * (-(nk[0] * C));
+      }
+      v[0] = v[0] - D * (nk[0] + tx);
+      v[1] = v[1] - D * (nk[1] + ty);
+    }
+  }
+  // G2P with each particle's own solid's velocities
+  for (size_t si = 0; si < g.solids.size(); si++) {
+    auto& s = g.solids[si];
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, true);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T L[4] = {0.0, 0.0, 0.0, 0.0};
+      T v0 = s.vel[2 * ip], v1 = s.vel[2 * ip + 1];
+      T x0 = s.pos[2 * ip], x1 = s.pos[2 * ip + 1];
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        int k = g.cbody[2 * in] == int(si) ? 0 : (g.cbody[2 * in + 1] == int(si) ? 1 : -1);
+        if (k < 0) continue;
+        T Ni = g.funcs[i];
+        T d1 = g.ders[2 * i], d2 = g.ders[2 * i + 1];
+        if (val(g.cm[2 * in + k]) > val(g.tol)) {
+          const T* vb = &g.cv[4 * in + 2 * k];
+          const T* vb0 = &g.cv0[4 * in + 2 * k];
+          v0 += Ni * (vb[0] - vb0[0]);
+          v1 += Ni * (vb[1] - vb0[1]);
+          x0 += Ni * vb[0] * dt;
+          x1 += Ni * vb[1] * dt;
+          L[0] += d1 * vb[0]; L[1] += d1 * vb[1]; L[2] += d2 * vb[0]; L[3] += d2 * vb[1];
+        }
+      }
+      s.vel[2 * ip] = v0; s.vel[2 * ip + 1] = v1;
+      s.pos[2 * ip] = x0; s.pos[2 * ip + 1] = x1;
+      T D[4] = {0.5 * (L[0] + L[0]), 0.5 * (L[1] + L[2]), 0.5 * (L[2] + L[1]), 0.5 * (L[3] + L[3])};
+      T* eps = &s.strain[4 * ip];
+      for (int k = 0; k < 4; k++) eps[k] = eps[k] + dt * D[k];
+      T A[4] = {1.0 + L[0] * dt, L[1] * dt, L[2] * dt, 1.0 + L[3] * dt};
+      T* F = &s.F[4 * ip];
+      T Fn[4];
+      matmul2(F, A, Fn);
+      for (int k = 0; k < 4; k++) F[k] = Fn[k];
+      T J = det2(F);
+      s.vol[ip] = J * s.vol0[ip];
+      if (val(J) < 0.0) g.flags |= JUMP_FLAG_NEGATIVE_J;
+      T* sg = &s.stress[4 * ip];
+      switch (s.mat.kind) {
+        case JUMP_MAT_ELASTIC: stress_hooke_2d(s.mat, eps, sg); break;
+        case JUMP_MAT_NEOHOOKEAN: stress_nh(2, s.mat, F, J, sg); break;
+        case JUMP_MAT_ELASTOPLASTIC: stress_j2_2d(s.mat, eps, sg, &s.pstrain[3 * ip], s.alpha[ip], s.vm[ip]); break;
+        default: break;
+      }
+      if (!std::isfinite(val(x0)) || !std::isfinite(val(x1)) || !std::isfinite(val(v0)) || !std::isfinite(val(v1)))
+        g.flags |= JUMP_FLAG_NONFINITE;
+    }
+  }
   (void)t;
 }
 

@@ -45,6 +45,12 @@ int orc_ctx_create(const jump_params* p, orc_ctx** out) {
   g.alpha = p->alpha;
   g.dt = p->dt;
   for (int k = 0; k < 3; k++) g.body[k] = p->body[k];
+  if (p->contact && p->dim != 2) {
+    delete h;
+    return JUMP_ERR_UNSUPPORTED;
+  }
+  g.contact = p->contact != 0;
+  g.fric = p->friction;
   g.nx = p->nx;
   g.ny = p->ny;
   g.nz = p->dim == 3 ? p->nz : 1;
@@ -91,6 +97,7 @@ int orc_add_solid(orc_ctx* h, const jump_solid_desc* ds, const jump_material* ma
   if (mat->kind < JUMP_MAT_RIGID || mat->kind > JUMP_MAT_JOHNSONCOOK) return fail(h, JUMP_ERR_ARG, "bad material");
   if (d == 3 && mat->kind == JUMP_MAT_RIGID) return fail(h, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid");
   if (d == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return fail(h, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");
+  if (g.contact && mat->kind == JUMP_MAT_RIGID) return fail(h, JUMP_ERR_UNSUPPORTED, "rigid solids do not combine with contact");
   if (g.basis == JUMP_BASIS_CPDI_Q4 && (!ds->corners || !ds->elems || ds->ncorners <= 0))
     return fail(h, JUMP_ERR_ARG, "CPDI solid needs corners and elems");
   orc::Solid<double> s;
@@ -144,7 +151,8 @@ int orc_step(orc_ctx* h, int64_t nsteps, double* t) {
   if (!h || !t || nsteps < 0) return JUMP_ERR_ARG;
   OCtx& g = h->c;
   for (int64_t s = 0; s < nsteps; s++) {
-    if (g.dim == 2) orc::step_usl_2d(g, *t);
+    if (g.dim == 2 && g.contact) orc::step_usl_2d_contact(g, *t);
+    else if (g.dim == 2) orc::step_usl_2d(g, *t);
     else orc::step_musl_3d(g);
     *t += g.dt;
   }

@@ -359,3 +359,88 @@ def test_batch_members_step_as_alone(hip_lib):
                 assert np.array_equal(sa[k], sm[k]), (i, s, k)
         assert (ke[i], se[i]) == a.energies(), i
     b.close()
+
+
+# ---------------------------------------------------------------- contact
+CONTACT_CASES = [(b, m) for b in (_ffi.BASIS_LINEAR, _ffi.BASIS_GIMP) for m in ("elastic", "neohookean", "j2")] + \
+    [(_ffi.BASIS_QUAD_BSPLINE, "elastic"), (_ffi.BASIS_CUBIC_BSPLINE, "elastic")]
+
+
+@pytest.mark.parametrize("basis,material", CONTACT_CASES)
+def test_contact_collision_matches_oracle(hip_lib, basis, material):
+    """Two disks colliding with multi-body contact, 800 steps through the
+    impact and rebound: the per-solid P2G slots, the correction and the
+    per-slot G2P equal the restatement bit for bit (grid totals included;
+    Neo-Hookean within 1e-10)."""
+    _say(f"contact head-on basis={basis} material={material}")
+    sc = scenes.disks_contact(basis=basis, material=material, steps=800)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=_exact(material))
+
+
+@pytest.mark.parametrize("friction", [0.0, 0.3, 1e6])
+def test_contact_oblique_friction_matches_oracle(hip_lib, friction):
+    _say(f"contact oblique friction={friction}")
+    sc = scenes.disks_contact(offset=0.15, friction=friction, steps=1000)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=True)
+
+
+def test_contact_binned_order_and_overflow(hip_lib):
+    """Binned summation order with contact (slots may fill in another order;
+    same physics) within 1e-9 of the restatement; a third solid at a node
+    raises the same overflow flag on both sides."""
+    from oracle.pyoracle import OracleEngine
+    sc = scenes.disks_contact(offset=0.15, friction=0.3, steps=600)
+    sc.params.order = _ffi.ORDER_BINNED
+    g = Engine(sc.params)
+    sc.load(g)
+    g.step(sc.steps, 0.0)
+    o = OracleEngine(sc.params)
+    sc.load(o)
+    o.step(sc.steps, 0.0)
+    for s in range(2):
+        a, b = g.get_solid(s), o.get_solid(s)
+        for k in ("pos", "vel", "F", "stress"):
+            _close(a[k], b[k], rtol=1e-9, what=f"binned contact solid {s} {k}")
+    sc = scenes.disks_contact(gap_cells=1, steps=300)
+    extra = dict(sc.solids[0])
+    pos = sc.solids[0]["pos"].copy()
+    d = sc.solids[1]["pos"][:, 0].mean() - pos[:, 0].mean()
+    pos += (0.5 * d, 0.5 * d)
+    extra["pos"] = pos
+    extra["vel"] = np.zeros_like(pos)
+    extra["vel"][:, 1] = -0.1
+    sc.solids.append(extra)
+    g = Engine(sc.params)
+    sc.load(g)
+    o = OracleEngine(sc.params)
+    sc.load(o)
+    _, rg = g.step(sc.steps, 0.0)
+    _, ro = o.step(sc.steps, 0.0)
+    assert rg == ro == _ffi.JUMP_ERR_NUMERIC
+    fg, fo = g.flags(), o.flags()  # flags are read and cleared
+    assert fg == fo and fg & _ffi.JUMP_FLAG_CONTACT_OVERFLOW, (fg, fo)
+    _compare(g, o, sc, exact=True)
+
+
+def test_contact_refusals(hip_lib):
+    """Contact is a 2D USL mode: 3D contexts, rigid solids, batches and the
+    reverse mode refuse it."""
+    from jump_amd.engine import Batch, make_material, make_params
+    p = make_params(3, _ffi.BASIS_LINEAR, _ffi.ALG_MUSL, 1e-3, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (5, 5, 5),
+                    contact=True)
+    with pytest.raises(JumpError):
+        Engine(p)
+    sc = scenes.disks_contact(steps=2)
+    g = Engine(sc.params)
+    with pytest.raises(JumpError):
+        g.add_solid(make_material(_ffi.MAT_RIGID, density=1.0), np.array([[0.5, 0.5]]), np.zeros((1, 2)),
+                    np.ones(1), np.ones(1), np.ones(1))
+    sc.load(g)
+    with pytest.raises(JumpError):
+        Batch([g])
+    from jump_amd import adjoint
+    with pytest.raises(JumpError):
+        adjoint.rrule(g, 2)
+    g.close()

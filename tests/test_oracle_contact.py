@@ -1,0 +1,122 @@
+"""Pins the oracle's multi-body contact step (the FEMP contact algorithm,
+SolveExplicitFEMP2DContact.jl:311-387, on the USL grid) with checks that do
+not retype its formulas: before the solids meet it reduces to the
+single-field step (grid bit for bit); the correction conserves momentum; a
+head-on collision rebounds without the disks passing through each other
+(which independent velocity fields without the correction would do); a
+frictionless oblique impact spins the disks far less than a sticking one
+(high friction, or the single shared field); a third solid at a node is
+flagged; unsupported combinations are refused."""
+import numpy as np
+import pytest
+
+from jump_amd import _ffi, scenes
+from jump_amd.engine import make_material, make_params
+from jump_amd.engine import JumpError
+from oracle.pyoracle import OracleEngine
+
+
+def _engine(scene):
+    o = OracleEngine(scene.params)
+    scene.load(o)
+    return o
+
+
+def _body_stats(o, s):
+    a = o.get_solid(s)
+    m = a["mass"]
+    c = (a["pos"] * m[:, None]).sum(0) / m.sum()
+    v = (a["vel"] * m[:, None]).sum(0) / m.sum()
+    r = a["pos"] - c
+    dv = a["vel"] - v
+    w = float(((r[:, 0] * dv[:, 1] - r[:, 1] * dv[:, 0]) * m).sum() / ((r * r).sum(1) * m).sum())
+    return c, v, w, a
+
+
+def test_contact_grid_equals_single_field_before_the_solids_meet():
+    kw = dict(gap_cells=6, steps=1)
+    a = _engine(scenes.disks_contact(contact=True, **kw))
+    b = _engine(scenes.disks_contact(contact=False, **kw))
+    a.step(1, 0.0)
+    b.step(1, 0.0)
+    ga, gb = a.get_grid(), b.get_grid()
+    for k in ("mass", "momentum0", "momentum", "force"):
+        assert np.array_equal(ga[k], gb[k]), k
+    a.step(149, 0.0)
+    b.step(149, 0.0)
+    for s in range(2):
+        sa, sb = a.get_solid(s), b.get_solid(s)
+        for k in ("pos", "vel", "F", "stress"):
+            scale = np.max(np.abs(sb[k]))
+            assert np.max(np.abs(sa[k] - sb[k])) <= 1e-12 * scale, (s, k)
+
+
+@pytest.mark.parametrize("offset,friction", [(0.0, 0.0), (0.15, 0.0), (0.15, 0.3)])
+def test_contact_conserves_momentum(offset, friction):
+    o = _engine(scenes.disks_contact(offset=offset, friction=friction))
+    ref = None
+    for _ in range(8):
+        o.step(100, 0.0)
+        P = np.zeros(2)
+        mv = 0.0
+        for s in range(2):
+            a = o.get_solid(s)
+            P += (a["vel"] * a["mass"][:, None]).sum(0)
+            mv += float((np.abs(a["vel"]).sum(1) * a["mass"]).sum())
+        ref = mv if ref is None else ref
+        assert np.max(np.abs(P)) <= 1e-12 * ref, P
+    assert o.flags() == 0
+
+
+def test_head_on_disks_rebound_without_passing_through():
+    sc = scenes.disks_contact()
+    o = _engine(sc)
+    r = float(np.max(np.abs(sc.solids[0]["pos"][:, 0] - sc.solids[0]["pos"][:, 0].mean())))
+    gaps = []
+    for _ in range(12):
+        o.step(100, 0.0)
+        c0, v0, _, _ = _body_stats(o, 0)
+        c1, v1, _, _ = _body_stats(o, 1)
+        gaps.append(c1[0] - c0[0])
+    assert min(gaps) > 0.9 * 2 * r
+    assert v0[0] < -0.05 and v1[0] > 0.05  # rebounded
+    assert abs(v0[1]) < 1e-12 and abs(v1[1]) < 1e-12  # symmetric impact stays on the line
+
+
+def test_frictionless_oblique_impact_transmits_little_spin():
+    spins = {}
+    for name, kw in (("smooth", dict(friction=0.0)), ("rough", dict(friction=1e6)),
+                     ("single", dict(contact=False))):
+        o = _engine(scenes.disks_contact(offset=0.15, **kw))
+        o.step(1000, 0.0)
+        spins[name] = _body_stats(o, 0)[2]
+    assert spins["rough"] > 0.3 and spins["single"] > 0.3, spins
+    assert abs(spins["smooth"]) < 0.1 * spins["rough"], spins
+
+
+def test_third_solid_at_a_node_is_flagged():
+    sc = scenes.disks_contact(gap_cells=1)
+    extra = dict(sc.solids[0])
+    pos = sc.solids[0]["pos"].copy()
+    pos[:, 1] += 0.5 * (sc.solids[1]["pos"][:, 0].mean() - pos[:, 0].mean())
+    pos[:, 0] += 0.5 * (sc.solids[1]["pos"][:, 0].mean() - sc.solids[0]["pos"][:, 0].mean())
+    extra["pos"] = pos
+    extra["vel"] = np.zeros_like(pos)
+    extra["vel"][:, 1] = -0.1
+    sc.solids.append(extra)
+    o = _engine(sc)
+    o.step(300, 0.0)
+    assert o.flags() & _ffi.JUMP_FLAG_CONTACT_OVERFLOW
+
+
+def test_contact_refuses_rigid_solids_and_3d():
+    p = make_params(3, _ffi.BASIS_LINEAR, _ffi.ALG_MUSL, 1e-3, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (5, 5, 5),
+                    contact=True)
+    with pytest.raises(JumpError):
+        OracleEngine(p)
+    p = make_params(2, _ffi.BASIS_LINEAR, _ffi.ALG_USL, 1e-3, (0.0, 0.0), (1.0, 1.0), (5, 5), contact=True)
+    o = OracleEngine(p)
+    pos = np.array([[0.5, 0.5]])
+    with pytest.raises(JumpError):
+        o.add_solid(make_material(_ffi.MAT_RIGID, density=1.0), pos, np.zeros((1, 2)), np.ones(1), np.ones(1),
+                    np.ones(1))
