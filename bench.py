@@ -142,12 +142,14 @@ def batched_leg(n_inst, steps, warmup):
             "speedup": t_plain / t_batch, "unit": "particle-steps/s", "step_flags": flags}
 
 
-def adjoint_leg(steps, K):
+def adjoint_leg(steps, K_recompute=10):
     """Reverse mode against the plain step on the two-rings impact at 1025 x
     513 nodes, 4 particles per cell and direction (1,844,952 particles,
     2D USL, GIMP, Neo-Hookean E = 2 MPa, 30 mm/s), `steps` steps.  plain:
-    jump_step; adjoint: the recorded run (snapshots every K steps) plus the
-    backward sweep, for L = sum |x_T - x_0|^2."""
+    jump_step; adjoint: the recorded run plus the backward sweep, for
+    L = sum |x_T - x_0|^2.  Two checkpoint intervals: automatic (the library
+    keeps every step's state when HBM holds them, so nothing is recomputed),
+    and every K_recompute steps (snapshot and recompute)."""
     import numpy as np
     import torch
 
@@ -164,44 +166,55 @@ def adjoint_leg(steps, K):
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     e.close()
-    e = Engine(sc.params, device=0)
-    sc.load(e)
     x0 = np.concatenate([s_["pos"] for s_ in sc.solids])
-    lib = e.lib
     ns = len(sc.solids)
-    dE, dnu, dv0 = np.zeros(ns), np.zeros(ns), np.zeros((ns, 2))
-    e.step(0, 0.0)
-    start = e.checkpoint()
-    # a first recorded run and sweep on this context allocate its buffers,
-    # which are kept; the timed pair then starts again from the same state
-    tt = ctypes.c_double(0.0)
-    e._check(lib.jump_adjoint_forward(e.h, steps, K, ctypes.byref(tt)))
-    xb = np.zeros_like(x0)
-    e._check(lib.jump_adjoint_backward(e.h, _ffi.dptr(xb), None, None, None, None))
-    e.restore(start)
-    e.step(0, 0.0)
-    torch.cuda.synchronize()
-    tt = ctypes.c_double(0.0)
-    t0 = time.perf_counter()
-    rc = lib.jump_adjoint_forward(e.h, steps, K, ctypes.byref(tt))
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    pos = np.concatenate([e.get_solid(s_)["pos"] for s_ in range(ns)])
-    xb = np.ascontiguousarray(2.0 * (pos - x0))
-    t0 = time.perf_counter()
-    e._check(lib.jump_adjoint_backward(e.h, _ffi.dptr(xb), None, _ffi.dptr(dE), _ffi.dptr(dnu), _ffi.dptr(dv0)))
-    torch.cuda.synchronize()
-    t_back = time.perf_counter() - t0
-    g = {"E": dE, "nu": dnu}
-    e.close()
-    return {"workload": f"two-rings 1025x513 nodes (2D USL, GIMP, Neo-Hookean), {sc.n_particles} particles",
-            "steps": steps, "checkpoint_every": K,
-            "plain_ms_per_step": t_plain * 1e3 / steps,
-            "recorded_forward_ms_per_step": t_fwd * 1e3 / steps,
-            "backward_ms_per_step": t_back * 1e3 / steps,
-            "adjoint_over_plain": (t_fwd + t_back) / t_plain,
-            "backward_over_plain": t_back / t_plain,
-            "dL_dE": float(g["E"][0]), "dL_dnu": float(g["nu"][0]), "step_flags": rc}
+
+    def timed(K):
+        e = Engine(sc.params, device=0)
+        sc.load(e)
+        lib = e.lib
+        dE, dnu, dv0 = np.zeros(ns), np.zeros(ns), np.zeros((ns, 2))
+        e.step(0, 0.0)
+        start = e.checkpoint()
+        # a first recorded run and sweep on this context allocate its
+        # buffers, which are kept; the timed pair starts again from the same
+        # state
+        tt = ctypes.c_double(0.0)
+        e._check(lib.jump_adjoint_forward(e.h, steps, K, ctypes.byref(tt)))
+        xb = np.zeros_like(x0)
+        e._check(lib.jump_adjoint_backward(e.h, _ffi.dptr(xb), None, None, None, None))
+        e.restore(start)
+        e.step(0, 0.0)
+        torch.cuda.synchronize()
+        tt = ctypes.c_double(0.0)
+        t0 = time.perf_counter()
+        rc = lib.jump_adjoint_forward(e.h, steps, K, ctypes.byref(tt))
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        pos = np.concatenate([e.get_solid(s_)["pos"] for s_ in range(ns)])
+        xb = np.ascontiguousarray(2.0 * (pos - x0))
+        t0 = time.perf_counter()
+        e._check(lib.jump_adjoint_backward(e.h, _ffi.dptr(xb), None, _ffi.dptr(dE), _ffi.dptr(dnu), _ffi.dptr(dv0)))
+        torch.cuda.synchronize()
+        t_back = time.perf_counter() - t0
+        k_used = ctypes.c_int32()
+        e._check(lib.jump_adjoint_interval(e.h, ctypes.byref(k_used)))
+        e.close()
+        return {"checkpoint_every": k_used.value,
+                "recorded_forward_ms_per_step": t_fwd * 1e3 / steps,
+                "backward_ms_per_step": t_back * 1e3 / steps,
+                "adjoint_over_plain": (t_fwd + t_back) / t_plain,
+                "backward_over_plain": t_back / t_plain,
+                "dL_dE": float(dE[0]), "dL_dnu": float(dnu[0]), "step_flags": rc}
+
+    auto = timed(0)
+    rec = timed(K_recompute)
+    out = {"workload": f"two-rings 1025x513 nodes (2D USL, GIMP, Neo-Hookean), {sc.n_particles} particles",
+           "steps": steps, "plain_ms_per_step": t_plain * 1e3 / steps}
+    out.update(auto)
+    out["recompute"] = rec
+    out["gradients_equal"] = (auto["dL_dE"], auto["dL_dnu"]) == (rec["dL_dE"], rec["dL_dnu"])
+    return out
 
 
 def main():
@@ -308,7 +321,7 @@ def main():
     if args.batch_instances > 0:
         out["batched"] = batched_leg(args.batch_instances, args.batch_steps, 2)
     if args.adjoint_steps > 0:
-        out["adjoint"] = adjoint_leg(args.adjoint_steps, 10)
+        out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -254,8 +254,13 @@ const char* jump_batch_last_error(const jump_batch* batch);
  * (the derivative with respect to a uniform change of that solid's initial
  * velocity, 2 per solid); outputs may be NULL.  Each step of the sweep
  * recomputes its states from the nearest snapshot (checkpoint and
- * recompute); the context is left in the recorded final state. */
+ * recompute); the context is left in the recorded final state.
+ * checkpoint_every = 0 picks the interval: the whole run when its per-step
+ * states, grids and bins fit in three quarters of the free device memory
+ * (no recomputation), else the largest interval that fits.
+ * jump_adjoint_interval reports the interval of the recorded run. */
 int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, double* t_inout);
+int jump_adjoint_interval(jump_ctx* ctx, int32_t* checkpoint_every);
 int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* vel_bar, double* dE,
                           double* dnu, double* dvel0);
 

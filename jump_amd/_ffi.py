@@ -163,7 +163,8 @@ def load():
                             ("jump_batch_destroy", ctypes.c_int, [H]),
                             ("jump_batch_last_error", ctypes.c_char_p, [H]),
                             ("jump_adjoint_forward", ctypes.c_int, [H, _i64, _i32, _pd]),
-                            ("jump_adjoint_backward", ctypes.c_int, [H, _pd, _pd, _pd, _pd, _pd])):
+                            ("jump_adjoint_backward", ctypes.c_int, [H, _pd, _pd, _pd, _pd, _pd]),
+                            ("jump_adjoint_interval", ctypes.c_int, [H, ctypes.POINTER(_i32)])):
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
@@ -179,5 +180,5 @@ EXPORTED = [
     "jump_abi_version", "jump_profile", "jump_profile_read",
     "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
     "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies", "jump_batch_destroy",
-    "jump_batch_last_error", "jump_adjoint_forward", "jump_adjoint_backward",
+    "jump_batch_last_error", "jump_adjoint_forward", "jump_adjoint_backward", "jump_adjoint_interval",
 ]

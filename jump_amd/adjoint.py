@@ -4,7 +4,10 @@
 rrule around solve_explicit_dynamics_2D (SolveExplicitSolidMechanics2D.jl:335):
 it runs the steps and returns the final particle positions and velocities
 with a pullback that maps their adjoints to gradients with respect to each
-solid's E, nu and initial velocity.
+solid's E, nu and initial velocity.  checkpoint_every = 0 lets the library
+pick the interval from the free device memory (the whole run when it fits,
+so the backward sweep recomputes nothing); `pullback.checkpoint_every` is
+the interval used.
 """
 import ctypes
 
@@ -14,7 +17,7 @@ from . import _ffi
 from .engine import JumpError
 
 
-def rrule(engine, nsteps, checkpoint_every=20, t=0.0):
+def rrule(engine, nsteps, checkpoint_every=0, t=0.0):
     lib = engine.lib
     tt = ctypes.c_double(t)
     rc = lib.jump_adjoint_forward(engine.h, int(nsteps), int(checkpoint_every), ctypes.byref(tt))
@@ -34,4 +37,7 @@ def rrule(engine, nsteps, checkpoint_every=20, t=0.0):
             engine._check(rc2)
         return {"E": dE, "nu": dnu, "v0": dv0}
 
+    k = ctypes.c_int32()
+    engine._check(lib.jump_adjoint_interval(engine.h, ctypes.byref(k)))
+    pullback.checkpoint_every = k.value
     return (pos, vel, tt.value, rc), pullback

@@ -1127,8 +1127,7 @@ int jump_batch_create(jump_ctx* const* ctxs, int32_t n, int device, jump_batch**
     for (size_t ii = 0; ii < grp.members.size(); ii++) {
       jump_ctx* c = ctxs[grp.members[ii]];
       if (c->on_device) {
-        hipSetDevice(c->device);
-        r = download(c);
+        r = hipSetDevice(c->device) == hipSuccess ? download(c) : JUMP_ERR_DEVICE;
         if (r) {
           jump_batch_destroy(b);
           return r;
@@ -1161,8 +1160,8 @@ int jump_batch_create(jump_ctx* const* ctxs, int32_t n, int device, jump_batch**
       jump_batch_destroy(b);
       return JUMP_ERR_UNSUPPORTED;
     }
-    hipSetDevice(device);
-    if (hipMemcpyAsync(in->d_fixed, in->hfixed.data(), in->hfixed.size(), hipMemcpyHostToDevice, in->stream) !=
+    if (hipSetDevice(device) != hipSuccess ||
+        hipMemcpyAsync(in->d_fixed, in->hfixed.data(), in->hfixed.size(), hipMemcpyHostToDevice, in->stream) !=
             hipSuccess ||
         hipStreamSynchronize(in->stream) != hipSuccess) {
       jump_batch_destroy(b);
@@ -1328,15 +1327,46 @@ template <int B> static void adj_step_back(jump_ctx* ctx) {
   LAUNCH((k_adj_p2g<B>), np, ctx->P, ctx->G, ctx->d_solids, an, ctx->adj.ap);
 }
 
-int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K, double* t_inout) {
+// Device bytes a recorded run of nsteps steps at interval K holds: nck
+// snapshots, K per-step states, grids and bins, the final state.
+static size_t adjoint_record_bytes(const jump_ctx* ctx, int64_t nsteps, int64_t K) {
+  const size_t snap = snap_doubles(ctx) * sizeof(double);
+  const size_t per_step = snap + 5 * size_t(ctx->NN) * sizeof(double) + bins_bytes(ctx) + 256;
+  const size_t nck = size_t((nsteps + K - 1) / K);
+  return (nck + 1) * snap + size_t(std::min(K, nsteps)) * per_step;
+}
+
+// The largest interval whose record fits in 3/4 of the device memory that is
+// free once this context's previous record is released.  The adjoint arrays
+// of the backward sweep (about 25 doubles per particle) are set aside first.
+static int adjoint_auto_interval(jump_ctx* ctx, int64_t nsteps, int64_t* K) {
+  size_t freeb = 0, total = 0;
+  HIP_TRY(hipMemGetInfo(&freeb, &total));
+  const auto& A = ctx->adj;
+  if (A.snap_np == ctx->np) {  // the buffers held now are freed or reused
+    const size_t snap = snap_doubles(ctx) * sizeof(double);
+    freeb += (A.ckpt.size() + A.segbuf.size() + (A.final_state ? 1 : 0)) * snap +
+             A.gridbuf.size() * 5 * size_t(ctx->NN) * sizeof(double) + A.binbuf.size() * (bins_bytes(ctx) + 256);
+  }
+  const size_t reserve = 25 * size_t(ctx->np) * sizeof(double) + 6 * size_t(ctx->NN) * sizeof(double);
+  const size_t budget = freeb > reserve ? (freeb - reserve) / 4 * 3 : 0;
+  int64_t k = std::min<int64_t>(nsteps, INT32_MAX);
+  while (k > 1 && adjoint_record_bytes(ctx, nsteps, k) > budget) k = k * 7 / 8;
+  *K = std::max<int64_t>(k, 1);
+  return JUMP_OK;
+}
+
+int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t_inout) {
   if (!ctx) return JUMP_ERR_ARG;
-  if (!t_inout || nsteps <= 0 || K <= 0) return set_err(ctx, JUMP_ERR_ARG, "bad adjoint run arguments");
+  if (!t_inout || nsteps <= 0 || K_arg < 0) return set_err(ctx, JUMP_ERR_ARG, "bad adjoint run arguments");
   int r = adjoint_supported(ctx);
   if (r) return r;
   HIP_TRY(hipSetDevice(ctx->device));
   r = ensure_device(ctx);
   if (r) return r;
   auto& A = ctx->adj;
+  int64_t K = K_arg;
+  if (K == 0 && (r = adjoint_auto_interval(ctx, nsteps, &K))) return r;
   const size_t sd = snap_doubles(ctx);
   const int64_t nck = (nsteps + K - 1) / K;
   const size_t nseg = size_t(std::min<int64_t>(K, nsteps));
@@ -1384,10 +1414,17 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K, double* t_ino
   for (int64_t s = 0; s < nsteps; s++) *t_inout += ctx->prm.dt;
   ctx->grid_valid = true;
   A.nsteps = nsteps;
-  A.K = K;
+  A.K = int(K);
   A.ready = true;
   ctx->flags |= fl;
   if (fl) return set_err(ctx, JUMP_ERR_NUMERIC, "particle error flags raised during the recorded run");
+  return JUMP_OK;
+}
+
+int jump_adjoint_interval(jump_ctx* ctx, int32_t* K) {
+  if (!ctx || !K) return JUMP_ERR_ARG;
+  if (!ctx->adj.ready) return set_err(ctx, JUMP_ERR_STATE, "no recorded run (call jump_adjoint_forward first)");
+  *K = int32_t(ctx->adj.K);
   return JUMP_OK;
 }
 

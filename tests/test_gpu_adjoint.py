@@ -62,10 +62,11 @@ def test_checkpoint_interval_does_not_change_the_gradient(hip_lib):
     sc = _scene(_ffi.BASIS_LINEAR, "neohookean")
     target = sc.solids[0]["pos"]
     res = []
-    for K in (1, 7, NSTEPS):
+    for K in (1, 7, NSTEPS, 0):
         e = Engine(sc.params)
         sc.load(e)
         (pos, vel, t, rc), pullback = rrule(e, NSTEPS, checkpoint_every=K)
+        assert pullback.checkpoint_every == (K or NSTEPS)  # automatic: the small run fits whole
         res.append(pullback(2.0 * (pos - target), vel_bar=np.ones_like(vel)))
         after = e.get_solid(0)["pos"]
         assert np.array_equal(after, pos)  # the context keeps the recorded final state
