@@ -239,6 +239,7 @@ __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& 
 template <int B> struct MaxW { static constexpr int W = Width<B>::W; };
 template <> struct MaxW<JUMP_BASIS_CPDI_Q4> { static constexpr int W = 5; };
 constexpr int kNoParticle = 0x7fffffff;
+constexpr int kStaleHead = -1;  // a consumed bin whose next head is not loaded yet
 
 // Particle fields one P2G pass reads, fetched as whole 16-byte slots.
 // PASS 0: x, m, V, v, sigma (every slot); PASS 1 (re-projection): x, m, v
@@ -376,7 +377,13 @@ __global__ void __launch_bounds__(kBlock) k_node_order(const int* __restrict__ p
 // LDS arrays s_cur/s_head of this thread's column), bin after bin for
 // JUMP_ORDER_BINNED.  start: the node's own instance's bins; off: that
 // instance's first node; sorted/bxyz: all particles.
-template <int D, int B, int ORDER, class Visit>
+// DEFER: a consumed bin's new head is loaded when the run of consecutive
+// indices ends (one load per bin the run touched, all issued together)
+// instead of after every particle.  Pays where the visit is light: the 3D
+// re-projection pass (1.62 -> 1.19 ms on the 4.2M-particle impact) and the
+// 2D force pass (two rings, 1.8M particles: step 1.52 -> 1.26 ms).  The 3D
+// force pass hides the per-particle load behind its work (2.41 vs 2.47 ms).
+template <int D, int B, int ORDER, bool DEFER, class Visit>
 __device__ __forceinline__ void for_window_particles(const Particles& P, const GridDesc& G, const int* start,
                                                      const unsigned long long* sorted,
                                                      const unsigned long long* bxyz, int off, int inst, int W,
@@ -425,10 +432,15 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
         const int a0 = b % MW, a1 = (b / MW) % MW, a2 = (D == 3) ? b / (MW * MW) : 0;
         const unsigned bin = unsigned(off + (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2)));
         const int c = s_cur[b][t] + 1;
-        const unsigned long long e = ldu(sorted, unsigned(c));  // sorted[np] is a sentinel
         s_cur[b][t] = c;
-        visit(p);
-        s_head[b][t] = unsigned(e >> 32) == bin ? int(unsigned(e)) : kNoParticle;
+        if (DEFER) {
+          s_head[b][t] = kStaleHead;
+          visit(p);
+        } else {
+          const unsigned long long e = ldu(sorted, unsigned(c));  // sorted[np] is a sentinel
+          visit(p);
+          s_head[b][t] = unsigned(e >> 32) == bin ? int(unsigned(e)) : kNoParticle;
+        }
         // is p+1 in the window?  (bin coordinates packed 21 bits per axis;
         // excluded particles hold all ones)
         const int q = p + 1;
@@ -443,6 +455,16 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
           break;
         b = int(ux) + MW * (int(uy) + (D == 3 ? MW * int(uz) : 0));
         p = q;
+      }
+      if (DEFER) {
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+          if (s_head[k][t] != kStaleHead) continue;
+          const int a0 = k % MW, a1 = (k / MW) % MW, a2 = (D == 3) ? k / (MW * MW) : 0;
+          const unsigned bin = unsigned(off + (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2)));
+          const unsigned long long e = ldu(sorted, unsigned(s_cur[k][t]));
+          s_head[k][t] = unsigned(e >> 32) == bin ? int(unsigned(e)) : kNoParticle;
+        }
       }
     }
   }
@@ -489,8 +511,9 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
   double am = 0.0, ap[D], af[D];
   for (int k = 0; k < D; k++) { ap[k] = 0.0; af[k] = 0.0; }
-  for_window_particles<D, B, ORDER>(P, G, start, sorted, bxyz, off, inst, W, ni, nj, nk, s_cur, s_head,
-                                   [&](int p) { p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af); });
+  constexpr bool DEFER = PASS == 1 || D == 2;
+  for_window_particles<D, B, ORDER, DEFER>(P, G, start, sorted, bxyz, off, inst, W, ni, nj, nk, s_cur, s_head,
+                                           [&](int p) { p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af); });
   if (PASS == 0) {
     const uint8_t fx = g.fixed[n];
     g.m[n] = am;
@@ -558,7 +581,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
     s.nr[0] += fM * dN[0];
     s.nr[1] += fM * dN[1];
   };
-  for_window_particles<2, B, ORDER>(P, G, start, sorted, bxyz, 0, 0, W, ni, nj, 0, s_cur, s_head, [&](int p) {
+  for_window_particles<2, B, ORDER, true>(P, G, start, sorted, bxyz, 0, 0, W, ni, nj, 0, s_cur, s_head, [&](int p) {
     double v[PairLayout<2>::NF + 1];
     P2GFields<2, 0>::load(P, unsigned(p), v);
     double lp[3] = {0.0, 0.0, 0.0};
