@@ -433,6 +433,13 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
         const unsigned bin = unsigned(off + (wx0 + a0) + G.nx * ((wy0 + a1) + G.ny * (wz0 + a2)));
         const int c = s_cur[b][t] + 1;
         s_cur[b][t] = c;
+        // is p+1 in the window?  (bin coordinates packed 21 bits per axis;
+        // excluded particles hold all ones)  The load that decides it is
+        // issued before the visit, so its latency runs under the visit's
+        // work: a run's critical path is this chain of loads (4.99 vs 5.33
+        // ms per step on the 4.2M-particle impact with it after the visit).
+        const int q = p + 1;
+        const unsigned long long cc = q < P.n ? ldu(bxyz, unsigned(q)) : ~0ull;
         if (DEFER) {
           s_head[b][t] = kStaleHead;
           visit(p);
@@ -441,11 +448,6 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
           visit(p);
           s_head[b][t] = unsigned(e >> 32) == bin ? int(unsigned(e)) : kNoParticle;
         }
-        // is p+1 in the window?  (bin coordinates packed 21 bits per axis;
-        // excluded particles hold all ones)
-        const int q = p + 1;
-        if (q >= P.n) break;
-        const unsigned long long cc = ldu(bxyz, unsigned(q));
         const unsigned ux = unsigned(int(cc & 0x1fffffu) - wx0);
         const unsigned uy = unsigned(int((cc >> 21) & 0x1fffffu) - wy0);
         const unsigned uz = unsigned(int((cc >> 42) & 0x1fffffu) - wz0);
