@@ -43,7 +43,8 @@ typedef enum {
   JUMP_ERR_DEVICE = -3,      /* HIP runtime failure                        */
   JUMP_ERR_NOMEM = -4,       /* allocation failure                         */
   JUMP_ERR_UNSUPPORTED = -5, /* combination the reference does not define  */
-  JUMP_ERR_NUMERIC = -6      /* step finished with a particle error flag   */
+  JUMP_ERR_NUMERIC = -6,     /* step finished with a particle error flag   */
+  JUMP_ERR_IO = -7           /* a dump file could not be written           */
 } jump_status;
 
 /* Particle error flags raised during a step (OR-ed; jump_flags()).
@@ -211,6 +212,21 @@ int jump_get_grid(jump_ctx* ctx, double* mass, double* momentum0,
 /* Kinetic and strain energy over all solids, compute(::EnergiesFix)
  * (Fix.jl:248-270). */
 int jump_energies(jump_ctx* ctx, double* ke, double* se);
+
+/* LAMMPS particle dump of all solids, the reference's OvitoOutput
+ * (plotParticles_2D/3D, Output.jl:146-260): header, then one row per
+ * particle "id type x y z <outs>" (type = 1-based solid index; 2D rows
+ * write z as 0) with numbers in Julia's shortest Float64 text, appended to
+ * `path` as the reference's open(file, "a") does.  `outs`: space-separated
+ * column names from pstrain, vonMises, pressure, vx, vy, and sigmaxx,
+ * sigmayy (2D) or vz (3D); others are JUMP_ERR_UNSUPPORTED.  Box bounds are
+ * the grid extents (grid.lx, ly, lz).  The state is copied to pinned host
+ * memory on the context's stream, ahead of any later step, and a worker
+ * thread formats and writes the text while the device goes on stepping.
+ * jump_dump_wait blocks until every queued dump is written and returns
+ * JUMP_ERR_IO if one failed. */
+int jump_dump_lammps(jump_ctx* ctx, const char* path, const char* outs);
+int jump_dump_wait(jump_ctx* ctx);
 
 /* Checkpoint / restart of the particle state (the reference has none; its
  * dumps, Output.jl:146-290, are write-only).  A checkpoint holds every

@@ -19,6 +19,7 @@ JUMP_ERR_DEVICE = -3
 JUMP_ERR_NOMEM = -4
 JUMP_ERR_UNSUPPORTED = -5
 JUMP_ERR_NUMERIC = -6
+JUMP_ERR_IO = -7
 
 JUMP_FLAG_OUT_OF_GRID = 1
 JUMP_FLAG_NEGATIVE_J = 2
@@ -164,7 +165,9 @@ def load():
                             ("jump_batch_last_error", ctypes.c_char_p, [H]),
                             ("jump_adjoint_forward", ctypes.c_int, [H, _i64, _i32, _pd]),
                             ("jump_adjoint_backward", ctypes.c_int, [H, _pd, _pd, _pd, _pd, _pd]),
-                            ("jump_adjoint_interval", ctypes.c_int, [H, ctypes.POINTER(_i32)])):
+                            ("jump_adjoint_interval", ctypes.c_int, [H, ctypes.POINTER(_i32)]),
+                            ("jump_dump_lammps", ctypes.c_int, [H, ctypes.c_char_p, ctypes.c_char_p]),
+                            ("jump_dump_wait", ctypes.c_int, [H])):
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
@@ -181,4 +184,5 @@ EXPORTED = [
     "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
     "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies", "jump_batch_destroy",
     "jump_batch_last_error", "jump_adjoint_forward", "jump_adjoint_backward", "jump_adjoint_interval",
+    "jump_dump_lammps", "jump_dump_wait",
 ]

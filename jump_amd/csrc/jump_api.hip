@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <memory>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -11,6 +13,7 @@
 #include <vector>
 
 #include "mpm_adjoint.hpp"
+#include "mpm_output.hpp"
 
 using namespace jh;
 
@@ -100,6 +103,18 @@ struct jump_ctx {
     AdjSources src{};
   } adj;
   uint32_t flags = 0;
+
+  // LAMMPS dumps (jump_dump_*): pinned host buffer sets, reused while free,
+  // and the writer thread.  A set is busy from its device copy until the
+  // writer has formatted it.
+  struct DumpBuf {
+    char* host = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    std::atomic<bool> busy{false};
+  };
+  std::vector<std::unique_ptr<DumpBuf>> dump_bufs;
+  DumpWriter dump_writer;
 
   // per-phase timing with HIP events on the context stream (jump_profile)
   bool prof = false;
@@ -615,6 +630,11 @@ int jump_ctx_destroy(jump_ctx* ctx) {
   if (!ctx) return JUMP_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->dump_writer.stop();  // writes what is queued
+  for (auto& b : ctx->dump_bufs) {
+    if (b->host) (void)hipHostFree(b->host);
+    if (b->ev) (void)hipEventDestroy(b->ev);
+  }
   free_list(ctx->adj.allocs);
   free_list(ctx->adj.adj_allocs);
   free_list(ctx->part_allocs);
@@ -973,6 +993,94 @@ int jump_energies(jump_ctx* ctx, double* ke, double* se) {
   }
   r = range_energies(ctx, 0, ctx->np, ke, se);
   return r;
+}
+
+int jump_dump_lammps(jump_ctx* ctx, const char* path, const char* outs) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (!path || !*path) return set_err(ctx, JUMP_ERR_ARG, "empty dump path");
+  const int D = ctx->D;
+  DumpJob job;
+  if (!parse_dump_cols(outs, D, job.names, job.cols))
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, std::string("unknown dump column in \"") + outs + "\"");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int r = ensure_device(ctx);
+  if (r) return r;
+  // the pair arrays and history fields the columns read (positions always)
+  bool want[7] = {}, walpha = false, wvm = false;
+  const int fv = D + 2, fs = 2 * D + 2;
+  for (int a = 0; a < D; a++) want[a / 2] = true;
+  for (int c : job.cols) {
+    if (c == kColPstrain) walpha = true;
+    if (c == kColVonMises) wvm = true;
+    if (c == kColVx || c == kColVy || c == kColVz)
+      for (int a = 0; a < D; a++) want[(fv + a) / 2] = true;
+    if (c == kColSxx || c == kColSyy || c == kColPressure)
+      for (int k = 0; k < D; k++) want[(fs + k) / 2] = true;
+  }
+  const size_t n = size_t(ctx->np);
+  size_t bytes = (walpha ? n * 8 : 0) + (wvm ? n * 8 : 0);
+  for (bool w : want) bytes += w ? n * 16 : 0;
+  bytes = std::max<size_t>(bytes, 16);
+  // a free buffer set (at most two: a third dump waits for the writer)
+  jump_ctx::DumpBuf* buf = nullptr;
+  for (auto& b : ctx->dump_bufs)
+    if (!b->busy.load()) { buf = b.get(); break; }
+  if (!buf && ctx->dump_bufs.size() < 2) {
+    ctx->dump_bufs.push_back(std::make_unique<jump_ctx::DumpBuf>());
+    buf = ctx->dump_bufs.back().get();
+  }
+  if (!buf) {
+    ctx->dump_writer.wait();
+    buf = ctx->dump_bufs[0].get();
+  }
+  if (buf->bytes < bytes) {
+    if (buf->host) HIP_TRY(hipHostFree(buf->host));
+    buf->host = nullptr;
+    buf->bytes = 0;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&buf->host), bytes, hipHostMallocDefault));
+    buf->bytes = bytes;
+  }
+  if (!buf->ev) HIP_TRY(hipEventCreateWithFlags(&buf->ev, hipEventDisableTiming));
+  buf->busy = true;
+  char* q = buf->host;
+  for (int k = 0; k < 7; k++) {
+    if (!want[k]) continue;
+    HIP_TRY(hipMemcpyAsync(q, ctx->P.pr[k], n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    job.pr[k] = reinterpret_cast<const double*>(q);
+    q += n * 16;
+  }
+  if (walpha) {
+    HIP_TRY(hipMemcpyAsync(q, ctx->P.alpha, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    job.alpha = reinterpret_cast<const double*>(q);
+    q += n * 8;
+  }
+  if (wvm) {
+    HIP_TRY(hipMemcpyAsync(q, ctx->P.vm, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    job.vm = reinterpret_cast<const double*>(q);
+  }
+  HIP_TRY(hipEventRecord(buf->ev, ctx->stream));
+  job.path = path;
+  job.dim = D;
+  job.np = ctx->np;
+  job.lims[0] = ctx->prm.xmax - ctx->prm.xmin;  // Grid.jl: lx = xmax - xmin
+  job.lims[1] = ctx->prm.ymax - ctx->prm.ymin;
+  job.lims[2] = D == 3 ? ctx->prm.zmax - ctx->prm.zmin : 0.0;
+  for (const HostSolid& hs : ctx->solids) {
+    job.solid_end.push_back(hs.offset + hs.n);
+    // Output.jl:176-177 (2D: elastic or rigid), :243 (3D: elastic)
+    job.elastic.push_back(hs.mat.kind == JUMP_MAT_ELASTIC || (D == 2 && hs.mat.kind == JUMP_MAT_RIGID));
+  }
+  hipEvent_t ev = buf->ev;
+  job.ready = [ev] { return hipEventSynchronize(ev) == hipSuccess; };
+  job.release = [buf] { buf->busy = false; };
+  ctx->dump_writer.push(std::move(job));
+  return JUMP_OK;
+}
+
+int jump_dump_wait(jump_ctx* ctx) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (!ctx->dump_writer.wait()) return set_err(ctx, JUMP_ERR_IO, "a LAMMPS dump could not be written");
+  return JUMP_OK;
 }
 
 int jump_get_bins(jump_ctx* ctx, int32_t* keys) {

@@ -5,6 +5,7 @@ reference's AoS layout: positions (n, d), matrices (n, d*d) column-major per
 particle (the memory of Vector{SMatrix{d,d,Float64}}).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -173,6 +174,17 @@ class Engine:
         ke, se = ctypes.c_double(), ctypes.c_double()
         self._check(self._f("energies")(self.h, ctypes.byref(ke), ctypes.byref(se)))
         return ke.value, se.value
+
+    def dump_lammps(self, path, outs=()):
+        """Queue a LAMMPS particle dump of the current state (OvitoOutput,
+        Output.jl:146-260) appended to `path`; formatted and written by the
+        library's writer thread (HIP library only)."""
+        cols = outs if isinstance(outs, str) else " ".join(outs)
+        self._check(self.lib.jump_dump_lammps(self.h, os.fsencode(path), cols.encode()))
+
+    def dump_wait(self):
+        """Block until every queued dump is on disk."""
+        self._check(self.lib.jump_dump_wait(self.h))
 
     def profile(self, enable=True):
         """Enable (and reset) per-phase HIP-event timing (HIP library only)."""

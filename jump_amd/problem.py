@@ -22,6 +22,7 @@ from .basis import GIMPBasis
 from .dirichlet import fix_Dirichlet_grid
 from .engine import Engine, make_params
 from .fix import EnergiesFix, closeFile, compute
+from .output import OvitoOutput
 
 
 def _fix_list(fixes):
@@ -108,9 +109,12 @@ def _time_loop(eng, output, fixes, data):
             flags = eng.flags()
             sys.stderr.write(f"jump: particle error flags 0x{flags:x} near t={tc}\n")
         if out:
+            if isinstance(output, OvitoOutput):  # plotParticles_2D/3D (:575-576)
+                eng.dump_lammps(output.dump_path(counter + k - 1), output.outs)
             compute(fl, tc)
         t = t_new
         counter += k
+    eng.dump_wait()
     data["time_end"] = t
     return counter
 
