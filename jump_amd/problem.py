@@ -31,7 +31,11 @@ def _fix_list(fixes):
     return list(fixes) if isinstance(fixes, (list, tuple)) else [fixes]
 
 
-def build_engine(grid, solids, basis, alg, data, device=0):
+def build_engine(grid, solids, basis, alg, data, device=0, engine=None):
+    """The configured context of a solve: parameters from the grid, basis,
+    algorithm and data; solids added in order; the Dirichlet mask of
+    fix_Dirichlet_grid.  `engine(params)` makes the context (default: the
+    HIP library's Engine on `device`)."""
     dim = grid.dim
     dt = float(data["dt"])
     if dim == 2:
@@ -43,7 +47,7 @@ def build_engine(grid, solids, basis, alg, data, device=0):
     params = make_params(dim, basis.code, alg.code, dt, grid.lo, grid.hi, grid.nodes,
                          tolerance=getattr(alg, "tolerance", 0.0), alpha=getattr(alg, "alpha", 1.0), body=body,
                          ghostcell=ghost)
-    eng = Engine(params, device=device)
+    eng = Engine(params, device=device) if engine is None else engine(params)
     for s in solids:
         lp = s.lp
         if isinstance(basis, GIMPBasis) and lp is None and not s.rigid:

@@ -128,11 +128,19 @@ class _SolidBase:
 
 class Solid2D(_SolidBase):
     """Solid.jl:67-259.  Solid2D(coords, mat); Solid2D(nodes, elems, mat) for
-    CPDI-Q4 particles (corners (nc, 2), elems (n, 4) 0-based)."""
+    CPDI-Q4 particles (corners (nc, 2), elems (n, 4) 0-based);
+    Solid2D(fileName, mat) for CPDI-Q4 particles of a Gmsh mesh
+    (Solid.jl:162-222: elements oriented counter-clockwise first)."""
 
     dim = 2
 
     def __init__(self, *args, lp=None):
+        if len(args) == 2 and isinstance(args[0], (str, bytes)):
+            from .mesh import loadGMSH, orient_q4
+            coords3, elems1 = loadGMSH(args[0])
+            nodes2 = np.ascontiguousarray(coords3[:2])
+            elems1 = orient_q4(nodes2, elems1)
+            args = (nodes2.T, elems1 - 1, args[1])
         if len(args) == 2:
             coords, mat = args
             self._init_common(coords, mat)
@@ -247,6 +255,13 @@ def buildParticleForBlock(fCenter, fWidth, fHeight, fThickness, fOffset):
 def assign_velocity(solid, v0):
     """Solid.jl:532-537"""
     solid.velocity[:] = np.asarray(v0, dtype=np.float64).reshape(1, -1)
+
+
+def move_cpdi(solid, dx):
+    """Solid.jl:539-548: moves the corner nodes and the particles."""
+    d = np.asarray(dx, dtype=np.float64).reshape(1, -1)
+    solid.nodes += d
+    solid.pos += d
 
 
 def move(solid, dx):

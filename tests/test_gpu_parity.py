@@ -444,3 +444,33 @@ def test_contact_refusals(hip_lib):
     with pytest.raises(JumpError):
         adjoint.rrule(g, 2)
     g.close()
+
+
+def test_cpdi_two_disks_from_gmsh(hip_lib):
+    """Main_TwoDisksCPDI.jl's set-up through the host mirror: two disks of
+    CPDI-Q4 domains read from disk.msh (Solid2D(fileName, mat)), placed with
+    move_cpdi, on the 1.5 x 1.5 grid of 11 x 11 nodes; 300 steps of the USL
+    step (the example selects the reference's 2D MUSL, which is not on this
+    path) equal the restatement bit for bit."""
+    _say("cpdi two disks from disk.msh")
+    from jump_amd import algorithm, basis, bodyforce, grid, material, problem, solid
+    from oracle.pyoracle import OracleEngine
+    path = os.path.join(GOLD, "disk.msh")
+    g = grid.Grid2D(0.0, 1.5, 0.0, 1.5, 11, 11)
+    m = material.ElasticMaterial(1000.0, 0.3, 1000.0, 0, 0, 1000)
+    s1, s2 = solid.Solid2D(path, m), solid.Solid2D(path, m)
+    solid.assign_velocity(s1, (0.1, 0.1))
+    solid.assign_velocity(s2, (-0.1, -0.1))
+    solid.move_cpdi(s1, (0.45, 0.45))
+    solid.move_cpdi(s2, (0.45 + 0.6, 0.45 + 0.6))
+    data = {"total_time": 3.0, "dt": 1e-3, "time": 0.0, "bodyforce": bodyforce.ConstantBodyForce2D((0.0, 0.0))}
+    engines = [problem.build_engine(g, [s1, s2], basis.CPDIQ4Basis(), algorithm.USL(1e-9), data, engine=cls)
+               for cls in (Engine, OracleEngine)]
+    for e in engines:
+        _, rc = e.step(300, 0.0)
+        assert rc == 0
+    for s in range(2):
+        a, b = engines[0].get_solid(s), engines[1].get_solid(s)
+        for k in ("pos", "vel", "volume", "F", "strain", "stress"):
+            _same(a[k], b[k], True, f"solid {s} {k}")
+    _close(engines[0].energies(), engines[1].energies(), rtol=1e-12, what="energies")
