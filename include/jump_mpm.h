@@ -104,13 +104,16 @@ typedef struct {
   int64_t nx, ny, nz; /* node counts iN_x, iN_y, iN_z (nz ignored in 2D)   */
   double body[3];     /* ConstantBodyForce2D/3D g (BodyForce.jl:56-91)     */
   int32_t order;      /* jump_order (0: reference summation order)          */
-  int32_t contact;    /* 2D USL: 1 = multi-body contact between solids     */
-  double friction;    /* Coulomb coefficient of that contact               */
+  int32_t contact;    /* 1 = multi-body contact between solids             */
+  double friction;    /* Coulomb coefficient of the 2D contact             */
 } jump_params;
 
 /* Multi-body contact (contact = 1).  The reference's ULMPM step has none:
  * all solids share one velocity field.  This is the algorithm of its FEMP
- * contact solver (SolveExplicitFEMP2DContact.jl:311-387) on the MPM grid:
+ * contact solvers (SolveExplicitFEMP2DContact.jl:311-387 on the 2D USL
+ * step; SolveExplicitFEMP3DContact.jl:398-511 on the 3D MUSL step, where
+ * the correction is normal-only, v -= D n, and friction is not used; the
+ * re-projected momentum is not corrected again) on the MPM grid:
  * each node keeps the mass, momenta, force and normal sum(m_p grad N) of at
  * most two solids (in particle order; a third raises
  * JUMP_FLAG_CONTACT_OVERFLOW); where two solids meet, the common normal is

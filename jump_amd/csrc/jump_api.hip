@@ -54,7 +54,8 @@ struct jump_ctx {
   // device memory
   Particles P{};
   GridArrays g{};
-  ContactArrays cg{};  // contact mode only
+  ContactArrays cg{};    // contact mode only, 2D
+  ContactArrays3 cg3{};  // contact mode only, 3D
   std::vector<void*> part_allocs;
   std::vector<void*> grid_allocs;
   double* d_corners = nullptr;
@@ -337,7 +338,16 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_sums, (nn + 1) / kScanItems + 2);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_flags, 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_wmax, 1);
-  if (ctx->prm.contact) {
+  if (ctx->prm.contact && D == 3) {
+    ContactArrays3& c = ctx->cg3;
+    for (int k = 0; k < 2 && !r; k++) {
+      r = dev_alloc(ctx, ctx->grid_allocs, &c.body[k], nn);
+      if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.m[k], nn);
+      for (int a = 0; a < 3 && !r; a++) r = dev_alloc(ctx, ctx->grid_allocs, &c.p0[k][a], nn);
+      for (int a = 0; a < 3 && !r; a++) r = dev_alloc(ctx, ctx->grid_allocs, &c.p[k][a], nn);
+      for (int a = 0; a < 3 && !r; a++) r = dev_alloc(ctx, ctx->grid_allocs, &c.p2[k][a], nn);
+    }
+  } else if (ctx->prm.contact) {
     ContactArrays& c = ctx->cg;
     for (int k = 0; k < 2 && !r; k++) {
       r = dev_alloc(ctx, ctx->grid_allocs, &c.body[k], nn);
@@ -480,6 +490,30 @@ template <int B, int ORDER> static void step_usl2d_contact(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_G2P);
 }
 
+// 3D MUSL with multi-body contact: slot P2G with the grid update and the
+// normal correction, G2P-1, slot re-projection and G2P-2 on the own slot.
+template <int B, int ORDER> static void step_musl3d_contact(jump_ctx* ctx) {
+  const int np = int(ctx->np);
+  const double dt = ctx->prm.dt;
+  prof_begin(ctx);
+  bin_and_sort<3, B, false>(ctx);
+  prof_end(ctx, JUMP_PHASE_SORT);
+  prof_begin(ctx);
+  LAUNCH_P2G((k_p2g_contact3d<B, ORDER>), ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->d_bxyz,
+             ctx->g, ctx->cg3, dt, ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_P2G);
+  prof_begin(ctx);
+  LAUNCH((k_g2p1_musl_contact<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->cg3, ctx->prm.alpha, ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_G2P1);
+  prof_begin(ctx);
+  LAUNCH_P2G((k_p2g2_contact3d<B, ORDER>), ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->d_bxyz,
+             ctx->g, ctx->cg3);
+  prof_end(ctx, JUMP_PHASE_P2G2);
+  prof_begin(ctx);
+  LAUNCH((k_g2p2_musl_contact<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->cg3, dt, ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_G2P2);
+}
+
 template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
@@ -547,8 +581,13 @@ static int run_steps(jump_ctx* ctx, int64_t nsteps) {
       switch (ctx->prm.basis) {
 #define MUSL_CASE(BB)                                                    \
   case BB:                                                               \
-    if (ref) step_musl3d<BB, JUMP_ORDER_REFERENCE>(ctx);                 \
-    else step_musl3d<BB, JUMP_ORDER_BINNED>(ctx);                        \
+    if (ctx->prm.contact) {                                              \
+      if (ref) step_musl3d_contact<BB, JUMP_ORDER_REFERENCE>(ctx);       \
+      else step_musl3d_contact<BB, JUMP_ORDER_BINNED>(ctx);              \
+    } else {                                                             \
+      if (ref) step_musl3d<BB, JUMP_ORDER_REFERENCE>(ctx);               \
+      else step_musl3d<BB, JUMP_ORDER_BINNED>(ctx);                      \
+    }                                                                    \
     break;
         MUSL_CASE(JUMP_BASIS_LINEAR)
         MUSL_CASE(JUMP_BASIS_QUAD_BSPLINE)
@@ -582,7 +621,6 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   if (p->order != JUMP_ORDER_REFERENCE && p->order != JUMP_ORDER_BINNED) return JUMP_ERR_ARG;
   if (p->nx >= 65536 || p->ny >= 65536 || nz >= 65536) return JUMP_ERR_UNSUPPORTED;
   if (p->contact != 0 && p->contact != 1) return JUMP_ERR_ARG;
-  if (p->contact && p->dim != 2) return JUMP_ERR_UNSUPPORTED;  // the 2D USL step only
   if (p->contact && !(p->friction >= 0.0 && std::isfinite(p->friction))) return JUMP_ERR_ARG;
   jump_ctx* ctx = new (std::nothrow) jump_ctx();
   if (!ctx) return JUMP_ERR_NOMEM;

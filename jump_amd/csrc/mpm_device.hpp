@@ -101,6 +101,15 @@ struct ContactArrays {
   double* v0[2][2];
   double* v[2][2];
 };
+// 3D (MUSL): per slot the momenta p0, p (after the force update, Dirichlet
+// and the contact correction) and the re-projected p2.  [slot][comp][node].
+struct ContactArrays3 {
+  int* body[2];
+  double* m[2];
+  double* p0[2][3];
+  double* p[2][3];
+  double* p2[2][3];
+};
 
 // A batch steps several independent 2D problems with the same launches.
 // Instance i has its own grid, whose nodes (and bins) occupy

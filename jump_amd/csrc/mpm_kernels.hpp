@@ -957,34 +957,10 @@ __global__ void __launch_bounds__(kBlock) k_g2p1_musl(Particles P, GridDesc G, c
   for (int k = 0; k < 3; k++) P.v[k][i] = alpha * P.v[k][i] + vx[k];
 }
 
-// G2P-2, SolveExplicitSolidMechanics3D.jl:161-211
-template <int B>
-__global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, const SolidDev* __restrict__ S,
-                                                      const double* __restrict__ corners, GridArrays g, double dt,
-                                                      unsigned* __restrict__ flags) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const SolidDev& s = S[P.sid[i]];
-  double xx[3] = {P.x[0][i], P.x[1][i], P.x[2][i]};
-  double L[9];
-  for (int k = 0; k < 9; k++) L[k] = 0.0;
-  bool ok = walk_stencil<3, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
-    const double m = g.m[n];
-    if (m > 0.0) {
-      const double mii = 1.0 / m;
-      const double vIx = mii * g.p2[0][n], vIy = mii * g.p2[1][n], vIz = mii * g.p2[2][n];
-      const double c = Ni * mii * dt;
-      for (int k = 0; k < 3; k++) xx[k] += c * g.p[k][n];
-      // L[a,b] = dN_a v_b (Basis gradient index first, as SMatrix{3,3}(...) at :189-191)
-      L[0] += dN[0] * vIx; L[1] += dN[1] * vIx; L[2] += dN[2] * vIx;
-      L[3] += dN[0] * vIy; L[4] += dN[1] * vIy; L[5] += dN[2] * vIy;
-      L[6] += dN[0] * vIz; L[7] += dN[1] * vIz; L[8] += dN[2] * vIz;
-    }
-  });
-  if (!ok) {
-    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
-    return;
-  }
+// The particle update of G2P-2 (SolveExplicitSolidMechanics3D.jl:186-211)
+// from the gathered position xx and velocity gradient L[a,b] = dN_a v_b.
+__device__ __forceinline__ void musl3d_particle_update(const Particles& P, const SolidDev& s, int i, double dt,
+                                                       const double* xx, const double* L, unsigned* flags) {
   for (int k = 0; k < 3; k++) P.x[k][i] = xx[k];
   // symmetric part: D[a,b] = 0.5 (L[a,b] + L[b,a])
   const double D[6] = {0.5 * (L[0] + L[0]), 0.5 * (L[4] + L[4]), 0.5 * (L[8] + L[8]),
@@ -1044,6 +1020,263 @@ __global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, c
   for (int k = 0; k < 6; k++) P.sig[k][i] = sg[k];
   if (!(finite3(xx[0], xx[1], xx[2]) && finite3(P.v[0][i], P.v[1][i], P.v[2][i]))) fl |= JUMP_FLAG_NONFINITE;
   if (fl) atomicOr(flags, fl);
+}
+
+// G2P-2, SolveExplicitSolidMechanics3D.jl:161-211
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                      const double* __restrict__ corners, GridArrays g, double dt,
+                                                      unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  double xx[3] = {P.x[0][i], P.x[1][i], P.x[2][i]};
+  double L[9];
+  for (int k = 0; k < 9; k++) L[k] = 0.0;
+  bool ok = walk_stencil<3, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
+    const double m = g.m[n];
+    if (m > 0.0) {
+      const double mii = 1.0 / m;
+      const double vIx = mii * g.p2[0][n], vIy = mii * g.p2[1][n], vIz = mii * g.p2[2][n];
+      const double c = Ni * mii * dt;
+      for (int k = 0; k < 3; k++) xx[k] += c * g.p[k][n];
+      // L[a,b] = dN_a v_b (Basis gradient index first, as SMatrix{3,3}(...) at :189-191)
+      L[0] += dN[0] * vIx; L[1] += dN[1] * vIx; L[2] += dN[2] * vIx;
+      L[3] += dN[0] * vIy; L[4] += dN[1] * vIy; L[5] += dN[2] * vIy;
+      L[6] += dN[0] * vIz; L[7] += dN[1] * vIz; L[8] += dN[2] * vIz;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  musl3d_particle_update(P, s, i, dt, xx, L, flags);
+}
+
+// ------------------------------------------------------------------ contact (3D)
+// The MUSL step with multi-body contact: per-solid node slots as in 2D; the
+// normal-only correction of SolveExplicitFEMP3DContact.jl:398-511 (v -= D n
+// for a slot approaching the centre-of-mass velocity along its normal, n =
+// (n0 - n1)/|n0 - n1| from the slots' sums of m grad N, -n for slot 1) at
+// the grid update; G2P-1, the re-projection and G2P-2 each on the particle's
+// own solid's slot.  The FEMP solver's gap test on its mesh's surface
+// positions has no counterpart on MPM solids and is not applied.
+struct ContactSlot3 {
+  int body = -1;
+  double m = 0.0, p0[3] = {0.0, 0.0, 0.0}, f[3] = {0.0, 0.0, 0.0}, nr[3] = {0.0, 0.0, 0.0};
+};
+
+template <int B, int ORDER>
+__global__ void __launch_bounds__(kP2GBlock) k_p2g_contact3d(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                             const int* __restrict__ start,
+                                                             const unsigned long long* __restrict__ sorted,
+                                                             const unsigned long long* __restrict__ bxyz,
+                                                             GridArrays g, ContactArrays3 c, double dt,
+                                                             unsigned* __restrict__ flags) {
+  constexpr int MW = MaxW<B>::W;
+  constexpr int NB = MW * MW * MW;
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  const int n = blockIdx.x * kP2GBlock + threadIdx.x;
+  if (n >= G.nn) return;
+  const int ni = n % G.nx, nj = (n / G.nx) % G.ny, nk = n / G.nxy;
+  ContactSlot3 s0, s1;
+  bool overflow = false;
+  auto add = [&](ContactSlot3& s, int sid, double N, const double* dN, const double* v) {
+    const double fM = v[3], fV = v[4];
+    const double* vp = v + 5;
+    const double* sg = v + 8;
+    const double Nim = N * fM;
+    s.body = sid;
+    s.m += Nim;
+    for (int a = 0; a < 3; a++) s.p0[a] += Nim * vp[a];
+    s.f[0] -= fV * (sg[0] * dN[0] + sg[3] * dN[1] + sg[4] * dN[2]);
+    s.f[1] -= fV * (sg[3] * dN[0] + sg[1] * dN[1] + sg[5] * dN[2]);
+    s.f[2] -= fV * (sg[4] * dN[0] + sg[5] * dN[1] + sg[2] * dN[2]);
+    for (int a = 0; a < 3; a++) s.nr[a] += fM * dN[a];
+  };
+  for_window_particles<3, B, ORDER, false>(P, G, start, sorted, bxyz, 0, 0, Width<B>::W, ni, nj, nk, s_cur, s_head,
+                                           [&](int p) {
+    double v[PairLayout<3>::NF + 1];
+    P2GFields<3, 0>::load(P, unsigned(p), v);
+    double lp[3] = {0.0, 0.0, 0.0};
+    const int sid = ldu(P.sid, unsigned(p));
+    if (B == JUMP_BASIS_GIMP)
+      for (int a = 0; a < 3; a++) lp[a] = S[sid].lp[a];
+    double N, dN[3];
+    bool member;
+    node_weight<3, B, false, true>(P, G, lp, nullptr, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
+    if (s0.body < 0 || s0.body == sid) add(s0, sid, N, dN, v);
+    else if (s1.body < 0 || s1.body == sid) add(s1, sid, N, dN, v);
+    else overflow = true;
+  });
+  if (overflow) atomicOr(flags, JUMP_FLAG_CONTACT_OVERFLOW);
+  const uint8_t fx = g.fixed[n];
+  double pk[2][3];
+  ContactSlot3* sl[2] = {&s0, &s1};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    ContactSlot3& s = *sl[k];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      pk[k][a] = s.p0[a] + s.f[a] * dt;
+      if (fx & (1u << a)) { s.p0[a] = 0.0; pk[k][a] = 0.0; }
+    }
+  }
+  g.m[n] = s0.m + s1.m;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    g.p0[a][n] = s0.p0[a] + s1.p0[a];
+    g.p[a][n] = pk[0][a] + pk[1][a];
+    g.f[a][n] = s0.f[a] + s1.f[a];
+  }
+  if (s0.body >= 0 && s1.body >= 0) {
+    const double mS = s0.m + s1.m;
+    double vcm[3], nc[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) vcm[a] = (pk[0][a] + pk[1][a]) / mS;
+#pragma unroll
+    for (int a = 0; a < 3; a++) nc[a] = s0.nr[a] - s1.nr[a];
+    const double nrm = sqrt(nc[0] * nc[0] + nc[1] * nc[1] + nc[2] * nc[2]);
+#pragma unroll
+    for (int a = 0; a < 3; a++) nc[a] = nc[a] / nrm;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const double m = sl[k]->m;
+      double v[3], nk[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        v[a] = pk[k][a] / m;
+        nk[a] = k == 0 ? nc[a] : -nc[a];
+      }
+      const double Dn = (v[0] - vcm[0]) * nk[0] + (v[1] - vcm[1]) * nk[1] + (v[2] - vcm[2]) * nk[2];
+      if (!(Dn > 0.0)) continue;
+#pragma unroll
+      for (int a = 0; a < 3; a++) pk[k][a] = (v[a] - Dn * nk[a]) * m;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    c.body[k][n] = sl[k]->body;
+    c.m[k][n] = sl[k]->m;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      c.p0[k][a][n] = sl[k]->p0[a];
+      c.p[k][a][n] = pk[k][a];
+    }
+  }
+}
+
+// Slot of solid sid at node n (-1: the solid was dropped there).
+__device__ __forceinline__ int contact_slot(const ContactArrays3& c, int n, int sid) {
+  return c.body[0][n] == sid ? 0 : (c.body[1][n] == sid ? 1 : -1);
+}
+
+// G2P-1 on the own slot (SolveExplicitSolidMechanics3D.jl:109-134).
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p1_musl_contact(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                              ContactArrays3 c, double alpha,
+                                                              unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int sid = P.sid[i];
+  const SolidDev& s = S[sid];
+  double vx[3] = {0.0, 0.0, 0.0};
+  bool ok = walk_stencil<3, B, true, false>(P, G, s, nullptr, i, [&](int n, double Ni, const double*) {
+    const int k = contact_slot(c, n, sid);
+    if (k < 0) return;
+    const double mI = (k ? c.m[1] : c.m[0])[n];
+    if (mI > 0.0) {
+      const double mii = (1.0 / mI) * Ni;
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        vx[a] += ((k ? c.p[1][a] : c.p[0][a])[n] - alpha * (k ? c.p0[1][a] : c.p0[0][a])[n]) * mii;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  for (int a = 0; a < 3; a++) P.v[a][i] = alpha * P.v[a][i] + vx[a];
+}
+
+// Re-projection per slot (SolveExplicitSolidMechanics3D.jl:135-138): the
+// node gathers its window's particles in index order, each into its own
+// solid's slot (as PASS 1 of k_p2g); the slot totals go to g.p2.
+template <int B, int ORDER>
+__global__ void __launch_bounds__(kP2GBlock) k_p2g2_contact3d(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                              const int* __restrict__ start,
+                                                              const unsigned long long* __restrict__ sorted,
+                                                              const unsigned long long* __restrict__ bxyz,
+                                                              GridArrays g, ContactArrays3 c) {
+  constexpr int MW = MaxW<B>::W;
+  constexpr int NB = MW * MW * MW;
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  const int n = blockIdx.x * kP2GBlock + threadIdx.x;
+  if (n >= G.nn) return;
+  const int ni = n % G.nx, nj = (n / G.nx) % G.ny, nk = n / G.nxy;
+  const int b0 = c.body[0][n], b1 = c.body[1][n];
+  double a0x = 0.0, a0y = 0.0, a0z = 0.0, a1x = 0.0, a1y = 0.0, a1z = 0.0;
+  for_window_particles<3, B, ORDER, true>(P, G, start, sorted, bxyz, 0, 0, Width<B>::W, ni, nj, nk, s_cur, s_head,
+                                          [&](int p) {
+    const int sid = ldu(P.sid, unsigned(p));
+    if (sid != b0 && sid != b1) return;  // dropped at this node (overflow)
+    double q[3] = {0.0, 0.0, 0.0};
+    double am = 0.0, af[3];
+    p2g_add<3, B, 1>(P, G, S, nullptr, p, ni, nj, nk, am, q, af);
+    // both sums formed, one kept: selects keep the six sums in registers
+    const bool first = sid == b0;
+    a0x = first ? a0x + q[0] : a0x;
+    a0y = first ? a0y + q[1] : a0y;
+    a0z = first ? a0z + q[2] : a0z;
+    a1x = first ? a1x : a1x + q[0];
+    a1y = first ? a1y : a1y + q[1];
+    a1z = first ? a1z : a1z + q[2];
+  });
+  c.p2[0][0][n] = a0x; c.p2[0][1][n] = a0y; c.p2[0][2][n] = a0z;
+  c.p2[1][0][n] = a1x; c.p2[1][1][n] = a1y; c.p2[1][2][n] = a1z;
+  g.p2[0][n] = a0x + a1x;
+  g.p2[1][n] = a0y + a1y;
+  g.p2[2][n] = a0z + a1z;
+}
+
+// G2P-2 on the own slot (SolveExplicitSolidMechanics3D.jl:161-211).
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p2_musl_contact(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                              ContactArrays3 c, double dt,
+                                                              unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int sid = P.sid[i];
+  const SolidDev& s = S[sid];
+  double xx[3] = {P.x[0][i], P.x[1][i], P.x[2][i]};
+  double L[9];
+  for (int k = 0; k < 9; k++) L[k] = 0.0;
+  bool ok = walk_stencil<3, B, false, true>(P, G, s, nullptr, i, [&](int n, double Ni, const double* dN) {
+    const int k = contact_slot(c, n, sid);
+    if (k < 0) return;
+    // the slot's arrays by selects (an index into the argument struct
+    // would copy it to scratch)
+    const double m = (k ? c.m[1] : c.m[0])[n];
+    if (m > 0.0) {
+      const double mii = 1.0 / m;
+      const double vIx = mii * (k ? c.p2[1][0] : c.p2[0][0])[n], vIy = mii * (k ? c.p2[1][1] : c.p2[0][1])[n],
+                   vIz = mii * (k ? c.p2[1][2] : c.p2[0][2])[n];
+      const double cc = Ni * mii * dt;
+      xx[0] += cc * (k ? c.p[1][0] : c.p[0][0])[n];
+      xx[1] += cc * (k ? c.p[1][1] : c.p[0][1])[n];
+      xx[2] += cc * (k ? c.p[1][2] : c.p[0][2])[n];
+      L[0] += dN[0] * vIx; L[1] += dN[1] * vIx; L[2] += dN[2] * vIx;
+      L[3] += dN[0] * vIy; L[4] += dN[1] * vIy; L[5] += dN[2] * vIy;
+      L[6] += dN[0] * vIz; L[7] += dN[1] * vIz; L[8] += dN[2] * vIz;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  musl3d_particle_update(P, s, i, dt, xx, L, flags);
 }
 
 // ------------------------------------------------------------------ energies

@@ -6,7 +6,7 @@ into any Engine (the HIP library, or the oracle in tests).
 
   two_disks      Main_TwoDisks.jl:40-116 at 20x20 cells, linear basis
   disks_contact  two disks colliding head-on or obliquely, with or without
-                 multi-body contact
+                 multi-body contact; spheres_contact: the same in 3D (MUSL)
   impact3d       3D high-velocity impact: elastic sphere on a J2 plate,
                  quadratic B-splines, MUSL (Main_HighVelocity_Impact_3D.jl)
   taylor_bar     3D Taylor bar (Main_TaylorBar.jl), J2 copper, GIMP
@@ -148,6 +148,34 @@ def disks_contact(cells=32, ppc=2, r=0.15, gap_cells=2, v=0.1, offset=0.0, basis
     p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (1.0, 1.0), (n, n), tolerance=1e-9, contact=contact,
                     friction=friction)
     return Scene("disks_contact", p, [s1, s2], steps=steps)
+
+
+def spheres_contact(cells=24, r=0.15, gap_cells=2, v=0.1, offset=0.0, basis=_ffi.BASIS_QUAD_BSPLINE,
+                    material="elastic", contact=True, steps=600, dt=1e-3, alpha=1.0):
+    """Two spheres (2 particles per cell and direction, Two-Disks material
+    E=1000, nu=0.3, rho=1000) on the x axis of the unit cube, `gap_cells`
+    cells apart, moving towards each other at +-v, 3D MUSL; `offset` shifts
+    the left sphere down in y and the right one up (an oblique impact).
+    Centres sit on grid nodes.  With contact on, the spheres interact only
+    through the normal contact correction
+    (SolveExplicitFEMP3DContact.jl:398-511)."""
+    n = cells + 1
+    dx = 1.0 / cells
+    f = dx / 2
+    rr = math.floor(r / f) * f
+    half = round((rr + 0.5 * gap_cells * dx) / dx) * dx
+    dyc = round(0.5 * offset / dx) * dx
+    c1 = buildParticleForSphere([0.5 - half, 0.5 - dyc, 0.5], r, f)
+    c2 = buildParticleForSphere([0.5 + half, 0.5 + dyc, 0.5], r, f)
+    mat = {"elastic": elastic, "neohookean": neohookean}[material](1000.0, 0.3, 1000.0) \
+        if material != "j2" else j2(1000.0, 0.3, 1000.0, 5.0, 100.0)
+    vol = f ** 3
+    lp = 0.5 * f if basis == _ffi.BASIS_GIMP else None
+    s1 = _solid(mat, c1, (v, 0.0, 0.0), vol, 1000.0, lp=lp)
+    s2 = _solid(mat, c2, (-v, 0.0, 0.0), vol, 1000.0, lp=lp)
+    p = make_params(3, basis, _ffi.ALG_MUSL, dt, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (n, n, n), alpha=alpha,
+                    contact=contact)
+    return Scene("spheres_contact", p, [s1, s2], steps=steps)
 
 
 def rings2d(cells=40, basis=_ffi.BASIS_LINEAR, material="neohookean", platen=True, steps=50, v=2.0, seed=0):

@@ -107,11 +107,13 @@ template <class T> struct Ctx {
   std::vector<uint8_t> fixed;              // dim*nn
   std::vector<Solid<T>> solids;
   uint32_t flags = 0;
-  bool contact = false;  // multi-body contact (2D USL)
+  bool contact = false;  // multi-body contact (2D USL, 3D MUSL)
   T fric;
-  // contact: per node two solid slots
+  // contact: per node two solid slots; per-slot vectors are indexed
+  // (2 node + slot) dim + component
   std::vector<int> cbody;                    // 2*nn, -1 empty
-  std::vector<T> cm, cp0, cf, cn, cv0, cv;   // 2*nn, 4*nn (slot, component)
+  std::vector<T> cm, cp0, cf, cn, cv0, cv;   // 2*nn, 2*dim*nn
+  std::vector<T> cp, cp2;                    // 3D: momentum, momentum2
   std::string err;
   // scratch for the support (Basis.jl initialise, :1658-1727)
   std::vector<i64> near;
@@ -1025,6 +1027,40 @@ template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
   (void)t;
 }
 
+// The particle update of G2P-2 (SolveExplicitSolidMechanics3D.jl:186-211)
+// from the gathered position xx and velocity gradient L[a,b] = dN_a v_b.
+template <class T> void musl3d_particle_update(Ctx<T>& g, Solid<T>& s, i64 ip, const T* xx, const T* L) {
+  const T dt = g.dt;
+  for (int k = 0; k < 3; k++) s.pos[3 * ip + k] = xx[k];
+  T D[9], dD[9];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) D[a + 3 * b] = 0.5 * (L[a + 3 * b] + L[b + 3 * a]);
+  T* eps = &s.strain[9 * ip];
+  for (int k = 0; k < 9; k++) { dD[k] = dt * D[k]; eps[k] = eps[k] + dD[k]; }
+  T A[9];
+  for (int k = 0; k < 9; k++) A[k] = L[k] * dt;
+  A[0] = 1.0 + A[0]; A[4] = 1.0 + A[4]; A[8] = 1.0 + A[8];
+  T* F = &s.F[9 * ip];
+  T Fn[9];
+  matmul3(F, A, Fn);
+  for (int k = 0; k < 9; k++) F[k] = Fn[k];
+  T J = det3(F);
+  s.vol[ip] = J * s.vol0[ip];
+  if (val(J) < 0.0) g.flags |= JUMP_FLAG_NEGATIVE_J;
+  T* sg = &s.stress[9 * ip];
+  switch (s.mat.kind) {
+    case JUMP_MAT_ELASTIC: stress_hooke_3d(s.mat, eps, sg, s.vm[ip]); break;
+    case JUMP_MAT_NEOHOOKEAN: stress_nh(3, s.mat, F, J, sg); break;
+    case JUMP_MAT_ELASTOPLASTIC: stress_j2_3d(s.mat, eps, sg, &s.pstrain[9 * ip], s.alpha[ip], s.vm[ip]); break;
+    case JUMP_MAT_JOHNSONCOOK: stress_jc(s.mat, eps, dD, dt, sg, s.alpha[ip], s.alphadot[ip], s.vm[ip]); break;
+    default: break;
+  }
+  if (!std::isfinite(val(xx[0])) || !std::isfinite(val(xx[1])) || !std::isfinite(val(xx[2])) ||
+      !std::isfinite(val(s.vel[3 * ip])) || !std::isfinite(val(s.vel[3 * ip + 1])) ||
+      !std::isfinite(val(s.vel[3 * ip + 2])))
+    g.flags |= JUMP_FLAG_NONFINITE;
+}
+
 // SolveExplicitSolidMechanics3D.jl:46-211 (one iteration, no output)
 template <class T> void step_musl_3d(Ctx<T>& g) {
   const T dt = g.dt, alpha = g.alpha;
@@ -1113,34 +1149,164 @@ template <class T> void step_musl_3d(Ctx<T>& g) {
           L[6] += dNx * vIz; L[7] += dNy * vIz; L[8] += dNz * vIz;
         }
       }
-      for (int k = 0; k < 3; k++) s.pos[3 * ip + k] = xx[k];
-      T D[9], dD[9];
-      for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) D[a + 3 * b] = 0.5 * (L[a + 3 * b] + L[b + 3 * a]);
-      T* eps = &s.strain[9 * ip];
-      for (int k = 0; k < 9; k++) { dD[k] = dt * D[k]; eps[k] = eps[k] + dD[k]; }
-      T A[9];
-      for (int k = 0; k < 9; k++) A[k] = L[k] * dt;
-      A[0] = 1.0 + A[0]; A[4] = 1.0 + A[4]; A[8] = 1.0 + A[8];
-      T* F = &s.F[9 * ip];
-      T Fn[9];
-      matmul3(F, A, Fn);
-      for (int k = 0; k < 9; k++) F[k] = Fn[k];
-      T J = det3(F);
-      s.vol[ip] = J * s.vol0[ip];
-      if (val(J) < 0.0) g.flags |= JUMP_FLAG_NEGATIVE_J;
-      T* sg = &s.stress[9 * ip];
-      switch (s.mat.kind) {
-        case JUMP_MAT_ELASTIC: stress_hooke_3d(s.mat, eps, sg, s.vm[ip]); break;
-        case JUMP_MAT_NEOHOOKEAN: stress_nh(3, s.mat, F, J, sg); break;
-        case JUMP_MAT_ELASTOPLASTIC: stress_j2_3d(s.mat, eps, sg, &s.pstrain[9 * ip], s.alpha[ip], s.vm[ip]); break;
-        case JUMP_MAT_JOHNSONCOOK: stress_jc(s.mat, eps, dD, dt, sg, s.alpha[ip], s.alphadot[ip], s.vm[ip]); break;
-        default: break;
+      musl3d_particle_update(g, s, ip, xx, L);
+    }
+  }
+}
+
+// The 3D MUSL step with multi-body contact: the normal-only correction of
+// the FEMP 3D contact solver for two deformable solids
+// (SolveExplicitFEMP3DContact.jl:398-511: n = (n1 - n2)/|n1 - n2|, -n for
+// the second; a solid whose velocity approaches the centre-of-mass velocity
+// along its normal, D = (v - v_cm).n > 0, gets v -= D n), at the grid update
+// of the MUSL step.  Per-solid slots as in 2D (first two solids of a node in
+// particle order); normals sum m grad N over the solid's particles.  The
+// FEMP solver also requires |(x1 - x2).n| < dy/40 for nodal positions
+// summed over its mesh's surface particles; an MPM solid has no surface
+// particles, and over all particles that test lets two spheres interpenetrate
+// by more than two cells before it passes, so it is not applied.  Dirichlet
+// is applied to the slot momenta before v_cm.  Then G2P-1, re-projection and
+// G2P-2 of SolveExplicitSolidMechanics3D.jl:109-211, each particle reading
+// its own solid's slot; the re-projected momentum is not corrected again.
+template <class T> void step_musl_3d_contact(Ctx<T>& g) {
+  const T dt = g.dt, alpha = g.alpha;
+  const i64 nn = g.nn;
+  g.cbody.assign(size_t(2 * nn), -1);
+  for (auto* v : {&g.cm}) v->assign(size_t(2 * nn), T(0.0));
+  for (auto* v : {&g.cp0, &g.cp, &g.cf, &g.cn, &g.cp2}) v->assign(size_t(6 * nn), T(0.0));
+  grid_reset(g, true);
+  auto slot_of = [&](i64 in, int si) {
+    return g.cbody[2 * in] == si ? 0 : (g.cbody[2 * in + 1] == si ? 1 : -1);
+  };
+  // P2G per solid slot (:55-84 per slot, normals and positions added)
+  for (size_t si = 0; si < g.solids.size(); si++) {
+    auto& s = g.solids[si];
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, true);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T fV = s.vol[ip], fM = s.mass[ip];
+      const T* vp = &s.vel[3 * ip];
+      const T* sg = &s.stress[9 * ip];
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        int k;
+        if (g.cbody[2 * in] < 0 || g.cbody[2 * in] == int(si)) k = 0;
+        else if (g.cbody[2 * in + 1] < 0 || g.cbody[2 * in + 1] == int(si)) k = 1;
+        else { g.flags |= JUMP_FLAG_CONTACT_OVERFLOW; continue; }
+        g.cbody[2 * in + k] = int(si);
+        const i64 c = 2 * in + k, c3 = 3 * c;
+        T Ni = g.funcs[i];
+        T dx = g.ders[3 * i], dy = g.ders[3 * i + 1], dz = g.ders[3 * i + 2];
+        T Nim = Ni * fM;
+        g.cm[c] += Nim;
+        for (int a = 0; a < 3; a++) g.cp0[c3 + a] += Nim * vp[a];
+        T f0 = sg[0] * dx + sg[3] * dy + sg[6] * dz;
+        T f1 = sg[3] * dx + sg[4] * dy + sg[7] * dz;
+        T f2 = sg[6] * dx + sg[7] * dy + sg[8] * dz;
+        g.cf[c3] -= fV * f0;
+        g.cf[c3 + 1] -= fV * f1;
+        g.cf[c3 + 2] -= fV * f2;
+        g.cn[c3] += fM * dx;
+        g.cn[c3 + 1] += fM * dy;
+        g.cn[c3 + 2] += fM * dz;
       }
-      if (!std::isfinite(val(xx[0])) || !std::isfinite(val(xx[1])) || !std::isfinite(val(xx[2])) ||
-          !std::isfinite(val(s.vel[3 * ip])) || !std::isfinite(val(s.vel[3 * ip + 1])) ||
-          !std::isfinite(val(s.vel[3 * ip + 2])))
-        g.flags |= JUMP_FLAG_NONFINITE;
+    }
+  }
+  // grid update + BC per slot (:89-105), totals, contact correction
+  for (i64 i = 0; i < nn; i++) {
+    for (int k = 0; k < 2; k++) {
+      const i64 c3 = 3 * (2 * i + k);
+      for (int a = 0; a < 3; a++) {
+        g.cp[c3 + a] = g.cp0[c3 + a] + g.cf[c3 + a] * dt;
+        if (g.fixed[3 * i + a]) { g.cp0[c3 + a] = 0.0; g.cp[c3 + a] = 0.0; }
+      }
+    }
+    g.gmass[i] = g.cm[2 * i] + g.cm[2 * i + 1];
+    for (int a = 0; a < 3; a++) {
+      g.gp0[3 * i + a] = g.cp0[6 * i + a] + g.cp0[6 * i + 3 + a];
+      g.gp[3 * i + a] = g.cp[6 * i + a] + g.cp[6 * i + 3 + a];
+      g.gf[3 * i + a] = g.cf[6 * i + a] + g.cf[6 * i + 3 + a];
+    }
+    if (g.cbody[2 * i] < 0 || g.cbody[2 * i + 1] < 0) continue;
+    const T mS = g.cm[2 * i] + g.cm[2 * i + 1];
+    T vcm[3], nc[3];
+    for (int a = 0; a < 3; a++) vcm[a] = (g.cp[6 * i + a] + g.cp[6 * i + 3 + a]) / mS;
+    for (int a = 0; a < 3; a++) nc[a] = g.cn[6 * i + a] - g.cn[6 * i + 3 + a];
+    const T nrm = sqrt(nc[0] * nc[0] + nc[1] * nc[1] + nc[2] * nc[2]);
+    for (int a = 0; a < 3; a++) nc[a] = nc[a] / nrm;
+    for (int k = 0; k < 2; k++) {
+      const i64 c3 = 3 * (2 * i + k);
+      const T m = g.cm[2 * i + k];
+      T v[3], nk[3];
+      for (int a = 0; a < 3; a++) {
+        v[a] = g.cp[c3 + a] / m;
+        nk[a] = k == 0 ? nc[a] : -nc[a];
+      }
+      const T Dn = (v[0] - vcm[0]) * nk[0] + (v[1] - vcm[1]) * nk[1] + (v[2] - vcm[2]) * nk[2];
+      if (!(val(Dn) > 0.0)) continue;
+      for (int a = 0; a < 3; a++) g.cp[c3 + a] = (v[a] - Dn * nk[a]) * m;
+    }
+  }
+  // G2P-1 and re-projection (:109-140), each particle on its own slot
+  for (size_t si = 0; si < g.solids.size(); si++) {
+    auto& s = g.solids[si];
+    for (i64 ip = 0; ip < s.n; ip++) {
+      T* vv = &s.vel[3 * ip];
+      T mp = s.mass[ip];
+      T vx[3] = {0.0, 0.0, 0.0};
+      int sup = support(g, s, ip, false);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        int k = slot_of(in, int(si));
+        if (k < 0) continue;
+        const i64 c = 2 * in + k;
+        T mI = g.cm[c];
+        if (val(mI) > 0.0) {
+          T mii = (1.0 / mI) * g.funcs[i];
+          for (int a = 0; a < 3; a++) vx[a] += (g.cp[3 * c + a] - alpha * g.cp0[3 * c + a]) * mii;
+        }
+      }
+      for (int a = 0; a < 3; a++) vv[a] = alpha * vv[a] + vx[a];
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        int k = slot_of(in, int(si));
+        if (k < 0) continue;
+        T Nm = g.funcs[i] * mp;
+        for (int a = 0; a < 3; a++) g.cp2[3 * (2 * in + k) + a] += Nm * vv[a];
+      }
+    }
+  }
+  for (i64 i = 0; i < nn; i++)
+    for (int a = 0; a < 3; a++) g.gp2[3 * i + a] = g.cp2[6 * i + a] + g.cp2[6 * i + 3 + a];
+  // G2P-2 (:161-211) on the own slot
+  for (size_t si = 0; si < g.solids.size(); si++) {
+    auto& s = g.solids[si];
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, true);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T L[9];
+      for (int k = 0; k < 9; k++) L[k] = 0.0;
+      T xx[3] = {s.pos[3 * ip], s.pos[3 * ip + 1], s.pos[3 * ip + 2]};
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        int k = slot_of(in, int(si));
+        if (k < 0) continue;
+        const i64 c = 2 * in + k;
+        T Ni = g.funcs[i];
+        T dNx = g.ders[3 * i], dNy = g.ders[3 * i + 1], dNz = g.ders[3 * i + 2];
+        T m = g.cm[c];
+        if (val(m) > 0.0) {
+          T mii = 1.0 / m;
+          T vIx = mii * g.cp2[3 * c], vIy = mii * g.cp2[3 * c + 1], vIz = mii * g.cp2[3 * c + 2];
+          T cc = Ni * mii * dt;
+          for (int a = 0; a < 3; a++) xx[a] += cc * g.cp[3 * c + a];
+          L[0] += dNx * vIx; L[1] += dNy * vIx; L[2] += dNz * vIx;
+          L[3] += dNx * vIy; L[4] += dNy * vIy; L[5] += dNz * vIy;
+          L[6] += dNx * vIz; L[7] += dNy * vIz; L[8] += dNz * vIz;
+        }
+      }
+      musl3d_particle_update(g, s, ip, xx, L);
     }
   }
 }

@@ -45,10 +45,6 @@ int orc_ctx_create(const jump_params* p, orc_ctx** out) {
   g.alpha = p->alpha;
   g.dt = p->dt;
   for (int k = 0; k < 3; k++) g.body[k] = p->body[k];
-  if (p->contact && p->dim != 2) {
-    delete h;
-    return JUMP_ERR_UNSUPPORTED;
-  }
   g.contact = p->contact != 0;
   g.fric = p->friction;
   g.nx = p->nx;
@@ -153,6 +149,7 @@ int orc_step(orc_ctx* h, int64_t nsteps, double* t) {
   for (int64_t s = 0; s < nsteps; s++) {
     if (g.dim == 2 && g.contact) orc::step_usl_2d_contact(g, *t);
     else if (g.dim == 2) orc::step_usl_2d(g, *t);
+    else if (g.contact) orc::step_musl_3d_contact(g);
     else orc::step_musl_3d(g);
     *t += g.dt;
   }

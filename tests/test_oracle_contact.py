@@ -6,7 +6,10 @@ head-on collision rebounds without the disks passing through each other
 (which independent velocity fields without the correction would do); a
 frictionless oblique impact spins the disks far less than a sticking one
 (high friction, or the single shared field); a third solid at a node is
-flagged; unsupported combinations are refused."""
+flagged; unsupported combinations are refused.  In 3D (the MUSL step with
+the normal-only correction) likewise: bitwise the single-field step while
+apart, momentum conserved, a rebound without interpenetration where the
+single shared field keeps the spheres together."""
 import numpy as np
 import pytest
 
@@ -109,14 +112,75 @@ def test_third_solid_at_a_node_is_flagged():
     assert o.flags() & _ffi.JUMP_FLAG_CONTACT_OVERFLOW
 
 
-def test_contact_refuses_rigid_solids_and_3d():
-    p = make_params(3, _ffi.BASIS_LINEAR, _ffi.ALG_MUSL, 1e-3, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (5, 5, 5),
-                    contact=True)
-    with pytest.raises(JumpError):
-        OracleEngine(p)
+def test_contact_refuses_rigid_solids():
     p = make_params(2, _ffi.BASIS_LINEAR, _ffi.ALG_USL, 1e-3, (0.0, 0.0), (1.0, 1.0), (5, 5), contact=True)
     o = OracleEngine(p)
     pos = np.array([[0.5, 0.5]])
     with pytest.raises(JumpError):
         o.add_solid(make_material(_ffi.MAT_RIGID, density=1.0), pos, np.zeros((1, 2)), np.ones(1), np.ones(1),
                     np.ones(1))
+
+
+# ---------------------------------------------------------------- 3D (MUSL)
+def _stats3(o, s):
+    a = o.get_solid(s)
+    m = a["mass"]
+    return (a["pos"] * m[:, None]).sum(0) / m.sum(), (a["vel"] * m[:, None]).sum(0) / m.sum(), a
+
+
+def test_contact3d_grid_equals_single_field_before_the_solids_meet():
+    kw = dict(gap_cells=6, steps=1)
+    a = _engine(scenes.spheres_contact(contact=True, **kw))
+    b = _engine(scenes.spheres_contact(contact=False, **kw))
+    a.step(1, 0.0)
+    b.step(1, 0.0)
+    ga, gb = a.get_grid(), b.get_grid()
+    for k in ("mass", "momentum0", "momentum", "momentum2", "force"):
+        assert np.array_equal(ga[k], gb[k]), k
+    a.step(99, 0.0)
+    b.step(99, 0.0)
+    for s in range(2):
+        sa, sb = a.get_solid(s), b.get_solid(s)
+        for k in ("pos", "vel", "F", "stress"):
+            assert np.array_equal(sa[k], sb[k]), (s, k)  # same sums in the same order while apart
+
+
+@pytest.mark.parametrize("offset", [0.0, 0.15])
+def test_contact3d_conserves_momentum_and_rebounds(offset):
+    sc = scenes.spheres_contact(gap_cells=4, offset=offset)
+    o = _engine(sc)
+    ref = sum(float((np.abs(s["vel"]).sum(1) * s["mass"]).sum()) for s in sc.solids)
+    overlaps = []
+    for _ in range(12):
+        o.step(100, 0.0)
+        P = np.zeros(3)
+        for s in range(2):
+            a = o.get_solid(s)
+            P += (a["vel"] * a["mass"][:, None]).sum(0)
+        assert np.max(np.abs(P)) <= 1e-12 * ref, P
+        c0, v0, a0 = _stats3(o, 0)
+        c1, v1, a1 = _stats3(o, 1)
+        overlaps.append(a0["pos"][:, 0].max() - a1["pos"][:, 0].min())
+    if offset == 0.0:
+        assert max(overlaps) < 0.0  # the spheres never interpenetrate
+        assert v0[0] < -0.05 and v1[0] > 0.05  # and rebound
+    assert o.flags() == 0
+
+
+def test_contact3d_frictionless_oblique_impact_transmits_little_spin():
+    """An oblique impact spins the spheres about z through the single shared
+    field (it sticks); the normal-only correction transmits no tangential
+    impulse."""
+    def spin(o):
+        a = o.get_solid(0)
+        m = a["mass"]
+        r = a["pos"] - (a["pos"] * m[:, None]).sum(0) / m.sum()
+        dv = a["vel"] - (a["vel"] * m[:, None]).sum(0) / m.sum()
+        return float(((r[:, 0] * dv[:, 1] - r[:, 1] * dv[:, 0]) * m).sum() / ((r[:, 0] ** 2 + r[:, 1] ** 2) * m).sum())
+    w = {}
+    for contact in (True, False):
+        o = _engine(scenes.spheres_contact(gap_cells=4, offset=0.15, contact=contact))
+        o.step(1200, 0.0)
+        w[contact] = spin(o)
+    assert w[False] > 0.3, w
+    assert abs(w[True]) < 0.1 * w[False], w
