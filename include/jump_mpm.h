@@ -5,6 +5,8 @@
  * solvers:
  *   solve_explicit_dynamics_2D(grid,solids,basis,alg::USL,output,fixes,data)
  *       SolveExplicitSolidMechanics2D.jl:335-585
+ *   solve_explicit_dynamics_2D(grid,solids,basis,alg::MUSL,output,fixes,data)
+ *       SolveExplicitSolidMechanics2D.jl:25-329 (blended consistent mass)
  *   solve_explicit_dynamics_3D(grid,solids,basis,alg::MUSL,output,fixes,data)
  *       SolveExplicitSolidMechanics3D.jl:14-223
  * both exported by module Problem (Problem.jl:163-166).  A host binding keeps
@@ -55,6 +57,7 @@ typedef enum {
 #define JUMP_FLAG_NEGATIVE_J 2u   /* det(F) < 0 after the update              */
 #define JUMP_FLAG_NONFINITE 4u    /* NaN/Inf in updated particle state        */
 #define JUMP_FLAG_CONTACT_OVERFLOW 8u /* contact: a node touched by > 2 solids  */
+#define JUMP_FLAG_SOLVER 16u        /* 2D MUSL: mass solve did not converge    */
 
 /* Basis types: Basis.jl:37-48 (LinearBasis, QuadBsplineBasis,
  * CubicBsplineBasis, CPDIQ4Basis) plus uGIMP, which the reference lacks. */
@@ -67,7 +70,13 @@ typedef enum {
 } jump_basis;
 
 /* Algorithm tags: Algorithm.jl:16-18 (USL(tolerance)), :37-39 (MUSL(alpha)).
- * The reference implements USL in 2D and MUSL in 3D on this path. */
+ * The reference implements USL and MUSL in 2D and MUSL in 3D on this path.
+ * The 2D MUSL step solves with the blended mass matrix
+ * B = 0.1 diag(m) + 0.9 M (M_IJ = sum m N_I N_J); the reference assembles it
+ * densely and factors it, this library keeps it as a (2W-1)^2-point stencil
+ * (W the stencil width) and solves by preconditioned conjugate gradients to
+ * a relative residual of 1e-14 (JUMP_FLAG_SOLVER if that takes more than
+ * 2000 iterations).  No contact and no rigid solids with it. */
 typedef enum { JUMP_ALG_USL = 0, JUMP_ALG_MUSL = 1 } jump_algorithm;
 
 /* Material kinds: Material.jl:29-31 (Rigid), :37-81 (Elastic/Hooke),
@@ -163,8 +172,9 @@ typedef struct jump_ctx jump_ctx;
 /* Create a context for one grid/basis/algorithm.  Replaces the set-up part of
  * solve_explicit_dynamics_2D/3D (SolveExplicitSolidMechanics2D.jl:335-371,
  * SolveExplicitSolidMechanics3D.jl:14-39) and Grid2D/Grid3D allocation
- * (Grid.jl:104-158, 198-267; the 2D dense mass matrix M is not allocated,
- * USL never reads it).  Selects device `device` (HIP ordinal). */
+ * (Grid.jl:104-158, 198-267; the 2D dense mass matrix M is not allocated:
+ * USL never reads it and 2D MUSL keeps its nonzero band only).  Selects
+ * device `device` (HIP ordinal). */
 int jump_ctx_create(const jump_params* params, int device, jump_ctx** out);
 int jump_ctx_destroy(jump_ctx* ctx);
 const char* jump_last_error(const jump_ctx* ctx);
@@ -182,7 +192,7 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* desc,
 int jump_set_fixed(jump_ctx* ctx, const uint8_t* mask);
 
 /* Run nsteps iterations of the reference's `while t < Tf` body
- * (SolveExplicitSolidMechanics2D.jl:373-581 for USL,
+ * (SolveExplicitSolidMechanics2D.jl:373-581 for USL, :86-327 for 2D MUSL,
  *  SolveExplicitSolidMechanics3D.jl:41-220 for MUSL) without output, each
  * ending with t += dt.  *t_inout holds t (data["time"]) and is advanced.
  * Returns JUMP_ERR_NUMERIC (after finishing all steps) if a particle error
@@ -306,7 +316,8 @@ typedef enum {
   JUMP_PHASE_G2P1 = 4, /* MUSL particle velocity                     */
   JUMP_PHASE_P2G2 = 5, /* MUSL momentum re-projection                */
   JUMP_PHASE_G2P2 = 6, /* MUSL position, deformation, stress         */
-  JUMP_PHASE_COUNT = 7
+  JUMP_PHASE_SOLVE = 7,/* 2D MUSL blended mass solve                 */
+  JUMP_PHASE_COUNT = 8
 } jump_phase;
 int jump_profile(jump_ctx* ctx, int32_t enable);
 int jump_profile_read(jump_ctx* ctx, int32_t phase, double* total_ms, int64_t* count);

@@ -25,6 +25,7 @@ JUMP_FLAG_OUT_OF_GRID = 1
 JUMP_FLAG_NEGATIVE_J = 2
 JUMP_FLAG_NONFINITE = 4
 JUMP_FLAG_CONTACT_OVERFLOW = 8
+JUMP_FLAG_SOLVER = 16
 
 BASIS_LINEAR = 0
 BASIS_QUAD_BSPLINE = 1
@@ -175,7 +176,7 @@ def load():
     return lib
 
 
-PHASES = ["sort", "p2g", "rigid", "g2p", "g2p1", "p2g2", "g2p2"]
+PHASES = ["sort", "p2g", "rigid", "g2p", "g2p1", "p2g2", "g2p2", "solve"]
 
 EXPORTED = [
     "jump_ctx_create", "jump_ctx_destroy", "jump_last_error", "jump_add_solid", "jump_set_fixed", "jump_step",

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstddef>
 #include <memory>
 #include <cstdio>
 #include <cstring>
@@ -56,6 +57,10 @@ struct jump_ctx {
   GridArrays g{};
   ContactArrays cg{};    // contact mode only, 2D
   ContactArrays3 cg3{};  // contact mode only, 3D
+  Musl2Arrays mu{};      // 2D MUSL only: matrix rows, right-hand sides, velocities
+  CgArrays cgm{};        // 2D MUSL only: the conjugate-gradient vectors and state
+  int* h_cg = nullptr;   // pinned: CgState::all_done and ::iters read back
+  int cg_check = 8;      // CG iterations before the first convergence read-back
   std::vector<void*> part_allocs;
   std::vector<void*> grid_allocs;
   double* d_corners = nullptr;
@@ -347,6 +352,25 @@ static int alloc_grid(jump_ctx* ctx) {
       for (int a = 0; a < 3 && !r; a++) r = dev_alloc(ctx, ctx->grid_allocs, &c.p[k][a], nn);
       for (int a = 0; a < 3 && !r; a++) r = dev_alloc(ctx, ctx->grid_allocs, &c.p2[k][a], nn);
     }
+  } else if (D == 2 && ctx->prm.algorithm == JUMP_ALG_MUSL) {
+    // rows of the blended mass matrix: (2R+1)^2 coefficients per node, R
+    // the stencil width less one (CPDI: the widest domain, up to 5 nodes)
+    const int W = ctx->prm.basis == JUMP_BASIS_CPDI_Q4 ? MaxW<JUMP_BASIS_CPDI_Q4>::W
+                  : ctx->prm.basis == JUMP_BASIS_LINEAR ? 2
+                  : ctx->prm.basis == JUMP_BASIS_CUBIC_BSPLINE ? 4
+                                                               : 3;
+    const size_t K1 = size_t(2 * W - 1);
+    Musl2Arrays& m = ctx->mu;
+    r = dev_alloc(ctx, ctx->grid_allocs, &m.brow, K1 * K1 * nn);
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &m.rhs, 4 * nn);
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &m.mv, 4 * nn);
+    CgArrays& c = ctx->cgm;
+    for (double** v : {&c.x, &c.r, &c.p, &c.s, &c.w})
+      if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, 4 * nn);
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.part, 12 * ((nn + kCgBlock - 1) / kCgBlock));
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.st, 1);
+    if (!r && hipHostMalloc(reinterpret_cast<void**>(&ctx->h_cg), 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+      r = set_err(ctx, JUMP_ERR_NOMEM, "hipHostMalloc failed");
   } else if (ctx->prm.contact) {
     ContactArrays& c = ctx->cg;
     for (int k = 0; k < 2 && !r; k++) {
@@ -538,6 +562,68 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_G2P2);
 }
 
+// 2D MUSL with the blended mass matrix: P2G with the matrix rows and the
+// grid fields, the solve for the nodal velocities, G2P-1, G2P-2.  R: the
+// matrix stencil radius (width - 1).
+constexpr double kMuslTol = 1e-14;  // CG: relative residual per right-hand side
+constexpr int kMuslMaxIt = 2000;
+// The blended mass solve: CG iterations of three launches each; the host
+// reads the convergence state back after as many iterations as the last
+// solve needed (then every 4), so a step usually waits for it once.
+template <int R> static void solve_musl2d(jump_ctx* ctx) {
+  const int nn = ctx->NN;
+  const int nblk = (nn + kCgBlock - 1) / kCgBlock;
+  CgArrays& c = ctx->cgm;
+  LAUNCH(k_cg_init, nn, nn, ctx->mu, c);
+  int check = std::max(ctx->cg_check, 4);
+  const char* st_done = reinterpret_cast<const char*>(c.st) + offsetof(CgState, all_done);
+  static_assert(offsetof(CgState, iters) == offsetof(CgState, all_done) + sizeof(int), "read-back layout");
+  for (int it = 0; it < kMuslMaxIt; it++) {
+    hipLaunchKernelGGL((k_cg_spmv<R>), dim3(nblk), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, ctx->g.m, c);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(kCgBlock), 0, ctx->stream, c, nblk, it, kMuslTol);
+    hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kCgBlock), 0, ctx->stream, nn, ctx->g.m, c);
+    if (it + 1 != check) continue;
+    if (hipMemcpyAsync(ctx->h_cg, st_done, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      break;  // the error stays with the stream and is reported by the step
+    if (ctx->h_cg[0]) {
+      ctx->cg_check = ctx->h_cg[1] + 1;  // the iteration that found every column converged
+      break;
+    }
+    check += 4;
+  }
+  LAUNCH(k_cg_finish, nn, nn, ctx->mu, c, ctx->d_flags);
+}
+template <int B, int ORDER, int R> static void step_musl2d(jump_ctx* ctx) {
+  const int np = int(ctx->np);
+  const double dt = ctx->prm.dt;
+  prof_begin(ctx);
+  bin_and_sort<2, B, false>(ctx);
+  prof_end(ctx, JUMP_PHASE_SORT);
+  prof_begin(ctx);
+  LAUNCH_P2G((k_p2g_musl2d<B, ORDER, R>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
+             ctx->d_bxyz, ctx->g, ctx->mu, ctx->cpdi_w, dt);
+  prof_end(ctx, JUMP_PHASE_P2G);
+  prof_begin(ctx);
+  solve_musl2d<R>(ctx);
+  prof_end(ctx, JUMP_PHASE_SOLVE);
+  prof_begin(ctx);
+  LAUNCH((k_g2p1_musl2d<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->mu, ctx->prm.alpha,
+         ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_G2P1);
+  prof_begin(ctx);
+  LAUNCH((k_g2p2_musl2d<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->mu, dt, ctx->d_flags);
+  prof_end(ctx, JUMP_PHASE_G2P2);
+}
+template <int ORDER> static void step_musl2d_cpdi(jump_ctx* ctx) {
+  switch (ctx->cpdi_w) {
+    case 2: step_musl2d<JUMP_BASIS_CPDI_Q4, ORDER, 1>(ctx); break;
+    case 3: step_musl2d<JUMP_BASIS_CPDI_Q4, ORDER, 2>(ctx); break;
+    case 4: step_musl2d<JUMP_BASIS_CPDI_Q4, ORDER, 3>(ctx); break;
+    default: step_musl2d<JUMP_BASIS_CPDI_Q4, ORDER, 4>(ctx); break;
+  }
+}
+
 static int prepare_cpdi(jump_ctx* ctx) {
   if (ctx->prm.basis != JUMP_BASIS_CPDI_Q4 || ctx->cpdi_w > 0) return JUMP_OK;
   int zero = 0;
@@ -555,7 +641,30 @@ static int prepare_cpdi(jump_ctx* ctx) {
 static int run_steps(jump_ctx* ctx, int64_t nsteps) {
   const bool ref = ctx->prm.order == JUMP_ORDER_REFERENCE;
   for (int64_t s = 0; s < nsteps; s++) {
-    if (ctx->D == 2) {
+    if (ctx->D == 2 && ctx->prm.algorithm == JUMP_ALG_MUSL) {
+      switch (ctx->prm.basis) {
+        case JUMP_BASIS_LINEAR:
+          if (ref) step_musl2d<JUMP_BASIS_LINEAR, JUMP_ORDER_REFERENCE, 1>(ctx);
+          else step_musl2d<JUMP_BASIS_LINEAR, JUMP_ORDER_BINNED, 1>(ctx);
+          break;
+        case JUMP_BASIS_QUAD_BSPLINE:
+          if (ref) step_musl2d<JUMP_BASIS_QUAD_BSPLINE, JUMP_ORDER_REFERENCE, 2>(ctx);
+          else step_musl2d<JUMP_BASIS_QUAD_BSPLINE, JUMP_ORDER_BINNED, 2>(ctx);
+          break;
+        case JUMP_BASIS_CUBIC_BSPLINE:
+          if (ref) step_musl2d<JUMP_BASIS_CUBIC_BSPLINE, JUMP_ORDER_REFERENCE, 3>(ctx);
+          else step_musl2d<JUMP_BASIS_CUBIC_BSPLINE, JUMP_ORDER_BINNED, 3>(ctx);
+          break;
+        case JUMP_BASIS_GIMP:
+          if (ref) step_musl2d<JUMP_BASIS_GIMP, JUMP_ORDER_REFERENCE, 2>(ctx);
+          else step_musl2d<JUMP_BASIS_GIMP, JUMP_ORDER_BINNED, 2>(ctx);
+          break;
+        case JUMP_BASIS_CPDI_Q4:
+          if (ref) step_musl2d_cpdi<JUMP_ORDER_REFERENCE>(ctx);
+          else step_musl2d_cpdi<JUMP_ORDER_BINNED>(ctx);
+          break;
+      }
+    } else if (ctx->D == 2) {
       switch (ctx->prm.basis) {
 #define USL_CASE(BB)                                                     \
   case BB:                                                               \
@@ -610,10 +719,11 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   if (p->dim != 2 && p->dim != 3) return JUMP_ERR_ARG;
   if (p->basis < JUMP_BASIS_LINEAR || p->basis > JUMP_BASIS_GIMP) return JUMP_ERR_ARG;
   if (p->dim == 3 && p->basis == JUMP_BASIS_CPDI_Q4) return JUMP_ERR_UNSUPPORTED;
-  // The reference implements USL in 2D and MUSL in 3D on this path (2D MUSL
-  // needs a consistent mass solve; 3D USL is marked unfinished and broken).
-  if (p->dim == 2 && p->algorithm != JUMP_ALG_USL) return JUMP_ERR_UNSUPPORTED;
+  // The reference implements USL and MUSL (blended mass matrix) in 2D and
+  // MUSL in 3D on this path (3D USL is marked unfinished and broken).
+  if (p->algorithm != JUMP_ALG_USL && p->algorithm != JUMP_ALG_MUSL) return JUMP_ERR_ARG;
   if (p->dim == 3 && p->algorithm != JUMP_ALG_MUSL) return JUMP_ERR_UNSUPPORTED;
+  if (p->dim == 2 && p->algorithm == JUMP_ALG_MUSL && p->contact) return JUMP_ERR_UNSUPPORTED;
   if (p->nx < 2 || p->ny < 2 || (p->dim == 3 && p->nz < 2)) return JUMP_ERR_ARG;
   const int64_t nz = p->dim == 3 ? p->nz : 1;
   if (p->nx * p->ny * nz >= (int64_t(1) << 31) - 2) return JUMP_ERR_UNSUPPORTED;
@@ -655,6 +765,7 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   int r = alloc_grid(ctx);
   if (r) {
     free_list(ctx->grid_allocs);
+    if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return r;
@@ -677,6 +788,7 @@ int jump_ctx_destroy(jump_ctx* ctx) {
   free_list(ctx->adj.adj_allocs);
   free_list(ctx->part_allocs);
   free_list(ctx->grid_allocs);
+  if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -694,6 +806,8 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
   if (mat->kind < JUMP_MAT_RIGID || mat->kind > JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_ARG, "bad material");
   if (D == 3 && mat->kind == JUMP_MAT_RIGID)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid (Solid3D has no rigid flag)");
+  if (ctx->D == 2 && ctx->prm.algorithm == JUMP_ALG_MUSL && mat->kind == JUMP_MAT_RIGID)
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the 2D MUSL step has no rigid bodies");
   if (ctx->prm.contact && mat->kind == JUMP_MAT_RIGID)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "rigid solids do not combine with contact");
   if (D == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");

@@ -1053,6 +1053,387 @@ __global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, c
   musl3d_particle_update(P, s, i, dt, xx, L, flags);
 }
 
+// ------------------------------------------------------------------ 2D MUSL
+// SolveExplicitSolidMechanics2D.jl:25-329: MUSL in 2D with the blended
+// (consistent) mass matrix B = 0.1 diag(m) + 0.9 M, M_IJ = sum m N_I N_J.
+// The reference assembles M densely and solves with a dense LU, which
+// bounds it to a few thousand nodes; here M is a (2R+1)^2-point stencil per
+// node (R = stencil width - 1: a node couples to the nodes of the particles
+// it shares) and B is solved by preconditioned conjugate gradients on the
+// device.  The nodal velocities v0 = B \ p0 and v = v0 + B \ (dt f) feed
+// both G2P passes; the Dirichlet mask zeroes only the nodal momenta (:184-
+// 199), which the G2P passes do not read.
+//   brow[k nn + n]: row n of B at offset k = (a+R) + (2R+1)(b+R), the
+//                   coefficient of node (i+a, j+b)
+//   rhs[c nn + n]:  p0x, p0y, dt fx, dt fy
+//   mv[c nn + n]:   vx0, vy0, vx, vy
+struct Musl2Arrays {
+  double* brow;
+  double* rhs;
+  double* mv;
+};
+constexpr double kMuslEps = 0.1;  // the lumped share of the blend (:150)
+
+// P2G: mass, momentum0, force as the USL pass, and the row of M, each node
+// in the reference's particle order (the row entries accumulate
+// (N_I m) N_J as massMatrix[id,idj] += Nim * Nj, :120-125).
+template <int B, int ORDER, int R>
+__global__ void __launch_bounds__(kP2GBlock) k_p2g_musl2d(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                          const double* __restrict__ corners,
+                                                          const int* __restrict__ start,
+                                                          const unsigned long long* __restrict__ sorted,
+                                                          const unsigned long long* __restrict__ bxyz, GridArrays g,
+                                                          Musl2Arrays mu, int Wrt, double dt) {
+  constexpr int MW = MaxW<B>::W;
+  constexpr int NB = MW * MW;
+  constexpr int K1 = 2 * R + 1;
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  const int n = blockIdx.x * kP2GBlock + threadIdx.x;
+  if (n >= G.nn) return;
+  const int ni = n % G.nx, nj = n / G.nx;
+  const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
+  double am = 0.0, ap[2] = {0.0, 0.0}, af[2] = {0.0, 0.0};
+  double mrow[K1 * K1];
+#pragma unroll
+  for (int k = 0; k < K1 * K1; k++) mrow[k] = 0.0;
+  const Axis X = axis_of(G, 0), Y = axis_of(G, 1);
+  for_window_particles<2, B, ORDER, true>(P, G, start, sorted, bxyz, 0, 0, W, ni, nj, 0, s_cur, s_head, [&](int p) {
+    double v[PairLayout<2>::NF + 1];
+    P2GFields<2, 0>::load(P, unsigned(p), v);
+    double lp[3] = {0.0, 0.0, 0.0};
+    if (B == JUMP_BASIS_GIMP) {
+      const SolidDev& s = S[P.sid[p]];
+      lp[0] = s.lp[0];
+      lp[1] = s.lp[1];
+    }
+    double N, dN[3];
+    bool member;
+    node_weight<2, B, false, true>(P, G, lp, corners, p, v, ni + 1, nj + 1, 1, N, dN, member);
+    if (!member) return;
+    const double fM = v[2], fV = v[3];
+    const double Nim = N * fM;
+    am += Nim;
+    ap[0] += Nim * v[4];
+    ap[1] += Nim * v[5];
+    af[0] += (-fV) * (v[6] * dN[0] + v[8] * dN[1]) + (fM * G.body[0]) * N;
+    af[1] += (-fV) * (v[8] * dN[0] + v[7] * dN[1]) + (fM * G.body[1]) * N;
+    if (B == JUMP_BASIS_CPDI_Q4) {
+      CpdiW c;
+      cpdi_setup(corners, P.elem[0][p], P.elem[1][p], P.elem[2][p], P.elem[3][p], G, c);
+#pragma unroll
+      for (int b = -R; b <= R; b++)
+#pragma unroll
+        for (int a = -R; a <= R; a++) {
+          if (!cpdi_member(c, ni + a + 1, nj + b + 1)) continue;
+          double NJ, dNJ[3];
+          cpdi_node(c, G, ni + a + 1, nj + b + 1, NJ, dNJ);
+          mrow[(a + R) + K1 * (b + R)] += Nim * NJ;
+        }
+    } else {
+      // the particle's stencil: nodes fx..fx+W-1, fy..fy+W-1 (0-based)
+      const unsigned long long cc = ldu(bxyz, unsigned(p));
+      const int fx = int(cc & 0x1fffffu), fy = int((cc >> 21) & 0x1fffffu);
+      double wx[K1], wy[K1], dw;
+#pragma unroll
+      for (int a = -R; a <= R; a++) {
+        wx[a + R] = 0.0;
+        wy[a + R] = 0.0;
+        if (ni + a >= fx && ni + a < fx + W) axis_weight<B, false>(v[0], 0, X, lp[0], ni + a + 1, ni + a + 1, wx[a + R], dw);
+        if (nj + a >= fy && nj + a < fy + W) axis_weight<B, false>(v[1], 1, Y, lp[1], nj + a + 1, nj + a + 1, wy[a + R], dw);
+      }
+#pragma unroll
+      for (int b = -R; b <= R; b++)
+#pragma unroll
+        for (int a = -R; a <= R; a++) {
+          if (ni + a < fx || ni + a >= fx + W || nj + b < fy || nj + b >= fy + W) continue;
+          mrow[(a + R) + K1 * (b + R)] += Nim * (wx[a + R] * wy[b + R]);
+        }
+    }
+  });
+  const int nn = G.nn;
+  // right-hand sides before the Dirichlet mask (:159-171)
+  mu.rhs[n] = ap[0];
+  mu.rhs[nn + n] = ap[1];
+  mu.rhs[2 * nn + n] = dt * af[0];
+  mu.rhs[3 * nn + n] = dt * af[1];
+  // the blended row (:150-156)
+  const double oneme = 1.0 - kMuslEps;
+#pragma unroll
+  for (int k = 0; k < K1 * K1; k++) {
+    double b = oneme * mrow[k];
+    if (k == R + K1 * R) b = am == 0.0 ? 1.0 : kMuslEps * am + b;
+    mu.brow[size_t(k) * nn + n] = b;
+  }
+  // grid fields as the reference leaves them (:184-199)
+  const uint8_t fx = g.fixed[n];
+  g.m[n] = am;
+  for (int k = 0; k < 2; k++) {
+    double p0 = ap[k];
+    double pk = p0 + af[k] * dt;
+    if (fx & (1u << k)) { p0 = 0.0; pk = 0.0; }
+    g.p0[k][n] = p0;
+    g.p[k][n] = pk;
+    g.f[k][n] = af[k];
+    g.p2[k][n] = 0.0;
+  }
+}
+
+// Conjugate gradients for B X = RHS preconditioned with the lumped mass
+// diag(m) (1 at massless nodes, whose rows are the identity), four
+// right-hand sides at once.  With non-negative shape functions
+// 0.1 diag(m) <= B <= diag(m) (x.Mx = sum_p m_p (N_p.x)^2 <= sum_I m_I x_I^2
+// since the N_p sum to one), so the preconditioned condition number is at
+// most 10 whatever the particle layout: about 40-50 iterations to 1e-14.
+// One iteration is three launches over the whole device:
+//   k_cg_spmv     z = D^-1 r, w = B z, per-block partial sums of (r, z),
+//                 (w, z), (r, r) for each column
+//   k_cg_scalars  one workgroup: the partials in a fixed order, the
+//                 convergence test |r| <= tol |b| per column, alpha, beta
+//   k_cg_update   p = z + beta p, s = w + beta s, x += alpha p, r -= alpha s
+// (the single-reduction form of Chronopoulos and Gear, 1989: s = B p is
+// carried, not recomputed).  Every sum has a fixed order, so the solve is
+// bitwise reproducible.  Once every column has converged the kernels return
+// at once; the host stops launching when it sees that.
+struct CgState {
+  double alpha[4], beta[4], gamma[4], bb[4];
+  int done[4];
+  int all_done;  // every column converged (or had a zero right-hand side)
+  int iters;     // iterations that updated at least one column
+};
+struct CgArrays {
+  double* x;  // four columns of nn each, likewise below
+  double* r;
+  double* p;
+  double* s;
+  double* w;
+  double* part;  // [block][12]: (r,z) x4, (w,z) x4, (r,r) x4
+  CgState* st;
+};
+constexpr int kCgBlock = 256;
+
+__global__ void __launch_bounds__(kBlock) k_cg_init(int nn, Musl2Arrays mu, CgArrays cg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    CgState z{};
+    *cg.st = z;
+  }
+  if (i >= nn) return;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const size_t k = size_t(c) * nn + i;
+    cg.x[k] = 0.0;
+    cg.r[k] = mu.rhs[k];
+    cg.p[k] = 0.0;
+    cg.s[k] = 0.0;
+  }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kCgBlock) k_cg_spmv(GridDesc G, Musl2Arrays mu, const double* __restrict__ gm,
+                                                      CgArrays cg) {
+  constexpr int K1 = 2 * R + 1;
+  constexpr int NW = kCgBlock / 64;
+  __shared__ double red[12][NW];
+  if (cg.st->all_done) return;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int i = blockIdx.x * kCgBlock + t;
+  const int nn = G.nn;
+  double acc[12];
+#pragma unroll
+  for (int c = 0; c < 12; c++) acc[c] = 0.0;
+  if (i < nn) {
+    const int ix = i % G.nx, iy = i / G.nx;
+    double y[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int b = -R; b <= R; b++) {
+      if (iy + b < 0 || iy + b >= G.ny) continue;
+#pragma unroll
+      for (int a = -R; a <= R; a++) {
+        if (ix + a < 0 || ix + a >= G.nx) continue;
+        const double cf = mu.brow[size_t((a + R) + K1 * (b + R)) * nn + i];
+        const int j = i + a + G.nx * b;
+        const double mj = gm[j];
+        const double dj = mj > 0.0 ? 1.0 / mj : 1.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) y[c] += cf * (cg.r[size_t(c) * nn + j] * dj);
+      }
+    }
+    const double mi = gm[i];
+    const double di = mi > 0.0 ? 1.0 / mi : 1.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const double ri = cg.r[size_t(c) * nn + i];
+      const double zi = ri * di;
+      cg.w[size_t(c) * nn + i] = y[c];
+      acc[c] = ri * zi;
+      acc[4 + c] = y[c] * zi;
+      acc[8 + c] = ri * ri;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    double v = acc[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[c][wv] = v;
+  }
+  __syncthreads();
+  if (t < 12) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) v += red[t][w];
+    cg.part[size_t(blockIdx.x) * 12 + t] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, int it, double tol) {
+  constexpr int NW = kCgBlock / 64;
+  __shared__ double red[12][NW];
+  __shared__ double tot[12];
+  CgState& st = *cg.st;
+  if (st.all_done) return;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double acc[12];
+#pragma unroll
+  for (int c = 0; c < 12; c++) acc[c] = 0.0;
+  for (int b = t; b < nblk; b += kCgBlock)
+#pragma unroll
+    for (int c = 0; c < 12; c++) acc[c] += cg.part[size_t(b) * 12 + c];
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    double v = acc[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[c][wv] = v;
+  }
+  __syncthreads();
+  if (t < 12) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) v += red[t][w];
+    tot[t] = v;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  bool all = true, any = false;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const double gam = tot[c], del = tot[4 + c], rr = tot[8 + c];
+    if (it == 0) {
+      st.bb[c] = rr;
+      st.done[c] = rr == 0.0;
+    } else if (!st.done[c]) {
+      st.done[c] = rr <= tol * tol * st.bb[c];
+    }
+    if (st.done[c]) {
+      st.alpha[c] = 0.0;
+      st.beta[c] = 0.0;
+      continue;
+    }
+    all = false;
+    any = true;
+    const double be = it == 0 ? 0.0 : gam / st.gamma[c];
+    const double al = it == 0 ? gam / del : gam / (del - be * gam / st.alpha[c]);
+    st.beta[c] = be;
+    st.alpha[c] = al;
+    st.gamma[c] = gam;
+  }
+  if (any) st.iters = it + 1;
+  st.all_done = all ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kCgBlock) k_cg_update(int nn, const double* __restrict__ gm, CgArrays cg) {
+  const CgState& st = *cg.st;
+  if (st.all_done) return;
+  const int i = blockIdx.x * kCgBlock + threadIdx.x;
+  if (i >= nn) return;
+  const double mi = gm[i];
+  const double di = mi > 0.0 ? 1.0 / mi : 1.0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    if (st.done[c]) continue;
+    const size_t k = size_t(c) * nn + i;
+    const double al = st.alpha[c], be = st.beta[c];
+    const double ri = cg.r[k];
+    const double pk = ri * di + be * cg.p[k];
+    const double sk = cg.w[k] + be * cg.s[k];
+    cg.p[k] = pk;
+    cg.s[k] = sk;
+    cg.x[k] += al * pk;
+    cg.r[k] = ri - al * sk;
+  }
+}
+
+// mv = (x0, x1, x0 + x2, x1 + x3); the solver flag if a column did not
+// converge.
+__global__ void __launch_bounds__(kBlock) k_cg_finish(int nn, Musl2Arrays mu, CgArrays cg,
+                                                      unsigned* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && cg.st->all_done != 1) atomicOr(flags, JUMP_FLAG_SOLVER);
+  if (i >= nn) return;
+  const double x0 = cg.x[i], x1 = cg.x[size_t(nn) + i];
+  mu.mv[i] = x0;
+  mu.mv[size_t(nn) + i] = x1;
+  mu.mv[2 * size_t(nn) + i] = x0 + cg.x[2 * size_t(nn) + i];
+  mu.mv[3 * size_t(nn) + i] = x1 + cg.x[3 * size_t(nn) + i];
+}
+
+// G2P-1 (getShapeFunctions, :203-226): v = alpha v + sum N (v_I - alpha v0_I)
+// over nodes with mass.
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p1_musl2d(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                        const double* __restrict__ corners, GridArrays g,
+                                                        Musl2Arrays mu, double alpha, unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  const int nn = G.nn;
+  double vx = 0.0, vy = 0.0;
+  bool ok = walk_stencil<2, B, true, false>(P, G, s, corners, i, [&](int n, double Ni, const double*) {
+    if (g.m[n] > 0.0) {
+      vx += Ni * (mu.mv[2 * nn + n] - alpha * mu.mv[n]);
+      vy += Ni * (mu.mv[3 * nn + n] - alpha * mu.mv[nn + n]);
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  P.v[0][i] = alpha * P.v[0][i] + vx;
+  P.v[1][i] = alpha * P.v[1][i] + vy;
+}
+
+// G2P-2 (getShapeAndGradient, :246-286): x += N v_I dt, L from v_I over
+// nodes with mass, then the USL particle update.
+template <int B>
+__global__ void __launch_bounds__(kBlock) k_g2p2_musl2d(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+                                                        const double* __restrict__ corners, GridArrays g,
+                                                        Musl2Arrays mu, double dt, unsigned* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const SolidDev& s = S[P.sid[i]];
+  const int nn = G.nn;
+  double x0 = P.x[0][i], x1 = P.x[1][i];
+  double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  bool ok = walk_stencil<2, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
+    if (g.m[n] > 0.0) {
+      const double vI0 = mu.mv[2 * nn + n], vI1 = mu.mv[3 * nn + n];
+      x0 += Ni * vI0 * dt;
+      x1 += Ni * vI1 * dt;
+      L0 += dN[0] * vI0;
+      L1 += dN[0] * vI1;
+      L2 += dN[1] * vI0;
+      L3 += dN[1] * vI1;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  usl2d_particle_update(P, s, i, dt, P.v[0][i], P.v[1][i], x0, x1, L0, L1, L2, L3, flags);
+}
+
 // ------------------------------------------------------------------ contact (3D)
 // The MUSL step with multi-body contact: per-solid node slots as in 2D; the
 // normal-only correction of SolveExplicitFEMP3DContact.jl:398-511 (v -= D n

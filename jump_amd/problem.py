@@ -3,6 +3,8 @@
 
   solve_explicit_dynamics_2D(grid, solids, basis, alg::USL, output, fixes, data)
       SolveExplicitSolidMechanics2D.jl:335-585
+  solve_explicit_dynamics_2D(grid, solids, basis, alg::MUSL, output, fixes, data)
+      SolveExplicitSolidMechanics2D.jl:25-329 (blended consistent mass)
   solve_explicit_dynamics_3D(grid, solids, basis, alg::MUSL, output, fixes, data)
       SolveExplicitSolidMechanics3D.jl:14-223
 
@@ -124,8 +126,8 @@ def _time_loop(eng, output, fixes, data):
 
 
 def solve_explicit_dynamics_2D(grid, solids, basis, alg, output, fixes, data):
-    if not isinstance(alg, USL):
-        raise NotImplementedError("2D MUSL (consistent-mass variant) is not part of the GPU step")
+    if not isinstance(alg, (USL, MUSL)):
+        raise TypeError("alg must be USL or MUSL")
     eng = build_engine(grid, solids, basis, alg, data)
     try:
         _time_loop(eng, output, fixes, data)

@@ -436,8 +436,10 @@ def block3d(nodes=(12, 12, 12), basis=_ffi.BASIS_QUAD_BSPLINE, material="elastic
     return Scene("block3d", p, [s], fixed=fixed, steps=steps)
 
 
-def block2d(cells=16, basis=_ffi.BASIS_LINEAR, material="elastic", steps=10, seed=0):
-    """Small 2D block with random velocity and stress (parity tests)."""
+def block2d(cells=16, basis=_ffi.BASIS_LINEAR, material="elastic", steps=10, seed=0, algorithm=_ffi.ALG_USL,
+            alpha=1.0):
+    """Small 2D block with random velocity and stress (parity tests); USL, or
+    MUSL with the blended mass matrix and FLIP fraction `alpha`."""
     rng = np.random.default_rng(seed)
     dx = 1.0 / cells
     f = dx / 2
@@ -454,7 +456,7 @@ def block2d(cells=16, basis=_ffi.BASIS_LINEAR, material="elastic", steps=10, see
     s["stress"] = np.ascontiguousarray(sig.transpose(0, 2, 1).reshape(n, 4))
     c = math.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) / rho)
     dt = 0.2 * dx / c
-    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (1.0, 1.0), (cells + 1, cells + 1), tolerance=1e-12,
-                    body=(0.0, -9.81))
+    p = make_params(2, basis, algorithm, dt, (0.0, 0.0), (1.0, 1.0), (cells + 1, cells + 1), tolerance=1e-12,
+                    alpha=alpha, body=(0.0, -9.81))
     fixed = _fixed_faces((cells + 1, cells + 1), ["bottom"], {"bottom": (0, 1)})
     return Scene("block2d", p, [s], fixed=fixed, steps=steps)

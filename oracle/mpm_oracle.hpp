@@ -114,6 +114,7 @@ template <class T> struct Ctx {
   std::vector<int> cbody;                    // 2*nn, -1 empty
   std::vector<T> cm, cp0, cf, cn, cv0, cv;   // 2*nn, 2*dim*nn
   std::vector<T> cp, cp2;                    // 3D: momentum, momentum2
+  std::vector<T> mv;                         // 2D MUSL: vx0, vy0, vx, vy per node
   std::string err;
   // scratch for the support (Basis.jl initialise, :1658-1727)
   std::vector<i64> near;
@@ -757,6 +758,36 @@ template <class T> void grid_reset(Ctx<T>& g, bool musl) {
   if (musl) std::fill(g.gp2.begin(), g.gp2.end(), T(0.0));
 }
 
+// The particle update after a USL 2D gather (SolveExplicitSolidMechanics2D.jl:
+// 506-532): new v and x, strain += dt D, F = F (I + L dt), V = det(F) V0,
+// stress; L = (dN1 v1, dN1 v2, dN2 v1, dN2 v2) column-major.
+template <class T> void usl2d_particle_update(Ctx<T>& g, Solid<T>& s, i64 ip, const T& v0, const T& v1, const T& x0,
+                                              const T& x1, const T* L) {
+  const T dt = g.dt;
+  s.vel[2 * ip] = v0; s.vel[2 * ip + 1] = v1;
+  s.pos[2 * ip] = x0; s.pos[2 * ip + 1] = x1;
+  T D[4] = {0.5 * (L[0] + L[0]), 0.5 * (L[1] + L[2]), 0.5 * (L[2] + L[1]), 0.5 * (L[3] + L[3])};
+  T* eps = &s.strain[4 * ip];
+  for (int k = 0; k < 4; k++) eps[k] = eps[k] + dt * D[k];
+  T A[4] = {1.0 + L[0] * dt, L[1] * dt, L[2] * dt, 1.0 + L[3] * dt};
+  T* F = &s.F[4 * ip];
+  T Fn[4];
+  matmul2(F, A, Fn);
+  for (int k = 0; k < 4; k++) F[k] = Fn[k];
+  T J = det2(F);
+  s.vol[ip] = J * s.vol0[ip];
+  if (val(J) < 0.0) g.flags |= JUMP_FLAG_NEGATIVE_J;
+  T* sg = &s.stress[4 * ip];
+  switch (s.mat.kind) {
+    case JUMP_MAT_ELASTIC: stress_hooke_2d(s.mat, eps, sg); break;
+    case JUMP_MAT_NEOHOOKEAN: stress_nh(2, s.mat, F, J, sg); break;
+    case JUMP_MAT_ELASTOPLASTIC: stress_j2_2d(s.mat, eps, sg, &s.pstrain[3 * ip], s.alpha[ip], s.vm[ip]); break;
+    default: break;
+  }
+  if (!std::isfinite(val(x0)) || !std::isfinite(val(x1)) || !std::isfinite(val(v0)) || !std::isfinite(val(v1)))
+    g.flags |= JUMP_FLAG_NONFINITE;
+}
+
 // SolveExplicitSolidMechanics2D.jl:381-572 (one iteration, no output)
 template <class T> void step_usl_2d(Ctx<T>& g, const T& t) {
   const T dt = g.dt;
@@ -836,28 +867,7 @@ template <class T> void step_usl_2d(Ctx<T>& g, const T& t) {
           L[0] += d1 * vI0; L[1] += d1 * vI1; L[2] += d2 * vI0; L[3] += d2 * vI1;
         }
       }
-      s.vel[2 * ip] = v0; s.vel[2 * ip + 1] = v1;
-      s.pos[2 * ip] = x0; s.pos[2 * ip + 1] = x1;
-      T D[4] = {0.5 * (L[0] + L[0]), 0.5 * (L[1] + L[2]), 0.5 * (L[2] + L[1]), 0.5 * (L[3] + L[3])};
-      T* eps = &s.strain[4 * ip];
-      for (int k = 0; k < 4; k++) eps[k] = eps[k] + dt * D[k];
-      T A[4] = {1.0 + L[0] * dt, L[1] * dt, L[2] * dt, 1.0 + L[3] * dt};
-      T* F = &s.F[4 * ip];
-      T Fn[4];
-      matmul2(F, A, Fn);
-      for (int k = 0; k < 4; k++) F[k] = Fn[k];
-      T J = det2(F);
-      s.vol[ip] = J * s.vol0[ip];
-      if (val(J) < 0.0) g.flags |= JUMP_FLAG_NEGATIVE_J;
-      T* sg = &s.stress[4 * ip];
-      switch (s.mat.kind) {
-        case JUMP_MAT_ELASTIC: stress_hooke_2d(s.mat, eps, sg); break;
-        case JUMP_MAT_NEOHOOKEAN: stress_nh(2, s.mat, F, J, sg); break;
-        case JUMP_MAT_ELASTOPLASTIC: stress_j2_2d(s.mat, eps, sg, &s.pstrain[3 * ip], s.alpha[ip], s.vm[ip]); break;
-        default: break;
-      }
-      if (!std::isfinite(val(x0)) || !std::isfinite(val(x1)) || !std::isfinite(val(v0)) || !std::isfinite(val(v1)))
-        g.flags |= JUMP_FLAG_NONFINITE;
+      usl2d_particle_update(g, s, ip, v0, v1, x0, x1, L);
     }
   }
   // rigid solids move (:539-551)
@@ -1000,28 +1010,7 @@ template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
           L[0] += d1 * vb[0]; L[1] += d1 * vb[1]; L[2] += d2 * vb[0]; L[3] += d2 * vb[1];
         }
       }
-      s.vel[2 * ip] = v0; s.vel[2 * ip + 1] = v1;
-      s.pos[2 * ip] = x0; s.pos[2 * ip + 1] = x1;
-      T D[4] = {0.5 * (L[0] + L[0]), 0.5 * (L[1] + L[2]), 0.5 * (L[2] + L[1]), 0.5 * (L[3] + L[3])};
-      T* eps = &s.strain[4 * ip];
-      for (int k = 0; k < 4; k++) eps[k] = eps[k] + dt * D[k];
-      T A[4] = {1.0 + L[0] * dt, L[1] * dt, L[2] * dt, 1.0 + L[3] * dt};
-      T* F = &s.F[4 * ip];
-      T Fn[4];
-      matmul2(F, A, Fn);
-      for (int k = 0; k < 4; k++) F[k] = Fn[k];
-      T J = det2(F);
-      s.vol[ip] = J * s.vol0[ip];
-      if (val(J) < 0.0) g.flags |= JUMP_FLAG_NEGATIVE_J;
-      T* sg = &s.stress[4 * ip];
-      switch (s.mat.kind) {
-        case JUMP_MAT_ELASTIC: stress_hooke_2d(s.mat, eps, sg); break;
-        case JUMP_MAT_NEOHOOKEAN: stress_nh(2, s.mat, F, J, sg); break;
-        case JUMP_MAT_ELASTOPLASTIC: stress_j2_2d(s.mat, eps, sg, &s.pstrain[3 * ip], s.alpha[ip], s.vm[ip]); break;
-        default: break;
-      }
-      if (!std::isfinite(val(x0)) || !std::isfinite(val(x1)) || !std::isfinite(val(v0)) || !std::isfinite(val(v1)))
-        g.flags |= JUMP_FLAG_NONFINITE;
+      usl2d_particle_update(g, s, ip, v0, v1, x0, x1, L);
     }
   }
   (void)t;
@@ -1152,6 +1141,151 @@ template <class T> void step_musl_3d(Ctx<T>& g) {
       musl3d_particle_update(g, s, ip, xx, L);
     }
   }
+}
+
+// Dense LU with partial pivoting (the factorisation Julia's `\` applies to
+// a general square matrix), in place on the n x n row-major A; then solves
+// A x = b for each right-hand side column of B (n x m, row-major), in place.
+// Returns false for a singular pivot.
+template <class T> bool lu_solve(std::vector<T>& A, i64 n, std::vector<T>& B, int m) {
+  std::vector<i64> piv(static_cast<size_t>(n));
+  for (i64 k = 0; k < n; k++) {
+    i64 p = k;
+    for (i64 i = k + 1; i < n; i++)
+      if (std::fabs(val(A[size_t(i * n + k)])) > std::fabs(val(A[size_t(p * n + k)]))) p = i;
+    piv[size_t(k)] = p;
+    if (val(A[size_t(p * n + k)]) == 0.0) return false;
+    if (p != k)
+      for (i64 j = 0; j < n; j++) std::swap(A[size_t(k * n + j)], A[size_t(p * n + j)]);
+    const T inv = 1.0 / A[size_t(k * n + k)];
+    for (i64 i = k + 1; i < n; i++) {
+      T& l = A[size_t(i * n + k)];
+      if (val(l) == 0.0) continue;
+      l = l * inv;
+      for (i64 j = k + 1; j < n; j++) A[size_t(i * n + j)] = A[size_t(i * n + j)] - l * A[size_t(k * n + j)];
+    }
+  }
+  for (int c = 0; c < m; c++) {
+    for (i64 k = 0; k < n; k++)
+      if (piv[size_t(k)] != k) std::swap(B[size_t(k * m + c)], B[size_t(piv[size_t(k)] * m + c)]);
+    for (i64 i = 0; i < n; i++)  // L y = b (unit diagonal)
+      for (i64 j = 0; j < i; j++) B[size_t(i * m + c)] = B[size_t(i * m + c)] - A[size_t(i * n + j)] * B[size_t(j * m + c)];
+    for (i64 i = n - 1; i >= 0; i--) {  // U x = y
+      for (i64 j = i + 1; j < n; j++) B[size_t(i * m + c)] = B[size_t(i * m + c)] - A[size_t(i * n + j)] * B[size_t(j * m + c)];
+      B[size_t(i * m + c)] = B[size_t(i * m + c)] / A[size_t(i * n + i)];
+    }
+  }
+  return true;
+}
+
+// SolveExplicitSolidMechanics2D.jl:25-329 (2D MUSL with the blended mass
+// matrix; one iteration, no output).  P2G adds the consistent mass matrix
+// M_IJ = sum m N_I N_J (:120-125); the blended B = 0.1 diag(m) + 0.9 M, with
+// 1 on the diagonal of massless nodes (:150-156), is solved for the nodal
+// velocities v0 = B \ p0 and v = v0 + B \ (dt f) (:160-178).  The Dirichlet
+// mask zeroes only the nodal momenta (:184-199), which the G2P passes do not
+// read.  G2P-1 (getShapeFunctions, :203-226): v = alpha v + sum N (v_I -
+// alpha v0_I) over nodes with mass; G2P-2 (:246-286): x += N v_I dt, L from
+// v_I, then the particle update.  The CPDI corner update reads the
+// re-projected momentum, which this step never fills, so corners stay put
+// (:289-309).  Dense, as the reference: up to 4096 nodes.
+template <class T> bool step_musl_2d(Ctx<T>& g) {
+  const T dt = g.dt, alpha = g.alpha;
+  const i64 nn = g.nn;
+  if (nn > 4096) return false;
+  grid_reset(g, true);
+  std::vector<T> M(size_t(nn * nn), T(0.0));
+  for (auto& s : g.solids) {
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, true);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T fV = s.vol[ip], fM = s.mass[ip];
+      const T* vp = &s.vel[2 * ip];
+      const T* sg = &s.stress[4 * ip];
+      T b0 = fM * g.body[0], b1 = fM * g.body[1];
+      for (int i = 0; i < sup; i++) {
+        i64 id = g.near[i];
+        T Ni = g.funcs[i];
+        T d1 = g.ders[2 * i], d2 = g.ders[2 * i + 1];
+        T Nim = Ni * fM;
+        g.gmass[id] += Nim;
+        for (int j = 0; j < sup; j++) M[size_t(id * nn + g.near[j])] += Nim * g.funcs[j];
+        g.gp0[2 * id] += Nim * vp[0];
+        g.gp0[2 * id + 1] += Nim * vp[1];
+        g.gf[2 * id] += (-fV) * (sg[0] * d1 + sg[2] * d2) + b0 * Ni;
+        g.gf[2 * id + 1] += (-fV) * (sg[1] * d1 + sg[3] * d2) + b1 * Ni;
+      }
+    }
+  }
+  const T eps = 0.1, oneme = 1.0 - eps;
+  for (i64 i = 0; i < nn; i++)
+    for (i64 j = 0; j < nn; j++) {
+      T& b = M[size_t(i * nn + j)];
+      b = i == j ? eps * g.gmass[i] + oneme * b : oneme * b;
+    }
+  for (i64 i = 0; i < nn; i++)
+    if (val(g.gmass[i]) == 0.0) M[size_t(i * nn + i)] = 1.0;
+  // right-hand sides p0x, p0y, dt fx, dt fy (before the Dirichlet mask)
+  std::vector<T> X(size_t(4 * nn));
+  for (i64 i = 0; i < nn; i++) {
+    X[size_t(4 * i)] = g.gp0[2 * i];
+    X[size_t(4 * i + 1)] = g.gp0[2 * i + 1];
+    X[size_t(4 * i + 2)] = dt * g.gf[2 * i];
+    X[size_t(4 * i + 3)] = dt * g.gf[2 * i + 1];
+  }
+  if (!lu_solve(M, nn, X, 4)) g.flags |= JUMP_FLAG_NONFINITE;
+  g.mv.assign(size_t(4 * nn), T(0.0));  // vx0, vy0, vx, vy
+  for (i64 i = 0; i < nn; i++) {
+    g.mv[size_t(4 * i)] = X[size_t(4 * i)];
+    g.mv[size_t(4 * i + 1)] = X[size_t(4 * i + 1)];
+    g.mv[size_t(4 * i + 2)] = X[size_t(4 * i)] + X[size_t(4 * i + 2)];
+    g.mv[size_t(4 * i + 3)] = X[size_t(4 * i + 1)] + X[size_t(4 * i + 3)];
+  }
+  for (i64 i = 0; i < nn; i++) {
+    for (int k = 0; k < 2; k++) g.gp[2 * i + k] = g.gp0[2 * i + k] + g.gf[2 * i + k] * dt;
+    for (int k = 0; k < 2; k++)
+      if (g.fixed[2 * i + k]) { g.gp0[2 * i + k] = 0.0; g.gp[2 * i + k] = 0.0; }
+  }
+  // G2P-1: particle velocity only
+  for (auto& s : g.solids) {
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, false);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T vx = 0.0, vy = 0.0;
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        T Ni = g.funcs[i];
+        if (val(g.gmass[in]) > 0.0) {
+          vx += Ni * (g.mv[size_t(4 * in + 2)] - alpha * g.mv[size_t(4 * in)]);
+          vy += Ni * (g.mv[size_t(4 * in + 3)] - alpha * g.mv[size_t(4 * in + 1)]);
+        }
+      }
+      s.vel[2 * ip] = alpha * s.vel[2 * ip] + vx;
+      s.vel[2 * ip + 1] = alpha * s.vel[2 * ip + 1] + vy;
+    }
+  }
+  // G2P-2: position, deformation, stress
+  for (auto& s : g.solids) {
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, true);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T L[4] = {0.0, 0.0, 0.0, 0.0};
+      T x0 = s.pos[2 * ip], x1 = s.pos[2 * ip + 1];
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        T Ni = g.funcs[i];
+        T d1 = g.ders[2 * i], d2 = g.ders[2 * i + 1];
+        if (val(g.gmass[in]) > 0.0) {
+          T vI0 = g.mv[size_t(4 * in + 2)], vI1 = g.mv[size_t(4 * in + 3)];
+          x0 += Ni * vI0 * dt;
+          x1 += Ni * vI1 * dt;
+          L[0] += d1 * vI0; L[1] += d1 * vI1; L[2] += d2 * vI0; L[3] += d2 * vI1;
+        }
+      }
+      usl2d_particle_update(g, s, ip, s.vel[2 * ip], s.vel[2 * ip + 1], x0, x1, L);
+    }
+  }
+  return true;
 }
 
 // The 3D MUSL step with multi-body contact: the normal-only correction of

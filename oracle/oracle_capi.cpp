@@ -32,7 +32,8 @@ int orc_ctx_create(const jump_params* p, orc_ctx** out) {
   if (p->nx < 2 || p->ny < 2 || (p->dim == 3 && p->nz < 2)) return JUMP_ERR_ARG;
   if (p->basis < 0 || p->basis > JUMP_BASIS_GIMP) return JUMP_ERR_ARG;
   if (p->dim == 3 && p->basis == JUMP_BASIS_CPDI_Q4) return JUMP_ERR_UNSUPPORTED;
-  if (p->dim == 2 && p->algorithm != JUMP_ALG_USL) return JUMP_ERR_UNSUPPORTED;
+  if (p->dim == 2 && p->algorithm == JUMP_ALG_MUSL && (p->contact || p->nx * p->ny > 4096))
+    return JUMP_ERR_UNSUPPORTED;  // the dense 2D MUSL restatement: no contact, up to 4096 nodes
   if (p->dim == 3 && p->algorithm != JUMP_ALG_MUSL) return JUMP_ERR_UNSUPPORTED;
   orc_ctx* h = new (std::nothrow) orc_ctx();
   if (!h) return JUMP_ERR_NOMEM;
@@ -94,6 +95,8 @@ int orc_add_solid(orc_ctx* h, const jump_solid_desc* ds, const jump_material* ma
   if (d == 3 && mat->kind == JUMP_MAT_RIGID) return fail(h, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid");
   if (d == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return fail(h, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");
   if (g.contact && mat->kind == JUMP_MAT_RIGID) return fail(h, JUMP_ERR_UNSUPPORTED, "rigid solids do not combine with contact");
+  if (d == 2 && g.alg == JUMP_ALG_MUSL && mat->kind == JUMP_MAT_RIGID)
+    return fail(h, JUMP_ERR_UNSUPPORTED, "the 2D MUSL step has no rigid-body treatment");
   if (g.basis == JUMP_BASIS_CPDI_Q4 && (!ds->corners || !ds->elems || ds->ncorners <= 0))
     return fail(h, JUMP_ERR_ARG, "CPDI solid needs corners and elems");
   orc::Solid<double> s;
@@ -148,6 +151,7 @@ int orc_step(orc_ctx* h, int64_t nsteps, double* t) {
   OCtx& g = h->c;
   for (int64_t s = 0; s < nsteps; s++) {
     if (g.dim == 2 && g.contact) orc::step_usl_2d_contact(g, *t);
+    else if (g.dim == 2 && g.alg == JUMP_ALG_MUSL) orc::step_musl_2d(g);
     else if (g.dim == 2) orc::step_usl_2d(g, *t);
     else if (g.contact) orc::step_musl_3d_contact(g);
     else orc::step_musl_3d(g);

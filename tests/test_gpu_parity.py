@@ -445,8 +445,8 @@ def test_cpdi_two_disks_from_gmsh(hip_lib):
     """Main_TwoDisksCPDI.jl's set-up through the host mirror: two disks of
     CPDI-Q4 domains read from disk.msh (Solid2D(fileName, mat)), placed with
     move_cpdi, on the 1.5 x 1.5 grid of 11 x 11 nodes; 300 steps of the USL
-    step (the example selects the reference's 2D MUSL, which is not on this
-    path) equal the restatement bit for bit."""
+    step equal the restatement bit for bit (the example itself selects the
+    2D MUSL step, whose parity test runs the same set-up)."""
     _say("cpdi two disks from disk.msh")
     from jump_amd import algorithm, basis, bodyforce, grid, material, problem, solid
     from oracle.pyoracle import OracleEngine
