@@ -552,8 +552,12 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
   prof_end(ctx, JUMP_PHASE_G2P1);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<3, B, 1, ORDER, true, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt());
+  {
+    const int rows = (ctx->NN + kP2GBlock - 1) / kP2GBlock;
+    hipLaunchKernelGGL((k_p2g<3, B, 1, ORDER, true, false>), dim3(xcd_grid(rows)), dim3(kP2GBlock),
+                       0, ctx->stream, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
+                       ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt());
+  }
   prof_end(ctx, JUMP_PHASE_P2G2);
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
   // zeroes components of momentum0/momentum already zeroed by the grid update.

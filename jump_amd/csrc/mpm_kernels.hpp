@@ -486,8 +486,15 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
   __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
   const int t = threadIdx.x;
-  const int i = blockIdx.x * kP2GBlock + t;
   const int NN = MULTI ? Bt.NN : G0.nn;
+  // XCD: the 3D re-projection pass deals its (compacted) blocks to the XCDs
+  // in runs of kXcdChunk (1.115 -> 1.066 ms on the 4.2M-particle impact;
+  // the force pass keeps the hardware order, it was slower with any run
+  // length tried, 2.12 -> 2.19-3.47 ms, as with 8x4x4 node tiles, 2.9-3.8)
+  constexpr bool XCD = D == 3 && !MULTI && COMPACT;
+  const int L = XCD ? xcd_block((NN + kP2GBlock - 1) / kP2GBlock) : int(blockIdx.x);
+  if (L < 0) return;
+  const int i = L * kP2GBlock + t;
   if (i >= NN) return;
   const int ng = COMPACT ? ldu(order, unsigned(i)) : i;  // node in the (concatenated) node arrays
   if (COMPACT && i >= ldu(nodepos, unsigned(NN))) {  // inactive node: no particle reaches it
