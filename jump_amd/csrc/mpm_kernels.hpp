@@ -392,7 +392,7 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
   const int t = threadIdx.x;
-  if (ORDER == JUMP_ORDER_BINNED) {
+  if constexpr (ORDER == JUMP_ORDER_BINNED) {
     const int k0 = (D == 3) ? max(nk - W + 1, 0) : 0;
     for (int bk = k0; bk <= nk; bk++)
       for (int bj = max(nj - W + 1, 0); bj <= nj; bj++)
