@@ -471,6 +471,41 @@ def test_cpdi_two_disks_from_gmsh(hip_lib):
     _close(engines[0].energies(), engines[1].energies(), rtol=1e-12, what="energies")
 
 
+def test_usl2d_tube_cpdi_platens(hip_lib):
+    """Main_LateralCompressionTube_CPDI.jl's set-up: the reference's tube.msh
+    (7898 CPDI-Q4 domains, J2 steel E=210 GPa, fy=310 MPa, k=10 MPa in mm-s
+    units) between rigid platens on the 100 x 60 mm grid of 51 x 21 nodes, the
+    top one moving down at 10 m/s, USL(0), dt = 0.2 dx/c; 300 steps equal the
+    restatement bit for bit.  The example predates the current Grid2D and
+    solve_explicit_dynamics_2D signatures; the platens get unit CPDI domains
+    (a rigid solid's grid coupling uses only its particles)."""
+    _say("usl2d tube.msh with platens")
+    from jump_amd import material, solid
+    from jump_amd.engine import make_params
+    rho, E, nu, fy, k1, v = 7800e-12, 210.0e3, 0.3, 310.0, 10.0, 10e3
+    dx = 100.0 / 50
+    f = dx / 2
+    m1 = material.ElastoPlasticMaterial(E, nu, rho, fy, k1, 7898)
+    tube = solid.Solid2D(os.path.join(GOLD, "tube.msh"), m1)
+    sol = [dict(material=m1.to_c(), pos=tube.pos, vel=tube.velocity, mass=tube.mass, volume=tube.volume,
+                volume0=tube.volumeInitial, corners=tube.nodes, elems=tube.elems)]
+    for yc, vy in ((60.0 - dx / 2, -v), (dx / 2, 0.0)):
+        pts = solid.buildParticleForRectangle([50.0, yc], 100.0, dx, f)
+        nc = pts.shape[0]
+        pc = np.concatenate([pts + [-f / 2, -f / 2], pts + [f / 2, -f / 2], pts + [f / 2, f / 2],
+                             pts + [-f / 2, f / 2]])
+        pe = np.stack([np.arange(nc) + k * nc for k in range(4)], axis=1)
+        sol.append(dict(material=material.RigidMaterial(0.0, vy, rho).to_c(), pos=pts, vel=np.zeros_like(pts),
+                        mass=np.full(nc, rho * f * f), volume=np.full(nc, f * f), volume0=np.full(nc, f * f),
+                        corners=pc, elems=pe))
+    dt = 0.2 * dx / math.sqrt((m1.lambda_ + 2 * m1.mu) / m1.density)
+    p = make_params(2, _ffi.BASIS_CPDI_Q4, _ffi.ALG_USL, dt, (0.0, 0.0), (100.0, 60.0), (51, 21), tolerance=0.0)
+    sc = scenes.Scene("tube_cpdi", p, sol, steps=300)
+    g, o = _run(sc, sc.steps, hip_lib)
+    assert o.get_state(0)["pstrain"].max() > 1e-3  # the platen has yielded the tube
+    _compare(g, o, sc, exact=True)
+
+
 CONTACT3D_CASES = [(_ffi.BASIS_QUAD_BSPLINE, m) for m in ("elastic", "neohookean", "j2")] + \
     [(_ffi.BASIS_LINEAR, "elastic"), (_ffi.BASIS_GIMP, "elastic"), (_ffi.BASIS_CUBIC_BSPLINE, "elastic")]
 

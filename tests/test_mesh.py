@@ -1,12 +1,14 @@
 """The mesh input of CPDI set-ups (jump_amd/mesh.py): the Gmsh v2 reader
-(Mesh.jl:634-714) on a hand-written file and on the reference's disk.msh
-(tests/golden/disk.msh, an input file of Main_TwoDisksCPDI.jl), the Q4
+(Mesh.jl:634-714) on a hand-written file and on the reference's disk.msh,
+ring.msh and tube.msh (tests/golden/, input files of Main_TwoDisksCPDI.jl,
+Main_TwoRings_CPDI.jl and Main_LateralCompressionTube_CPDI.jl), the Q4
 orientation fix and areas of Solid2D(fileName, mat) (Solid.jl:162-222), and
 the rectangle mesher (Mesh.jl:523-569) against closed forms."""
 import math
 import os
 
 import numpy as np
+import pytest
 
 from jump_amd import material, mesh, solid
 
@@ -85,6 +87,27 @@ def test_disk_mesh_solid():
     solid.move_cpdi(s, (0.45, 0.45))
     assert np.array_equal(s.pos, s.nodes[s.elems].mean(axis=1)) or \
         np.max(np.abs(s.pos - s.nodes[s.elems].mean(axis=1))) < 1e-15
+
+
+@pytest.mark.parametrize("name,centre,ri,ro,nodes,nel", [
+    ("ring.msh", (0.0, 0.0), 30.0, 40.0, 3108, 2884),           # Main_TwoRings_CPDI.jl
+    ("tube.msh", (50.0, 30.0), 47.9 / 2, 47.9 / 2 + 1.48, 8679, 7898),  # Main_LateralCompressionTube_CPDI.jl
+])
+def test_annulus_meshes(name, centre, ri, ro, nodes, nel):
+    """The reference's ring and tube meshes: every Q4 positive after the
+    orientation fix, every element node on or between the two circles (the
+    files also hold the arcs' centre points, which no element uses), and the
+    areas summing to the annulus less the inscribed polygons' deficit."""
+    m = material.ElasticMaterial(1000.0, 0.3, 1000.0, 0, 0, 1000)
+    s = solid.Solid2D(os.path.join(GOLD, name), m)
+    coords, elems = mesh.loadGMSH(os.path.join(GOLD, name))
+    assert coords.shape == (3, nodes) and elems.shape == (nel, 4)
+    areas = np.array([_area(s.nodes[e]) for e in s.elems])
+    assert np.all(areas > 0.0) and np.array_equal(s.volume, areas)
+    r = np.hypot(*(s.nodes[np.unique(s.elems)] - np.asarray(centre)).T)
+    assert r.min() > ri - 1e-9 * ro and r.max() < ro + 1e-9 * ro
+    ann = math.pi * (ro * ro - ri * ri)
+    assert ann * (1.0 - 1e-5) < areas.sum() <= ann * (1.0 + 1e-12)
 
 
 def test_rectangle_mesher():
