@@ -13,7 +13,7 @@ its mean duration, from HIP events on the library's stream over the timed
 region), and the CPU restatement timed on a bounded sample on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--order reference|binned]
-                    [--batch-instances N] [--batch-steps K] [--adjoint-steps K]
+                    [--batch-instances N] [--batch-steps K] [--adjoint-steps K] [--musl2d-steps K]
     python bench.py --copy-probe MIB   # device-to-device copy of MIB MiB, to
                                        # calibrate HBM byte counters
 """
@@ -142,6 +142,40 @@ def batched_leg(n_inst, steps, warmup):
             "speedup": t_plain / t_batch, "unit": "particle-steps/s", "step_flags": flags}
 
 
+def musl2d_leg(steps, warmup, cells=256):
+    """The 2D MUSL step with the blended consistent mass matrix
+    (SolveExplicitSolidMechanics2D.jl:25-329) on a 257 x 257 grid: a
+    stress-free block of 94,249 particles, quadratic B-splines (scenes.
+    musl_block2d).  Per-phase times from HIP events; the solve is the
+    conjugate-gradient solve of the blended mass matrix on the device."""
+    import torch
+
+    from jump_amd import scenes
+    from jump_amd.engine import Engine
+
+    sc = scenes.musl_block2d(cells=cells, steps=steps)
+    e = Engine(sc.params, device=0)
+    sc.load(e)
+    e.step(warmup, 0.0)
+    e.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(steps, 0.0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = e.profile_read()
+    e.profile(False)
+    flags = e.flags()
+    e.close()
+    n = sc.n_particles
+    return {"workload": f"musl_block2d: stress-free block, quadratic B-spline, MUSL(0.99), {cells + 1}x{cells + 1} "
+                        "nodes, blended mass solved by device CG",
+            "particles": n, "grid_nodes": [cells + 1, cells + 1], "steps": steps,
+            "value": n * steps / dt, "unit": "particle-steps/s", "ms_per_step": dt * 1e3 / steps,
+            "phases_ms": {k: round(ms / c, 4) for k, (ms, c) in prof.items() if c},
+            "step_flags": flags}
+
+
 def adjoint_leg(steps, K_recompute=10):
     """Reverse mode against the plain step on the two-rings impact at 1025 x
     513 nodes, 4 particles per cell and direction (1,844,952 particles,
@@ -230,6 +264,7 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=256, help="0 skips the batched leg")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--adjoint-steps", type=int, default=50, help="0 skips the adjoint leg")
+    ap.add_argument("--musl2d-steps", type=int, default=50, help="0 skips the 2D MUSL leg")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -322,6 +357,8 @@ def main():
         out["batched"] = batched_leg(args.batch_instances, args.batch_steps, 2)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
+    if args.musl2d_steps > 0:
+        out["musl2d"] = musl2d_leg(args.musl2d_steps, 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)
