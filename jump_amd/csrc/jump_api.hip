@@ -367,7 +367,8 @@ static int alloc_grid(jump_ctx* ctx) {
     CgArrays& c = ctx->cgm;
     for (double** v : {&c.x, &c.r, &c.p, &c.s, &c.w})
       if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, 4 * nn);
-    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.part, 12 * ((nn + kCgBlock - 1) / kCgBlock));
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.dinv, nn);
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.part, 12 * size_t(cg_tiles(ctx->G)));
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.st, 1);
     if (!r && hipHostMalloc(reinterpret_cast<void**>(&ctx->h_cg), 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
       r = set_err(ctx, JUMP_ERR_NOMEM, "hipHostMalloc failed");
@@ -577,15 +578,16 @@ constexpr int kMuslMaxIt = 2000;
 template <int R> static void solve_musl2d(jump_ctx* ctx) {
   const int nn = ctx->NN;
   const int nblk = (nn + kCgBlock - 1) / kCgBlock;
+  const int ntile = cg_tiles(ctx->G);
   CgArrays& c = ctx->cgm;
-  LAUNCH(k_cg_init, nn, nn, ctx->mu, c);
+  LAUNCH(k_cg_init, nn, nn, ctx->mu, ctx->g.m, c);
   int check = std::max(ctx->cg_check, 4);
   const char* st_done = reinterpret_cast<const char*>(c.st) + offsetof(CgState, all_done);
   static_assert(offsetof(CgState, iters) == offsetof(CgState, all_done) + sizeof(int), "read-back layout");
   for (int it = 0; it < kMuslMaxIt; it++) {
-    hipLaunchKernelGGL((k_cg_spmv<R>), dim3(nblk), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, ctx->g.m, c);
-    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(kCgBlock), 0, ctx->stream, c, nblk, it, kMuslTol);
-    hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kCgBlock), 0, ctx->stream, nn, ctx->g.m, c);
+    hipLaunchKernelGGL((k_cg_spmv<R>), dim3(ntile), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, c);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(kCgBlock), 0, ctx->stream, c, ntile, it, kMuslTol);
+    hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kCgBlock), 0, ctx->stream, nn, c);
     if (it + 1 != check) continue;
     if (hipMemcpyAsync(ctx->h_cg, st_done, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)

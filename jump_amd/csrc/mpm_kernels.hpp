@@ -1209,7 +1209,8 @@ struct CgState {
   int iters;     // iterations that updated at least one column
 };
 struct CgArrays {
-  double* x;  // four columns of nn each, likewise below
+  double* dinv;  // the preconditioner: 1/m, 1 at massless nodes (nn)
+  double* x;     // four columns of nn each, likewise below
   double* r;
   double* p;
   double* s;
@@ -1218,14 +1219,24 @@ struct CgArrays {
   CgState* st;
 };
 constexpr int kCgBlock = 256;
+// The SpMV works on 32 x 8 node tiles, staging z = D^-1 r over the tile and
+// its R-wide halo in LDS (each z then feeds up to (2R+1)^2 rows).
+constexpr int kCgTX = 32, kCgTY = 8;
+static_assert(kCgTX * kCgTY == kCgBlock, "one node per thread");
+__host__ __device__ inline int cg_tiles(const GridDesc& G) {
+  return ((G.nx + kCgTX - 1) / kCgTX) * ((G.ny + kCgTY - 1) / kCgTY);
+}
 
-__global__ void __launch_bounds__(kBlock) k_cg_init(int nn, Musl2Arrays mu, CgArrays cg) {
+__global__ void __launch_bounds__(kBlock) k_cg_init(int nn, Musl2Arrays mu, const double* __restrict__ gm,
+                                                    CgArrays cg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     CgState z{};
     *cg.st = z;
   }
   if (i >= nn) return;
+  const double mi = gm[i];
+  cg.dinv[i] = mi > 0.0 ? 1.0 / mi : 1.0;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const size_t k = size_t(c) * nn + i;
@@ -1237,20 +1248,35 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nn, Musl2Arrays mu, CgAr
 }
 
 template <int R>
-__global__ void __launch_bounds__(kCgBlock) k_cg_spmv(GridDesc G, Musl2Arrays mu, const double* __restrict__ gm,
-                                                      CgArrays cg) {
+__global__ void __launch_bounds__(kCgBlock) k_cg_spmv(GridDesc G, Musl2Arrays mu, CgArrays cg) {
   constexpr int K1 = 2 * R + 1;
   constexpr int NW = kCgBlock / 64;
+  constexpr int LX = kCgTX + 2 * R, LY = kCgTY + 2 * R;
+  __shared__ double zs[4][LY][LX];
   __shared__ double red[12][NW];
-  if (cg.st->all_done) return;
+  if (cg.st->all_done) return;  // uniform: every workgroup returns
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int i = blockIdx.x * kCgBlock + t;
   const int nn = G.nn;
+  const int ntx = (G.nx + kCgTX - 1) / kCgTX;
+  const int x0 = int(blockIdx.x % unsigned(ntx)) * kCgTX, y0 = int(blockIdx.x / unsigned(ntx)) * kCgTY;
+  // z over the tile and its halo (zero outside the grid; never read there)
+  for (int e = t; e < LX * LY; e += kCgBlock) {
+    const int lx = e % LX, ly = e / LX;
+    const int gx = x0 + lx - R, gy = y0 + ly - R;
+    const bool in = gx >= 0 && gx < G.nx && gy >= 0 && gy < G.ny;
+    const int j = in ? gx + G.nx * gy : 0;
+    const double dj = in ? cg.dinv[j] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) zs[c][ly][lx] = in ? cg.r[size_t(c) * nn + j] * dj : 0.0;
+  }
+  __syncthreads();
+  const int tx = t % kCgTX, ty = t / kCgTX;
+  const int ix = x0 + tx, iy = y0 + ty;
   double acc[12];
 #pragma unroll
   for (int c = 0; c < 12; c++) acc[c] = 0.0;
-  if (i < nn) {
-    const int ix = i % G.nx, iy = i / G.nx;
+  if (ix < G.nx && iy < G.ny) {
+    const int i = ix + G.nx * iy;
     double y[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int b = -R; b <= R; b++) {
@@ -1259,19 +1285,14 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(GridDesc G, Musl2Arrays mu
       for (int a = -R; a <= R; a++) {
         if (ix + a < 0 || ix + a >= G.nx) continue;
         const double cf = mu.brow[size_t((a + R) + K1 * (b + R)) * nn + i];
-        const int j = i + a + G.nx * b;
-        const double mj = gm[j];
-        const double dj = mj > 0.0 ? 1.0 / mj : 1.0;
 #pragma unroll
-        for (int c = 0; c < 4; c++) y[c] += cf * (cg.r[size_t(c) * nn + j] * dj);
+        for (int c = 0; c < 4; c++) y[c] += cf * zs[c][ty + b + R][tx + a + R];
       }
     }
-    const double mi = gm[i];
-    const double di = mi > 0.0 ? 1.0 / mi : 1.0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       const double ri = cg.r[size_t(c) * nn + i];
-      const double zi = ri * di;
+      const double zi = zs[c][ty + R][tx + R];
       cg.w[size_t(c) * nn + i] = y[c];
       acc[c] = ri * zi;
       acc[4 + c] = y[c] * zi;
@@ -1350,13 +1371,12 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, 
   st.all_done = all ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(kCgBlock) k_cg_update(int nn, const double* __restrict__ gm, CgArrays cg) {
+__global__ void __launch_bounds__(kCgBlock) k_cg_update(int nn, CgArrays cg) {
   const CgState& st = *cg.st;
   if (st.all_done) return;
   const int i = blockIdx.x * kCgBlock + threadIdx.x;
   if (i >= nn) return;
-  const double mi = gm[i];
-  const double di = mi > 0.0 ? 1.0 / mi : 1.0;
+  const double di = cg.dinv[i];
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     if (st.done[c]) continue;
