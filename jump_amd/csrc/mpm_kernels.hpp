@@ -219,8 +219,8 @@ __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& 
   }
   Axis X = axis_of(G, 0), Y = axis_of(G, 1);
   double wx, dwx, wy, dwy;
-  axis_weight<B, QUIRK>(xp[0], 0, X, lp[0], ii, ii, wx, dwx);
-  axis_weight<B, QUIRK>(xp[1], 1, Y, lp[1], jj, jj, wy, dwy);
+  axis_weight<B, QUIRK, true>(xp[0], 0, X, lp[0], ii, ii, wx, dwx);
+  axis_weight<B, QUIRK, true>(xp[1], 1, Y, lp[1], jj, jj, wy, dwy);
   if (D == 2) {
     if (GRAD) combine2<B>(X, Y, wx, dwx, wy, dwy, N, dN);
     else N = wx * wy;
@@ -228,7 +228,7 @@ __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& 
     Axis Z = axis_of(G, 2);
     double wz, dwz;
     int sel4 = (QUIRK && B == JUMP_BASIS_CUBIC_BSPLINE) ? jj : kk;
-    axis_weight<B, QUIRK>(xp[2], 2, Z, lp[2], kk, sel4, wz, dwz);
+    axis_weight<B, QUIRK, true>(xp[2], 2, Z, lp[2], kk, sel4, wz, dwz);
     if (GRAD) combine3<B>(X, Y, Z, wx, dwx, wy, dwy, wz, dwz, N, dN);
     else N = wx * wy * wz;
   }

@@ -1,5 +1,6 @@
 // Host build of the device shape-function kernels: checks that the
-// branch-free quadratic B-spline evaluation equals the piecewise definition
+// branch-free quadratic B-spline evaluation, and its split form with the
+// interior-node path, equal the piecewise definition
 // (Bsplines.jl:18-83) bit for bit, for every boundary type, at every piece
 // boundary, at its neighbouring doubles and on a dense grid.  Prints the
 // number of points checked and exits 1 on the first mismatch.
@@ -22,12 +23,14 @@ int main() {
   long checked = 0;
   for (int type = 1; type <= 4; type++) {
     auto check = [&](double r) {
-      double n0, d0, n1, d1;
+      double n0, d0, n1, d1, n2, d2;
       jh::qbs_eval_branchy(type, r, n0, d0);
       jh::qbs_eval(type, r, n1, d1);
+      jh::qbs_eval_split(type, r, n2, d2);  // the interior path for type 3
       checked++;
-      if (!same(n0, n1) || !same(d0, d1)) {
-        printf("mismatch type %d r=%.17g: %.17g/%.17g vs %.17g/%.17g\n", type, r, n0, d0, n1, d1);
+      if (!same(n0, n1) || !same(d0, d1) || !same(n0, n2) || !same(d0, d2)) {
+        printf("mismatch type %d r=%.17g: %.17g/%.17g vs %.17g/%.17g vs %.17g/%.17g\n", type, r, n0, d0, n1, d1, n2,
+               d2);
         return false;
       }
       return true;
