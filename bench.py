@@ -47,15 +47,15 @@ def algorithmic_bytes(phase, n_plate, n_sphere, n_active):
     if phase == "sort":
         return 32 * n  # read x (24), write key + perm (8)
     if phase == "p2g":
-        return 112 * n + 80 * n_active  # read x,v,m,V,sigma; write m,p0,p,f
+        return 112 * n + 112 * n_active  # read x,v,m,V,sigma; write m,p0,p,f and minv,q
     if phase == "g2p1":
-        return 72 * n + 56 * n_active  # read x,v; write v; read m,p,p0
+        return 72 * n + 32 * n_active  # read x,v; write v; read minv,q
     if phase == "p2g2":
         return 56 * n + 24 * n_active  # read x,m,v; write p2
     if phase == "g2p2":
         plate = 208 + 264  # read x,V0,F,eps,alpha,ep; write x,V,F,eps,sig,alpha,ep,vm
         sphere = 200 + 208  # read x,V0,F,eps,sig; write x,V,F,eps,sig,vm
-        return plate * n_plate + sphere * n_sphere + 56 * n_active  # + read m,p,p2
+        return plate * n_plate + sphere * n_sphere + 56 * n_active  # + read minv,p,p2
     return 0
 
 

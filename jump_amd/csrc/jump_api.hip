@@ -343,6 +343,10 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_sums, (nn + 1) / kScanItems + 2);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_flags, 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_wmax, 1);
+  if (!r && D == 3) {
+    r = dev_alloc(ctx, ctx->grid_allocs, &g.minv, nn);
+    for (int k = 0; k < 3 && !r; k++) r = dev_alloc(ctx, ctx->grid_allocs, &g.q[k], nn);
+  }
   if (ctx->prm.contact && D == 3) {
     ContactArrays3& c = ctx->cg3;
     for (int k = 0; k < 2 && !r; k++) {
@@ -482,7 +486,7 @@ template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   LAUNCH_P2G((k_p2g<2, B, 0, ORDER, MULTI, MULTI>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt());
+             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt(), 0.0);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   for (int r = 0; r <= ctx->max_rigid_rank; r++)
@@ -547,7 +551,7 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   LAUNCH_P2G((k_p2g<3, B, 0, ORDER, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt());
+             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
@@ -557,7 +561,7 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
     const int rows = (ctx->NN + kP2GBlock - 1) / kP2GBlock;
     hipLaunchKernelGGL((k_p2g<3, B, 1, ORDER, true, false>), dim3(xcd_grid(rows)), dim3(kP2GBlock),
                        0, ctx->stream, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
-                       ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt());
+                       ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha);
   }
   prof_end(ctx, JUMP_PHASE_P2G2);
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)

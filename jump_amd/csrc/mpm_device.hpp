@@ -89,6 +89,13 @@ struct GridArrays {
   double* p2[3];
   double* f[3];
   const uint8_t* fixed;  // bit k set: direction k fixed
+  // 3D MUSL, written by the force pass for the G2P passes: minv = 1/m (0
+  // where m <= 0) and q = p - alpha p0, the node factors the reference forms
+  // without the particle's weight (SolveExplicitSolidMechanics3D.jl:124-126,
+  // 176-180), so G2P-1 loads four fields per node instead of seven and
+  // neither pass divides
+  double* minv;
+  double* q[3];
 };
 
 // Multi-body contact: each node keeps up to two solids ("slots", filled in

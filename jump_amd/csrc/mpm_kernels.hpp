@@ -478,7 +478,8 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
                                                    const unsigned long long* __restrict__ sorted,
                                                    const unsigned long long* __restrict__ bxyz,
                                                    const int* __restrict__ order, const int* __restrict__ nodepos,
-                                                   GridArrays g0, int Wrt, double dt0, BatchDev Bt) {
+                                                   GridArrays g0, int Wrt, double dt0, BatchDev Bt,
+                                                   double alpha) {
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
   // Reference order: per window bin, the position in `sorted` of its next
@@ -526,6 +527,8 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   if (PASS == 0) {
     const uint8_t fx = g.fixed[n];
     g.m[n] = am;
+    const bool g2p = D == 3 && !MULTI;
+    if (g2p) g.minv[n] = am > 0.0 ? 1.0 / am : 0.0;
     for (int k = 0; k < D; k++) {
       double p0 = ap[k];
       double pk = p0 + af[k] * dt;
@@ -533,6 +536,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
       g.p0[k][n] = p0;
       g.p[k][n] = pk;
       g.f[k][n] = af[k];
+      if (g2p) g.q[k][n] = pk - alpha * p0;
     }
   } else {
     for (int k = 0; k < D; k++) g.p2[k][n] = ap[k];
@@ -951,10 +955,10 @@ __global__ void __launch_bounds__(kBlock) k_g2p1_musl(Particles P, GridDesc G, c
   const SolidDev& s = S[P.sid[i]];
   double vx[3] = {0.0, 0.0, 0.0};
   bool ok = walk_stencil<3, B, true, false>(P, G, s, corners, i, [&](int n, double Ni, const double*) {
-    const double mI = g.m[n];
-    if (mI > 0.0) {
-      const double mii = (1.0 / mI) * Ni;
-      for (int k = 0; k < 3; k++) vx[k] += (g.p[k][n] - alpha * g.p0[k][n]) * mii;
+    const double minv = g.minv[n];  // 1/m, 0 without mass
+    if (minv > 0.0) {
+      const double mii = minv * Ni;
+      for (int k = 0; k < 3; k++) vx[k] += g.q[k][n] * mii;  // q = p - alpha p0
     }
   });
   if (!ok) {
@@ -1041,9 +1045,8 @@ __global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, c
   double L[9];
   for (int k = 0; k < 9; k++) L[k] = 0.0;
   bool ok = walk_stencil<3, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
-    const double m = g.m[n];
-    if (m > 0.0) {
-      const double mii = 1.0 / m;
+    const double mii = g.minv[n];  // 1/m, 0 without mass
+    if (mii > 0.0) {
       const double vIx = mii * g.p2[0][n], vIy = mii * g.p2[1][n], vIz = mii * g.p2[2][n];
       const double c = Ni * mii * dt;
       for (int k = 0; k < 3; k++) xx[k] += c * g.p[k][n];
