@@ -47,15 +47,15 @@ def algorithmic_bytes(phase, n_plate, n_sphere, n_active):
     if phase == "sort":
         return 32 * n  # read x (24), write key + perm (8)
     if phase == "p2g":
-        return 112 * n + 112 * n_active  # read x,v,m,V,sigma; write m,p0,p,f and minv,q
+        return 112 * n + 128 * n_active  # read x,v,m,V,sigma; write m,p0,p,f and the G2P pairs (48 B)
     if phase == "g2p1":
-        return 72 * n + 32 * n_active  # read x,v; write v; read minv,q
+        return 72 * n + 32 * n_active  # read x,v; write v; read the pairs (minv,q0) (q1,q2)
     if phase == "p2g2":
-        return 56 * n + 24 * n_active  # read x,m,v; write p2
+        return 56 * n + 56 * n_active  # read x,m,v; write p2 and the pairs (p2x,p2y) (pz,p2z)
     if phase == "g2p2":
         plate = 208 + 264  # read x,V0,F,eps,alpha,ep; write x,V,F,eps,sig,alpha,ep,vm
         sphere = 200 + 208  # read x,V0,F,eps,sig; write x,V,F,eps,sig,vm
-        return plate * n_plate + sphere * n_sphere + 56 * n_active  # + read minv,p,p2
+        return plate * n_plate + sphere * n_sphere + 64 * n_active  # + read 4 pairs (minv,p,p2)
     return 0
 
 

@@ -344,8 +344,8 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_flags, 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_wmax, 1);
   if (!r && D == 3) {
-    r = dev_alloc(ctx, ctx->grid_allocs, &g.minv, nn);
-    for (int k = 0; k < 3 && !r; k++) r = dev_alloc(ctx, ctx->grid_allocs, &g.q[k], nn);
+    for (double2** v : {&g.mq, &g.qq, &g.pxy, &g.p2xy, &g.pz2})
+      if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, nn);
   }
   if (ctx->prm.contact && D == 3) {
     ContactArrays3& c = ctx->cg3;

@@ -89,13 +89,18 @@ struct GridArrays {
   double* p2[3];
   double* f[3];
   const uint8_t* fixed;  // bit k set: direction k fixed
-  // 3D MUSL, written by the force pass for the G2P passes: minv = 1/m (0
-  // where m <= 0) and q = p - alpha p0, the node factors the reference forms
-  // without the particle's weight (SolveExplicitSolidMechanics3D.jl:124-126,
-  // 176-180), so G2P-1 loads four fields per node instead of seven and
-  // neither pass divides
-  double* minv;
-  double* q[3];
+  // 3D MUSL, for the G2P passes: the node factors the reference forms
+  // without the particle's weight, minv = 1/m (0 where m <= 0) and
+  // q = p - alpha p0 (SolveExplicitSolidMechanics3D.jl:124-126, 176-180),
+  // and the momenta, in pairs of their own arrays (16 bytes per node, each
+  // array contiguous): G2P-1 loads two pairs per node, G2P-2 four.
+  //   mq = (minv, q0), qq = (q1, q2), pxy = (px, py)    by the force pass
+  //   p2xy = (p2x, p2y), pz2 = (pz, p2z)                 by the re-projection
+  double2* mq;
+  double2* qq;
+  double2* pxy;
+  double2* p2xy;
+  double2* pz2;
 };
 
 // Multi-body contact: each node keeps up to two solids ("slots", filled in

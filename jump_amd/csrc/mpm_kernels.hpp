@@ -503,6 +503,8 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
       g0.m[ng] = 0.0;
       for (int k = 0; k < D; k++) { g0.p0[k][ng] = 0.0; g0.p[k][ng] = 0.0; g0.f[k][ng] = 0.0; }
     } else {
+      // (p2xy, pz2 are left alone: G2P reads them only where 1/m > 0, and
+      // a node with mass has a particle in its window)
       for (int k = 0; k < D; k++) g0.p2[k][ng] = 0.0;
     }
     return;
@@ -527,8 +529,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   if (PASS == 0) {
     const uint8_t fx = g.fixed[n];
     g.m[n] = am;
-    const bool g2p = D == 3 && !MULTI;
-    if (g2p) g.minv[n] = am > 0.0 ? 1.0 / am : 0.0;
+    double q[D], pn[D];
     for (int k = 0; k < D; k++) {
       double p0 = ap[k];
       double pk = p0 + af[k] * dt;
@@ -536,10 +537,20 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
       g.p0[k][n] = p0;
       g.p[k][n] = pk;
       g.f[k][n] = af[k];
-      if (g2p) g.q[k][n] = pk - alpha * p0;
+      q[k] = pk - alpha * p0;
+      pn[k] = pk;
+    }
+    if (D == 3 && !MULTI) {
+      g.mq[n] = make_double2(am > 0.0 ? 1.0 / am : 0.0, q[0]);
+      g.qq[n] = make_double2(q[1], q[D - 1]);
+      g.pxy[n] = make_double2(pn[0], pn[1]);
     }
   } else {
     for (int k = 0; k < D; k++) g.p2[k][n] = ap[k];
+    if (D == 3 && !MULTI) {
+      g.p2xy[n] = make_double2(ap[0], ap[1]);
+      g.pz2[n] = make_double2(g.p[D - 1][n], ap[D - 1]);
+    }
   }
 }
 
@@ -955,10 +966,13 @@ __global__ void __launch_bounds__(kBlock) k_g2p1_musl(Particles P, GridDesc G, c
   const SolidDev& s = S[P.sid[i]];
   double vx[3] = {0.0, 0.0, 0.0};
   bool ok = walk_stencil<3, B, true, false>(P, G, s, corners, i, [&](int n, double Ni, const double*) {
-    const double minv = g.minv[n];  // 1/m, 0 without mass
-    if (minv > 0.0) {
-      const double mii = minv * Ni;
-      for (int k = 0; k < 3; k++) vx[k] += g.q[k][n] * mii;  // q = p - alpha p0
+    const double2 a = g.mq[n];  // (1/m, q0), 1/m = 0 without mass; q = p - alpha p0
+    if (a.x > 0.0) {
+      const double mii = a.x * Ni;
+      const double2 b = g.qq[n];
+      vx[0] += a.y * mii;
+      vx[1] += b.x * mii;
+      vx[2] += b.y * mii;
     }
   });
   if (!ok) {
@@ -1045,11 +1059,14 @@ __global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, c
   double L[9];
   for (int k = 0; k < 9; k++) L[k] = 0.0;
   bool ok = walk_stencil<3, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
-    const double mii = g.minv[n];  // 1/m, 0 without mass
+    const double mii = g.mq[n].x;  // 1/m, 0 without mass
     if (mii > 0.0) {
-      const double vIx = mii * g.p2[0][n], vIy = mii * g.p2[1][n], vIz = mii * g.p2[2][n];
+      const double2 a = g.p2xy[n], b = g.pz2[n], c2 = g.pxy[n];
+      const double vIx = mii * a.x, vIy = mii * a.y, vIz = mii * b.y;
       const double c = Ni * mii * dt;
-      for (int k = 0; k < 3; k++) xx[k] += c * g.p[k][n];
+      xx[0] += c * c2.x;
+      xx[1] += c * c2.y;
+      xx[2] += c * b.x;
       // L[a,b] = dN_a v_b (Basis gradient index first, as SMatrix{3,3}(...) at :189-191)
       L[0] += dN[0] * vIx; L[1] += dN[1] * vIx; L[2] += dN[2] * vIx;
       L[3] += dN[0] * vIy; L[4] += dN[1] * vIy; L[5] += dN[2] * vIy;
