@@ -65,7 +65,7 @@ struct jump_ctx {
   std::vector<void*> grid_allocs;
   double* d_corners = nullptr;
   SolidDev* d_solids = nullptr;
-  int *d_key = nullptr, *d_cnt = nullptr, *d_start = nullptr, *d_cursor = nullptr,
+  int *d_key = nullptr, *d_rank = nullptr, *d_cnt = nullptr, *d_start = nullptr,
       *d_sums = nullptr;
   unsigned long long* d_bxyz = nullptr;
   int *d_nodepos = nullptr, *d_order = nullptr;  // P2G work order: scan of active flags, node per slot
@@ -229,6 +229,7 @@ static int alloc_particles(jump_ctx* ctx) {
   if (ctx->prm.basis == JUMP_BASIS_CPDI_Q4)
     for (int k = 0; k < 4 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.elem[k], n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_key, n);
+  if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_rank, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_sorted, n + 1);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_bxyz, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_red, 2 * size_t(blocks_for(ctx->np) + 1));
@@ -337,7 +338,6 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_fixed, nn);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_cnt, nn + 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_start, nn + 1);
-  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_cursor, nn + 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_nodepos, nn + 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_order, nn);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_sums, (nn + 1) / kScanItems + 2);
@@ -462,11 +462,10 @@ template <int D, int B, bool MULTI> static int bin_and_sort(jump_ctx* ctx) {
   const int nn = ctx->NN;
   const int np = int(ctx->np);
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cnt, 0, nn + 1);
-  LAUNCH(k_fill_i32, nn + 1, ctx->d_cursor, 0, nn + 1);
-  LAUNCH((k_bin<D, B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_bxyz,
-         ctx->d_cnt, ctx->d_flags, ctx->bt());
+  LAUNCH((k_bin<D, B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_rank,
+         ctx->d_bxyz, ctx->d_cnt, ctx->d_flags, ctx->bt());
   scan_nodes(ctx, ctx->d_cnt, ctx->d_start);
-  LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_cursor, ctx->d_sorted, np, nn);
+  LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_rank, ctx->d_sorted, np, nn);
   LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_sorted, nn);
   // P2G work order: active nodes first (d_cnt is free again as the flags)
   const int W = (B == JUMP_BASIS_CPDI_Q4) ? ctx->cpdi_w : Width<B>::W;

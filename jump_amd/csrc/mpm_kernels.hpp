@@ -56,8 +56,8 @@ __global__ void k_fill_i32(int* __restrict__ a, int v, int n) {
 template <int D, int B, bool MULTI>
 __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
                                                 const double* __restrict__ corners, int* __restrict__ key,
-                                                unsigned long long* __restrict__ bxyz, int* __restrict__ cnt,
-                                                unsigned* __restrict__ flags, BatchDev Bt) {
+                                                int* __restrict__ rank, unsigned long long* __restrict__ bxyz,
+                                                int* __restrict__ cnt, unsigned* __restrict__ flags, BatchDev Bt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
     }
     if (ok) {
       k = off + (f[0] - 1) + G.nx * ((f[1] - 1) + (D == 3 ? G.ny * (f[2] - 1) : 0));
-      atomicAdd(&cnt[k], 1);
+      rank[i] = atomicAdd(&cnt[k], 1);  // its slot in the bin (any order; k_binsort orders it)
     } else {
       atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
     }
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
   }
   key[i] = -1;
   bxyz[i] = ~0ull;
-  atomicAdd(&cnt[NN], 1);
+  rank[i] = atomicAdd(&cnt[NN], 1);
 }
 
 // CPDI stencil extent (static corners): max bounding-box width in nodes.
@@ -173,13 +173,13 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(int* __restrict__ out, cons
 // from the entry itself.  Particles outside the grid go to the extra bin nn,
 // so every entry of sorted[0, np) is written.
 __global__ void __launch_bounds__(kBlock) k_scatter(const int* __restrict__ key, const int* __restrict__ start,
-                                                    int* __restrict__ cursor, unsigned long long* __restrict__ sorted,
-                                                    int np, int nn) {
+                                                    const int* __restrict__ rank,
+                                                    unsigned long long* __restrict__ sorted, int np, int nn) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int k = key[i];
   if (k < 0) k = nn;
-  int pos = start[k] + atomicAdd(&cursor[k], 1);
+  int pos = start[k] + rank[i];
   sorted[pos] = ((unsigned long long)unsigned(k) << 32) | unsigned(i);
 }
 
