@@ -115,26 +115,34 @@ __global__ void __launch_bounds__(kBlock) k_cpdi_width(Particles P, GridDesc G, 
 // Exclusive scan, 1024 items per block (256 threads x 4), three passes.
 constexpr int kScanItems = 1024;
 
+// Exclusive scan over the block (kBlock threads): a shuffle scan per wave,
+// then the wave totals through LDS.  lds: at least kBlock / 64 ints.
 __device__ __forceinline__ int block_excl_scan(int v, int* lds, int& total) {
-  // lds: kBlock ints
-  int t = threadIdx.x;
-  lds[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
-    int add = t >= off ? lds[t - off] : 0;
-    __syncthreads();
-    lds[t] += add;
-    __syncthreads();
+  constexpr int NW = kBlock / 64;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  total = lds[kBlock - 1];
-  int incl = lds[t];
+  if (lane == 63) lds[w] = x;
   __syncthreads();
-  return incl - v;
+  int pre = 0;
+  total = 0;
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    const int s = lds[j];
+    if (j < w) pre += s;
+    total += s;
+  }
+  __syncthreads();  // lds is free again
+  return pre + x - v;
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_block(const int* __restrict__ in, int* __restrict__ out,
                                                        int* __restrict__ sums, int n) {
-  __shared__ int lds[kBlock];
+  __shared__ int lds[kBlock / 64];
   int base = blockIdx.x * kScanItems + threadIdx.x * 4;
   int v[4];
   for (int j = 0; j < 4; j++) v[j] = (base + j < n) ? in[base + j] : 0;
@@ -148,18 +156,21 @@ __global__ void __launch_bounds__(kBlock) k_scan_block(const int* __restrict__ i
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// Single block: exclusive scan of the block sums in place.
+// Single block: exclusive scan of the block sums in place.  Each thread
+// scans a contiguous run of ceil(nb / kBlock) sums, then the runs' totals
+// are scanned across the block.
 __global__ void __launch_bounds__(kBlock) k_scan_sums(int* __restrict__ sums, int nb) {
-  __shared__ int lds[kBlock];
-  int carry = 0;
-  for (int base = 0; base < nb; base += kBlock) {
-    int i = base + threadIdx.x;
-    int v = i < nb ? sums[i] : 0;
-    int total;
-    int ex = block_excl_scan(v, lds, total);
-    if (i < nb) sums[i] = ex + carry;
-    carry += total;
-    __syncthreads();
+  __shared__ int lds[kBlock / 64];
+  const int per = (nb + kBlock - 1) / kBlock;
+  const int lo = min(nb, int(threadIdx.x) * per), hi = min(nb, lo + per);
+  int run = 0;
+  for (int i = lo; i < hi; i++) run += sums[i];
+  int total;
+  int acc = block_excl_scan(run, lds, total);
+  for (int i = lo; i < hi; i++) {
+    const int v = sums[i];
+    sums[i] = acc;
+    acc += v;
   }
 }
 
