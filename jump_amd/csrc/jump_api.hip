@@ -59,6 +59,7 @@ struct jump_ctx {
   ContactArrays3 cg3{};  // contact mode only, 3D
   Musl2Arrays mu{};      // 2D MUSL only: matrix rows, right-hand sides, velocities
   CgArrays cgm{};        // 2D MUSL only: the conjugate-gradient vectors and state
+  RunList runs{};        // 3D: the force pass's particle runs per node, replayed by the re-projection
   int* h_cg = nullptr;   // pinned: CgState::all_done and ::iters read back
   int cg_check = 8;      // CG iterations before the first convergence read-back
   std::vector<void*> part_allocs;
@@ -346,6 +347,8 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r && D == 3) {
     for (double2** v : {&g.mq, &g.qq, &g.pxy, &g.p2xy, &g.pz2})
       if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, nn);
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->runs.runs, size_t(kRunCap) * nn);
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->runs.nruns, nn);
   }
   if (ctx->prm.contact && D == 3) {
     ContactArrays3& c = ctx->cg3;
@@ -485,7 +488,8 @@ template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   LAUNCH_P2G((k_p2g<2, B, 0, ORDER, MULTI, MULTI>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt(), 0.0);
+             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt(), 0.0,
+             RunList{});
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   for (int r = 0; r <= ctx->max_rigid_rank; r++)
@@ -550,7 +554,8 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   LAUNCH_P2G((k_p2g<3, B, 0, ORDER, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha);
+             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha,
+             ctx->runs);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
@@ -560,7 +565,8 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
     const int rows = (ctx->NN + kP2GBlock - 1) / kP2GBlock;
     hipLaunchKernelGGL((k_p2g<3, B, 1, ORDER, true, false>), dim3(xcd_grid(rows)), dim3(kP2GBlock),
                        0, ctx->stream, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
-                       ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha);
+                       ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha,
+                       ctx->runs);
   }
   prof_end(ctx, JUMP_PHASE_P2G2);
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
