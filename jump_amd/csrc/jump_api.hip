@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <climits>
 #include <cstddef>
 #include <memory>
 #include <cstdio>
@@ -60,6 +61,9 @@ struct jump_ctx {
   Musl2Arrays mu{};      // 2D MUSL only: matrix rows, right-hand sides, velocities
   CgArrays cgm{};        // 2D MUSL only: the conjugate-gradient vectors and state
   RunList runs{};        // 3D: the force pass's particle runs per node, replayed by the re-projection
+  int* d_dirty = nullptr;  // 3D: per node, the stamp of the step its window last changed
+  int runs_gen = 0;        // stamp of the last step
+  bool runs_valid = false; // the recorded runs match the particles' current bins
   int* h_cg = nullptr;   // pinned: CgState::all_done and ::iters read back
   int cg_check = 8;      // CG iterations before the first convergence read-back
   std::vector<void*> part_allocs;
@@ -290,6 +294,7 @@ static int upload(jump_ctx* ctx) {
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->on_device = true;
   ctx->cpdi_w = 0;
+  ctx->runs_valid = false;  // new particle state: every node merges again
   return JUMP_OK;
 }
 
@@ -349,6 +354,8 @@ static int alloc_grid(jump_ctx* ctx) {
       if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->runs.runs, size_t(kRunCap) * nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->runs.nruns, nn);
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_dirty, nn);
+    if (!r) HIP_TRY(hipMemsetAsync(ctx->d_dirty, 0, nn * sizeof(int), ctx->stream));
   }
   if (ctx->prm.contact && D == 3) {
     ContactArrays3& c = ctx->cg3;
@@ -461,12 +468,13 @@ static void scan_nodes(jump_ctx* ctx, const int* in, int* out) {
   LAUNCH(k_scan_add, n, out, ctx->d_sums, n);
 }
 
-template <int D, int B, bool MULTI> static int bin_and_sort(jump_ctx* ctx) {
+// dirty, gen: 3D run replay (k_bin stamps nodes whose window changed)
+template <int D, int B, bool MULTI> static int bin_and_sort(jump_ctx* ctx, int* dirty = nullptr, int gen = 0) {
   const int nn = ctx->NN;
   const int np = int(ctx->np);
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cnt, 0, nn + 1);
   LAUNCH((k_bin<D, B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_rank,
-         ctx->d_bxyz, ctx->d_cnt, ctx->d_flags, ctx->bt());
+         ctx->d_bxyz, ctx->d_cnt, ctx->d_flags, ctx->bt(), dirty, gen);
   scan_nodes(ctx, ctx->d_cnt, ctx->d_start);
   LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_rank, ctx->d_sorted, np, nn);
   LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_sorted, nn);
@@ -549,13 +557,27 @@ template <int B, int ORDER> static void step_musl3d_contact(jump_ctx* ctx) {
 template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
+  // run replay (reference order): this step's stamp; after an upload every
+  // node merges and records
+  RunList rl = ctx->runs;
+  if (ORDER == JUMP_ORDER_REFERENCE) {
+    if (ctx->runs_gen == INT_MAX) {  // stamps wrapped: start over
+      ctx->runs_gen = 0;
+      ctx->runs_valid = false;
+      LAUNCH(k_fill_i32, ctx->NN, ctx->d_dirty, 0, ctx->NN);
+    }
+    rl.dirty = ctx->d_dirty;
+    rl.gen = ++ctx->runs_gen;
+    rl.all = ctx->runs_valid ? 0 : 1;
+  }
   prof_begin(ctx);
-  bin_and_sort<3, B, false>(ctx);
+  bin_and_sort<3, B, false>(ctx, ORDER == JUMP_ORDER_REFERENCE ? ctx->d_dirty : nullptr, rl.gen);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   LAUNCH_P2G((k_p2g<3, B, 0, ORDER, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
              ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha,
-             ctx->runs);
+             rl);
+  if (ORDER == JUMP_ORDER_REFERENCE) ctx->runs_valid = true;
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
@@ -566,7 +588,7 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
     hipLaunchKernelGGL((k_p2g<3, B, 1, ORDER, true, false>), dim3(xcd_grid(rows)), dim3(kP2GBlock),
                        0, ctx->stream, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
                        ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha,
-                       ctx->runs);
+                       rl);
   }
   prof_end(ctx, JUMP_PHASE_P2G2);
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
@@ -1274,6 +1296,7 @@ int jump_get_bins(jump_ctx* ctx, int32_t* keys) {
       case JUMP_BASIS_GIMP: bin_and_sort<3, JUMP_BASIS_GIMP, false>(ctx); break;
     }
   }
+  ctx->runs_valid = false;  // the bins now differ from those the runs were recorded on
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(keys, ctx->d_key, size_t(ctx->np) * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));

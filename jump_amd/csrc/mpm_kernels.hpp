@@ -57,9 +57,23 @@ template <int D, int B, bool MULTI>
 __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
                                                 const double* __restrict__ corners, int* __restrict__ key,
                                                 int* __restrict__ rank, unsigned long long* __restrict__ bxyz,
-                                                int* __restrict__ cnt, unsigned* __restrict__ flags, BatchDev Bt) {
+                                                int* __restrict__ cnt, unsigned* __restrict__ flags, BatchDev Bt,
+                                                int* __restrict__ dirty, int gen) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
+  // 3D: a particle whose bin changed since the last binning changes the
+  // windows of the nodes that contain its old or new bin; stamp them
+  const int old = (D == 3 && !MULTI && dirty) ? key[i] : -1;
+  auto mark = [&](int b) {
+    if (D != 3 || MULTI || !dirty || b < 0) return;
+    constexpr int W = Width<B>::W;
+    const int bx = b % G0.nx, by = (b / G0.nx) % G0.ny, bz = b / G0.nxy;
+    for (int c = 0; c < W; c++)
+      for (int y = 0; y < W; y++)
+        for (int a = 0; a < W; a++)
+          if (bx + a < G0.nx && by + y < G0.ny && bz + c < G0.nz)
+            dirty[(bx + a) + G0.nx * (by + y) + G0.nxy * (bz + c)] = gen;
+  };
   const SolidDev& s = S[P.sid[i]];
   const GridDesc& G = MULTI ? Bt.inst[s.inst].G : G0;
   const int off = MULTI ? Bt.node_off[s.inst] : 0;
@@ -85,6 +99,10 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
       atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
     }
     if (ok) {
+      if (k != old) {
+        mark(old);
+        mark(k);
+      }
       key[i] = k;
       // bin coordinates 21 bits per axis; a 2D batch keeps the instance in
       // the third field, so a window of one instance never admits another's
@@ -93,6 +111,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
       return;
     }
   }
+  if (old >= 0) mark(old);
   key[i] = -1;
   bxyz[i] = ~0ull;
   rank[i] = atomicAdd(&cnt[NN], 1);
@@ -398,17 +417,23 @@ struct NoRuns {
   __device__ void operator()(int, int) const {}
 };
 // The 3D re-projection pass visits exactly the particles of the force
-// pass's merge, in the same order (same bins, same windows).  The force pass
-// records each node's runs of consecutive particle indices, and the
-// re-projection replays them instead of merging again (the merge is about
-// 0.84 of the force pass's 1.97 ms on the 4.2M-particle impact).  A run is
-// one 32-bit entry, first particle << 3 | (length - 1), longer runs taking
-// several; entry r of the node at compacted position i is runs[r NN + i].
-// A node with more than kRunCap entries keeps the merge in both passes.
+// pass's merge, in the same order (same bins, same windows), and a node's
+// window holds the same particles from one step to the next unless one of
+// them changed bin.  The force pass records each node's runs of consecutive
+// particle indices; the re-projection replays them, and so does the next
+// step's force pass at every node whose window no particle entered or left
+// (k_bin stamps the others).  The merge is about 0.84 of the force pass's
+// 1.97 ms on the 4.2M-particle impact.  A run is one 32-bit entry, first
+// particle << 3 | (length - 1), longer runs taking several; entry r of node
+// n is runs[r NN + n].  A node with more than kRunCap entries keeps the
+// merge in both passes.
 constexpr int kRunCap = 48;
 struct RunList {
-  unsigned* runs;  // kRunCap planes of NN entries
-  int* nruns;      // per compacted node: entries recorded (> kRunCap: overflow)
+  unsigned* runs;    // kRunCap planes of NN entries, by node
+  int* nruns;        // per node: entries recorded (> kRunCap: overflow)
+  const int* dirty;  // per node: the step stamp of its last window change
+  int gen;           // this step's stamp
+  int all;           // every node merges (the first step after an upload)
 };
 template <int D, int B, int ORDER, bool DEFER, class Visit, class OnRun = NoRuns>
 __device__ __forceinline__ void for_window_particles(const Particles& P, const GridDesc& G, const int* start,
@@ -557,31 +582,32 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   constexpr bool DEFER = PASS == 1 || D == 2;
   constexpr bool RUNS = D == 3 && !MULTI && ORDER == JUMP_ORDER_REFERENCE;
   auto visit = [&](int p) { p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af); };
-  if (RUNS && PASS == 1 && rl.nruns[i] <= kRunCap) {
-    // replay the force pass's runs (i: this node's compacted position)
-    const int nr = rl.nruns[i];
+  auto replay = [&](int nr) {
     for (int r = 0; r < nr; r++) {
-      const unsigned e = rl.runs[size_t(r) * NN + i];
+      const unsigned e = rl.runs[size_t(r) * NN + n];
       const int p0 = int(e >> 3), len = int(e & 7u) + 1;
       for (int k = 0; k < len; k++) visit(p0 + k);
     }
+  };
+  if (RUNS && PASS == 1 && rl.nruns[n] <= kRunCap) {
+    replay(rl.nruns[n]);  // the runs this step's force pass left
+  } else if (RUNS && PASS == 0 && !rl.all && rl.dirty[n] != rl.gen && rl.nruns[n] <= kRunCap) {
+    replay(rl.nruns[n]);  // no particle entered or left the window since they were recorded
   } else if (RUNS && PASS == 0) {
-    // record this node's runs at its compacted position (an active node:
-    // one whose window holds a particle, exactly the nodes with runs)
+    // merge, recording the runs (none for a node whose window is empty)
     int nr = 0;
-    const int pos = ldu(nodepos, unsigned(n));
     for_window_particles<D, B, ORDER, DEFER>(P, G, start, sorted, bxyz, off, inst, W, ni, nj, nk, s_cur, s_head, visit,
                                              [&](int p0, int len) {
                                                while (len > 0) {
                                                  const int c = min(len, 8);
                                                  if (nr < kRunCap)
-                                                   rl.runs[size_t(nr) * NN + pos] = (unsigned(p0) << 3) | unsigned(c - 1);
+                                                   rl.runs[size_t(nr) * NN + n] = (unsigned(p0) << 3) | unsigned(c - 1);
                                                  nr++;
                                                  p0 += c;
                                                  len -= c;
                                                }
                                              });
-    if (nr > 0) rl.nruns[pos] = nr;
+    rl.nruns[n] = nr;
   } else {
     for_window_particles<D, B, ORDER, DEFER>(P, G, start, sorted, bxyz, off, inst, W, ni, nj, nk, s_cur, s_head,
                                              visit);
