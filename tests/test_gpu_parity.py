@@ -471,6 +471,35 @@ def test_cpdi_two_disks_from_gmsh(hip_lib):
     _close(engines[0].energies(), engines[1].energies(), rtol=1e-12, what="energies")
 
 
+@pytest.mark.parametrize("basis", [_ffi.BASIS_QUAD_BSPLINE, _ffi.BASIS_LINEAR, _ffi.BASIS_GIMP])
+def test_run_replay_equals_merging_every_step(hip_lib, basis):
+    """The 3D passes replay recorded runs of particles wherever no particle
+    entered or left a node's window since the runs were recorded.  A block
+    translating at about a tenth of a cell per step (so a large share of the
+    particles changes bin) is stepped 40 times in one context, and in a twin
+    whose bins are read before every step, which makes every step merge at
+    every node: the two agree bit for bit, grid included."""
+    _say(f"run replay vs merging, basis={basis}")
+    sc = scenes.block3d(nodes=(14, 14, 14), basis=basis, v=(15.0, -20.0, 10.0), steps=40)
+    a = Engine(sc.params)
+    sc.load(a)
+    b = Engine(sc.params)
+    sc.load(b)
+    k0 = b.bins()
+    for _ in range(sc.steps):
+        a.step(1, 0.0)
+        b.bins()
+        b.step(1, 0.0)
+    assert a.flags() == b.flags() == 0
+    moved = int(np.count_nonzero(b.bins() != k0))
+    assert moved > k0.size // 4, moved
+    for k in ("pos", "vel", "volume", "F", "strain", "stress"):
+        _same(a.get_solid(0)[k], b.get_solid(0)[k], True, k)
+    ga, gb = a.get_grid(), b.get_grid()
+    for k in ("mass", "momentum0", "momentum", "momentum2", "force"):
+        _same(ga[k], gb[k], True, f"grid {k}")
+
+
 def test_usl2d_tube_cpdi_platens(hip_lib):
     """Main_LateralCompressionTube_CPDI.jl's set-up: the reference's tube.msh
     (7898 CPDI-Q4 domains, J2 steel E=210 GPa, fy=310 MPa, k=10 MPa in mm-s
