@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE x1.0); table and commands in profiles/hbm_traffic_impact3d.md.
 # Counters cannot be read from inside a timed run, so the figure is carried
 # here and must be re-measured when a kernel changes.
-MEASURED_TRAFFIC = {"p2g": 5.338e9, "p2g2": 3.600e9, "g2p1": 0.671e9, "g2p2": 4.696e9}
+MEASURED_TRAFFIC = {"p2g": 4.580e9, "p2g2": 2.231e9, "g2p1": 0.672e9, "g2p2": 4.698e9}
 
 
 def algorithmic_bytes(phase, n_plate, n_sphere, n_active):
