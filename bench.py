@@ -33,10 +33,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each phase's kernel on this workload (reference-order
 # P2G), measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE and
 # scaled by what the same counters read for a 1 GiB device copy (FETCH x2.0,
-# WRITE x1.0); table and commands in profiles/hbm_traffic_impact3d.md.
+# WRITE x1.0), mean of the launches after the first (the first step merges
+# at every node; the timed steps replay); table and commands in
+# profiles/hbm_traffic_impact3d.md.
 # Counters cannot be read from inside a timed run, so the figure is carried
 # here and must be re-measured when a kernel changes.
-MEASURED_TRAFFIC = {"p2g": 4.580e9, "p2g2": 2.231e9, "g2p1": 0.672e9, "g2p2": 4.698e9}
+MEASURED_TRAFFIC = {"p2g": 5.437e9, "p2g2": 2.134e9, "g2p1": 0.667e9, "g2p2": 4.697e9}
 
 
 def algorithmic_bytes(phase, n_plate, n_sphere, n_active):

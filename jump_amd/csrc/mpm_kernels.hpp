@@ -426,8 +426,11 @@ struct NoRuns {
 // 1.97 ms on the 4.2M-particle impact.  A run is one 32-bit entry, first
 // particle << 3 | (length - 1), longer runs taking several; entry r of node
 // n is runs[r NN + n].  A node with more than kRunCap entries keeps the
-// merge in both passes.
-constexpr int kRunCap = 48;
+// merge in both passes.  On the impact the plate's interior nodes hold 36
+// entries (6 x 6 rows of 6 particles); nodes where the sphere has entered
+// the crater reached 57 by step 400.  At 48 such nodes merged again in the
+// re-projection and lengthened its tail (0.70 -> 0.93 ms after step 160).
+constexpr int kRunCap = 64;
 struct RunList {
   unsigned* runs;    // kRunCap planes of NN entries, by node
   int* nruns;        // per node: entries recorded (> kRunCap: overflow)
@@ -549,8 +552,15 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   // the force pass keeps the hardware order, it was slower with any run
   // length tried, 2.12 -> 2.19-3.47 ms, as with 8x4x4 node tiles, 2.9-3.8)
   constexpr bool XCD = D == 3 && !MULTI && COMPACT;
-  const int L = XCD ? xcd_block((NN + kP2GBlock - 1) / kP2GBlock) : int(blockIdx.x);
-  if (L < 0) return;
+  // 3D: blocks in descending node order.  The costliest nodes (the crater,
+  // where windows change and merge again, and the sphere's mixed windows)
+  // sit at the top of the plate, the highest node ids; started last they
+  // set the kernel's tail.  Both grids have exactly nlog (logical) blocks.
+  constexpr bool REV = D == 3 && !MULTI;
+  const int nlog = (NN + kP2GBlock - 1) / kP2GBlock;
+  const int L0 = XCD ? xcd_block(nlog) : int(blockIdx.x);
+  if (L0 < 0) return;
+  const int L = REV ? nlog - 1 - L0 : L0;
   const int i = L * kP2GBlock + t;
   if (i >= NN) return;
   const int ng = COMPACT ? ldu(order, unsigned(i)) : i;  // node in the (concatenated) node arrays
