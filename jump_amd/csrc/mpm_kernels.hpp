@@ -564,7 +564,15 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   const int i = L * kP2GBlock + t;
   if (i >= NN) return;
   const int ng = COMPACT ? ldu(order, unsigned(i)) : i;  // node in the (concatenated) node arrays
+  constexpr bool RUNS = D == 3 && !MULTI && ORDER == JUMP_ORDER_REFERENCE;
+  // A node whose window held no particle when its runs were recorded, and
+  // which no particle has entered since, still holds the zeros that both
+  // passes wrote in the step that emptied it (or in the first step after an
+  // upload, when every node merges and writes): skip the writes.  3.6M of
+  // the impact's 4.2M nodes, 0.46 GB of writes per force pass.
+  const bool still_empty = RUNS && !rl.all && rl.nruns[ng] == 0 && rl.dirty[ng] != rl.gen;
   if (COMPACT && i >= ldu(nodepos, unsigned(NN))) {  // inactive node: no particle reaches it
+    if (PASS == 1 && still_empty) return;
     if (PASS == 0) {
       g0.m[ng] = 0.0;
       for (int k = 0; k < D; k++) { g0.p0[k][ng] = 0.0; g0.p[k][ng] = 0.0; g0.f[k][ng] = 0.0; }
@@ -590,7 +598,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   double am = 0.0, ap[D], af[D];
   for (int k = 0; k < D; k++) { ap[k] = 0.0; af[k] = 0.0; }
   constexpr bool DEFER = PASS == 1 || D == 2;
-  constexpr bool RUNS = D == 3 && !MULTI && ORDER == JUMP_ORDER_REFERENCE;
+  if (PASS == 0 && still_empty) return;
   auto visit = [&](int p) { p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af); };
   auto replay = [&](int nr) {
     for (int r = 0; r < nr; r++) {
