@@ -550,7 +550,8 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   // XCD: the 3D re-projection pass deals its (compacted) blocks to the XCDs
   // in runs of kXcdChunk (1.115 -> 1.066 ms on the 4.2M-particle impact;
   // the force pass keeps the hardware order, it was slower with any run
-  // length tried, 2.12 -> 2.19-3.47 ms, as with 8x4x4 node tiles, 2.9-3.8)
+  // length tried, 2.12 -> 2.19-3.47 ms, as with 8x4x4 node tiles, 2.9-3.8,
+  // and again once it replayed runs, 1.47 -> 1.92 ms with runs of 16)
   constexpr bool XCD = D == 3 && !MULTI && COMPACT;
   // 3D: blocks in descending node order.  The costliest nodes (the crater,
   // where windows change and merge again, and the sphere's mixed windows)
