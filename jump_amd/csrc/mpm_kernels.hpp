@@ -294,13 +294,11 @@ template <int D, int PASS> struct P2GFields {
 // One particle's contribution to the node accumulators, in the reference's
 // operation order (SolveExplicitSolidMechanics2D.jl:411-421;
 // SolveExplicitSolidMechanics3D.jl:69-82 and :135-138).
+// v: the particle's fields as P2GFields loads them.
 template <int D, int B, int PASS>
-__device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, const SolidDev* S,
-                                        const double* corners, int p, int ni, int nj, int nk, double& am,
-                                        double* ap, double* af) {
-  constexpr int NF = P2GFields<D, PASS>::NF;
-  double v[NF + 1];
-  P2GFields<D, PASS>::load(P, unsigned(p), v);
+__device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G, const SolidDev* S,
+                                          const double* corners, int p, const double* v, int ni, int nj, int nk,
+                                          double& am, double* ap, double* af) {
   double lp[3] = {0.0, 0.0, 0.0};
   if (B == JUMP_BASIS_GIMP) {
     const SolidDev& s = S[P.sid[p]];
@@ -330,6 +328,15 @@ __device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, c
     const double Nm = N * fM;
     for (int k = 0; k < D; k++) ap[k] += Nm * v[D + 2 + k];
   }
+}
+
+template <int D, int B, int PASS>
+__device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, const SolidDev* S,
+                                        const double* corners, int p, int ni, int nj, int nk, double& am,
+                                        double* ap, double* af) {
+  double v[P2GFields<D, PASS>::NF + 1];
+  P2GFields<D, PASS>::load(P, unsigned(p), v);
+  p2g_add_v<D, B, PASS>(P, G, S, corners, p, v, ni, nj, nk, am, ap, af);
 }
 
 // PASS 0: mass, momentum0, force, then p = p0 + f dt and the Dirichlet mask
@@ -601,11 +608,33 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   constexpr bool DEFER = PASS == 1 || D == 2;
   if (PASS == 0 && still_empty) return;
   auto visit = [&](int p) { p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af); };
+  // Replay.  The re-projection loads the next particle's fields (and the
+  // next run's entry) while the current particle is summed; the sums keep
+  // their order (0.683 -> 0.651 ms on the impact).  The force pass keeps
+  // the plain loop: with the prefetch it went 1.46 -> 1.53 ms.
   auto replay = [&](int nr) {
+    if constexpr (PASS == 0) {
+      for (int r = 0; r < nr; r++) {
+        const unsigned e = rl.runs[size_t(r) * NN + n];
+        const int p0 = int(e >> 3), len = int(e & 7u) + 1;
+        for (int k = 0; k < len; k++) visit(p0 + k);
+      }
+      return;
+    }
+    constexpr int NF = P2GFields<D, PASS>::NF;
+    unsigned e = nr > 0 ? rl.runs[n] : 0u;
     for (int r = 0; r < nr; r++) {
-      const unsigned e = rl.runs[size_t(r) * NN + n];
       const int p0 = int(e >> 3), len = int(e & 7u) + 1;
-      for (int k = 0; k < len; k++) visit(p0 + k);
+      if (r + 1 < nr) e = rl.runs[size_t(r + 1) * NN + n];
+      double v[NF + 1], w[NF + 1];
+      P2GFields<D, PASS>::load(P, unsigned(p0), v);
+      for (int k = 0; k < len; k++) {
+        if (k + 1 < len) P2GFields<D, PASS>::load(P, unsigned(p0 + k + 1), w);
+        p2g_add_v<D, B, PASS>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+        if (k + 1 < len)
+#pragma unroll
+          for (int q = 0; q < NF + 1; q++) v[q] = w[q];
+      }
     }
   };
   if (RUNS && PASS == 1 && rl.nruns[n] <= kRunCap) {
