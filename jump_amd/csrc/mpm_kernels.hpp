@@ -53,6 +53,24 @@ __global__ void k_fill_i32(int* __restrict__ a, int v, int n) {
 // for particles that take no part in P2G (2D rigid bodies, out of grid).
 // bxyz[i] packs the origin's 0-based coordinates, 21 bits per axis (all
 // ones for excluded particles).
+// A slot in bin kb for each active lane: consecutive lanes with the same bin
+// (neighbouring particles usually share one) take theirs with one atomic,
+// the first lane of the run adding the run's length.  Active lanes of a
+// wave are contiguous from lane 0 (the grid is cut at the particle count).
+__device__ __forceinline__ int wave_slot(int* cnt, int kb) {
+  const int lane = int(__lane_id());
+  const int kprev = __shfl_up(kb, 1u);
+  const unsigned long long heads = __ballot(lane == 0 || kprev != kb);
+  const unsigned long long brk = heads | ~__ballot(1);  // a run ends at the next head or inactive lane
+  const int leader = 63 - __clzll((long long)(heads & ((2ull << lane) - 1ull)));
+  const unsigned long long after = brk & ~((2ull << leader) - 1ull);
+  const int end = after ? __ffsll((long long)after) - 1 : 64;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(&cnt[kb], end - leader);
+  base = __shfl(base, leader);
+  return base + (lane - leader);
+}
+
 template <int D, int B, bool MULTI>
 __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
                                                 const double* __restrict__ corners, int* __restrict__ key,
@@ -79,9 +97,10 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
   const int off = MULTI ? Bt.node_off[s.inst] : 0;
   const int NN = MULTI ? Bt.NN : G0.nn;
   int k = -1;
+  bool ok = false;
+  int f[3] = {1, 1, 1};
   if (!(D == 2 && s.rigid)) {
-    bool ok = true;
-    int f[3] = {1, 1, 1};
+    ok = true;
     if (B == JUMP_BASIS_CPDI_Q4) {
       CpdiW c;
       ok = cpdi_setup(corners, P.elem[0][i], P.elem[1][i], P.elem[2][i], P.elem[3][i], G, c);
@@ -92,29 +111,29 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
     } else {
       for (int a = 0; a < D; a++) ok = ok && first_node<B, D>(P.x[a][i], a, axis_of(G, a), s.lp[a], f[a]);
     }
-    if (ok) {
+    if (ok)
       k = off + (f[0] - 1) + G.nx * ((f[1] - 1) + (D == 3 ? G.ny * (f[2] - 1) : 0));
-      rank[i] = atomicAdd(&cnt[k], 1);  // its slot in the bin (any order; k_binsort orders it)
-    } else {
+    else
       atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+  }
+  // its slot in the bin, any order (k_binsort orders each bin); unbinned
+  // particles count in the bin past the last node
+  rank[i] = wave_slot(cnt, ok ? k : NN);
+  if (ok) {
+    if (k != old) {
+      mark(old);
+      mark(k);
     }
-    if (ok) {
-      if (k != old) {
-        mark(old);
-        mark(k);
-      }
-      key[i] = k;
-      // bin coordinates 21 bits per axis; a 2D batch keeps the instance in
-      // the third field, so a window of one instance never admits another's
-      bxyz[i] = (unsigned long long)(f[0] - 1) | ((unsigned long long)(f[1] - 1) << 21) |
-                ((unsigned long long)(D == 3 ? f[2] - 1 : (MULTI ? s.inst : 0)) << 42);
-      return;
-    }
+    key[i] = k;
+    // bin coordinates 21 bits per axis; a 2D batch keeps the instance in
+    // the third field, so a window of one instance never admits another's
+    bxyz[i] = (unsigned long long)(f[0] - 1) | ((unsigned long long)(f[1] - 1) << 21) |
+              ((unsigned long long)(D == 3 ? f[2] - 1 : (MULTI ? s.inst : 0)) << 42);
+    return;
   }
   if (old >= 0) mark(old);
   key[i] = -1;
   bxyz[i] = ~0ull;
-  rank[i] = atomicAdd(&cnt[NN], 1);
 }
 
 // CPDI stencil extent (static corners): max bounding-box width in nodes.
