@@ -234,11 +234,37 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int* __restrict__ key,
 
 // Order each bin segment by particle index so the P2G sum order is fixed
 // (the bin half of every entry in a segment is equal).
+constexpr int kSortReg = 16;  // bins up to this size sort in registers
 __global__ void __launch_bounds__(kBlock) k_binsort(const int* __restrict__ start,
                                                     unsigned long long* __restrict__ sorted, int nbins) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbins) return;
   int s = start[b], e = start[b + 1];
+  if (e - s <= 1) return;
+  if (e - s <= kSortReg) {
+    // in registers: odd-even transposition sort, padded with the largest key
+    // (about 8 particles per bin at 2 per cell and axis)
+    const int n = e - s;
+    unsigned long long r[kSortReg];
+#pragma unroll
+    for (int q = 0; q < kSortReg; q++) {
+      r[q] = ~0ull;
+      if (q < n) r[q] = sorted[s + q];
+    }
+#pragma unroll
+    for (int round = 0; round < kSortReg; round++)
+#pragma unroll
+      for (int q = round & 1; q + 1 < kSortReg; q += 2) {
+        const unsigned long long lo = r[q] < r[q + 1] ? r[q] : r[q + 1];
+        const unsigned long long hi = r[q] < r[q + 1] ? r[q + 1] : r[q];
+        r[q] = lo;
+        r[q + 1] = hi;
+      }
+#pragma unroll
+    for (int q = 0; q < kSortReg; q++)
+      if (q < n) sorted[s + q] = r[q];
+    return;
+  }
   for (int i = s + 1; i < e; i++) {
     unsigned long long v = sorted[i];
     int j = i - 1;
