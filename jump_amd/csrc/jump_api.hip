@@ -1753,9 +1753,8 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
     if (!r) r = alloc(&ap.lam, n);
     if (!r) r = alloc(&ap.mu, n);
     AdjSources& src = A.src;
-    for (int k = 0; k < 2 && !r; k++) r = alloc(&src.xb[k], n);
-    for (int k = 0; k < 2 && !r; k++) r = alloc(&src.vb[k], n);
-    for (int k = 0; k < 4 && !r; k++) r = alloc(&src.Lb[k], n);
+    for (double2** v : {&src.xb, &src.vb, &src.Lb[0], &src.Lb[1]})
+      if (!r) r = dev_alloc(ctx, A.adj_allocs, v, n);
     if (!r) r = alloc(&an.mb, nn);
     for (int k = 0; k < 2 && !r; k++) r = alloc(&an.Pb[k], nn);
     for (int k = 0; k < 2 && !r; k++) r = alloc(&an.P0b[k], nn);

@@ -45,10 +45,11 @@ struct AdjNodes {
 };
 // Per-particle sources of the G2P node adjoints: the adjoints of x' and v'
 // and of L, written by k_adj_g2p and gathered per node by k_adj_nodes.
+// In pairs, one 16-byte load each where k_adj_nodes gathers them.
 struct AdjSources {
-  double* xb[2];
-  double* vb[2];
-  double* Lb[4];
+  double2* xb;     // (x, y)
+  double2* vb;     // (x, y)
+  double2* Lb[2];  // (Lb0, Lb1), (Lb2, Lb3)
 };
 
 // Weight, first and second derivative of node id (1-based) on one axis.
@@ -114,8 +115,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
     }
   });
   if (!ok) {  // the forward left this particle unchanged: identity adjoint, no node sources
-    for (int k = 0; k < 2; k++) { src.xb[k][i] = 0.0; src.vb[k][i] = 0.0; }
-    for (int k = 0; k < 4; k++) src.Lb[k][i] = 0.0;
+    src.xb[i] = make_double2(0.0, 0.0);
+    src.vb[i] = make_double2(0.0, 0.0);
+    src.Lb[0][i] = make_double2(0.0, 0.0);
+    src.Lb[1][i] = make_double2(0.0, 0.0);
     return;
   }
   const double e[3] = {P.eps[0][i] + dt * L0, P.eps[1][i] + dt * L3, P.eps[2][i] + dt * (0.5 * (L2 + L1))};
@@ -176,14 +179,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
   const double Lb2 = dt * Ab[2] + 0.5 * dt * eb[2];
   const double xb1[2] = {ad.xb[0][i], ad.xb[1][i]};
   const double vb1[2] = {ad.vb[0][i], ad.vb[1][i]};
-  src.xb[0][i] = xb1[0];
-  src.xb[1][i] = xb1[1];
-  src.vb[0][i] = vb1[0];
-  src.vb[1][i] = vb1[1];
-  src.Lb[0][i] = Lb0;
-  src.Lb[1][i] = Lb1;
-  src.Lb[2][i] = Lb2;
-  src.Lb[3][i] = Lb3;
+  src.xb[i] = make_double2(xb1[0], xb1[1]);
+  src.vb[i] = make_double2(vb1[0], vb1[1]);
+  src.Lb[0][i] = make_double2(Lb0, Lb1);
+  src.Lb[1][i] = make_double2(Lb2, Lb3);
   double xb[2] = {xb1[0], xb1[1]};
   walk_stencil_h<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
     const double mI = g.m[n];
@@ -234,15 +233,30 @@ __global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, c
       for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
         const int bin = bi + G.nx * bj;
         const int e = start[bin + 1];
-        for (int c = start[bin]; c < e; c++) {
-          const int q = int(unsigned(sorted[c]));
-          const SolidDev& s = S[P.sid[q]];
+        int c = start[bin];
+        if (c >= e) continue;
+        // the next particle's index, solid and position are loaded while
+        // the current one is summed (the order of the sums is unchanged)
+        int q = int(unsigned(sorted[c]));
+        int sq = P.sid[q];
+        double xq = P.x[0][q], yq = P.x[1][q];
+        for (; c < e; c++) {
+          int qn = q, sn = sq;
+          double xn = xq, yn = yq;
+          if (c + 1 < e) {
+            qn = int(unsigned(sorted[c + 1]));
+            sn = P.sid[qn];
+            xn = P.x[0][qn];
+            yn = P.x[1][qn];
+          }
+          const SolidDev& s = S[sq];
           double wx, d1x, d2x, wy, d1y, d2y;
-          axis_w2<B>(P.x[0][q], X, s.lp[0], ni + 1, wx, d1x, d2x);
-          axis_w2<B>(P.x[1][q], Y, s.lp[1], nj + 1, wy, d1y, d2y);
+          axis_w2<B>(xq, X, s.lp[0], ni + 1, wx, d1x, d2x);
+          axis_w2<B>(yq, Y, s.lp[1], nj + 1, wy, d1y, d2y);
           const double N = wx * wy, gx = d1x * wy, gy = wx * d1y;
-          const double xb[2] = {src.xb[0][q], src.xb[1][q]}, vb[2] = {src.vb[0][q], src.vb[1][q]};
-          const double Lb[4] = {src.Lb[0][q], src.Lb[1][q], src.Lb[2][q], src.Lb[3][q]};
+          const double2 xb2 = src.xb[q], vb2 = src.vb[q], L01 = src.Lb[0][q], L23 = src.Lb[1][q];
+          const double xb[2] = {xb2.x, xb2.y}, vb[2] = {vb2.x, vb2.y};
+          const double Lb[4] = {L01.x, L01.y, L23.x, L23.y};
           const double ub0 = N * dt * xb[0] + Lb[0] * gx + Lb[2] * gy;
           const double ub1 = N * dt * xb[1] + Lb[1] * gx + Lb[3] * gy;
           const double ab0 = N * vb[0], ab1 = N * vb[1];
@@ -251,6 +265,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, c
           P0b0 -= ab0 * iM;
           P0b1 -= ab1 * iM;
           mb -= (u[0] * ub0 + u[1] * ub1 + a[0] * ab0 + a[1] * ab1) * iM;
+          q = qn;
+          sq = sn;
+          xq = xn;
+          yq = yn;
         }
       }
   }
