@@ -174,7 +174,9 @@ typedef struct jump_ctx jump_ctx;
  * SolveExplicitSolidMechanics3D.jl:14-39) and Grid2D/Grid3D allocation
  * (Grid.jl:104-158, 198-267; the 2D dense mass matrix M is not allocated:
  * USL never reads it and 2D MUSL keeps its nonzero band only).  Selects
- * device `device` (HIP ordinal). */
+ * device `device` (HIP ordinal).  A grid (or, in jump_batch_create, the
+ * grids of a batch together) has at most 2^30 - 2 nodes and at most 65535
+ * per axis; larger ones are JUMP_ERR_UNSUPPORTED. */
 int jump_ctx_create(const jump_params* params, int device, jump_ctx** out);
 int jump_ctx_destroy(jump_ctx* ctx);
 const char* jump_last_error(const jump_ctx* ctx);
@@ -237,7 +239,7 @@ int jump_energies(jump_ctx* ctx, double* ke, double* se);
  * memory on the context's stream, ahead of any later step, and a worker
  * thread formats and writes the text while the device goes on stepping.
  * jump_dump_wait blocks until every queued dump is written and returns
- * JUMP_ERR_IO if one failed. */
+ * JUMP_ERR_IO if any dump since the previous jump_dump_wait failed. */
 int jump_dump_lammps(jump_ctx* ctx, const char* path, const char* outs);
 int jump_dump_wait(jump_ctx* ctx);
 
@@ -283,7 +285,11 @@ const char* jump_batch_last_error(const jump_batch* batch);
  * (the derivative with respect to a uniform change of that solid's initial
  * velocity, 2 per solid); outputs may be NULL.  Each step of the sweep
  * recomputes its states from the nearest snapshot (checkpoint and
- * recompute); the context is left in the recorded final state.
+ * recompute); the context is left in the recorded final state.  The record
+ * belongs to the particle state the run ended in: jump_step (nsteps > 0),
+ * jump_checkpoint_load, jump_add_solid or jump_batch_sync on the context
+ * discard it, and jump_adjoint_backward then returns JUMP_ERR_STATE instead
+ * of rewinding the particles. 
  * checkpoint_every = 0 picks the interval: the whole run when its per-step
  * states, grids and bins fit in three quarters of the free device memory
  * (no recomputation), else the largest interval that fits.

@@ -28,8 +28,16 @@ def rrule(engine, nsteps, checkpoint_every=0, t=0.0):
 
     def pullback(pos_bar, vel_bar=None):
         ns = len(engine.counts)
-        pb = np.ascontiguousarray(pos_bar, dtype=np.float64).reshape(-1, 2)
-        vb = None if vel_bar is None else np.ascontiguousarray(vel_bar, dtype=np.float64).reshape(-1, 2)
+        npart = int(sum(engine.counts))
+        # the library reads 2 doubles per particle from each array
+        pb = np.ascontiguousarray(pos_bar, dtype=np.float64)
+        if pb.shape != (npart, 2):
+            raise ValueError(f"pos_bar must have shape ({npart}, 2), got {pb.shape}")
+        vb = None
+        if vel_bar is not None:
+            vb = np.ascontiguousarray(vel_bar, dtype=np.float64)
+            if vb.shape != (npart, 2):
+                raise ValueError(f"vel_bar must have shape ({npart}, 2), got {vb.shape}")
         dE, dnu, dv0 = np.zeros(ns), np.zeros(ns), np.zeros((ns, 2))
         rc2 = lib.jump_adjoint_backward(engine.h, _ffi.dptr(pb), _ffi.dptr(vb), _ffi.dptr(dE), _ffi.dptr(dnu),
                                         _ffi.dptr(dv0))

@@ -126,6 +126,7 @@ struct jump_ctx {
   };
   std::vector<std::unique_ptr<DumpBuf>> dump_bufs;
   DumpWriter dump_writer;
+  bool dump_failed = false;  // a dump failed before jump_dump_wait was called
 
   // per-phase timing with HIP events on the context stream (jump_profile)
   bool prof = false;
@@ -763,10 +764,10 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   if (p->dim == 2 && p->algorithm == JUMP_ALG_MUSL && p->contact) return JUMP_ERR_UNSUPPORTED;
   if (p->nx < 2 || p->ny < 2 || (p->dim == 3 && p->nz < 2)) return JUMP_ERR_ARG;
   const int64_t nz = p->dim == 3 ? p->nz : 1;
-  if (p->nx * p->ny * nz >= (int64_t(1) << 31) - 2) return JUMP_ERR_UNSUPPORTED;
+  if (p->nx >= 65536 || p->ny >= 65536 || nz >= 65536) return JUMP_ERR_UNSUPPORTED;
+  if (p->nx * p->ny * nz > kMaxNodes) return JUMP_ERR_UNSUPPORTED;
   if (!(p->dt > 0.0)) return JUMP_ERR_ARG;
   if (p->order != JUMP_ORDER_REFERENCE && p->order != JUMP_ORDER_BINNED) return JUMP_ERR_ARG;
-  if (p->nx >= 65536 || p->ny >= 65536 || nz >= 65536) return JUMP_ERR_UNSUPPORTED;
   if (p->contact != 0 && p->contact != 1) return JUMP_ERR_ARG;
   if (p->contact && !(p->friction >= 0.0 && std::isfinite(p->friction))) return JUMP_ERR_ARG;
   jump_ctx* ctx = new (std::nothrow) jump_ctx();
@@ -945,6 +946,8 @@ int jump_step(jump_ctx* ctx, int64_t nsteps, double* t_inout) {
   HIP_TRY(hipSetDevice(ctx->device));
   int r = ensure_device(ctx);
   if (r) return r;
+  // the particles move on from a recorded run: its pullback would rewind them
+  if (nsteps > 0) ctx->adj.ready = false;
   if (ctx->np == 0) {
     for (int64_t s = 0; s < nsteps; s++) *t_inout += ctx->prm.dt;
     return JUMP_OK;
@@ -1052,6 +1055,7 @@ int jump_checkpoint_load(jump_ctx* ctx, const void* buf, int64_t bytes) {
   ctx->on_device = false;  // the next call that needs the device uploads the restored state
   ctx->grid_valid = false;
   ctx->flags = 0;
+  ctx->adj.ready = false;  // a recorded run no longer matches the particles
   return JUMP_OK;
 }
 
@@ -1219,7 +1223,8 @@ int jump_dump_lammps(jump_ctx* ctx, const char* path, const char* outs) {
     buf = ctx->dump_bufs.back().get();
   }
   if (!buf) {
-    ctx->dump_writer.wait();
+    // wait() reports (and clears) a failed earlier dump: keep it for jump_dump_wait
+    if (!ctx->dump_writer.wait()) ctx->dump_failed = true;
     buf = ctx->dump_bufs[0].get();
   }
   if (buf->bytes < bytes) {
@@ -1268,7 +1273,9 @@ int jump_dump_lammps(jump_ctx* ctx, const char* path, const char* outs) {
 
 int jump_dump_wait(jump_ctx* ctx) {
   if (!ctx) return JUMP_ERR_ARG;
-  if (!ctx->dump_writer.wait()) return set_err(ctx, JUMP_ERR_IO, "a LAMMPS dump could not be written");
+  const bool ok = ctx->dump_writer.wait() && !ctx->dump_failed;
+  ctx->dump_failed = false;
+  if (!ok) return set_err(ctx, JUMP_ERR_IO, "a LAMMPS dump could not be written");
   return JUMP_OK;
 }
 
@@ -1407,7 +1414,7 @@ int jump_batch_create(jump_ctx* const* ctxs, int32_t n, int device, jump_batch**
       const jump_ctx* c = ctxs[m];
       in->hinst.push_back(InstDev{c->G, c->prm.dt, c->prm.tolerance});
       nn += c->G.nn;
-      if (nn >= (int64_t(1) << 31) - 2) {
+      if (nn > kMaxNodes) {
         jump_batch_destroy(b);
         return JUMP_ERR_UNSUPPORTED;
       }
@@ -1522,6 +1529,7 @@ int jump_batch_sync(jump_batch* b) {
       put(c->hvm, in->hvm);
       c->on_device = false;  // the member's own device copy (if any) is stale now
       c->grid_valid = false;
+      c->adj.ready = false;
     }
   }
   return JUMP_OK;

@@ -167,6 +167,10 @@ __device__ __forceinline__ GridArrays grid_view(GridArrays g, int off) {
 // < 2^32 (the library caps a context at kMaxParticles particles, so that
 // also 16-byte elements stay in range).
 constexpr long long kMaxParticles = 1ll << 28;
+// Node-indexed int arrays (bin starts, work order, active-node scan) hold
+// NN + 1 entries read through ldu, so (NN + 1) * 4 < 2^32: a context or batch
+// has at most kMaxNodes nodes.
+constexpr long long kMaxNodes = (1ll << 30) - 2;
 template <class T> __device__ __forceinline__ T ldu(const T* __restrict__ base, unsigned i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * unsigned(sizeof(T)));
 }

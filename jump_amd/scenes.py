@@ -441,7 +441,10 @@ def block3d(nodes=(12, 12, 12), basis=_ffi.BASIS_QUAD_BSPLINE, material="elastic
     pts = buildParticleForBlock(ctr, 0.45, 0.45, 0.45, f)
     rho, E, nu = 1000.0, 1.0e6, 0.3
     mat = {"elastic": elastic(E, nu, rho), "neohookean": neohookean(E, nu, rho),
-           "j2": j2(E, nu, rho, 2.0e3, 1.0e4), "jc": johnson_cook(E, nu, rho, 2.0e3, 1.0e4, 0.0, 1.0, 1.0, dx)}[material]
+           "j2": j2(E, nu, rho, 2.0e3, 1.0e4), "jc": johnson_cook(E, nu, rho, 2.0e3, 1.0e4, 0.0, 1.0, 1.0, dx),
+           # rate-dependent Johnson-Cook: C != 0 and a small reference rate, so
+           # the (1 + rate ratio)^C factor and the alphadot filter take part
+           "jc_rate": johnson_cook(E, nu, rho, 2.0e3, 1.0e4, 0.1, 0.5, 1.0e-5, dx)}[material]
     lp = 0.5 * f if basis == _ffi.BASIS_GIMP else None
     s = _solid(mat, pts, v, f ** 3, rho, lp=lp)
     s["vel"] = s["vel"] + rng.normal(scale=0.5, size=s["vel"].shape)

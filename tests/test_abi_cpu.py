@@ -95,3 +95,67 @@ def test_energy_record_format():
     assert julia_float(1e-5) == "1.0e-5"
     assert julia_float(2.6) == "2.6"
     assert julia_float(0.0) == "0.0"
+
+
+def test_grid_node_cap_is_refused_before_any_device_call():
+    """Node-indexed int arrays are read with 32-bit byte offsets, so a grid
+    of 2^30 - 1 nodes or more is refused (JUMP_ERR_UNSUPPORTED) by argument
+    checks that run before the library touches a device."""
+    from jump_amd.engine import make_params
+    lib = _ffi.load()
+    out = ctypes.c_void_p()
+    for nodes in ((32768, 32768), (1024, 1024, 1024), (65535, 16385)):
+        d = len(nodes)
+        p = make_params(d, _ffi.BASIS_LINEAR if d == 2 else _ffi.BASIS_QUAD_BSPLINE,
+                        _ffi.ALG_USL if d == 2 else _ffi.ALG_MUSL, 1e-3, (0.0,) * d, (1.0,) * d, nodes)
+        assert lib.jump_ctx_create(ctypes.byref(p), 0, ctypes.byref(out)) == _ffi.JUMP_ERR_UNSUPPORTED, nodes
+        assert not out.value
+    p = make_params(2, _ffi.BASIS_LINEAR, _ffi.ALG_USL, 1e-3, (0.0, 0.0), (1.0, 1.0), (65536, 4))
+    assert lib.jump_ctx_create(ctypes.byref(p), 0, ctypes.byref(out)) == _ffi.JUMP_ERR_UNSUPPORTED
+
+
+class _StubLib:
+    """Stands in for libjumphip.so: records the pullback call."""
+
+    def __init__(self):
+        self.backward_calls = 0
+
+    def jump_adjoint_forward(self, h, n, k, t):
+        return 0
+
+    def jump_adjoint_interval(self, h, k):
+        return 0
+
+    def jump_adjoint_backward(self, h, pb, vb, dE, dnu, dv0):
+        self.backward_calls += 1
+        return 0
+
+
+class _StubEngine:
+    def __init__(self, counts):
+        self.lib = _StubLib()
+        self.h = None
+        self.counts = list(counts)
+
+    def get_solid(self, s):
+        return {"pos": np.zeros((self.counts[s], 2)), "vel": np.zeros((self.counts[s], 2))}
+
+    def _check(self, rc):
+        assert rc == 0
+
+
+def test_pullback_checks_adjoint_shapes():
+    """The library reads 2 doubles per particle from pos_bar and vel_bar, so
+    the pullback refuses arrays of any other shape before calling it."""
+    from jump_amd.adjoint import rrule
+    eng = _StubEngine([3, 2])
+    _, pullback = rrule(eng, 4)
+    for bad in (np.zeros((4, 2)), np.zeros(10), np.zeros((5, 3)), np.zeros((2, 5))):
+        with pytest.raises(ValueError):
+            pullback(bad)
+        with pytest.raises(ValueError):
+            pullback(np.zeros((5, 2)), bad)
+    assert eng.lib.backward_calls == 0
+    pullback(np.zeros((5, 2)), np.ones((5, 2)))
+    pullback(np.zeros((5, 2)))
+    assert eng.lib.backward_calls == 2

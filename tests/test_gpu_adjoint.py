@@ -74,3 +74,23 @@ def test_checkpoint_interval_does_not_change_the_gradient(hip_lib):
         for k in ("E", "nu", "v0"):
             print(k, r[k], res[0][k])
             assert np.array_equal(r[k], res[0][k]), k  # every adjoint sum has a fixed order
+
+
+def test_stepping_after_the_recorded_run_discards_it(hip_lib):
+    """The pullback belongs to the state the recorded run ended in: after a
+    further jump_step it is refused (JUMP_ERR_STATE) and the particles keep
+    the stepped state instead of being rewound."""
+    from jump_amd.engine import JumpError
+    sc = _scene(_ffi.BASIS_LINEAR, "elastic")
+    e = Engine(sc.params)
+    sc.load(e)
+    (pos, vel, t, rc), pullback = rrule(e, 5, checkpoint_every=2)
+    assert rc == 0
+    e.step(3, t)
+    moved = e.get_solid(0)["pos"]
+    assert not np.array_equal(moved, pos)
+    with pytest.raises(JumpError) as ei:
+        pullback(np.ones_like(pos))
+    assert ei.value.code == _ffi.JUMP_ERR_STATE
+    assert np.array_equal(e.get_solid(0)["pos"], moved)
+    e.close()

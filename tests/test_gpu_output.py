@@ -112,3 +112,24 @@ def test_problem_two_disks_with_output(hip_lib, tmp_path):
         want = _expected(e, sc, ["vx", "vy"], (1.0, 1.0, 0.0))
         assert open(os.path.join(str(tmp_path), f"dump_p.{c}.LAMMPS")).read() == want, c
     e.close()
+
+
+def test_dump_failure_is_reported_after_buffer_reuse(hip_lib, tmp_path):
+    """Three dumps to an unwritable path in a row: the third waits for a
+    free buffer set, which consumes the writer's failure flag; the failure
+    must still reach jump_dump_wait."""
+    print("dump failure sticky", flush=True)
+    sc = scenes.rings2d(cells=20, basis=_ffi.BASIS_LINEAR, material="elastic", steps=1)
+    a = Engine(sc.params)
+    sc.load(a)
+    bad = str(tmp_path / "no_such_dir" / "x.LAMMPS")
+    for _ in range(3):
+        a.dump_lammps(bad, ["vx"])
+    with pytest.raises(JumpError) as ei:
+        a.dump_wait()
+    assert ei.value.code == _ffi.JUMP_ERR_IO
+    good = str(tmp_path / "ok.LAMMPS")
+    a.dump_lammps(good, ["vx"])
+    a.dump_wait()  # the failure was reported once; a good dump succeeds again
+    assert os.path.getsize(good) > 0
+    a.close()
