@@ -299,6 +299,22 @@ int jump_adjoint_interval(jump_ctx* ctx, int32_t* checkpoint_every);
 int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* vel_bar, double* dE,
                           double* dnu, double* dvel0);
 
+/* Kinetic and strain energy of particles held in the caller's device memory,
+ * allocating nothing.  No counterpart in the reference: this is the reduction
+ * of jump_energies (Fix.jl:248-270, the same sums in the same order, so the
+ * same bits as jump_energies on the same particles) for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  All pointers are device pointers on the
+ * current device: mass, volume (n), vel (dim*n), strain, stress (dim*dim*n,
+ * column-major, symmetric), in the AoS layout of jump_add_solid; ke_se
+ * receives KE then SE.  The work is queued on `stream` (a hipStream_t; NULL:
+ * the null stream) and is complete when the stream reaches it.  Partial sums
+ * live in a fixed device array of the library (1 MiB per device, held until
+ * the process ends): n is at most 16,777,216 (else JUMP_ERR_UNSUPPORTED),
+ * and calls that overlap on one device race on that array. */
+int jump_energies_noalloc(int32_t dim, int64_t n, const double* mass, const double* vel, const double* volume,
+                          const double* strain, const double* stress, double* ke_se, void* stream);
+
 /* Diagnostics (no counterpart in the reference, which recomputes the support
  * inside every shape-function call, Basis.jl:63-78,390-425).
  * jump_get_bins: runs the binning of the current particle positions and

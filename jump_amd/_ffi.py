@@ -168,7 +168,9 @@ def load():
                             ("jump_adjoint_backward", ctypes.c_int, [H, _pd, _pd, _pd, _pd, _pd]),
                             ("jump_adjoint_interval", ctypes.c_int, [H, ctypes.POINTER(_i32)]),
                             ("jump_dump_lammps", ctypes.c_int, [H, ctypes.c_char_p, ctypes.c_char_p]),
-                            ("jump_dump_wait", ctypes.c_int, [H])):
+                            ("jump_dump_wait", ctypes.c_int, [H]),
+                            ("jump_energies_noalloc", ctypes.c_int,
+                             [_i32, _i64, H, H, H, H, H, H, H])):
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
@@ -185,5 +187,5 @@ EXPORTED = [
     "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
     "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies", "jump_batch_destroy",
     "jump_batch_last_error", "jump_adjoint_forward", "jump_adjoint_backward", "jump_adjoint_interval",
-    "jump_dump_lammps", "jump_dump_wait",
+    "jump_dump_lammps", "jump_dump_wait", "jump_energies_noalloc",
 ]

@@ -8,6 +8,7 @@
 #include <climits>
 #include <cstddef>
 #include <memory>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -1162,8 +1163,8 @@ static int range_energies(jump_ctx* ctx, int64_t begin, int64_t end, double* ke,
   }
   const int nb = blocks_for(end - begin);
   double* out = ctx->d_red + 2 * size_t(nb);
-  if (ctx->D == 2) LAUNCH(k_energy_partial<2>, end - begin, ctx->P, int(begin), int(end), ctx->d_red);
-  else LAUNCH(k_energy_partial<3>, end - begin, ctx->P, int(begin), int(end), ctx->d_red);
+  if (ctx->D == 2) LAUNCH((k_energy_partial<2, EnergySoA<2>>), end - begin, EnergySoA<2>{ctx->P}, int(begin), int(end), ctx->d_red);
+  else LAUNCH((k_energy_partial<3, EnergySoA<3>>), end - begin, EnergySoA<3>{ctx->P}, int(begin), int(end), ctx->d_red);
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_red, nb, out);
   double h[2];
   HIP_TRY(hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1187,6 +1188,68 @@ int jump_energies(jump_ctx* ctx, double* ke, double* se) {
   r = range_energies(ctx, 0, ctx->np, ke, se);
   return r;
 }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ energies without allocation
+// jump_energies_noalloc: the reduction of jump_energies over a caller's device
+// arrays, with its per-block partial sums in `scratch`, a fixed device array
+// of this library (2 doubles per block of kBlock particles, 1 MiB: at most
+// kNoAllocMaxBlocks * kBlock = 16,777,216 particles per call).  Each device
+// holds its own copy of the array; its address is looked up once per device
+// (hipGetSymbolAddress, with that device current) and kept.  Every call on a
+// device shares that device's array, so calls that overlap on one device
+// race; the partials are fully rewritten by each call, nothing has to start
+// at zero.
+namespace {
+constexpr int kNoAllocMaxBlocks = 1 << 16;
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxBlocks];
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of this device's `scratch` (the array's name in host
+// code is a host-side stand-in, not a device address).
+int scratch_ptr(int dev, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return JUMP_ERR_DEVICE;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return JUMP_OK;
+}
+}  // namespace
+
+extern "C" int jump_energies_noalloc(int32_t dim, int64_t n, const double* mass, const double* vel,
+                                     const double* volume, const double* strain, const double* stress,
+                                     double* ke_se, void* stream) {
+  if (dim != 2 && dim != 3) return JUMP_ERR_ARG;
+  if (n < 0 || !ke_se) return JUMP_ERR_ARG;
+  if (n > 0 && (!mass || !vel || !volume || !strain || !stress)) return JUMP_ERR_ARG;
+  if (n > int64_t(kNoAllocMaxBlocks) * kBlock) return JUMP_ERR_UNSUPPORTED;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return JUMP_ERR_DEVICE;
+  if (dev >= kNoAllocMaxDevices) return JUMP_ERR_UNSUPPORTED;
+  double* part = nullptr;
+  int r = scratch_ptr(dev, &part);
+  if (r) return r;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nb = blocks_for(n);
+  if (nb > 0) {
+    if (dim == 2)
+      hipLaunchKernelGGL((k_energy_partial<2, EnergyAoS<2>>), dim3(nb), dim3(kBlock), 0, st,
+                         EnergyAoS<2>{mass, vel, volume, strain, stress}, 0, int(n), part);
+    else
+      hipLaunchKernelGGL((k_energy_partial<3, EnergyAoS<3>>), dim3(nb), dim3(kBlock), 0, st,
+                         EnergyAoS<3>{mass, vel, volume, strain, stress}, 0, int(n), part);
+  }
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kBlock), 0, st, part, nb, ke_se);  // nb = 0 writes zeros
+  return hipGetLastError() == hipSuccess ? JUMP_OK : JUMP_ERR_DEVICE;
+}
+
+extern "C" {
 
 int jump_dump_lammps(jump_ctx* ctx, const char* path, const char* outs) {
   if (!ctx) return JUMP_ERR_ARG;

@@ -1901,29 +1901,46 @@ __global__ void __launch_bounds__(kBlock) k_g2p2_musl_contact(Particles P, GridD
 // ------------------------------------------------------------------ energies
 // Fix.jl:248-270.  Per-block sums in a fixed tree order, then one block sums
 // the partials in block order: deterministic.
-template <int D>
+//
+// Particle fields of the sums: the context's layout (SoA, symmetric tensors
+// compressed) or a caller's arrays in the reference's AoS layout (vel d*n,
+// strain and stress d*d*n column-major).  The strain-energy term runs over
+// the full tensor in column-major order, so both layouts give the same bits
+// for the symmetric tensors the reference keeps.
+template <int D> struct EnergySoA {
+  Particles P;
+  __device__ double m(int i) const { return P.m[i]; }
+  __device__ double V(int i) const { return P.V[i]; }
+  __device__ double v(int i, int k) const { return P.v[k][i]; }
+  __device__ double se(int i, int r, int c) const { return P.sig[sym<D>(r, c)][i] * P.eps[sym<D>(r, c)][i]; }
+};
+template <int D> struct EnergyAoS {
+  const double *mass, *vel, *vol, *strain, *stress;
+  __device__ double m(int i) const { return mass[i]; }
+  __device__ double V(int i) const { return vol[i]; }
+  __device__ double v(int i, int k) const { return vel[size_t(D) * i + k]; }
+  __device__ double se(int i, int r, int c) const {
+    const size_t q = size_t(D * D) * i + r + D * c;
+    return stress[q] * strain[q];
+  }
+};
 // Partial sums over blocks of kBlock particles of [begin, end), the blocks
 // aligned at begin (a batch instance reduces exactly as a context of its own).
-__global__ void __launch_bounds__(kBlock) k_energy_partial(Particles P, int begin, int end, double* __restrict__ part) {
+template <int D, class Src>
+__global__ void __launch_bounds__(kBlock) k_energy_partial(Src src, int begin, int end, double* __restrict__ part) {
   __shared__ double ske[kBlock], sse[kBlock];
   int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
   double ke = 0.0, se = 0.0;
   if (i < end) {
-    double vv = P.v[0][i] * P.v[0][i];
-    for (int k = 1; k < D; k++) vv = vv + P.v[k][i] * P.v[k][i];
-    ke = 0.5 * P.m[i] * vv;
-    double acc;
-    if (D == 2) {
-      // column-major sum(sigma .* eps): 11, 21, 12, 22
-      const double a = P.sig[0][i] * P.eps[0][i], b = P.sig[2][i] * P.eps[2][i], c = P.sig[1][i] * P.eps[1][i];
-      acc = a + b + b + c;
-    } else {
-      // 11 21 31 12 22 32 13 23 33
-      const int order[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
-      acc = P.sig[0][i] * P.eps[0][i];
-      for (int k = 1; k < 9; k++) acc = acc + P.sig[order[k]][i] * P.eps[order[k]][i];
-    }
-    se = 0.5 * P.V[i] * acc;
+    double vv = src.v(i, 0) * src.v(i, 0);
+    for (int k = 1; k < D; k++) vv = vv + src.v(i, k) * src.v(i, k);
+    ke = 0.5 * src.m(i) * vv;
+    // sum(sigma .* eps), column-major: 11 21 (31) 12 22 (32) (13 23 33)
+    double acc = src.se(i, 0, 0);
+    for (int c = 0; c < D; c++)
+      for (int r = 0; r < D; r++)
+        if (r + c > 0) acc = acc + src.se(i, r, c);
+    se = 0.5 * src.V(i) * acc;
   }
   ske[threadIdx.x] = ke;
   sse[threadIdx.x] = se;

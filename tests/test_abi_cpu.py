@@ -43,7 +43,8 @@ def test_library_loads_and_exports_every_declared_symbol():
 def test_oracle_exports_the_same_entry_points(oracle_lib):
     for name in _declared():
         if name in ("jump_get_bins", "jump_abi_version", "jump_profile", "jump_profile_read",
-                    "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load") or \
+                    "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
+                    "jump_energies_noalloc") or \
                 name.startswith(("jump_batch_", "jump_adjoint_", "jump_dump_")):
             continue  # diagnostics, device-state transfer, batching, reverse mode, dumps: GPU library only
         assert hasattr(oracle_lib, name.replace("jump_", "orc_")), name
