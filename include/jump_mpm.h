@@ -131,8 +131,14 @@ typedef struct {
  * (D = (v - v_cm).n > 0) gets v -= D (n + mu'/|C| (n_y C, -n_x C)),
  * C = (v - v_cm) x n, mu' = min(friction, |C|/D) (the tangential term is 0
  * when C = 0, where the reference divides 0 by 0).  G2P then reads each
- * particle's own solid's nodal velocities.  Rigid solids, batches and the
- * reverse mode do not combine with contact. */
+ * particle's own solid's nodal velocities.  A 2D rigid solid
+ * (SolveExplicitFEMP2DContact.jl:272-304, :330-358) reaches the four nodes
+ * of each of its particles' hat cells and brings only its normal
+ * sum(m_p grad N); where it meets a deformable solid, n is the rigid body's
+ * normal (the deformable solid uses -n), v_cm is the rigid velocity (0 if
+ * fixed) and only the deformable solid is corrected; rigid solids move with
+ * their prescribed velocity.  Batches and the reverse mode do not combine
+ * with contact. */
 
 /* Material parameters.  Lame constants are derived exactly as the reference
  * constructors do (Material.jl:54-57,112-113,159-161,313-325). */
@@ -319,7 +325,9 @@ int jump_energies_noalloc(int32_t dim, int64_t n, const double* mass, const doub
  * inside every shape-function call, Basis.jl:63-78,390-425).
  * jump_get_bins: runs the binning of the current particle positions and
  * returns, per particle in insertion order, the 0-based linear index of the
- * first node of its stencil (-1: rigid in 2D or outside the grid).  For
+ * first node of its stencil (-1: outside the grid, or rigid in 2D outside
+ * a contact context; a contact context bins a rigid particle by its hat
+ * cell).  For
  * tests of the bit-exact support rule. */
 int jump_get_bins(jump_ctx* ctx, int32_t* keys);
 int jump_abi_version(void);

@@ -476,7 +476,7 @@ template <int D, int B, bool MULTI> static int bin_and_sort(jump_ctx* ctx, int* 
   const int np = int(ctx->np);
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cnt, 0, nn + 1);
   LAUNCH((k_bin<D, B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_rank,
-         ctx->d_bxyz, ctx->d_cnt, ctx->d_flags, ctx->bt(), dirty, gen);
+         ctx->d_bxyz, ctx->d_cnt, ctx->d_flags, ctx->bt(), dirty, gen, ctx->prm.contact);
   scan_nodes(ctx, ctx->d_cnt, ctx->d_start);
   LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_rank, ctx->d_sorted, np, nn);
   LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_sorted, nn);
@@ -530,6 +530,9 @@ template <int B, int ORDER> static void step_usl2d_contact(jump_ctx* ctx) {
   LAUNCH((k_g2p_usl2d_contact<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->cg, ctx->prm.tolerance, dt,
          ctx->d_flags);
   prof_end(ctx, JUMP_PHASE_G2P);
+  prof_begin(ctx);
+  if (ctx->max_rigid_rank >= 0) LAUNCH((k_rigid_move<false>), np, ctx->P, ctx->d_solids, dt, ctx->bt());
+  prof_end(ctx, JUMP_PHASE_RIGID);
 }
 
 // 3D MUSL with multi-body contact: slot P2G with the grid update and the
@@ -847,8 +850,6 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid (Solid3D has no rigid flag)");
   if (ctx->D == 2 && ctx->prm.algorithm == JUMP_ALG_MUSL && mat->kind == JUMP_MAT_RIGID)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the 2D MUSL step has no rigid bodies");
-  if (ctx->prm.contact && mat->kind == JUMP_MAT_RIGID)
-    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "rigid solids do not combine with contact");
   if (D == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");
   if (ctx->np + ds->n > kMaxParticles) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "too many particles (at most 2^28 per context)");
   const bool cpdi = ctx->prm.basis == JUMP_BASIS_CPDI_Q4;

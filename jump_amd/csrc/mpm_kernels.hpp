@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
                                                 const double* __restrict__ corners, int* __restrict__ key,
                                                 int* __restrict__ rank, unsigned long long* __restrict__ bxyz,
                                                 int* __restrict__ cnt, unsigned* __restrict__ flags, BatchDev Bt,
-                                                int* __restrict__ dirty, int gen) {
+                                                int* __restrict__ dirty, int gen, int rigid_hat) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   // 3D: a particle whose bin changed since the last binning changes the
@@ -99,9 +99,15 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
   int k = -1;
   bool ok = false;
   int f[3] = {1, 1, 1};
-  if (!(D == 2 && s.rigid)) {
+  // 2D rigid solids take no part in P2G, except in a contact step
+  // (rigid_hat), where they are binned by their hat cell: the nodes their
+  // velocity is prescribed at
+  const bool hat = D == 2 && s.rigid;
+  if (!hat || rigid_hat) {
     ok = true;
-    if (B == JUMP_BASIS_CPDI_Q4) {
+    if (hat) {
+      for (int a = 0; a < 2; a++) ok = ok && first_node<JUMP_BASIS_LINEAR, 2>(P.x[a][i], a, axis_of(G, a), 0.0, f[a]);
+    } else if (B == JUMP_BASIS_CPDI_Q4) {
       CpdiW c;
       ok = cpdi_setup(corners, P.elem[0][i], P.elem[1][i], P.elem[2][i], P.elem[3][i], G, c);
       if (ok) {
@@ -759,10 +765,37 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
 // its normal (D = (v - v_cm).n > 0) gets v -= D (n + mu'/|C| (n_y C, -n_x C)),
 // C = (v - v_cm) x n, mu' = min(mu, |C|/D) (:366-381; no tangential term
 // when C = 0).  The single-field arrays receive the slot totals.
+//
+// Rigid solids (SolveExplicitFEMP2DContact.jl:272-304, :330-358): a rigid
+// particle reaches the four nodes of its hat cell, as in the rigid override
+// of the ULMPM step, and adds only its normal sum(m_p grad N_hat) to its
+// slot (no mass, momentum or force).  Where a rigid and a deformable solid
+// meet, the common normal is the rigid body's own, normalised (the
+// deformable slot uses -n), the centre-of-mass velocity is the rigid
+// velocity, formed as (m_S v_r)/m_S from the deformable mass m_S as the
+// reference overrides the system momentum, and only the deformable slot is
+// corrected.  A rigid slot holds the rigid velocity (0 if fixed) as v and v0.
 struct ContactSlot {
   int body = -1;
+  bool rigid = false;
   double m = 0.0, p0[2] = {0.0, 0.0}, f[2] = {0.0, 0.0}, nr[2] = {0.0, 0.0};
 };
+
+// Hat weight and gradient of node (ii, jj) (1-based) for a particle at x;
+// false when the node is not a corner of the particle's hat cell
+// (Basis.jl:63-78, 219-251).
+__device__ __forceinline__ bool hat_node(const GridDesc& G, const double* x, int ii, int jj, double& N, double* dN) {
+  const Axis X = axis_of(G, 0), Y = axis_of(G, 1);
+  int ci, cj;
+  if (!first_node<JUMP_BASIS_LINEAR, 2>(x[0], 0, X, 0.0, ci) || !first_node<JUMP_BASIS_LINEAR, 2>(x[1], 1, Y, 0.0, cj))
+    return false;
+  if (ii - ci < 0 || ii - ci > 1 || jj - cj < 0 || jj - cj > 1) return false;
+  double wx, dwx, wy, dwy;
+  axis_weight<JUMP_BASIS_LINEAR, false>(x[0], 0, X, 0.0, ii, ii, wx, dwx);
+  axis_weight<JUMP_BASIS_LINEAR, false>(x[1], 1, Y, 0.0, jj, jj, wy, dwy);
+  combine2<JUMP_BASIS_LINEAR>(X, Y, wx, dwx, wy, dwy, N, dN);
+  return true;
+}
 
 template <int B, int ORDER>
 __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc G, const SolidDev* __restrict__ S,
@@ -806,6 +839,19 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
       lp[1] = S[sid].lp[1];
     }
     double N, dN[3];
+    if (S[sid].rigid) {
+      if (!hat_node(G, v, ni + 1, nj + 1, N, dN)) return;
+      ContactSlot* s = (s0.body < 0 || s0.body == sid) ? &s0 : ((s1.body < 0 || s1.body == sid) ? &s1 : nullptr);
+      if (!s) {
+        overflow = true;
+        return;
+      }
+      s->body = sid;
+      s->rigid = true;
+      s->nr[0] += v[2] * dN[0];
+      s->nr[1] += v[2] * dN[1];
+      return;
+    }
     bool member;
     node_weight<2, B, false, true>(P, G, lp, corners, p, v, ni + 1, nj + 1, 1, N, dN, member);
     if (!member) return;
@@ -828,7 +874,11 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
     double v0[2] = {0.0, 0.0};
     v[k][0] = 0.0;
     v[k][1] = 0.0;
-    if (s.body >= 0) {
+    if (s.rigid) {
+      const SolidDev& r = S[s.body];
+      v0[0] = v[k][0] = r.fixed ? 0.0 : r.vx;
+      v0[1] = v[k][1] = r.fixed ? 0.0 : r.vy;
+    } else if (s.body >= 0) {
       const double invM = 1.0 / s.m;
 #pragma unroll
       for (int a = 0; a < 2; a++) {
@@ -848,15 +898,30 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
     g.p[a][n] = pk[0][a] + pk[1][a];
     g.f[a][n] = s0.f[a] + s1.f[a];
   }
-  if (s0.body >= 0 && s1.body >= 0) {
+  if (s0.body >= 0 && s1.body >= 0 && !(s0.rigid && s1.rigid)) {
     const double mS = s0.m + s1.m;
-    const double vcm[2] = {(pk[0][0] + pk[1][0]) / mS, (pk[0][1] + pk[1][1]) / mS};
-    double nc[2] = {s0.nr[0] - s1.nr[0], s0.nr[1] - s1.nr[1]};
-    const double nrm = sqrt(nc[0] * nc[0] + nc[1] * nc[1]);
-    nc[0] = nc[0] / nrm;
-    nc[1] = nc[1] / nrm;
+    double vcm[2], nc[2];
+    if (s0.rigid || s1.rigid) {
+      // the rigid body's velocity and normal; slot 1 rigid: slot 0 takes -n
+      const ContactSlot& r = s0.rigid ? s0 : s1;
+      const double vr[2] = {v[s0.rigid ? 0 : 1][0], v[s0.rigid ? 0 : 1][1]};
+      vcm[0] = (mS * vr[0]) / mS;
+      vcm[1] = (mS * vr[1]) / mS;
+      const double nrm = sqrt(r.nr[0] * r.nr[0] + r.nr[1] * r.nr[1]);
+      nc[0] = s0.rigid ? r.nr[0] / nrm : -(r.nr[0] / nrm);
+      nc[1] = s0.rigid ? r.nr[1] / nrm : -(r.nr[1] / nrm);
+    } else {
+      vcm[0] = (pk[0][0] + pk[1][0]) / mS;
+      vcm[1] = (pk[0][1] + pk[1][1]) / mS;
+      nc[0] = s0.nr[0] - s1.nr[0];
+      nc[1] = s0.nr[1] - s1.nr[1];
+      const double nrm = sqrt(nc[0] * nc[0] + nc[1] * nc[1]);
+      nc[0] = nc[0] / nrm;
+      nc[1] = nc[1] / nrm;
+    }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
+      if (sl[k]->rigid) continue;  // a rigid body is never corrected
       const double nk[2] = {k == 0 ? nc[0] : -nc[0], k == 0 ? nc[1] : -nc[1]};
       const double dv[2] = {v[k][0] - vcm[0], v[k][1] - vcm[1]};
       const double Dn = dv[0] * nk[0] + dv[1] * nk[1];
@@ -1121,6 +1186,7 @@ __global__ void __launch_bounds__(kBlock) k_g2p_usl2d_contact(Particles P, GridD
   if (i >= P.n) return;
   const int sid = P.sid[i];
   const SolidDev& s = S[sid];
+  if (s.rigid) return;  // moved by k_rigid_move
   double v0 = P.v[0][i], v1 = P.v[1][i];
   double x0 = P.x[0][i], x1 = P.x[1][i];
   double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;

@@ -150,6 +150,35 @@ def disks_contact(cells=32, ppc=2, r=0.15, gap_cells=2, v=0.1, offset=0.0, basis
     return Scene("disks_contact", p, [s1, s2], steps=steps)
 
 
+def disk_wall(cells=32, ppc=2, r=0.15, gap_cells=2, v=(0.0, -0.1), wall_v=(0.0, 0.0), fixed=False,
+              basis=_ffi.BASIS_LINEAR, material="elastic", friction=0.0, steps=600, dt=1e-3, wall=True):
+    """A Two-Disks disk (E=1000, nu=0.3, rho=1000) moving at v onto a rigid
+    wall (RigidMaterial(vx, vy, rho), Main_LateralCompression_Cellular.jl:
+    58-59) two particles thick below it, `gap_cells` cells apart, with
+    multi-body contact (SolveExplicitFEMP2DContact.jl:272-358: the rigid
+    body's normal, the rigid velocity as the centre-of-mass velocity, only
+    the disk corrected).  The disk centre sits on a grid node, the wall's
+    particles at quarter cells, so no particle lies on a grid line."""
+    n = cells + 1
+    dx = 1.0 / cells
+    fOffset = dx / ppc
+    rr = math.floor(r / fOffset) * fOffset
+    cy = round((0.5 + 0.5 * gap_cells * dx) / dx) * dx
+    c = buildParticleForCircle([0.5, cy], r, fOffset)
+    mat = {"elastic": elastic, "neohookean": neohookean}[material](1000.0, 0.3, 1000.0) \
+        if material != "j2" else j2(1000.0, 0.3, 1000.0, 5.0, 100.0)
+    vol = fOffset * fOffset
+    lp = 0.5 * fOffset if basis == _ffi.BASIS_GIMP else None
+    solids = [_solid(mat, c, v, vol, 1000.0, lp=lp)]
+    if wall:
+        top = cy - rr - gap_cells * dx  # the wall's upper surface
+        w = buildParticleForRectangle([0.5, top - fOffset], 24 * dx, 2 * fOffset, fOffset)
+        solids.append(_solid(rigid(wall_v[0], wall_v[1], 7800.0, fixed=fixed), w, (0.0, 0.0), vol, 7800.0))
+    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (1.0, 1.0), (n, n), tolerance=1e-9, contact=True,
+                    friction=friction)
+    return Scene("disk_wall", p, solids, steps=steps)
+
+
 def spheres_contact(cells=24, r=0.15, gap_cells=2, v=0.1, offset=0.0, basis=_ffi.BASIS_QUAD_BSPLINE,
                     material="elastic", contact=True, steps=600, dt=1e-3, alpha=1.0):
     """Two spheres (2 particles per cell and direction, Two-Disks material
@@ -279,11 +308,17 @@ def _ring_q4(cx, cy, ri, ro, nr, nt, jitter, rng):
     return corners, np.asarray(elems, dtype=np.int64)
 
 
-def cpdi_tubes(rings=(2, 2), cells=64, nr=4, nt=96, jitter_cells=0.1, steps=20, seed=20261202, platens=True):
+def cpdi_tubes(rings=(2, 2), cells=64, nr=4, nt=96, jitter_cells=0.1, steps=20, seed=20261202, platens=True,
+               contact=False, friction=0.0):
     """Lateral compression of a tube array with CPDI-Q4 domains
-    (Main_LateralCompressionTube_CPDI.jl): Neo-Hookean E=1 GPa, nu=0.3, outer
-    radius 5 mm, wall 0.5 mm, rigid platens at 1 m/s; corner jitter
-    +-0.1 cell (seeded)."""
+    (Main_LateralCompression_Cellular.jl:26-86, Main_LateralCompressionTube_CPDI.jl):
+    rings of outer radius 5 mm and wall 0.5 mm side by side, each its own
+    solid, Neo-Hookean E=1 GPa, nu=0.3, rho=1000, between rigid platens
+    closing at 1 m/s each; ring corners jittered by +-0.1 cell (seeded); nr x
+    nt Q4 domains per ring.  With contact on, rings and platens interact
+    through the multi-body contact correction (rigid platens by their own
+    normal); otherwise every solid shares one velocity field.  The platens'
+    particles sit at quarter cells, off the grid lines."""
     rng = np.random.default_rng(seed)
     ro, wall = 5e-3, 0.5e-3
     nxr, nyr = rings
@@ -295,33 +330,33 @@ def cpdi_tubes(rings=(2, 2), cells=64, nr=4, nt=96, jitter_cells=0.1, steps=20, 
     rho, E, nu = 1000.0, 1.0e9, 0.3
     solids = []
     jit = jitter_cells * dx
-    corners_all, elems_all = [], []
-    base = 0
     for j in range(nyr):
         for i in range(nxr):
             cx = margin + ro + 2 * ro * i
             cy = margin + ro + 2 * ro * j
-            c, e = _ring_q4(cx, cy, ro - wall, ro, nr, nt, jit, rng)
-            corners_all.append(c)
-            elems_all.append(e + base)
-            base += c.shape[0]
-    corners = np.concatenate(corners_all)
-    elems = np.concatenate(elems_all)
-    cc = corners[elems]
-    x, y = cc[:, :, 0], cc[:, :, 1]
-    area = 0.5 * (x[:, 0] * y[:, 1] - x[:, 1] * y[:, 0] + x[:, 1] * y[:, 2] - x[:, 2] * y[:, 1]
-                  + x[:, 2] * y[:, 3] - x[:, 3] * y[:, 2] + x[:, 3] * y[:, 0] - x[:, 0] * y[:, 3])
-    centre = cc.mean(axis=1)
-    s = dict(material=neohookean(E, nu, rho), pos=np.ascontiguousarray(centre), vel=np.zeros_like(centre),
-             mass=rho * area, volume=area.copy(), volume0=area.copy(), corners=corners, elems=elems)
-    solids.append(s)
+            corners, elems = _ring_q4(cx, cy, ro - wall, ro, nr, nt, jit, rng)
+            cc = corners[elems]
+            x, y = cc[:, :, 0], cc[:, :, 1]
+            area = 0.5 * (x[:, 0] * y[:, 1] - x[:, 1] * y[:, 0] + x[:, 1] * y[:, 2] - x[:, 2] * y[:, 1]
+                          + x[:, 2] * y[:, 3] - x[:, 3] * y[:, 2] + x[:, 3] * y[:, 0] - x[:, 0] * y[:, 3])
+            centre = cc.mean(axis=1)
+            solids.append(dict(material=neohookean(E, nu, rho), pos=np.ascontiguousarray(centre),
+                               vel=np.zeros_like(centre), mass=rho * area, volume=area.copy(), volume0=area.copy(),
+                               corners=corners, elems=elems))
     if platens:
         f = dx / 2
-        top_y = margin + height + 2 * f
-        bot_y = margin - 2 * f
-        for yy, vy in ((top_y, -1.0), (bot_y, 1.0)):
-            pts = buildParticleForRectangle([margin + width / 2, yy], width + 4 * dx, 2 * f, f)
-            # rigid platens use the linear-basis stencil only; give them unit
+        # two rows of particles at quarter cells just outside the top and
+        # bottom ring rows, 4 cells wider than the array on each side
+        i0 = int(math.floor((margin - 4 * dx) / dx))
+        i1 = int(math.ceil((margin + width + 4 * dx) / dx))
+        xs = np.concatenate([(np.arange(i0, i1) + q) * dx for q in (0.25, 0.75)])
+        xs.sort()
+        jt = int(math.ceil((margin + height) / dx))
+        jb = int(math.floor(margin / dx)) - 1
+        for rows, vy in (((jt + 0.25, jt + 0.75), -1.0), ((jb + 0.25, jb + 0.75), 1.0)):
+            X, Y = np.meshgrid(xs, np.asarray(rows) * dx, indexing="xy")
+            pts = np.stack([X.ravel(), Y.ravel()], axis=1)
+            # rigid platens use the hat-cell stencil only; give them unit
             # CPDI domains so the solid is well formed
             nc = pts.shape[0]
             pc = np.concatenate([pts + [-f / 2, -f / 2], pts + [f / 2, -f / 2], pts + [f / 2, f / 2],
@@ -333,8 +368,8 @@ def cpdi_tubes(rings=(2, 2), cells=64, nr=4, nt=96, jitter_cells=0.1, steps=20, 
     c = math.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) / rho)
     dt = 0.2 * dx / c
     p = make_params(2, _ffi.BASIS_CPDI_Q4, _ffi.ALG_USL, dt, (0.0, 0.0), (L, L), (cells + 1, cells + 1),
-                    tolerance=1e-14)
-    return Scene("cpdi_tubes", p, solids, steps=steps)
+                    tolerance=1e-14, contact=contact, friction=friction)
+    return Scene("cpdi_tubes", p, solids, steps=steps, meta={"rings": rings, "dx": dx})
 
 
 def musl_block2d(cells=256, basis=_ffi.BASIS_QUAD_BSPLINE, steps=20, seed=0, alpha=0.99):

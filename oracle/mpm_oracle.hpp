@@ -894,6 +894,14 @@ template <class T> void step_usl_2d(Ctx<T>& g, const T& t) {
 //   n = n_a - n_b; n /= sqrt(n.n)   (slot b uses -n)
 //   dv = v - vcm; D = dv.n; C = dv_x n_y - dv_y n_x; mu' = min(fric, |C|/D)
 //   v = v - D (n + (mu'/|C|) (n_y C, -(n_x C)))   (tangential 0 if C == 0)
+// Rigid solids (SolveExplicitFEMP2DContact.jl:272-304, :330-358): a rigid
+// particle reaches the four nodes of its hat cell (Basis.jl:63-78, the nodes
+// of the ULMPM rigid override) and adds only normal += (fM * dN_hat) to its
+// slot.  Where a rigid and a deformable solid meet:
+//   vcm = (mS * v_r) / mS (mS: the deformable mass; v_r = 0 if fixed)
+//   n = n_r / sqrt(n_r.n_r) for the rigid slot, -n for the deformable one
+//   and only the deformable slot is corrected.  G2P skips rigid solids; they
+//   move by x += dt v_r as in the ULMPM step.
 template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
   const T dt = g.dt;
   const i64 nn = g.nn;
@@ -908,8 +916,23 @@ template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
   for (size_t si = 0; si < g.solids.size(); si++) {
     auto& s = g.solids[si];
     for (i64 ip = 0; ip < s.n; ip++) {
-      int sup = support(g, s, ip, true);
+      int sup = s.rigid ? sup_linear_2d(g, &s.pos[2 * ip]) : support(g, s, ip, true);
       if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      if (s.rigid) {
+        const T fM = s.mass[ip];
+        for (int i = 0; i < sup; i++) {
+          i64 in = g.near[i];
+          int k;
+          if (g.cbody[2 * in] < 0 || g.cbody[2 * in] == int(si)) k = 0;
+          else if (g.cbody[2 * in + 1] < 0 || g.cbody[2 * in + 1] == int(si)) k = 1;
+          else { g.flags |= JUMP_FLAG_CONTACT_OVERFLOW; continue; }
+          g.cbody[2 * in + k] = int(si);
+          const i64 c2 = 4 * in + 2 * k;
+          g.cn[c2] += fM * g.ders[2 * i];
+          g.cn[c2 + 1] += fM * g.ders[2 * i + 1];
+        }
+        continue;
+      }
       T fV = s.vol[ip], fM = s.mass[ip];
       const T* vp = &s.vel[2 * ip];
       const T* sg = &s.stress[4 * ip];
@@ -943,7 +966,12 @@ template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
         p[k][a] = g.cp0[c2 + a] + g.cf[c2 + a] * dt;
         if (g.fixed[2 * i + a]) { g.cp0[c2 + a] = 0.0; p[k][a] = 0.0; }
       }
-      if (g.cbody[2 * i + k] >= 0) {
+      const int b = g.cbody[2 * i + k];
+      if (b >= 0 && g.solids[size_t(b)].rigid) {
+        const auto& rm = g.solids[size_t(b)].mat;
+        g.cv0[c2] = g.cv[c2] = rm.fixed ? T(0.0) : rm.vx;
+        g.cv0[c2 + 1] = g.cv[c2 + 1] = rm.fixed ? T(0.0) : rm.vy;
+      } else if (b >= 0) {
         T invM = 1.0 / g.cm[2 * i + k];
         for (int a = 0; a < 2; a++) {
           g.cv0[c2 + a] = g.cp0[c2 + a] * invM;
@@ -959,14 +987,31 @@ template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
       g.gf[2 * i + a] = g.cf[4 * i + a] + g.cf[4 * i + 2 + a];
     }
     if (g.cbody[2 * i] < 0 || g.cbody[2 * i + 1] < 0) continue;
+    const bool r0 = g.solids[size_t(g.cbody[2 * i])].rigid, r1 = g.solids[size_t(g.cbody[2 * i + 1])].rigid;
+    if (r0 && r1) continue;  // rigid bodies are never corrected
     // contact node
     const T mS = g.cm[2 * i] + g.cm[2 * i + 1];
-    const T vcm[2] = {(p[0][0] + p[1][0]) / mS, (p[0][1] + p[1][1]) / mS};
-    T nc[2] = {g.cn[4 * i] - g.cn[4 * i + 2], g.cn[4 * i + 1] - g.cn[4 * i + 3]};
-    const T nrm = sqrt(nc[0] * nc[0] + nc[1] * nc[1]);
-    nc[0] = nc[0] / nrm;
-    nc[1] = nc[1] / nrm;
+    T vcm[2], nc[2];
+    if (r0 || r1) {
+      const i64 rr = r0 ? 0 : 1;
+      const T* vr = &g.cv[4 * i + 2 * rr];
+      vcm[0] = (mS * vr[0]) / mS;
+      vcm[1] = (mS * vr[1]) / mS;
+      const T* nr = &g.cn[4 * i + 2 * rr];
+      const T nrm = sqrt(nr[0] * nr[0] + nr[1] * nr[1]);
+      nc[0] = r0 ? nr[0] / nrm : -(nr[0] / nrm);
+      nc[1] = r0 ? nr[1] / nrm : -(nr[1] / nrm);
+    } else {
+      vcm[0] = (p[0][0] + p[1][0]) / mS;
+      vcm[1] = (p[0][1] + p[1][1]) / mS;
+      nc[0] = g.cn[4 * i] - g.cn[4 * i + 2];
+      nc[1] = g.cn[4 * i + 1] - g.cn[4 * i + 3];
+      const T nrm = sqrt(nc[0] * nc[0] + nc[1] * nc[1]);
+      nc[0] = nc[0] / nrm;
+      nc[1] = nc[1] / nrm;
+    }
     for (int k = 0; k < 2; k++) {
+      if ((k == 0 && r0) || (k == 1 && r1)) continue;
       const T nk[2] = {k == 0 ? nc[0] : -nc[0], k == 0 ? nc[1] : -nc[1]};
       T* v = &g.cv[4 * i + 2 * k];
       const T dv[2] = {v[0] - vcm[0], v[1] - vcm[1]};
@@ -988,6 +1033,7 @@ template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
   // G2P with each particle's own solid's velocities
   for (size_t si = 0; si < g.solids.size(); si++) {
     auto& s = g.solids[si];
+    if (s.rigid) continue;
     for (i64 ip = 0; ip < s.n; ip++) {
       int sup = support(g, s, ip, true);
       if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
@@ -1011,6 +1057,14 @@ template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
         }
       }
       usl2d_particle_update(g, s, ip, v0, v1, x0, x1, L);
+    }
+  }
+  // rigid solids move (SolveExplicitSolidMechanics2D.jl:539-551)
+  for (auto& s : g.solids) {
+    if (!s.rigid) continue;
+    for (i64 ip = 0; ip < s.n; ip++) {
+      s.pos[2 * ip] = s.pos[2 * ip] + dt * s.mat.vx;
+      s.pos[2 * ip + 1] = s.pos[2 * ip + 1] + dt * s.mat.vy;
     }
   }
   (void)t;

@@ -94,7 +94,6 @@ int orc_add_solid(orc_ctx* h, const jump_solid_desc* ds, const jump_material* ma
   if (mat->kind < JUMP_MAT_RIGID || mat->kind > JUMP_MAT_JOHNSONCOOK) return fail(h, JUMP_ERR_ARG, "bad material");
   if (d == 3 && mat->kind == JUMP_MAT_RIGID) return fail(h, JUMP_ERR_UNSUPPORTED, "3D solids cannot be rigid");
   if (d == 2 && mat->kind == JUMP_MAT_JOHNSONCOOK) return fail(h, JUMP_ERR_UNSUPPORTED, "Johnson-Cook is 3D only");
-  if (g.contact && mat->kind == JUMP_MAT_RIGID) return fail(h, JUMP_ERR_UNSUPPORTED, "rigid solids do not combine with contact");
   if (d == 2 && g.alg == JUMP_ALG_MUSL && mat->kind == JUMP_MAT_RIGID)
     return fail(h, JUMP_ERR_UNSUPPORTED, "the 2D MUSL step has no rigid-body treatment");
   if (g.basis == JUMP_BASIS_CPDI_Q4 && (!ds->corners || !ds->elems || ds->ncorners <= 0))

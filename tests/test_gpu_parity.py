@@ -491,14 +491,80 @@ def test_contact_binned_order_and_overflow(hip_lib):
     _compare(g, o, sc, exact=True)
 
 
+RIGID_WALL_CASES = [(_ffi.BASIS_LINEAR, dict(fixed=True)),
+                    (_ffi.BASIS_LINEAR, dict(v=(0.08, -0.1), fixed=True, friction=0.3)),
+                    (_ffi.BASIS_GIMP, dict(wall_v=(0.1, 0.0), friction=1e6)),
+                    (_ffi.BASIS_QUAD_BSPLINE, dict(v=(0.0, 0.0), wall_v=(0.0, 0.2), gap_cells=1)),
+                    (_ffi.BASIS_CUBIC_BSPLINE, dict(fixed=True, material="j2"))]
+
+
+@pytest.mark.parametrize("basis,kw", RIGID_WALL_CASES)
+def test_contact_rigid_wall_matches_oracle(hip_lib, basis, kw):
+    """A disk against a rigid wall with multi-body contact (the rigid body's
+    own normal and velocity, the rigid body never corrected): 1000 steps
+    through impact and rebound, bitwise equal to the restatement."""
+    _say(f"rigid wall basis={basis} {kw}")
+    sc = scenes.disk_wall(basis=basis, steps=1000, **kw)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=True)
+    assert g.flags() == 0
+
+
+def test_cpdi_tube_array_contact_matches_oracle(hip_lib):
+    """A 3 x 3 array of CPDI rings, each its own solid, between rigid
+    platens, with contact between rings and with the platens (the set-up of
+    Main_LateralCompression_Cellular.jl at a small size): 200 steps equal to
+    the restatement within 1e-10 (Neo-Hookean)."""
+    _say("cpdi tube array 3x3 with contact")
+    sc = scenes.cpdi_tubes(rings=(3, 3), cells=96, nr=2, nt=64, steps=200, contact=True)
+    g, o = _run(sc, sc.steps, hip_lib)
+    _compare(g, o, sc, exact=False)
+    assert g.flags() == 0
+
+
+def test_cpdi_tube_array_full_size_properties(hip_lib):
+    """Config c3 at its stated shape: an 8 x 8 array of CPDI rings (64
+    solids, 1,239,784 particles with the platens) on a 2048 x 2048 grid, rigid
+    platens at 1 m/s, contact between rings and platens, corners jittered by
+    0.1 cell: no flag is raised, the rings' x momentum stays zero (ring-ring
+    contact conserves momentum; the frictionless platens push along y only),
+    the platens close the array symmetrically, and the run is bitwise
+    reproducible."""
+    _say("cpdi tube array 8x8 at 2048^2")
+    sc = scenes.cpdi_tubes(rings=(8, 8), cells=2047, nr=16, nt=1200, steps=60, contact=True)
+    runs = []
+    for rep in range(2):
+        g = Engine(sc.params)
+        sc.load(g)
+        t, rc = g.step(sc.steps, 0.0)
+        assert rc == 0 and g.flags() == 0
+        vs = [g.get_solid(k) for k in range(64)]
+        runs.append(vs)
+        g.close()
+    for a, b in zip(runs[0], runs[1]):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+    m = np.concatenate([v["mass"] for v in runs[0]])
+    vel = np.concatenate([v["vel"] for v in runs[0]])
+    scale = float((m * np.abs(vel).sum(1)).sum())
+    assert scale > 0.0
+    assert abs(float((m * vel[:, 0]).sum())) <= 1e-10 * scale
+    vy = np.array([float((v["mass"] * v["vel"][:, 1]).sum() / v["mass"].sum()) for v in runs[0]]).reshape(8, 8)
+    assert np.all(vy[0] > 0.0) and np.all(vy[-1] < 0.0)  # bottom row pushed up, top row down
+
+
 def test_contact_refusals(hip_lib):
-    """Rigid solids, batches and the reverse mode refuse contact."""
+    """Batches and the reverse mode refuse contact; 3D rigid solids do not
+    exist (Solid3D has no rigid flag)."""
     from jump_amd.engine import Batch, make_material
+    sc3 = scenes.spheres_contact(steps=2)
+    g3 = Engine(sc3.params)
+    with pytest.raises(JumpError):
+        g3.add_solid(make_material(_ffi.MAT_RIGID, density=1.0), np.array([[0.5, 0.5, 0.5]]), np.zeros((1, 3)),
+                     np.ones(1), np.ones(1), np.ones(1))
+    g3.close()
     sc = scenes.disks_contact(steps=2)
     g = Engine(sc.params)
-    with pytest.raises(JumpError):
-        g.add_solid(make_material(_ffi.MAT_RIGID, density=1.0), np.array([[0.5, 0.5]]), np.zeros((1, 2)),
-                    np.ones(1), np.ones(1), np.ones(1))
     sc.load(g)
     with pytest.raises(JumpError):
         Batch([g])

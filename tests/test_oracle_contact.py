@@ -6,7 +6,10 @@ head-on collision rebounds without the disks passing through each other
 (which independent velocity fields without the correction would do); a
 frictionless oblique impact spins the disks far less than a sticking one
 (high friction, or the single shared field); a third solid at a node is
-flagged; unsupported combinations are refused.  In 3D (the MUSL step with
+flagged; a rigid wall (its own normal, its velocity as the centre-of-mass
+velocity, never corrected) changes nothing from afar, rebounds a disk
+without letting it cross, keeps tangential momentum when frictionless and
+pushes a resting disk when it moves.  In 3D (the MUSL step with
 the normal-only correction) likewise: bitwise the single-field step while
 apart, momentum conserved, a rebound without interpenetration where the
 single shared field keeps the spheres together."""
@@ -112,13 +115,92 @@ def test_third_solid_at_a_node_is_flagged():
     assert o.flags() & _ffi.JUMP_FLAG_CONTACT_OVERFLOW
 
 
-def test_contact_refuses_rigid_solids():
-    p = make_params(2, _ffi.BASIS_LINEAR, _ffi.ALG_USL, 1e-3, (0.0, 0.0), (1.0, 1.0), (5, 5), contact=True)
-    o = OracleEngine(p)
-    pos = np.array([[0.5, 0.5]])
-    with pytest.raises(JumpError):
-        o.add_solid(make_material(_ffi.MAT_RIGID, density=1.0), pos, np.zeros((1, 2)), np.ones(1), np.ones(1),
-                    np.ones(1))
+def _wall_top(sc):
+    return float(sc.solids[1]["pos"][:, 1].max())
+
+
+def test_rigid_wall_far_away_changes_nothing():
+    """While the disk is more than a stencil from the wall, no node holds
+    both: the run equals the one without the wall, bit for bit."""
+    a = _engine(scenes.disk_wall(gap_cells=8, steps=1))
+    b = _engine(scenes.disk_wall(gap_cells=8, steps=1, wall=False))
+    a.step(40, 0.0)
+    b.step(40, 0.0)
+    sa, sb = a.get_solid(0), b.get_solid(0)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    ga, gb = a.get_grid(), b.get_grid()
+    for k in ("mass", "momentum0", "momentum", "force"):
+        assert np.array_equal(ga[k], gb[k]), k  # the rigid solid adds no mass or momentum
+
+
+@pytest.mark.parametrize("basis", [_ffi.BASIS_LINEAR, _ffi.BASIS_QUAD_BSPLINE])
+def test_disk_rebounds_from_a_fixed_rigid_wall(basis):
+    """Normal impact on a fixed wall: the disk rebounds (its vertical
+    velocity changes sign) without its particles crossing the wall's
+    surface, and the wall does not move."""
+    sc = scenes.disk_wall(basis=basis, fixed=True)
+    o = _engine(sc)
+    top = _wall_top(sc)
+    wall0 = sc.solids[1]["pos"].copy()
+    lows = []
+    for _ in range(10):
+        o.step(100, 0.0)
+        lows.append(float(o.get_solid(0)["pos"][:, 1].min()))
+    c, v, _, _ = _body_stats(o, 0)
+    assert v[1] > 0.05, v  # rebounded (it came in at -0.1)
+    assert abs(v[0]) < 1e-12  # the symmetric impact stays on its line
+    assert min(lows) > top, (min(lows), top)
+    assert np.array_equal(o.get_solid(1)["pos"], wall0)
+
+
+def test_frictionless_wall_keeps_tangential_momentum():
+    """An oblique impact on a frictionless wall: the correction removes only
+    the normal component at the contact nodes, so the disk's tangential
+    momentum is what it would be without the wall (no body force, internal
+    forces sum to zero); a sticking wall (friction 1e6) takes much of it."""
+    res = {}
+    for fr in (0.0, 1e6):
+        sc = scenes.disk_wall(v=(0.08, -0.1), fixed=True, friction=fr)
+        o = _engine(sc)
+        m = sc.solids[0]["mass"]
+        px0 = float((m * sc.solids[0]["vel"][:, 0]).sum())
+        o.step(1000, 0.0)
+        a = o.get_solid(0)
+        res[fr] = float((a["mass"] * a["vel"][:, 0]).sum()) / px0
+        assert float(a["vel"][:, 1].mean()) > 0.0  # rebounded
+    assert abs(res[0.0] - 1.0) < 1e-12, res
+    assert res[1e6] < 0.8, res
+
+
+def test_sticking_wall_drags_the_disk_with_its_own_velocity():
+    """With sticking friction the wall's tangential velocity is the contact
+    nodes' centre-of-mass velocity: a wall sliding at 0.1 drags a falling
+    disk along; the same wall marked fixed has zero velocity and drags
+    nothing."""
+    vx = {}
+    for fixed in (False, True):
+        o = _engine(scenes.disk_wall(wall_v=(0.1, 0.0), fixed=fixed, friction=1e6))
+        o.step(1000, 0.0)
+        vx[fixed] = _body_stats(o, 0)[1][0]
+    assert vx[False] > 0.02, vx
+    assert abs(vx[True]) < 1e-12, vx
+
+
+def test_moving_rigid_platen_pushes_a_resting_disk():
+    """A platen rising at 0.2 meets a resting disk: the disk is driven
+    upwards and stays above the platen's moving surface."""
+    sc = scenes.disk_wall(v=(0.0, 0.0), wall_v=(0.0, 0.2), gap_cells=1)
+    o = _engine(sc)
+    top0 = _wall_top(sc)
+    t = 0.0
+    for _ in range(6):
+        t, _ = o.step(100, t)
+        top = top0 + 0.2 * t
+        assert float(o.get_solid(0)["pos"][:, 1].min()) > top - 0.25 / 32
+    c, v, _, _ = _body_stats(o, 0)
+    assert v[1] > 0.1, v
+    assert abs(v[0]) < 1e-12
 
 
 # ---------------------------------------------------------------- 3D (MUSL)
