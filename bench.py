@@ -14,6 +14,7 @@ region), and the CPU restatement timed on a bounded sample on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--order reference|binned]
                     [--batch-instances N] [--batch-steps K] [--adjoint-steps K] [--musl2d-steps K]
+                    [--tubes-steps K] [--cpu-steps K] [--cpu-omp-steps K]
     python bench.py --copy-probe MIB   # device-to-device copy of MIB MiB, to
                                        # calibrate HBM byte counters
 """
@@ -61,19 +62,37 @@ def algorithmic_bytes(phase, n_plate, n_sphere, n_active):
     return 0
 
 
-def cpu_baseline(steps):
+def cpu_baseline(steps, omp_steps):
+    """The CPU restatement on this host, on the same workload at full size:
+    serial (the reference's loops are single-threaded) and parallel over
+    particles with OpenMP on the threads this process may use
+    (OMP_NUM_THREADS, else every core), scatters coloured by stencil-origin
+    blocks.  The reported value is the parallel one."""
     from jump_amd import scenes
     from oracle.pyoracle import OracleEngine
     sc = scenes.impact3d()
+    nproc = os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
     eng = OracleEngine(sc.params)
     sc.load(eng)
     t0 = time.perf_counter()
     eng.step(steps, 0.0)
-    dt = time.perf_counter() - t0
+    dt_serial = time.perf_counter() - t0
     eng.close()
-    return {"value": sc.n_particles * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"the same impact workload at full size ({sc.n_particles} particles), {steps} steps, "
-                      f"serial C++ restatement of the reference loops ({dt:.1f} s)"}
+    eng = OracleEngine(sc.params)
+    sc.load(eng)
+    t0 = time.perf_counter()
+    eng.step_parallel(omp_steps, 0.0, threads)
+    dt_omp = time.perf_counter() - t0
+    eng.close()
+    n = sc.n_particles
+    return {"value": n * omp_steps / dt_omp, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "nproc": nproc,
+            "sample": f"the same impact workload at full size ({n} particles): {omp_steps} steps of the C++ "
+                      f"restatement parallel over particles on {threads} OpenMP threads ({dt_omp:.1f} s; "
+                      f"os.cpu_count() = {nproc})",
+            "serial": {"value": n * steps / dt_serial, "cores": 1,
+                       "sample": f"{steps} steps of the serial restatement ({dt_serial:.1f} s)"}}
 
 
 def copy_probe(mib, reps=5):
@@ -142,6 +161,39 @@ def batched_leg(n_inst, steps, warmup):
             "plain_sequential": {"value": npart * steps / t_plain, "ms_per_step": t_plain * 1e3 / steps},
             "batched": {"value": npart * steps / t_batch, "ms_per_step": t_batch * 1e3 / steps},
             "speedup": t_plain / t_batch, "unit": "particle-steps/s", "step_flags": flags}
+
+
+def tubes_leg(steps, warmup):
+    """Config c3: lateral compression of an 8 x 8 array of CPDI-Q4 rings
+    (outer radius 5 mm, wall 0.5 mm, 16 x 1200 domains each, corners jittered
+    by 0.1 cell, Neo-Hookean E = 1 GPa) between rigid platens closing at 1 m/s,
+    2D USL with multi-body contact between the 64 rings and the platens, on a
+    2048 x 2048 grid (scenes.cpdi_tubes)."""
+    import torch
+
+    from jump_amd import scenes
+    from jump_amd.engine import Engine
+
+    sc = scenes.cpdi_tubes(rings=(8, 8), cells=2047, nr=16, nt=1200, steps=steps, contact=True)
+    e = Engine(sc.params, device=0)
+    sc.load(e)
+    e.step(warmup, 0.0)
+    e.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(steps, 0.0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = e.profile_read()
+    e.profile(False)
+    flags = e.flags()
+    e.close()
+    n = sc.n_particles
+    return {"workload": "cpdi_tubes: 8x8 CPDI-Q4 rings (64 solids) + 2 rigid platens at 1 m/s, Neo-Hookean, "
+                        "2D USL with multi-body contact, 2048x2048 nodes",
+            "particles": n, "solids": len(sc.solids), "steps": steps,
+            "value": n * steps / dt, "unit": "particle-steps/s", "ms_per_step": dt * 1e3 / steps,
+            "phases_ms": {k: round(ms / c, 4) for k, (ms, c) in prof.items() if c}, "step_flags": flags}
 
 
 def musl2d_leg(steps, warmup, cells=256):
@@ -258,7 +310,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="serial CPU baseline steps")
+    ap.add_argument("--cpu-omp-steps", type=int, default=5, help="OpenMP CPU baseline steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", choices=["reference", "binned"], default="reference",
                     help="P2G summation order: the reference's particle order (bit-identical) or bin order")
@@ -267,6 +320,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--adjoint-steps", type=int, default=50, help="0 skips the adjoint leg")
     ap.add_argument("--musl2d-steps", type=int, default=50, help="0 skips the 2D MUSL leg")
+    ap.add_argument("--tubes-steps", type=int, default=100, help="0 skips the CPDI tube-array leg")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -361,8 +415,10 @@ def main():
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if args.musl2d_steps > 0:
         out["musl2d"] = musl2d_leg(args.musl2d_steps, 3)
+    if args.tubes_steps > 0:
+        out["tubes"] = tubes_leg(args.tubes_steps, 2)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_omp_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -10,6 +10,7 @@
 #include <new>
 
 #include "mpm_oracle.hpp"
+#include "omp_step.hpp"
 
 using orc::i64;
 typedef orc::Ctx<double> OCtx;
@@ -154,6 +155,20 @@ int orc_step(orc_ctx* h, int64_t nsteps, double* t) {
     else if (g.dim == 2) orc::step_usl_2d(g, *t);
     else if (g.contact) orc::step_musl_3d_contact(g);
     else orc::step_musl_3d(g);
+    *t += g.dt;
+  }
+  return g.flags ? JUMP_ERR_NUMERIC : JUMP_OK;
+}
+
+// The 3D MUSL step on nthreads OpenMP threads (omp_step.hpp: the CPU
+// baseline at many cores; equal to orc_step up to the rounding of the node
+// sums, which it forms block by block).  Other steps: JUMP_ERR_UNSUPPORTED.
+int orc_step_parallel(orc_ctx* h, int64_t nsteps, double* t, int nthreads) {
+  if (!h || !t || nsteps < 0 || nthreads < 1) return JUMP_ERR_ARG;
+  OCtx& g = h->c;
+  if (g.dim != 3 || g.contact) return JUMP_ERR_UNSUPPORTED;
+  for (int64_t s = 0; s < nsteps; s++) {
+    orc::step_musl_3d_omp(g, nthreads);
     *t += g.dt;
   }
   return g.flags ? JUMP_ERR_NUMERIC : JUMP_OK;

@@ -48,6 +48,8 @@ def load():
     lib.orc_stress.restype = ctypes.c_int
     lib.orc_stress.argtypes = [ctypes.c_int, ctypes.POINTER(_ffi.JumpMaterial)] + [ctypes.POINTER(ctypes.c_double)] * 3 + \
         [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.orc_step_parallel.restype = ctypes.c_int
+    lib.orc_step_parallel.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     lib.orc_det.restype = ctypes.c_double
     lib.orc_det.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     _LIB = lib
@@ -58,6 +60,15 @@ class OracleEngine(Engine):
     def __init__(self, params):
         lib = load()
         super().__init__(params, lib=lib, prefix="orc_", create=lib.orc_ctx_create)
+
+    def step_parallel(self, nsteps, t=0.0, threads=1):
+        """3D MUSL steps on `threads` OpenMP threads (the many-core CPU
+        baseline; node sums formed block by block)."""
+        tt = ctypes.c_double(t)
+        rc = self.lib.orc_step_parallel(self.h, int(nsteps), ctypes.byref(tt), int(threads))
+        if rc not in (0, _ffi.JUMP_ERR_NUMERIC):
+            self._check(rc)
+        return tt.value, rc
 
     def support(self, s, p, grad=True):
         nd = self.dim
