@@ -116,10 +116,14 @@ def batched_leg(n_inst, steps, warmup):
     linear or GIMP, grids 64..512 nodes wide, 5k..80k particles each).
     plain: every instance in a context of its own, stepped one after the
     other (one solve per problem, as the reference runs them); batched: all
-    of them in one jump_batch.  Same steps, inputs resident in HBM."""
+    of them in one jump_batch, with its step captured in HIP graphs and the
+    gather's launch shape chosen by the autotuner (the default), and again
+    launched eagerly in the default shape.  Same steps, inputs resident in
+    HBM."""
+    import numpy as np
     import torch
 
-    from jump_amd import scenes
+    from jump_amd import _ffi, scenes
     from jump_amd.engine import Batch, Engine
 
     scs = [scenes.two_rings_instance(1000 + i, steps=steps) for i in range(n_inst)]
@@ -140,27 +144,42 @@ def batched_leg(n_inst, steps, warmup):
     for e in plain:
         flags |= e.flags()
         e.close()
-    members = []
-    for sc in scs:
-        e = Engine(sc.params, device=0)
-        sc.load(e)
-        members.append(e)
-    b = Batch(members)
-    b.step(warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    b.step(steps)
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
-    b.close()
-    for e in members:
-        flags |= e.flags()
-        e.close()
+
+    def batched(graph, autotune):
+        members = []
+        for sc in scs:
+            e = Engine(sc.params, device=0)
+            sc.load(e)
+            members.append(e)
+        b = Batch(members)
+        b.options(graph=graph, autotune=autotune, shape=0)
+        b.step(warmup)  # the autotuner and the graph capture run here
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        tun = b.tuning() if autotune else []
+        b.close()
+        fl = 0
+        for e in members:
+            fl |= e.flags()
+            e.close()
+        return t, tun, fl
+
+    t_eager, _, f1 = batched(False, False)
+    t_batch, tun, f2 = batched(True, True)
     return {"workload": f"{n_inst} two-rings instances (seeds 1000..{1000 + n_inst - 1}), 2D USL, linear/GIMP",
             "particles": npart, "steps": steps,
             "plain_sequential": {"value": npart * steps / t_plain, "ms_per_step": t_plain * 1e3 / steps},
-            "batched": {"value": npart * steps / t_batch, "ms_per_step": t_batch * 1e3 / steps},
-            "speedup": t_plain / t_batch, "unit": "particle-steps/s", "step_flags": flags}
+            "batched": {"value": npart * steps / t_batch, "ms_per_step": t_batch * 1e3 / steps,
+                        "launch": "HIP graph per group, autotuned gather shape"},
+            "batched_eager": {"value": npart * steps / t_eager, "ms_per_step": t_eager * 1e3 / steps,
+                              "launch": "kernel launches, default gather shape"},
+            "autotuner": [{"shape": _ffi.P2G_SHAPE_NAMES[sh], "ms_per_launch": dict(zip(_ffi.P2G_SHAPE_NAMES,
+                                                                                     np.round(ms, 4).tolist()))}
+                          for sh, ms in tun],
+            "speedup": t_plain / t_batch, "unit": "particle-steps/s", "step_flags": flags | f1 | f2}
 
 
 def tubes_leg(steps, warmup):
@@ -317,7 +336,7 @@ def main():
                     help="P2G summation order: the reference's particle order (bit-identical) or bin order")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--batch-instances", type=int, default=256, help="0 skips the batched leg")
-    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--adjoint-steps", type=int, default=50, help="0 skips the adjoint leg")
     ap.add_argument("--musl2d-steps", type=int, default=50, help="0 skips the 2D MUSL leg")
     ap.add_argument("--tubes-steps", type=int, default=100, help="0 skips the CPDI tube-array leg")

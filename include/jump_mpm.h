@@ -280,6 +280,24 @@ int jump_batch_energies(jump_batch* batch, double* ke, double* se);
 int jump_batch_destroy(jump_batch* batch);
 const char* jump_batch_last_error(const jump_batch* batch);
 
+/* Launches of a batch (no counterpart in the reference, which runs one
+ * problem per call).  graph != 0 (the default): each group's step sequence
+ * is captured once into a HIP graph (one step and a chunk of 8) and replayed.
+ * autotune != 0 (the default): at the first jump_batch_step the batched P2G
+ * gather is timed in each of the JUMP_P2G_SHAPES launch shapes on the
+ * batch's own particles and the fastest is kept; autotune == 0: every group
+ * uses `shape` (JUMP_ERR_ARG outside 0..JUMP_P2G_SHAPES-1).  Shapes: 0 active nodes
+ * compacted, 128 threads per workgroup; 1 compacted, 64; 2 compacted, 256;
+ * 3 every node in grid-row order, 128; 4 rows, 64; 5 rows, 256.  Every shape
+ * sums each node in the same order, so neither option changes any result.
+ * jump_batch_tuning reports group g's chosen shape and the measured ms per
+ * launch of each shape (ms: JUMP_P2G_SHAPES doubles, may be NULL) and the
+ * number of groups (members with the same basis and summation order, in
+ * order of first appearance); JUMP_ERR_STATE before the group is tuned. */
+#define JUMP_P2G_SHAPES 6
+int jump_batch_options(jump_batch* batch, int32_t graph, int32_t autotune, int32_t shape);
+int jump_batch_tuning(jump_batch* batch, int32_t group, int32_t* ngroups, int32_t* shape, double* ms);
+
 /* Reverse mode of the 2D USL step (the reference has no derivatives; this is
  * the pullback a ChainRulesCore rrule around solve_explicit_dynamics_2D,
  * SolveExplicitSolidMechanics2D.jl:335, would call).  Covers the linear and

@@ -36,6 +36,9 @@ BASIS_GIMP = 4
 ALG_USL = 0
 ALG_MUSL = 1
 
+P2G_SHAPES = 6  # JUMP_P2G_SHAPES
+P2G_SHAPE_NAMES = ["compact/128", "compact/64", "compact/256", "rows/128", "rows/64", "rows/256"]
+
 ORDER_REFERENCE = 0
 ORDER_BINNED = 1
 
@@ -169,6 +172,8 @@ def load():
                             ("jump_adjoint_interval", ctypes.c_int, [H, ctypes.POINTER(_i32)]),
                             ("jump_dump_lammps", ctypes.c_int, [H, ctypes.c_char_p, ctypes.c_char_p]),
                             ("jump_dump_wait", ctypes.c_int, [H]),
+                            ("jump_batch_options", ctypes.c_int, [H, _i32, _i32, _i32]),
+                            ("jump_batch_tuning", ctypes.c_int, [H, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), _pd]),
                             ("jump_energies_noalloc", ctypes.c_int,
                              [_i32, _i64, H, H, H, H, H, H, H])):
         f = getattr(lib, name)
@@ -187,5 +192,5 @@ EXPORTED = [
     "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
     "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies", "jump_batch_destroy",
     "jump_batch_last_error", "jump_adjoint_forward", "jump_adjoint_backward", "jump_adjoint_interval",
-    "jump_dump_lammps", "jump_dump_wait", "jump_energies_noalloc",
+    "jump_dump_lammps", "jump_dump_wait", "jump_energies_noalloc", "jump_batch_options", "jump_batch_tuning",
 ]

@@ -82,6 +82,8 @@ struct jump_ctx {
   uint8_t* d_fixed = nullptr;
   int cpdi_w = 0;
   bool grid_valid = false;
+  int alloc_gen = 0;    // bumped whenever the particle arrays are reallocated (captured graphs hold their addresses)
+  int p2g_shape = 0;    // batched gather launch shape (kP2GShapes), chosen by the autotuner
 
   // Batch instances (an internal context of jump_batch holds several 2D
   // problems; an ordinary context is one instance with grid G).  NN: nodes
@@ -156,6 +158,12 @@ static int set_err(jump_ctx* c, int code, const std::string& msg) {
     hipError_t e_ = (expr);                                                                    \
     if (e_ != hipSuccess)                                                                      \
       return batch_err(b, JUMP_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));   \
+  } while (0)
+
+#define HIP_TRY_CTX_LAST(c)                                                                        \
+  do {                                                                                             \
+    hipError_t e_ = hipGetLastError();                                                             \
+    if (e_ != hipSuccess) return set_err((c), JUMP_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e_)); \
   } while (0)
 
 static int nsym(int D) { return D == 2 ? 3 : 6; }
@@ -247,6 +255,7 @@ static int alloc_particles(jump_ctx* ctx) {
   HIP_TRY(hipMemsetAsync(ctx->d_sorted + n, 0xff, sizeof(unsigned long long), ctx->stream));
   P.n = int(ctx->np);
   ctx->dev_capacity = ctx->np;
+  ctx->alloc_gen++;
   return r;
 }
 
@@ -488,8 +497,39 @@ template <int D, int B, bool MULTI> static int bin_and_sort(jump_ctx* ctx, int* 
   return JUMP_OK;
 }
 
-// MULTI (a batch of 2D instances): the gather runs over the compacted active
-// nodes, since the instances' grids are mostly larger than their solids.
+// Launch shapes of the batched gather (P2G over the nodes of all instances):
+// the node order (active nodes compacted, or every node in grid-row order)
+// and the workgroup size.  Every shape sums each node in the same order, so
+// the choice changes the time only; the autotuner times them on the batch's
+// own particles (tune_batched_p2g).
+struct P2GShape {
+  bool compact;
+  int block;
+};
+constexpr P2GShape kP2GShapes[JUMP_P2G_SHAPES] = {{true, 128}, {true, 64}, {true, 256},
+                                                  {false, 128}, {false, 64}, {false, 256}};
+
+template <int B, int ORDER, bool COMPACT, int BLK> static void p2g_batched_as(jump_ctx* ctx) {
+  hipLaunchKernelGGL((k_p2g<2, B, 0, ORDER, COMPACT, true, BLK>), dim3((ctx->NN + BLK - 1) / BLK), dim3(BLK), 0,
+                     ctx->stream, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
+                     ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, ctx->prm.dt, ctx->bt(), 0.0,
+                     RunList{});
+}
+template <int B, int ORDER> static void p2g_batched(jump_ctx* ctx, int shape) {
+  static_assert(JUMP_P2G_SHAPES == 6, "one case per shape");
+  switch (shape) {
+    case 1: p2g_batched_as<B, ORDER, true, 64>(ctx); break;
+    case 2: p2g_batched_as<B, ORDER, true, 256>(ctx); break;
+    case 3: p2g_batched_as<B, ORDER, false, 128>(ctx); break;
+    case 4: p2g_batched_as<B, ORDER, false, 64>(ctx); break;
+    case 5: p2g_batched_as<B, ORDER, false, 256>(ctx); break;
+    default: p2g_batched_as<B, ORDER, true, 128>(ctx); break;
+  }
+}
+
+// MULTI (a batch of 2D instances): the gather runs over all instances' nodes
+// in the launch shape the autotuner chose (by default the compacted active
+// nodes, since the instances' grids are mostly larger than their solids).
 template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
@@ -497,9 +537,13 @@ template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   bin_and_sort<2, B, MULTI>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<2, B, 0, ORDER, MULTI, MULTI>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt(), 0.0,
-             RunList{});
+  if constexpr (MULTI && B != JUMP_BASIS_CPDI_Q4) {  // (batches refuse CPDI)
+    p2g_batched<B, ORDER>(ctx, ctx->p2g_shape);
+  } else {
+    LAUNCH_P2G((k_p2g<2, B, 0, ORDER, MULTI, MULTI>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+               ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt(), 0.0,
+               RunList{});
+  }
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   for (int r = 0; r <= ctx->max_rigid_rank; r++)
@@ -1407,17 +1451,107 @@ int jump_abi_version(void) { return JUMP_ABI_VERSION; }
 // works in that instance's grid.  Each member's particles keep their order
 // and each member's reductions their tree, so a member steps bit for bit as
 // it would alone.
+//
+// Launches.  Each group's step is captured once into a HIP graph (one step,
+// and a chunk of kGraphChunk steps) and replayed, so a step costs one graph
+// launch per group instead of about fifteen kernel launches; the graphs are
+// captured again if the group's particle arrays move.  Before the first step
+// the autotuner times the batched gather's launch shapes on the group's own
+// particles and keeps the fastest (kernel times: about 0.1 to 1 ms per
+// launch on 256 instances, where shapes differ by up to tens of percent).
+constexpr int kGraphChunk = 8;
 struct jump_batch {
   int device = 0;
   std::string err;
   std::vector<jump_ctx*> members;
+  bool use_graph = true, autotune = true;
   struct Group {
     jump_ctx* inner = nullptr;
     std::vector<int> members;        // member index of each instance
     std::vector<int64_t> part_off;   // first particle of each instance in the group
+    bool tuned = false;
+    double tune_ms[JUMP_P2G_SHAPES] = {};  // the autotuner's ms per launch of each shape
+    hipGraphExec_t exec1 = nullptr, execN = nullptr;  // one step, kGraphChunk steps
+    int graph_gen = -1, graph_shape = -1;              // what the graphs were captured with
   };
   std::vector<Group> groups;
 };
+
+static void drop_graphs(jump_batch::Group& g) {
+  if (g.exec1) (void)hipGraphExecDestroy(g.exec1);
+  if (g.execN) (void)hipGraphExecDestroy(g.execN);
+  g.exec1 = g.execN = nullptr;
+  g.graph_gen = g.graph_shape = -1;
+}
+
+// Capture nsteps steps of the group's launch sequence (no host
+// synchronisation, memset or allocation happens inside a 2D USL step).
+static int capture_steps(jump_ctx* in, int64_t nsteps, hipGraphExec_t* out) {
+  hipGraph_t graph = nullptr;
+  if (hipStreamBeginCapture(in->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return set_err(in, JUMP_ERR_DEVICE, "hipStreamBeginCapture failed");
+  const int r = run_steps(in, nsteps);
+  const hipError_t e = hipStreamEndCapture(in->stream, &graph);
+  if (r || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r ? r : set_err(in, JUMP_ERR_DEVICE, "hipStreamEndCapture failed");
+  }
+  const hipError_t e2 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e2 != hipSuccess) return set_err(in, JUMP_ERR_DEVICE, "hipGraphInstantiate failed");
+  return JUMP_OK;
+}
+
+// The autotuner: bin once, then time each batched gather shape (one warm-up
+// launch and kTuneReps timed ones, HIP events on the group's stream) and
+// keep the fastest.  The gather only writes the grid, which the next step
+// rebuilds, so the particle state is untouched.
+constexpr int kTuneReps = 3;
+template <int B, int ORDER> static int tune_batched_p2g(jump_ctx* in, double* ms) {
+  bin_and_sort<2, B, true>(in);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return set_err(in, JUMP_ERR_DEVICE, "hipEventCreate failed");
+  }
+  int best = 0, r = JUMP_OK;
+  for (int sh = 0; sh < JUMP_P2G_SHAPES && !r; sh++) {
+    p2g_batched<B, ORDER>(in, sh);
+    float t = 0.0f;
+    if (hipEventRecord(e0, in->stream) != hipSuccess) r = JUMP_ERR_DEVICE;
+    for (int k = 0; k < kTuneReps && !r; k++) p2g_batched<B, ORDER>(in, sh);
+    if (!r && (hipEventRecord(e1, in->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+               hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      r = JUMP_ERR_DEVICE;
+    ms[sh] = double(t) / kTuneReps;
+    if (ms[sh] < ms[best]) best = sh;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (r) return set_err(in, r, "autotuner timing failed");
+  in->p2g_shape = best;
+  HIP_TRY_CTX_LAST(in);
+  return JUMP_OK;
+}
+static int tune_group(jump_ctx* in, double* ms) {
+  const bool ref = in->prm.order == JUMP_ORDER_REFERENCE;
+  switch (in->prm.basis) {
+    case JUMP_BASIS_LINEAR:
+      return ref ? tune_batched_p2g<JUMP_BASIS_LINEAR, JUMP_ORDER_REFERENCE>(in, ms)
+                 : tune_batched_p2g<JUMP_BASIS_LINEAR, JUMP_ORDER_BINNED>(in, ms);
+    case JUMP_BASIS_QUAD_BSPLINE:
+      return ref ? tune_batched_p2g<JUMP_BASIS_QUAD_BSPLINE, JUMP_ORDER_REFERENCE>(in, ms)
+                 : tune_batched_p2g<JUMP_BASIS_QUAD_BSPLINE, JUMP_ORDER_BINNED>(in, ms);
+    case JUMP_BASIS_CUBIC_BSPLINE:
+      return ref ? tune_batched_p2g<JUMP_BASIS_CUBIC_BSPLINE, JUMP_ORDER_REFERENCE>(in, ms)
+                 : tune_batched_p2g<JUMP_BASIS_CUBIC_BSPLINE, JUMP_ORDER_BINNED>(in, ms);
+    case JUMP_BASIS_GIMP:
+      return ref ? tune_batched_p2g<JUMP_BASIS_GIMP, JUMP_ORDER_REFERENCE>(in, ms)
+                 : tune_batched_p2g<JUMP_BASIS_GIMP, JUMP_ORDER_BINNED>(in, ms);
+    default:
+      return set_err(in, JUMP_ERR_UNSUPPORTED, "no batched gather for this basis");
+  }
+}
 
 static int batch_err(jump_batch* b, int code, const std::string& m) {
   b->err = m;
@@ -1428,7 +1562,12 @@ const char* jump_batch_last_error(const jump_batch* b) { return b ? b->err.c_str
 
 int jump_batch_destroy(jump_batch* b) {
   if (!b) return JUMP_OK;
-  for (auto& g : b->groups) jump_ctx_destroy(g.inner);
+  (void)hipSetDevice(b->device);
+  for (auto& g : b->groups) {
+    if (g.inner && g.inner->stream) (void)hipStreamSynchronize(g.inner->stream);
+    drop_graphs(g);
+    jump_ctx_destroy(g.inner);
+  }
   delete b;
   return JUMP_OK;
 }
@@ -1550,7 +1689,28 @@ int jump_batch_step(jump_batch* b, int64_t nsteps, double* t_inout) {
   for (auto& grp : b->groups) {  // every group's launches are queued before any wait
     jump_ctx* in = grp.inner;
     int r = ensure_device(in);
-    if (!r) r = run_steps(in, nsteps);
+    if (!r && b->autotune && !grp.tuned && nsteps > 0) {
+      r = tune_group(in, grp.tune_ms);
+      grp.tuned = r == JUMP_OK;
+    }
+    if (!r && b->use_graph && nsteps > 0) {
+      if (grp.graph_gen != in->alloc_gen || grp.graph_shape != in->p2g_shape) {
+        drop_graphs(grp);
+        r = capture_steps(in, 1, &grp.exec1);
+        if (!r) r = capture_steps(in, kGraphChunk, &grp.execN);
+        if (!r) {
+          grp.graph_gen = in->alloc_gen;
+          grp.graph_shape = in->p2g_shape;
+        }
+      }
+      int64_t k = 0;
+      for (; !r && k + kGraphChunk <= nsteps; k += kGraphChunk)
+        if (hipGraphLaunch(grp.execN, in->stream) != hipSuccess) r = set_err(in, JUMP_ERR_DEVICE, "hipGraphLaunch");
+      for (; !r && k < nsteps; k++)
+        if (hipGraphLaunch(grp.exec1, in->stream) != hipSuccess) r = set_err(in, JUMP_ERR_DEVICE, "hipGraphLaunch");
+    } else if (!r) {
+      r = run_steps(in, nsteps);
+    }
     if (r) return batch_err(b, r, in->err);
   }
   for (auto& grp : b->groups) {
@@ -1613,6 +1773,32 @@ int jump_batch_energies(jump_batch* b, double* ke, double* se) {
       if (r) return batch_err(b, r, in->err);
     }
   }
+  return JUMP_OK;
+}
+
+int jump_batch_options(jump_batch* b, int32_t graph, int32_t autotune, int32_t shape) {
+  if (!b) return JUMP_ERR_ARG;
+  if (!autotune && (shape < 0 || shape >= JUMP_P2G_SHAPES)) return batch_err(b, JUMP_ERR_ARG, "bad gather shape");
+  b->use_graph = graph != 0;
+  b->autotune = autotune != 0;
+  for (auto& grp : b->groups) {
+    if (!b->autotune) {
+      grp.inner->p2g_shape = shape;
+      grp.tuned = false;
+    }
+  }
+  return JUMP_OK;
+}
+
+int jump_batch_tuning(jump_batch* b, int32_t g, int32_t* ngroups, int32_t* shape, double* ms) {
+  if (!b) return JUMP_ERR_ARG;
+  if (ngroups) *ngroups = int32_t(b->groups.size());
+  if (g < 0 || size_t(g) >= b->groups.size()) return batch_err(b, JUMP_ERR_ARG, "bad group index");
+  const auto& grp = b->groups[size_t(g)];
+  if (!grp.tuned) return batch_err(b, JUMP_ERR_STATE, "the group has not been tuned (no step with autotune on)");
+  if (shape) *shape = grp.inner->p2g_shape;
+  if (ms)
+    for (int k = 0; k < JUMP_P2G_SHAPES; k++) ms[k] = grp.tune_ms[k];
   return JUMP_OK;
 }
 

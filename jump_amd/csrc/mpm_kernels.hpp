@@ -496,12 +496,12 @@ struct RunList {
   int gen;           // this step's stamp
   int all;           // every node merges (the first step after an upload)
 };
-template <int D, int B, int ORDER, bool DEFER, class Visit, class OnRun = NoRuns>
+template <int D, int B, int ORDER, bool DEFER, int BLK, class Visit, class OnRun = NoRuns>
 __device__ __forceinline__ void for_window_particles(const Particles& P, const GridDesc& G, const int* start,
                                                      const unsigned long long* sorted,
                                                      const unsigned long long* bxyz, int off, int inst, int W,
-                                                     int ni, int nj, int nk, int (*s_cur)[kP2GBlock],
-                                                     int (*s_head)[kP2GBlock], Visit&& visit,
+                                                     int ni, int nj, int nk, int (*s_cur)[BLK],
+                                                     int (*s_head)[BLK], Visit&& visit,
                                                      OnRun&& on_run = OnRun{}) {
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
@@ -589,8 +589,10 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
   }
 }
 
-template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI>
-__global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
+// BLK: threads (nodes) per workgroup; the batched step lets the autotuner
+// choose it (the single-problem steps use kP2GBlock).
+template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI, int BLK = kP2GBlock>
+__global__ void __launch_bounds__(BLK) k_p2g(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
                                                    const double* __restrict__ corners, const int* __restrict__ start0,
                                                    const unsigned long long* __restrict__ sorted,
                                                    const unsigned long long* __restrict__ bxyz,
@@ -601,8 +603,8 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
   // Reference order: per window bin, the position in `sorted` of its next
   // unconsumed particle and that particle (kNoParticle when exhausted).
-  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
-  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
   const int t = threadIdx.x;
   const int NN = MULTI ? Bt.NN : G0.nn;
   // XCD: the 3D re-projection pass deals its (compacted) blocks to the XCDs
@@ -616,11 +618,11 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g(Particles P, GridDesc G0, con
   // sit at the top of the plate, the highest node ids; started last they
   // set the kernel's tail.  Both grids have exactly nlog (logical) blocks.
   constexpr bool REV = D == 3 && !MULTI;
-  const int nlog = (NN + kP2GBlock - 1) / kP2GBlock;
+  const int nlog = (NN + BLK - 1) / BLK;
   const int L0 = XCD ? xcd_block(nlog) : int(blockIdx.x);
   if (L0 < 0) return;
   const int L = REV ? nlog - 1 - L0 : L0;
-  const int i = L * kP2GBlock + t;
+  const int i = L * BLK + t;
   if (i >= NN) return;
   const int ng = COMPACT ? ldu(order, unsigned(i)) : i;  // node in the (concatenated) node arrays
   constexpr bool RUNS = D == 3 && !MULTI && ORDER == JUMP_ORDER_REFERENCE;

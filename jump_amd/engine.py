@@ -250,6 +250,25 @@ class Batch:
     def sync(self):
         self._check(self.lib.jump_batch_sync(self.h))
 
+    def options(self, graph=True, autotune=True, shape=0):
+        """HIP-graph replay of the step and the gather-shape autotuner (both
+        on by default; neither changes results); without the autotuner every
+        group uses `shape` (_ffi.P2G_SHAPE_NAMES)."""
+        self._check(self.lib.jump_batch_options(self.h, 1 if graph else 0, 1 if autotune else 0, int(shape)))
+
+    def tuning(self):
+        """The autotuner's record: per group, (chosen shape, ms per launch of
+        each shape)."""
+        ng = ctypes.c_int32()
+        self.lib.jump_batch_tuning(self.h, 0, ctypes.byref(ng), None, None)
+        out = []
+        for g in range(ng.value):
+            sh = ctypes.c_int32()
+            ms = np.zeros(_ffi.P2G_SHAPES)
+            self._check(self.lib.jump_batch_tuning(self.h, g, None, ctypes.byref(sh), _ffi.dptr(ms)))
+            out.append((sh.value, ms))
+        return out
+
     def energies(self):
         ke, se = np.zeros(len(self.engines)), np.zeros(len(self.engines))
         self._check(self.lib.jump_batch_energies(self.h, _ffi.dptr(ke), _ffi.dptr(se)))

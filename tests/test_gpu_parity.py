@@ -428,6 +428,61 @@ def test_batch_members_step_as_alone(hip_lib):
     b.close()
 
 
+def test_batch_graph_replay_and_gather_shapes_equal_eager(hip_lib):
+    """HIP-graph replay (one step and chunks of 8) and every launch shape of
+    the batched gather leave each member bitwise equal to the eager batch in
+    the default shape, through calls of 3, 8, 1 and 12 steps with energies
+    and state reads between them; the autotuner keeps the shape it measured
+    fastest."""
+    from jump_amd.engine import Batch
+    _say("batch graphs and gather shapes")
+    scs = [scenes.two_rings_instance(1000 + i, steps=24) for i in (0, 3, 5, 7)]
+    scs.append(scenes.rings2d(cells=24, basis=_ffi.BASIS_LINEAR, material="j2", steps=24))
+    scs.append(scenes.block2d(cells=12, basis=_ffi.BASIS_QUAD_BSPLINE, material="elastic", steps=24))
+
+    def make():
+        ms = []
+        for sc in scs:
+            m = Engine(sc.params)
+            sc.load(m)
+            ms.append(m)
+        return ms, Batch(ms)
+
+    ref_members, ref = make()
+    ref.options(graph=False, autotune=False, shape=0)
+    variants = [("graph+autotune", dict())] + \
+        [(f"graph shape {k}", dict(graph=True, autotune=False, shape=k)) for k in range(_ffi.P2G_SHAPES)] + \
+        [("eager shape 5", dict(graph=False, autotune=False, shape=5))]
+    runs = []
+    for name, kw in variants:
+        members, b = make()
+        if kw:
+            b.options(**kw)
+        runs.append((name, members, b))
+    for n in (3, 8, 1, 12):
+        ref.step(n)
+        kr = ref.energies()
+        ref.sync()
+        for name, members, b in runs:
+            b.step(n)
+            assert all(np.array_equal(x, y) for x, y in zip(b.energies(), kr)), (name, n)
+            b.sync()
+            for i, (a, m) in enumerate(zip(ref_members, members)):
+                for s_ in range(len(scs[i].solids)):
+                    sa, sm = a.get_solid(s_), m.get_solid(s_)
+                    for k in sa:
+                        assert np.array_equal(sa[k], sm[k]), (name, n, i, k)
+    tun = runs[0][2].tuning()
+    assert len(tun) == 3  # linear, GIMP and quadratic B-spline groups
+    for shape, ms in tun:
+        print("autotuner", _ffi.P2G_SHAPE_NAMES[shape], np.round(ms, 4))
+        assert np.all(ms > 0.0) and ms[shape] == ms.min()
+    for _, members, b in runs + [("ref", ref_members, ref)]:
+        b.close()
+        for m in members:
+            m.close()
+
+
 # ---------------------------------------------------------------- contact
 CONTACT_CASES = [(b, m) for b in (_ffi.BASIS_LINEAR, _ffi.BASIS_GIMP) for m in ("elastic", "neohookean", "j2")] + \
     [(_ffi.BASIS_QUAD_BSPLINE, "elastic"), (_ffi.BASIS_CUBIC_BSPLINE, "elastic")]
