@@ -293,7 +293,9 @@ const char* jump_batch_last_error(const jump_batch* batch);
  * jump_batch_tuning reports group g's chosen shape and the measured ms per
  * launch of each shape (ms: JUMP_P2G_SHAPES doubles, may be NULL) and the
  * number of groups (members with the same basis and summation order, in
- * order of first appearance); JUMP_ERR_STATE before the group is tuned. */
+ * order of first appearance); JUMP_ERR_STATE before the group is tuned.  A
+ * group of one member steps as a single context and has no batched gather:
+ * it is not tuned, and its shape is reported as -1. */
 #define JUMP_P2G_SHAPES 6
 int jump_batch_options(jump_batch* batch, int32_t graph, int32_t autotune, int32_t shape);
 int jump_batch_tuning(jump_batch* batch, int32_t group, int32_t* ngroups, int32_t* shape, double* ms);

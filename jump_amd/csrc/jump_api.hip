@@ -1508,6 +1508,10 @@ static int capture_steps(jump_ctx* in, int64_t nsteps, hipGraphExec_t* out) {
 // rebuilds, so the particle state is untouched.
 constexpr int kTuneReps = 3;
 template <int B, int ORDER> static int tune_batched_p2g(jump_ctx* in, double* ms) {
+  // the batched kernels read the instance table, which only a group of two
+  // or more members has (a group of one steps as a single context)
+  if (in->ninst < 2 || !in->d_inst || !in->d_node_off)
+    return set_err(in, JUMP_ERR_STATE, "only a group of several instances has a batched gather to tune");
   bin_and_sort<2, B, true>(in);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1689,7 +1693,7 @@ int jump_batch_step(jump_batch* b, int64_t nsteps, double* t_inout) {
   for (auto& grp : b->groups) {  // every group's launches are queued before any wait
     jump_ctx* in = grp.inner;
     int r = ensure_device(in);
-    if (!r && b->autotune && !grp.tuned && nsteps > 0) {
+    if (!r && b->autotune && !grp.tuned && nsteps > 0 && in->ninst > 1) {
       r = tune_group(in, grp.tune_ms);
       grp.tuned = r == JUMP_OK;
     }
@@ -1795,6 +1799,12 @@ int jump_batch_tuning(jump_batch* b, int32_t g, int32_t* ngroups, int32_t* shape
   if (ngroups) *ngroups = int32_t(b->groups.size());
   if (g < 0 || size_t(g) >= b->groups.size()) return batch_err(b, JUMP_ERR_ARG, "bad group index");
   const auto& grp = b->groups[size_t(g)];
+  if (grp.inner->ninst < 2) {  // one member: stepped as a single context, no batched gather
+    if (shape) *shape = -1;
+    if (ms)
+      for (int k = 0; k < JUMP_P2G_SHAPES; k++) ms[k] = 0.0;
+    return JUMP_OK;
+  }
   if (!grp.tuned) return batch_err(b, JUMP_ERR_STATE, "the group has not been tuned (no step with autotune on)");
   if (shape) *shape = grp.inner->p2g_shape;
   if (ms)

@@ -473,8 +473,11 @@ def test_batch_graph_replay_and_gather_shapes_equal_eager(hip_lib):
                     for k in sa:
                         assert np.array_equal(sa[k], sm[k]), (name, n, i, k)
     tun = runs[0][2].tuning()
-    assert len(tun) == 3  # linear, GIMP and quadratic B-spline groups
-    for shape, ms in tun:
+    assert len(tun) == 3  # GIMP, linear and quadratic B-spline groups
+    for g, (shape, ms) in enumerate(tun):
+        if g == 2:  # the quadratic B-spline group has one member: a single context, nothing to tune
+            assert shape == -1 and np.all(ms == 0.0)
+            continue
         print("autotuner", _ffi.P2G_SHAPE_NAMES[shape], np.round(ms, 4))
         assert np.all(ms > 0.0) and ms[shape] == ms.min()
     for _, members, b in runs + [("ref", ref_members, ref)]:
