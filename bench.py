@@ -111,7 +111,7 @@ def copy_probe(mib, reps=5):
     print(json.dumps({"copy_probe_MiB": mib, "launches": reps}), flush=True)
 
 
-def batched_leg(n_inst, steps, warmup):
+def batched_leg(n_inst, steps, warmup, draw):
     """The batched workload: n_inst seeded two-rings instances (2D USL,
     linear or GIMP, grids 64..512 nodes wide, 5k..80k particles each).
     plain: every instance in a context of its own, stepped one after the
@@ -126,7 +126,7 @@ def batched_leg(n_inst, steps, warmup):
     from jump_amd import _ffi, scenes
     from jump_amd.engine import Batch, Engine
 
-    scs = [scenes.two_rings_instance(1000 + i, steps=steps) for i in range(n_inst)]
+    scs = [scenes.two_rings_instance(1000 + i, steps=steps, draw=draw) for i in range(n_inst)]
     npart = sum(sc.n_particles for sc in scs)
     plain = []
     for sc in scs:
@@ -169,7 +169,9 @@ def batched_leg(n_inst, steps, warmup):
 
     t_eager, _, f1 = batched(False, False)
     t_batch, tun, f2 = batched(True, True)
-    return {"workload": f"{n_inst} two-rings instances (seeds 1000..{1000 + n_inst - 1}), 2D USL, linear/GIMP",
+    what = "GIMP, 1..10 mm/s" if draw == "stable" else "linear or GIMP, 10..100 mm/s"
+    return {"workload": f"{n_inst} two-rings instances (seeds 1000..{1000 + n_inst - 1}, draw {draw}: {what}), "
+                        "2D USL, Neo-Hookean",
             "particles": npart, "steps": steps,
             "plain_sequential": {"value": npart * steps / t_plain, "ms_per_step": t_plain * 1e3 / steps},
             "batched": {"value": npart * steps / t_batch, "ms_per_step": t_batch * 1e3 / steps,
@@ -337,6 +339,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--batch-instances", type=int, default=256, help="0 skips the batched leg")
     ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--batch-draw", choices=["stable", "mixed"], default="stable",
+                    help="instance draw of the batched leg (scenes.two_rings_instance)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="0 skips the adjoint leg")
     ap.add_argument("--musl2d-steps", type=int, default=50, help="0 skips the 2D MUSL leg")
     ap.add_argument("--tubes-steps", type=int, default=100, help="0 skips the CPDI tube-array leg")
@@ -429,7 +433,7 @@ def main():
     sys.stdout.flush()
     eng.close()
     if args.batch_instances > 0:
-        out["batched"] = batched_leg(args.batch_instances, args.batch_steps, 2)
+        out["batched"] = batched_leg(args.batch_instances, args.batch_steps, 2, args.batch_draw)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if args.musl2d_steps > 0:
