@@ -59,24 +59,29 @@ template <class T> struct MatConst {
   T vx, vy;
 };
 
-// Material.jl:54-57, 112-113, 159-161, 313-325
-template <class T> MatConst<T> make_material(const jump_material& in) {
-  MatConst<T> c;
-  c.kind = in.kind;
-  c.fixed = in.fixed != 0;
-  c.E = in.E; c.nu = in.nu; c.density = in.density;
+// Material.jl:54-57, 112-113, 159-161, 313-325: the constants derived from
+// E, nu and the density (separate so that a dual-number instantiation can
+// seed E and nu and derive the rest)
+template <class T> void derive_constants(MatConst<T>& c) {
   c.lambda = c.E * c.nu / (1.0 + c.nu) / (1.0 - 2.0 * c.nu);
   c.mu = c.E / 2.0 / (1.0 + c.nu);
   c.kappa = c.lambda + c.mu;
-  c.fy = in.fy; c.k1 = in.k1;
-  c.A = in.A; c.B = in.B; c.C = in.C; c.n = in.n; c.epsdot0 = in.epsdot0;
-  c.m = in.m; c.chi = in.chi; c.Cp = in.Cp; c.cellsize = in.cellsize;
-  if (in.kind == JUMP_MAT_JOHNSONCOOK) {
+  if (c.kind == JUMP_MAT_JOHNSONCOOK) {
     c.velo = sqrt((c.lambda + 2.0 * c.mu) / c.density);
     c.delta = c.chi / (c.density * c.Cp);
   } else {
     c.velo = 0.0; c.delta = 0.0;
   }
+}
+template <class T> MatConst<T> make_material(const jump_material& in) {
+  MatConst<T> c;
+  c.kind = in.kind;
+  c.fixed = in.fixed != 0;
+  c.E = in.E; c.nu = in.nu; c.density = in.density;
+  c.fy = in.fy; c.k1 = in.k1;
+  c.A = in.A; c.B = in.B; c.C = in.C; c.n = in.n; c.epsdot0 = in.epsdot0;
+  c.m = in.m; c.chi = in.chi; c.Cp = in.Cp; c.cellsize = in.cellsize;
+  derive_constants(c);
   c.vx = in.vx; c.vy = in.vy;
   return c;
 }

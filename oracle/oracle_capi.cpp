@@ -11,6 +11,7 @@
 
 #include "mpm_oracle.hpp"
 #include "omp_step.hpp"
+#include "dual.hpp"
 
 using orc::i64;
 typedef orc::Ctx<double> OCtx;
@@ -172,6 +173,84 @@ int orc_step_parallel(orc_ctx* h, int64_t nsteps, double* t, int nthreads) {
     *t += g.dt;
   }
   return g.flags ? JUMP_ERR_NUMERIC : JUMP_OK;
+}
+
+// ForwardDiff-style gradient of the CPU path.  For the linear functional
+// L = sum_p pos_bar_p . x_p + vel_bar_p . v_p of the state after nsteps steps
+// (all solids' particles in order, AoS), the derivatives of L with respect
+// to solid `solid`'s E, nu, yield stress fy, hardening k1 and the components
+// of a uniform change of its initial velocity (grad: 4 + dim values), from a
+// run of the restatement on dual numbers seeded on those parameters.  The
+// context itself is not stepped.
+}  // extern "C"
+typedef orc::Dual<7> D7;
+static void to_dual(const orc::MatConst<double>& a, orc::MatConst<D7>& b) {
+  b.kind = a.kind; b.fixed = a.fixed;
+  b.E = a.E; b.nu = a.nu; b.density = a.density; b.lambda = a.lambda; b.mu = a.mu; b.kappa = a.kappa;
+  b.fy = a.fy; b.k1 = a.k1; b.A = a.A; b.B = a.B; b.C = a.C; b.n = a.n; b.epsdot0 = a.epsdot0; b.m = a.m;
+  b.chi = a.chi; b.Cp = a.Cp; b.cellsize = a.cellsize; b.velo = a.velo; b.delta = a.delta; b.vx = a.vx; b.vy = a.vy;
+}
+template <class V> static std::vector<D7> dvec(const std::vector<V>& a) { return std::vector<D7>(a.begin(), a.end()); }
+extern "C" {
+
+int orc_dual_gradient(orc_ctx* h, int32_t solid, int64_t nsteps, const double* pos_bar, const double* vel_bar,
+                      double* grad) {
+  if (!h || !pos_bar || !grad || nsteps < 0) return JUMP_ERR_ARG;
+  const OCtx& g = h->c;
+  if (solid < 0 || size_t(solid) >= g.solids.size()) return fail(h, JUMP_ERR_ARG, "bad solid index");
+  if (g.contact || (g.dim == 2 && g.alg == JUMP_ALG_MUSL)) return fail(h, JUMP_ERR_UNSUPPORTED, "dual run: USL 2D, MUSL 3D");
+  orc::Ctx<D7> q;
+  q.dim = g.dim; q.basis = g.basis; q.alg = g.alg; q.ghostcell = g.ghostcell;
+  q.tol = g.tol; q.alpha = g.alpha; q.dt = g.dt;
+  for (int k = 0; k < 3; k++) q.body[k] = g.body[k];
+  q.nx = g.nx; q.ny = g.ny; q.nz = g.nz; q.nxy = g.nxy; q.nn = g.nn;
+  q.xmin = g.xmin; q.ymin = g.ymin; q.zmin = g.zmin;
+  q.dx = g.dx; q.dy = g.dy; q.dz = g.dz; q.dxI = g.dxI; q.dyI = g.dyI; q.dzI = g.dzI;
+  q.gmass = dvec(g.gmass); q.gp0 = dvec(g.gp0); q.gp = dvec(g.gp); q.gp2 = dvec(g.gp2); q.gf = dvec(g.gf);
+  q.fixed = g.fixed;
+  q.near = g.near;
+  q.funcs = dvec(g.funcs);
+  q.ders = dvec(g.ders);
+  const int d = g.dim;
+  for (size_t si = 0; si < g.solids.size(); si++) {
+    const auto& a = g.solids[si];
+    orc::Solid<D7> b;
+    b.n = a.n; b.rigid = a.rigid;
+    to_dual(a.mat, b.mat);
+    b.mass = dvec(a.mass); b.vol0 = dvec(a.vol0); b.vol = dvec(a.vol); b.pos = dvec(a.pos); b.vel = dvec(a.vel);
+    b.F = dvec(a.F); b.strain = dvec(a.strain); b.stress = dvec(a.stress);
+    b.alpha = dvec(a.alpha); b.alphadot = dvec(a.alphadot); b.vm = dvec(a.vm); b.pstrain = dvec(a.pstrain);
+    b.corners = dvec(a.corners); b.elems = a.elems;
+    for (int k = 0; k < 3; k++) b.lp[k] = a.lp[k];
+    if (int(si) == solid) {  // seeds: E, nu, fy, k1, v0
+      b.mat.E.d[0] = 1.0;
+      b.mat.nu.d[1] = 1.0;
+      orc::derive_constants(b.mat);
+      b.mat.fy.d[2] = 1.0;
+      b.mat.k1.d[3] = 1.0;
+      for (orc::i64 ip = 0; ip < b.n; ip++)
+        for (int k = 0; k < d; k++) b.vel[size_t(d * ip + k)].d[4 + k] = 1.0;
+    }
+    q.solids.push_back(std::move(b));
+  }
+  for (int64_t s = 0; s < nsteps; s++) {
+    if (d == 2) orc::step_usl_2d(q, D7(0.0));
+    else orc::step_musl_3d(q);
+  }
+  for (int k = 0; k < 4 + d; k++) grad[k] = 0.0;
+  size_t off = 0;
+  for (const auto& b : q.solids) {
+    for (orc::i64 ip = 0; ip < b.n; ip++)
+      for (int c = 0; c < d; c++) {
+        const D7& x = b.pos[size_t(d * ip + c)];
+        const D7& v = b.vel[size_t(d * ip + c)];
+        const double pb = pos_bar[d * (off + size_t(ip)) + size_t(c)];
+        const double vb = vel_bar ? vel_bar[d * (off + size_t(ip)) + size_t(c)] : 0.0;
+        for (int k = 0; k < 4 + d; k++) grad[k] += pb * x.d[k] + vb * v.d[k];
+      }
+    off += size_t(b.n);
+  }
+  return q.flags ? JUMP_ERR_NUMERIC : JUMP_OK;
 }
 
 int orc_flags(orc_ctx* h, uint32_t* flags) {

@@ -48,6 +48,9 @@ def load():
     lib.orc_stress.restype = ctypes.c_int
     lib.orc_stress.argtypes = [ctypes.c_int, ctypes.POINTER(_ffi.JumpMaterial)] + [ctypes.POINTER(ctypes.c_double)] * 3 + \
         [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.orc_dual_gradient.restype = ctypes.c_int
+    lib.orc_dual_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.orc_step_parallel.restype = ctypes.c_int
     lib.orc_step_parallel.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     lib.orc_det.restype = ctypes.c_double
@@ -69,6 +72,20 @@ class OracleEngine(Engine):
         if rc not in (0, _ffi.JUMP_ERR_NUMERIC):
             self._check(rc)
         return tt.value, rc
+
+    def dual_gradient(self, solid, nsteps, pos_bar, vel_bar=None):
+        """dL/d(E, nu, fy, k1, v0) of solid `solid` for L = sum pos_bar . x +
+        vel_bar . v after nsteps steps from the current state, by the
+        restatement run on dual numbers (the context is not stepped)."""
+        d = self.dim
+        n = int(sum(self.counts))
+        pb = np.ascontiguousarray(pos_bar, dtype=np.float64).reshape(n, d)
+        vb = None if vel_bar is None else np.ascontiguousarray(vel_bar, dtype=np.float64).reshape(n, d)
+        g = np.zeros(4 + d)
+        rc = self.lib.orc_dual_gradient(self.h, int(solid), int(nsteps), _ffi.dptr(pb), _ffi.dptr(vb), _ffi.dptr(g))
+        if rc not in (0, _ffi.JUMP_ERR_NUMERIC):
+            self._check(rc)
+        return {"E": g[0], "nu": g[1], "fy": g[2], "k1": g[3], "v0": g[4:].copy()}
 
     def support(self, s, p, grad=True):
         nd = self.dim
