@@ -300,30 +300,36 @@ const char* jump_batch_last_error(const jump_batch* batch);
 int jump_batch_options(jump_batch* batch, int32_t graph, int32_t autotune, int32_t shape);
 int jump_batch_tuning(jump_batch* batch, int32_t group, int32_t* ngroups, int32_t* shape, double* ms);
 
-/* Reverse mode of the 2D USL step (the reference has no derivatives; this is
- * the pullback a ChainRulesCore rrule around solve_explicit_dynamics_2D,
- * SolveExplicitSolidMechanics2D.jl:335, would call).  Covers the linear and
- * GIMP bases and Hooke / Neo-Hookean solids (else JUMP_ERR_UNSUPPORTED).
- * jump_adjoint_forward runs nsteps steps like jump_step and records the
- * particle state every K steps.  jump_adjoint_backward takes the adjoint of
- * the final positions (pos_bar, AoS n x 2, required) and velocities
- * (vel_bar, may be NULL) and writes, per solid, dL/dE, dL/dnu and dL/dv0
- * (the derivative with respect to a uniform change of that solid's initial
- * velocity, 2 per solid); outputs may be NULL.  Each step of the sweep
- * recomputes its states from the nearest snapshot (checkpoint and
- * recompute); the context is left in the recorded final state.  The record
- * belongs to the particle state the run ended in: jump_step (nsteps > 0),
- * jump_checkpoint_load, jump_add_solid or jump_batch_sync on the context
- * discard it, and jump_adjoint_backward then returns JUMP_ERR_STATE instead
- * of rewinding the particles. 
+/* Reverse mode of the 2D USL step and of the 3D MUSL step (the reference has
+ * no derivatives; this is the pullback a ChainRulesCore rrule around
+ * solve_explicit_dynamics_2D/3D, SolveExplicitSolidMechanics2D.jl:335 and
+ * SolveExplicitSolidMechanics3D.jl:14, would call).  Covers the linear and
+ * GIMP bases and Hooke / Neo-Hookean solids, and in 3D J2 solids
+ * (Material.jl:228-275: the derivative of the return-map branch each
+ * particle takes), else JUMP_ERR_UNSUPPORTED.  jump_adjoint_forward runs
+ * nsteps steps like jump_step and records the particle state every K steps.
+ * jump_adjoint_backward takes the adjoint of the final positions (pos_bar,
+ * AoS n x dim, required) and velocities (vel_bar, may be NULL) and writes,
+ * per solid, dL/dE, dL/dnu and dL/dv0 (the derivative with respect to a
+ * uniform change of that solid's initial velocity, dim per solid); outputs
+ * may be NULL.  Each step of the sweep recomputes its states from the
+ * nearest snapshot (checkpoint and recompute); the context is left in the
+ * recorded final state.  The record belongs to the particle state the run
+ * ended in: jump_step (nsteps > 0), jump_checkpoint_load, jump_add_solid or
+ * jump_batch_sync on the context discard it, and jump_adjoint_backward then
+ * returns JUMP_ERR_STATE instead of rewinding the particles.
  * checkpoint_every = 0 picks the interval: the whole run when its per-step
  * states, grids and bins fit in three quarters of the free device memory
  * (no recomputation), else the largest interval that fits.
- * jump_adjoint_interval reports the interval of the recorded run. */
+ * jump_adjoint_interval reports the interval of the recorded run.
+ * jump_adjoint_plastic gives, per solid, dL/dfy (yield stress) and dL/dk1
+ * (linear hardening) from the last jump_adjoint_backward of the current
+ * record (0 for solids without J2 and in 2D; JUMP_ERR_STATE before it). */
 int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, double* t_inout);
 int jump_adjoint_interval(jump_ctx* ctx, int32_t* checkpoint_every);
 int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* vel_bar, double* dE,
                           double* dnu, double* dvel0);
+int jump_adjoint_plastic(jump_ctx* ctx, double* dfy, double* dk1);
 
 /* Kinetic and strain energy of particles held in the caller's device memory,
  * allocating nothing.  No counterpart in the reference: this is the reduction

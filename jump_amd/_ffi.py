@@ -170,6 +170,7 @@ def load():
                             ("jump_adjoint_forward", ctypes.c_int, [H, _i64, _i32, _pd]),
                             ("jump_adjoint_backward", ctypes.c_int, [H, _pd, _pd, _pd, _pd, _pd]),
                             ("jump_adjoint_interval", ctypes.c_int, [H, ctypes.POINTER(_i32)]),
+                            ("jump_adjoint_plastic", ctypes.c_int, [H, _pd, _pd]),
                             ("jump_dump_lammps", ctypes.c_int, [H, ctypes.c_char_p, ctypes.c_char_p]),
                             ("jump_dump_wait", ctypes.c_int, [H]),
                             ("jump_batch_options", ctypes.c_int, [H, _i32, _i32, _i32]),
@@ -192,5 +193,6 @@ EXPORTED = [
     "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
     "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies", "jump_batch_destroy",
     "jump_batch_last_error", "jump_adjoint_forward", "jump_adjoint_backward", "jump_adjoint_interval",
+    "jump_adjoint_plastic",
     "jump_dump_lammps", "jump_dump_wait", "jump_energies_noalloc", "jump_batch_options", "jump_batch_tuning",
 ]

@@ -1,13 +1,15 @@
-"""Reverse mode of the 2D USL step through the C ABI (jump_adjoint_*).
+"""Reverse mode of the 2D USL and 3D MUSL steps through the C ABI
+(jump_adjoint_*).
 
 `rrule(engine, nsteps, checkpoint_every)` is the shape of a ChainRulesCore
-rrule around solve_explicit_dynamics_2D (SolveExplicitSolidMechanics2D.jl:335):
-it runs the steps and returns the final particle positions and velocities
-with a pullback that maps their adjoints to gradients with respect to each
-solid's E, nu and initial velocity.  checkpoint_every = 0 lets the library
-pick the interval from the free device memory (the whole run when it fits,
-so the backward sweep recomputes nothing); `pullback.checkpoint_every` is
-the interval used.
+rrule around solve_explicit_dynamics_2D/3D (SolveExplicitSolidMechanics2D.jl:
+335, SolveExplicitSolidMechanics3D.jl:14): it runs the steps and returns the
+final particle positions and velocities with a pullback that maps their
+adjoints to gradients with respect to each solid's E, nu, initial velocity
+and, for J2 solids in 3D, yield stress fy and hardening k1.
+checkpoint_every = 0 lets the library pick the interval from the free device
+memory (the whole run when it fits, so the backward sweep recomputes
+nothing); `pullback.checkpoint_every` is the interval used.
 """
 import ctypes
 
@@ -29,21 +31,24 @@ def rrule(engine, nsteps, checkpoint_every=0, t=0.0):
     def pullback(pos_bar, vel_bar=None):
         ns = len(engine.counts)
         npart = int(sum(engine.counts))
-        # the library reads 2 doubles per particle from each array
+        d = getattr(engine, "dim", 2)
+        # the library reads dim doubles per particle from each array
         pb = np.ascontiguousarray(pos_bar, dtype=np.float64)
-        if pb.shape != (npart, 2):
-            raise ValueError(f"pos_bar must have shape ({npart}, 2), got {pb.shape}")
+        if pb.shape != (npart, d):
+            raise ValueError(f"pos_bar must have shape ({npart}, {d}), got {pb.shape}")
         vb = None
         if vel_bar is not None:
             vb = np.ascontiguousarray(vel_bar, dtype=np.float64)
-            if vb.shape != (npart, 2):
-                raise ValueError(f"vel_bar must have shape ({npart}, 2), got {vb.shape}")
-        dE, dnu, dv0 = np.zeros(ns), np.zeros(ns), np.zeros((ns, 2))
+            if vb.shape != (npart, d):
+                raise ValueError(f"vel_bar must have shape ({npart}, {d}), got {vb.shape}")
+        dE, dnu, dv0 = np.zeros(ns), np.zeros(ns), np.zeros((ns, d))
         rc2 = lib.jump_adjoint_backward(engine.h, _ffi.dptr(pb), _ffi.dptr(vb), _ffi.dptr(dE), _ffi.dptr(dnu),
                                         _ffi.dptr(dv0))
         if rc2 != 0:
             engine._check(rc2)
-        return {"E": dE, "nu": dnu, "v0": dv0}
+        dfy, dk1 = np.zeros(ns), np.zeros(ns)
+        engine._check(lib.jump_adjoint_plastic(engine.h, _ffi.dptr(dfy), _ffi.dptr(dk1)))
+        return {"E": dE, "nu": dnu, "v0": dv0, "fy": dfy, "k1": dk1}
 
     k = ctypes.c_int32()
     engine._check(lib.jump_adjoint_interval(engine.h, ctypes.byref(k)))

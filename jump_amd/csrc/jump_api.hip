@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "mpm_adjoint.hpp"
+#include "mpm_adjoint3.hpp"
 #include "mpm_output.hpp"
 
 using namespace jh;
@@ -115,6 +115,10 @@ struct jump_ctx {
     AdjParticles ap{};
     AdjNodes an{};
     AdjSources src{};
+    AdjParticles3 ap3{};  // 3D
+    AdjNodes3 an3{};
+    AdjSources3 src3{};
+    std::vector<double> dfy, dk1;  // per solid, from the last backward sweep
   } adj;
   uint32_t flags = 0;
 
@@ -1818,10 +1822,16 @@ int jump_batch_tuning(jump_batch* b, int32_t g, int32_t* ngroups, int32_t* shape
 // each segment's states from its snapshot (keeping the grid each step used),
 // then walks the segment's steps backwards.  Memory: (nsteps/K + K + 1)
 // snapshots of 136 bytes per particle and K grids of 40 bytes per node.
-static size_t snap_doubles(const jump_ctx* ctx) { return size_t(ctx->np) * (2 * PairLayout<2>::NP + 7); }
+// A snapshot is the particle state a step reads: the pair arrays (x, m, V,
+// v, sigma), F and the strain, and in 3D the J2 history (alpha and the
+// plastic strain): 17 doubles per particle in 2D, 36 in 3D.
+static size_t snap_doubles(const jump_ctx* ctx) {
+  return size_t(ctx->np) * (ctx->D == 2 ? 2 * PairLayout<2>::NP + 7 : 2 * PairLayout<3>::NP + 9 + 6 + 1 + 6);
+}
 
 static int snap_copy(jump_ctx* ctx, double* buf, bool save) {
   const size_t n = size_t(ctx->np);
+  const int D = ctx->D;
   double* q = buf;
   auto cp = [&](void* dev, size_t bytes) {
     hipError_t e = save ? hipMemcpyAsync(q, dev, bytes, hipMemcpyDeviceToDevice, ctx->stream)
@@ -1829,24 +1839,35 @@ static int snap_copy(jump_ctx* ctx, double* buf, bool save) {
     q += bytes / sizeof(double);
     return e;
   };
-  for (int j = 0; j < PairLayout<2>::NP; j++)
-    if (cp(ctx->P.pr[j], n * sizeof(double2)) != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "snapshot copy");
-  for (int k = 0; k < 4; k++)
-    if (cp(ctx->P.F[k], n * sizeof(double)) != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "snapshot copy");
-  for (int k = 0; k < 3; k++)
-    if (cp(ctx->P.eps[k], n * sizeof(double)) != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "snapshot copy");
+  std::vector<std::pair<void*, size_t>> arrays;
+  const int npair = D == 2 ? PairLayout<2>::NP : PairLayout<3>::NP;
+  for (int j = 0; j < npair; j++) arrays.push_back({ctx->P.pr[j], n * sizeof(double2)});
+  for (int k = 0; k < D * D; k++) arrays.push_back({ctx->P.F[k], n * sizeof(double)});
+  for (int k = 0; k < nsym(D); k++) arrays.push_back({ctx->P.eps[k], n * sizeof(double)});
+  if (D == 3) {
+    arrays.push_back({ctx->P.alpha, n * sizeof(double)});
+    for (int k = 0; k < 6; k++) arrays.push_back({ctx->P.ep[k], n * sizeof(double)});
+  }
+  for (auto& a : arrays)
+    if (cp(a.first, a.second) != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "snapshot copy");
+  if (!save) ctx->runs_valid = false;  // a restored state: the next step merges at every node
   return JUMP_OK;
 }
 
 static int adjoint_supported(jump_ctx* ctx) {
-  if (ctx->D != 2 || ctx->prm.algorithm != JUMP_ALG_USL || ctx->ninst != 1)
-    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the 2D USL step");
+  const bool usl2d = ctx->D == 2 && ctx->prm.algorithm == JUMP_ALG_USL;
+  const bool musl3d = ctx->D == 3 && ctx->prm.algorithm == JUMP_ALG_MUSL;
+  if (!(usl2d || musl3d) || ctx->ninst != 1)
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the 2D USL and 3D MUSL steps");
   if (ctx->prm.contact) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint does not cover contact");
   if (ctx->prm.basis != JUMP_BASIS_LINEAR && ctx->prm.basis != JUMP_BASIS_GIMP)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the linear and GIMP bases");
-  for (auto& s : ctx->solids)
-    if (s.dev.kind != JUMP_MAT_ELASTIC && s.dev.kind != JUMP_MAT_NEOHOOKEAN)
-      return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers Hooke and Neo-Hookean solids");
+  for (auto& s : ctx->solids) {
+    const int k = s.dev.kind;
+    if (k != JUMP_MAT_ELASTIC && k != JUMP_MAT_NEOHOOKEAN && !(musl3d && k == JUMP_MAT_ELASTOPLASTIC))
+      return set_err(ctx, JUMP_ERR_UNSUPPORTED,
+                     "the adjoint covers Hooke and Neo-Hookean solids, and J2 solids in 3D");
+  }
   return JUMP_OK;
 }
 
@@ -1867,11 +1888,18 @@ static int bins_copy(jump_ctx* ctx, void* buf, bool save) {
   return JUMP_OK;
 }
 
-// The grid a step's G2P used: nodal mass and momenta (5 arrays of NN).
+// The grid a step's G2P used: nodal mass and momenta (2D: m, p0, p, 5
+// arrays of NN; 3D: m, p0, p and the re-projected p2, 10 arrays).
+static int grid_arrays(const jump_ctx* ctx) { return ctx->D == 2 ? 5 : 10; }
 static int grid_copy(jump_ctx* ctx, double* buf, bool save) {
   const size_t nn = size_t(ctx->NN);
-  double* arrs[5] = {ctx->g.m, ctx->g.p0[0], ctx->g.p0[1], ctx->g.p[0], ctx->g.p[1]};
-  for (int a = 0; a < 5; a++) {
+  double* arrs[10] = {ctx->g.m, ctx->g.p0[0], ctx->g.p0[1], ctx->g.p[0], ctx->g.p[1]};
+  if (ctx->D == 3) {
+    double* a3[10] = {ctx->g.m, ctx->g.p0[0], ctx->g.p0[1], ctx->g.p0[2], ctx->g.p[0],
+                      ctx->g.p[1], ctx->g.p[2], ctx->g.p2[0], ctx->g.p2[1], ctx->g.p2[2]};
+    for (int a = 0; a < 10; a++) arrs[a] = a3[a];
+  }
+  for (int a = 0; a < grid_arrays(ctx); a++) {
     hipError_t e = save ? hipMemcpyAsync(buf + a * nn, arrs[a], nn * sizeof(double), hipMemcpyDeviceToDevice,
                                          ctx->stream)
                         : hipMemcpyAsync(arrs[a], buf + a * nn, nn * sizeof(double), hipMemcpyDeviceToDevice,
@@ -1879,6 +1907,19 @@ static int grid_copy(jump_ctx* ctx, double* buf, bool save) {
     if (e != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "grid copy");
   }
   return JUMP_OK;
+}
+
+template <int B> static void adj_step_back3(jump_ctx* ctx) {
+  const int np = int(ctx->np), nn = ctx->NN;
+  const double dt = ctx->prm.dt, alpha = ctx->prm.alpha;
+  auto& A = ctx->adj;
+  LAUNCH((k_adj3_g2p2<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, dt, A.ap3, A.src3);
+  LAUNCH((k_adj3_nodes2<B>), nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g, dt, A.src3, A.an3);
+  LAUNCH((k_adj3_g2p1<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, alpha, A.an3, A.ap3, A.src3);
+  LAUNCH((k_adj3_nodes1<B>), nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g, alpha, A.src3,
+         A.an3);
+  LAUNCH(k_adj3_grid, nn, nn, ctx->d_fixed, dt, A.an3);
+  LAUNCH((k_adj3_p2g<B>), np, ctx->P, ctx->G, ctx->d_solids, A.an3, A.ap3);
 }
 
 template <int B> static void adj_step_back(jump_ctx* ctx) {
@@ -1897,7 +1938,7 @@ template <int B> static void adj_step_back(jump_ctx* ctx) {
 // snapshots, K per-step states, grids and bins, the final state.
 static size_t adjoint_record_bytes(const jump_ctx* ctx, int64_t nsteps, int64_t K) {
   const size_t snap = snap_doubles(ctx) * sizeof(double);
-  const size_t per_step = snap + 5 * size_t(ctx->NN) * sizeof(double) + bins_bytes(ctx) + 256;
+  const size_t per_step = snap + size_t(grid_arrays(ctx)) * size_t(ctx->NN) * sizeof(double) + bins_bytes(ctx) + 256;
   const size_t nck = size_t((nsteps + K - 1) / K);
   return (nck + 1) * snap + size_t(std::min(K, nsteps)) * per_step;
 }
@@ -1912,9 +1953,13 @@ static int adjoint_auto_interval(jump_ctx* ctx, int64_t nsteps, int64_t* K) {
   if (A.snap_np == ctx->np) {  // the buffers held now are freed or reused
     const size_t snap = snap_doubles(ctx) * sizeof(double);
     freeb += (A.ckpt.size() + A.segbuf.size() + (A.final_state ? 1 : 0)) * snap +
-             A.gridbuf.size() * 5 * size_t(ctx->NN) * sizeof(double) + A.binbuf.size() * (bins_bytes(ctx) + 256);
+             A.gridbuf.size() * size_t(grid_arrays(ctx)) * size_t(ctx->NN) * sizeof(double) +
+             A.binbuf.size() * (bins_bytes(ctx) + 256);
   }
-  const size_t reserve = 25 * size_t(ctx->np) * sizeof(double) + 6 * size_t(ctx->NN) * sizeof(double);
+  // the adjoint arrays of the backward sweep: 2D 25 doubles per particle and
+  // 6 per node; 3D 54 and 10
+  const size_t reserve = (ctx->D == 2 ? 25 : 54) * size_t(ctx->np) * sizeof(double) +
+                         (ctx->D == 2 ? 6 : 10) * size_t(ctx->NN) * sizeof(double);
   const size_t budget = freeb > reserve ? (freeb - reserve) / 4 * 3 : 0;
   int64_t k = std::min<int64_t>(nsteps, INT32_MAX);
   while (k > 1 && adjoint_record_bytes(ctx, nsteps, k) > budget) k = k * 7 / 8;
@@ -1952,7 +1997,7 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
       if ((r = alloc(&c, sd))) return r;
     A.gridbuf.resize(nseg);
     for (auto& c : A.gridbuf)
-      if ((r = alloc(&c, 5 * size_t(ctx->NN)))) return r;
+      if ((r = alloc(&c, size_t(grid_arrays(ctx)) * size_t(ctx->NN)))) return r;
     A.binbuf.resize(nseg);
     for (auto& c : A.binbuf) {
       char* q = nullptr;
@@ -1963,6 +2008,8 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
     A.snap_np = ctx->np;
   }
   A.ready = false;
+  A.dfy.clear();
+  A.dk1.clear();
   unsigned fl = 0;
   const int64_t last0 = (nck - 1) * K;  // the last segment's states and grids are kept as they go by
   for (int64_t s = 0; s < nsteps; s++) {
@@ -2002,47 +2049,82 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
   if (!pos_bar) return set_err(ctx, JUMP_ERR_ARG, "pos_bar is required");
   HIP_TRY(hipSetDevice(ctx->device));
   const size_t n = size_t(ctx->np), nn = size_t(ctx->NN);
+  const int D = ctx->D;
   int r = JUMP_OK;
-  AdjParticles& ap = A.ap;
-  AdjNodes& an = A.an;
   if (A.adj_np != ctx->np) {
     free_list(A.adj_allocs);
-    ap = AdjParticles{};
-    an = AdjNodes{};
+    A.ap = AdjParticles{};
+    A.an = AdjNodes{};
     A.src = AdjSources{};
+    A.ap3 = AdjParticles3{};
+    A.an3 = AdjNodes3{};
+    A.src3 = AdjSources3{};
     A.adj_np = ctx->np;
     auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.adj_allocs, p, count); };
-    for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.xb[k], n);
-    for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.vb[k], n);
-    for (int k = 0; k < 4 && !r; k++) r = alloc(&ap.Fb[k], n);
-    for (int k = 0; k < 3 && !r; k++) r = alloc(&ap.eb[k], n);
-    for (int k = 0; k < 3 && !r; k++) r = alloc(&ap.sb[k], n);
-    if (!r) r = alloc(&ap.Vb, n);
-    if (!r) r = alloc(&ap.lam, n);
-    if (!r) r = alloc(&ap.mu, n);
-    AdjSources& src = A.src;
-    for (double2** v : {&src.xb, &src.vb, &src.Lb[0], &src.Lb[1]})
-      if (!r) r = dev_alloc(ctx, A.adj_allocs, v, n);
-    if (!r) r = alloc(&an.mb, nn);
-    for (int k = 0; k < 2 && !r; k++) r = alloc(&an.Pb[k], nn);
-    for (int k = 0; k < 2 && !r; k++) r = alloc(&an.P0b[k], nn);
+    if (D == 2) {
+      AdjParticles& ap = A.ap;
+      AdjNodes& an = A.an;
+      for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.xb[k], n);
+      for (int k = 0; k < 2 && !r; k++) r = alloc(&ap.vb[k], n);
+      for (int k = 0; k < 4 && !r; k++) r = alloc(&ap.Fb[k], n);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&ap.eb[k], n);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&ap.sb[k], n);
+      if (!r) r = alloc(&ap.Vb, n);
+      if (!r) r = alloc(&ap.lam, n);
+      if (!r) r = alloc(&ap.mu, n);
+      AdjSources& src = A.src;
+      for (double2** v : {&src.xb, &src.vb, &src.Lb[0], &src.Lb[1]})
+        if (!r) r = dev_alloc(ctx, A.adj_allocs, v, n);
+      if (!r) r = alloc(&an.mb, nn);
+      for (int k = 0; k < 2 && !r; k++) r = alloc(&an.Pb[k], nn);
+      for (int k = 0; k < 2 && !r; k++) r = alloc(&an.P0b[k], nn);
+    } else {
+      AdjParticles3& ap = A.ap3;
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&ap.xb[k], n);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&ap.vb[k], n);
+      for (int k = 0; k < 9 && !r; k++) r = alloc(&ap.Fb[k], n);
+      for (int k = 0; k < 6 && !r; k++) r = alloc(&ap.eb[k], n);
+      for (int k = 0; k < 6 && !r; k++) r = alloc(&ap.sb[k], n);
+      for (int k = 0; k < 6 && !r; k++) r = alloc(&ap.epb[k], n);
+      for (double** v : {&ap.Vb, &ap.ab, &ap.lam, &ap.mu, &ap.fy, &ap.k1})
+        if (!r) r = alloc(v, n);
+      AdjSources3& src = A.src3;
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&src.xb[k], n);
+      for (int k = 0; k < 9 && !r; k++) r = alloc(&src.Lb[k], n);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&src.vb[k], n);
+      AdjNodes3& an = A.an3;
+      if (!r) r = alloc(&an.mb, nn);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&an.Pb[k], nn);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&an.P0b[k], nn);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&an.P2b[k], nn);
+    }
     if (r) return r;
   }
   // adjoints of the final state: positions and velocities as given, the rest 0
   {
     std::vector<double> h(n);
-    for (int k = 0; k < 2; k++) {
-      for (size_t i = 0; i < n; i++) h[i] = pos_bar[2 * i + k];
-      HIP_TRY(hipMemcpyAsync(ap.xb[k], h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    for (int k = 0; k < D; k++) {
+      double* xbk = D == 2 ? A.ap.xb[k] : A.ap3.xb[k];
+      double* vbk = D == 2 ? A.ap.vb[k] : A.ap3.vb[k];
+      for (size_t i = 0; i < n; i++) h[i] = pos_bar[D * i + k];
+      HIP_TRY(hipMemcpyAsync(xbk, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
       HIP_TRY(hipStreamSynchronize(ctx->stream));
-      for (size_t i = 0; i < n; i++) h[i] = vel_bar ? vel_bar[2 * i + k] : 0.0;
-      HIP_TRY(hipMemcpyAsync(ap.vb[k], h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      for (size_t i = 0; i < n; i++) h[i] = vel_bar ? vel_bar[D * i + k] : 0.0;
+      HIP_TRY(hipMemcpyAsync(vbk, h.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
       HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
-    double* zs[12] = {ap.Fb[0], ap.Fb[1], ap.Fb[2], ap.Fb[3], ap.eb[0], ap.eb[1], ap.eb[2],
-                      ap.sb[0], ap.sb[1], ap.sb[2], ap.Vb, ap.lam};
+    std::vector<double*> zs;
+    if (D == 2) {
+      const AdjParticles& ap = A.ap;
+      zs = {ap.Fb[0], ap.Fb[1], ap.Fb[2], ap.Fb[3], ap.eb[0], ap.eb[1], ap.eb[2],
+            ap.sb[0], ap.sb[1], ap.sb[2], ap.Vb, ap.lam, ap.mu};
+    } else {
+      const AdjParticles3& ap = A.ap3;
+      for (int k = 0; k < 9; k++) zs.push_back(ap.Fb[k]);
+      for (int k = 0; k < 6; k++) { zs.push_back(ap.eb[k]); zs.push_back(ap.sb[k]); zs.push_back(ap.epb[k]); }
+      for (double* z : {ap.Vb, ap.ab, ap.lam, ap.mu, ap.fy, ap.k1}) zs.push_back(z);
+    }
     for (double* z : zs) LAUNCH(k_fill_f64, n, z, 0.0, int(n));
-    LAUNCH(k_fill_f64, n, ap.mu, 0.0, int(n));
   }
   const int64_t nseg = (A.nsteps + A.K - 1) / A.K;
   for (int64_t seg = nseg - 1; seg >= 0 && !r; seg--) {
@@ -2061,8 +2143,13 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
       if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], false))) break;
       if ((r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], false))) break;
       if ((r = bins_copy(ctx, A.binbuf[size_t(j - s0)], false))) break;
-      if (ctx->prm.basis == JUMP_BASIS_LINEAR) adj_step_back<JUMP_BASIS_LINEAR>(ctx);
-      else adj_step_back<JUMP_BASIS_GIMP>(ctx);
+      if (D == 2) {
+        if (ctx->prm.basis == JUMP_BASIS_LINEAR) adj_step_back<JUMP_BASIS_LINEAR>(ctx);
+        else adj_step_back<JUMP_BASIS_GIMP>(ctx);
+      } else {
+        if (ctx->prm.basis == JUMP_BASIS_LINEAR) adj_step_back3<JUMP_BASIS_LINEAR>(ctx);
+        else adj_step_back3<JUMP_BASIS_GIMP>(ctx);
+      }
     }
   }
   if (r) return r;
@@ -2070,8 +2157,20 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
   // leave the recorded final state in place
   if ((r = snap_copy(ctx, A.final_state, false))) return r;
   HIP_TRY(hipMemsetAsync(ctx->d_flags, 0, sizeof(unsigned), ctx->stream));
-  // per-solid sums on the device
+  // per-solid sums on the device: lambda, mu, (fy, k1,) v0
   const size_t ns = ctx->solids.size();
+  AdjSumArrays arr{};
+  int nsum = 0;
+  if (D == 2) {
+    for (const double* a : {(const double*)A.ap.lam, (const double*)A.ap.mu, (const double*)A.ap.vb[0],
+                            (const double*)A.ap.vb[1]})
+      arr.a[nsum++] = a;
+  } else {
+    for (const double* a : {(const double*)A.ap3.lam, (const double*)A.ap3.mu, (const double*)A.ap3.fy,
+                            (const double*)A.ap3.k1, (const double*)A.ap3.vb[0], (const double*)A.ap3.vb[1],
+                            (const double*)A.ap3.vb[2]})
+      arr.a[nsum++] = a;
+  }
   std::vector<long long> range(2 * ns);
   for (size_t si = 0; si < ns; si++) {
     range[2 * si] = ctx->solids[si].offset;
@@ -2081,34 +2180,52 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
   double* d_sums = nullptr;
   std::vector<void*> tmp;
   if ((r = dev_alloc(ctx, tmp, &d_range, range.size()))) return r;
-  if ((r = dev_alloc(ctx, tmp, &d_sums, 4 * ns))) {
+  if ((r = dev_alloc(ctx, tmp, &d_sums, size_t(nsum) * ns))) {
     free_list(tmp);
     return r;
   }
-  std::vector<double> sums(4 * ns);
+  std::vector<double> sums(size_t(nsum) * ns);
   hipError_t he = hipMemcpyAsync(d_range, range.data(), range.size() * sizeof(long long), hipMemcpyHostToDevice,
                                  ctx->stream);
   if (he == hipSuccess) {
-    hipLaunchKernelGGL(k_adj_solid_sums, dim3(unsigned(ns)), dim3(kBlock), 0, ctx->stream, d_range, ap, d_sums);
+    hipLaunchKernelGGL(k_adj_sums, dim3(unsigned(ns)), dim3(kBlock), 0, ctx->stream, d_range, arr, nsum, d_sums);
     he = hipMemcpyAsync(sums.data(), d_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   }
   if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
   free_list(tmp);
   if (he != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "adjoint sums");
+  A.dfy.assign(ns, 0.0);
+  A.dk1.assign(ns, 0.0);
   for (size_t si = 0; si < ns; si++) {
     const HostSolid& hs = ctx->solids[si];
-    const double lb = sums[4 * si], mb = sums[4 * si + 1], vb0 = sums[4 * si + 2], vb1 = sums[4 * si + 3];
+    const double* sm = &sums[size_t(nsum) * si];
+    const double lb = sm[0], mb = sm[1];
     // lambda = E nu / ((1+nu)(1-2nu)),  mu = E / (2(1+nu))
     const double E = hs.mat.E, nu = hs.mat.nu, den = (1.0 + nu) * (1.0 - 2.0 * nu);
     const double dl_dE = nu / den, dm_dE = 1.0 / (2.0 * (1.0 + nu));
     const double dl_dnu = E * (1.0 + 2.0 * nu * nu) / (den * den), dm_dnu = -E / (2.0 * (1.0 + nu) * (1.0 + nu));
     if (dE) dE[si] = lb * dl_dE + mb * dm_dE;
     if (dnu) dnu[si] = lb * dl_dnu + mb * dm_dnu;
-    if (dvel0) {
-      dvel0[2 * si] = vb0;
-      dvel0[2 * si + 1] = vb1;
+    const int vo = D == 2 ? 2 : 4;
+    if (D == 3) {
+      A.dfy[si] = sm[2];
+      A.dk1[si] = sm[3];
     }
+    if (dvel0)
+      for (int k = 0; k < D; k++) dvel0[D * si + size_t(k)] = sm[vo + k];
   }
   ctx->grid_valid = false;
+  return JUMP_OK;
+}
+
+int jump_adjoint_plastic(jump_ctx* ctx, double* dfy, double* dk1) {
+  if (!ctx) return JUMP_ERR_ARG;
+  const auto& A = ctx->adj;
+  if (!A.ready || A.dfy.size() != ctx->solids.size())
+    return set_err(ctx, JUMP_ERR_STATE, "no backward sweep of the current recorded run");
+  for (size_t si = 0; si < ctx->solids.size(); si++) {
+    if (dfy) dfy[si] = A.dfy[si];
+    if (dk1) dk1[si] = A.dk1[si];
+  }
   return JUMP_OK;
 }

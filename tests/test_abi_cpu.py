@@ -131,6 +131,9 @@ class _StubLib:
         self.backward_calls += 1
         return 0
 
+    def jump_adjoint_plastic(self, h, dfy, dk1):
+        return 0
+
 
 class _StubEngine:
     def __init__(self, counts):

@@ -94,3 +94,27 @@ def test_stepping_after_the_recorded_run_discards_it(hip_lib):
     assert ei.value.code == _ffi.JUMP_ERR_STATE
     assert np.array_equal(e.get_solid(0)["pos"], moved)
     e.close()
+
+
+@pytest.mark.parametrize("basis,material", [(_ffi.BASIS_LINEAR, "neohookean"), (_ffi.BASIS_GIMP, "elastic")])
+def test_adjoint2d_matches_dual_numbers(hip_lib, basis, material):
+    """The 2D reverse mode against the dual-number run of the CPU
+    restatement (whose forward equals the GPU's): relative 1e-9."""
+    from oracle.pyoracle import OracleEngine
+    sc = _scene(basis, material)
+    rng = np.random.default_rng(12)
+    n = sc.n_particles
+    pb, vb = rng.normal(size=(n, 2)), rng.normal(size=(n, 2))
+    e = Engine(sc.params)
+    sc.load(e)
+    (pos, vel, t, rc), pullback = rrule(e, NSTEPS, checkpoint_every=7)
+    g = pullback(pb, vb)
+    e.close()
+    o = OracleEngine(sc.params)
+    sc.load(o)
+    want = o.dual_gradient(0, NSTEPS, pb, vb)
+    got = {"E": g["E"][0], "nu": g["nu"][0], "v0": g["v0"][0, 0], "v1": g["v0"][0, 1]}
+    ref = {"E": want["E"], "nu": want["nu"], "v0": want["v0"][0], "v1": want["v0"][1]}
+    print({k: (got[k], ref[k]) for k in got})
+    for k in got:
+        assert abs(got[k] - ref[k]) <= 1e-9 * abs(ref[k]), (k, got[k], ref[k])
