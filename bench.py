@@ -326,6 +326,73 @@ def adjoint_leg(steps, K_recompute=10):
     return out
 
 
+def taylor_leg(steps, K):
+    """Config c5: the differentiable Taylor bar (Main_TaylorBar.jl:43-130;
+    scenes.taylor_bar: copper cylinder r = 3.2 mm, L = 32.4 mm, 190 m/s onto
+    a rigid wall, J2, GIMP, 3D MUSL, 496,871 particles).  A run with hidden
+    parameters (E, fy, hardening drawn with seed 7) gives the target
+    positions x*; the loss is L = sum |x_T - x*|^2 for the nominal
+    parameters, and its gradient with respect to E, nu, fy, k1 and the
+    initial velocity comes from the recorded run and the backward sweep with
+    a snapshot every K steps (checkpoint and recompute).  Timed against
+    `steps` plain steps."""
+    import numpy as np
+    import torch
+
+    from jump_amd import scenes
+    from jump_amd.adjoint import rrule
+    from jump_amd.engine import Engine
+
+    rng = np.random.default_rng(7)
+    hidden = {"E": 117e9 * rng.uniform(0.9, 1.1), "fy": 400e6 * rng.uniform(0.8, 1.2),
+              "H": 100e6 * rng.uniform(0.7, 1.3)}
+    tgt = scenes.taylor_bar(steps=steps, E=hidden["E"], fy=hidden["fy"], H=hidden["H"])
+    e = Engine(tgt.params, device=0)
+    tgt.load(e)
+    e.step(steps, 0.0)
+    xstar = e.get_solid(0)["pos"]
+    e.close()
+    sc = scenes.taylor_bar(steps=steps)
+    e = Engine(sc.params, device=0)
+    sc.load(e)
+    e.step(0, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(steps, 0.0)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    e.close()
+
+    def run(K):
+        e = Engine(sc.params, device=0)
+        sc.load(e)
+        e.step(0, 0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        (pos, vel, t, rc), pullback = rrule(e, steps, checkpoint_every=K)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g = pullback(2.0 * (pos - xstar))
+        torch.cuda.synchronize()
+        t_back = time.perf_counter() - t0
+        loss = float(((pos - xstar) ** 2).sum())
+        e.close()
+        return t_fwd, t_back, g, loss, pullback.checkpoint_every, rc
+
+    run(K)  # the first run allocates the record and adjoint buffers, kept by the library
+    t_fwd, t_back, g, loss, k_used, rc = run(K)
+    return {"workload": f"taylor_bar: copper cylinder, J2, GIMP, 3D MUSL, {sc.n_particles} particles, "
+                        f"{sc.params.nx}x{sc.params.ny}x{sc.params.nz} nodes",
+            "particles": sc.n_particles, "steps": steps, "checkpoint_every": k_used,
+            "hidden": hidden, "loss": loss,
+            "gradient": {"E": float(g["E"][0]), "nu": float(g["nu"][0]), "fy": float(g["fy"][0]),
+                         "k1": float(g["k1"][0]), "v0": [float(x) for x in g["v0"][0]]},
+            "plain_ms_per_step": t_plain * 1e3 / steps,
+            "recorded_forward_ms_per_step": t_fwd * 1e3 / steps, "backward_ms_per_step": t_back * 1e3 / steps,
+            "adjoint_over_plain": (t_fwd + t_back) / t_plain, "step_flags": rc}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +411,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=50, help="0 skips the adjoint leg")
     ap.add_argument("--musl2d-steps", type=int, default=50, help="0 skips the 2D MUSL leg")
     ap.add_argument("--tubes-steps", type=int, default=100, help="0 skips the CPDI tube-array leg")
+    ap.add_argument("--taylor-steps", type=int, default=400, help="0 skips the Taylor-bar reverse-mode leg")
+    ap.add_argument("--taylor-checkpoint", type=int, default=20)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -440,6 +509,8 @@ def main():
         out["musl2d"] = musl2d_leg(args.musl2d_steps, 3)
     if args.tubes_steps > 0:
         out["tubes"] = tubes_leg(args.tubes_steps, 2)
+    if args.taylor_steps > 0:
+        out["taylor"] = taylor_leg(args.taylor_steps, args.taylor_checkpoint)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_omp_steps)
     print(json.dumps(out), flush=True)
