@@ -452,19 +452,33 @@ def impact3d(nodes=(256, 256, 64), plate_mm=(100.0, 100.0, 20.0), sphere_r_mm=5.
     return Scene("impact3d", p, solids, fixed=fixed, steps=steps)
 
 
-def taylor_bar(scale=1.0, steps=400, basis=_ffi.BASIS_GIMP, material="j2", E=117e9, fy=400e6, H=100e6, v0=190.0):
+def taylor_bar(scale=1.0, steps=400, basis=_ffi.BASIS_GIMP, material="j2", E=117e9, fy=400e6, H=100e6, v0=190.0,
+               alpha=0.95):
     """Taylor bar (Main_TaylorBar.jl:43-130): copper cylinder r=3.2 mm,
-    L=32.4 mm along y, v0=190 m/s onto the grid bottom (fixed in y), J2,
-    GIMP.  `scale` multiplies the particle spacing (1.0: ~0.128 mm)."""
+    L=32.4 mm along y, v0=190 m/s onto a rigid wall, J2, GIMP.  `scale`
+    multiplies the particle spacing (1.0: ~0.128 mm, 2 particles per cell and
+    axis).  The wall is the reference's Dirichlet bottom (nodes fixed in y),
+    taken as the node planes j <= 2 so that particles landing on it keep
+    their GIMP window inside the grid (a particle needs its half-width of
+    room above the lowest node); the bar starts half a cell above it and
+    lands within the first tens of steps.  MUSL(alpha) with alpha = 0.95:
+    the reference's 3D step zeroes momentum0 and momentum on fixed nodes but
+    not momentum2 (SolveExplicitSolidMechanics3D.jl:89-105, 142-156), and
+    G2P-2 takes the velocity gradient from momentum2, so with pure FLIP
+    (alpha = 1) a wall changes no particle velocity and the bar develops no
+    strain against it; with the PIC share the wall's zero momentum damps
+    the velocities next to it."""
     r, L = 3.2e-3, 32.4e-3
     f = 0.128e-3 * scale
     dx = 2 * f
     rho, nu = 8930.0, 0.35
     pts = buildParticleForCylinder([0.0, 0.0, 0.0], r, 0.5 * L, f, f)
     margin = 4 * dx
+    jw = 2  # the wall: node planes 0..jw fixed in y
     ext = (2 * r + 2 * margin, L * 1.2 + 2 * margin, 2 * r + 2 * margin)
     nodes = tuple(int(math.ceil(e / dx)) + 1 for e in ext)
-    pts = pts + np.array([0.5 * (nodes[0] - 1) * dx + 0.25 * dx, 0.5 * L + 2.25 * dx, 0.5 * (nodes[2] - 1) * dx + 0.25 * dx])
+    shift_y = (jw + 0.5) * dx - pts[:, 1].min() + 0.5 * f
+    pts = pts + np.array([0.5 * (nodes[0] - 1) * dx + 0.25 * dx, shift_y, 0.5 * (nodes[2] - 1) * dx + 0.25 * dx])
     mat = j2(E, nu, rho, fy, H) if material == "j2" else johnson_cook(E, nu, rho, fy, H, 0.0, 1.0, 1.0, dx)
     lp = 0.5 * f if basis == _ffi.BASIS_GIMP else None
     solids = [_solid(mat, pts, (0.0, -v0, 0.0), f ** 3, rho, lp=lp)]
@@ -473,8 +487,10 @@ def taylor_bar(scale=1.0, steps=400, basis=_ffi.BASIS_GIMP, material="j2", E=117
     c = math.sqrt((lam + 2 * mu) / rho)
     dt = 0.2 * dx / c
     hi = tuple((n - 1) * dx for n in nodes)
-    p = make_params(3, basis, _ffi.ALG_MUSL, dt, (0.0, 0.0, 0.0), hi, nodes, alpha=1.0)
-    fixed = _fixed_faces(nodes, ["bottom"], {"bottom": (0, 1, 0)})
+    p = make_params(3, basis, _ffi.ALG_MUSL, dt, (0.0, 0.0, 0.0), hi, nodes, alpha=alpha)
+    fixed = np.zeros((int(np.prod(nodes)), 3), dtype=np.uint8)
+    j = (np.arange(fixed.shape[0]) // nodes[0]) % nodes[1]
+    fixed[j <= jw, 1] = 1
     return Scene("taylor_bar", p, solids, fixed=fixed, steps=steps)
 
 

@@ -109,3 +109,58 @@ def test_adjoint3d_checkpoint_interval_does_not_change_the_gradient(hip_lib):
     for r in res[1:]:
         for k in ("E", "nu", "fy", "k1", "v0"):
             assert np.array_equal(r[k], res[0][k]), k
+
+
+def test_taylor_bar_gradients_reduced_size(hip_lib):
+    """Config c5 at reduced size (particle spacing x4: 7,700 particles,
+    11 x 41 x 11 nodes, 60 steps, a snapshot every 20): the inverse problem's
+    loss L = sum |x_T - x*|^2 against a run with other (E, fy, hardening);
+    its gradient by the reverse mode equals the dual-number run of the CPU
+    restatement to 1e-9 and central differences of the GPU forward to 1e-6,
+    and plastic flow takes part (dL/dfy != 0)."""
+    print("taylor bar reduced size", flush=True)
+    steps = 60
+    tgt = scenes.taylor_bar(scale=4.0, steps=steps, E=125e9, fy=450e6, H=150e6)
+    e = Engine(tgt.params)
+    tgt.load(e)
+    e.step(steps, 0.0)
+    xstar = e.get_solid(0)["pos"]
+    e.close()
+    sc = scenes.taylor_bar(scale=4.0, steps=steps)
+    e = Engine(sc.params)
+    sc.load(e)
+    (pos, vel, t, rc), pullback = rrule(e, steps, checkpoint_every=20)
+    assert rc == 0
+    pb = 2.0 * (pos - xstar)
+    g = pullback(pb)
+    e.close()
+    o = OracleEngine(sc.params)
+    sc.load(o)
+    want = o.dual_gradient(0, steps, pb)
+    o.close()
+    got = {"E": g["E"][0], "nu": g["nu"][0], "fy": g["fy"][0], "k1": g["k1"][0], "vy": g["v0"][0, 1]}
+    ref = {"E": want["E"], "nu": want["nu"], "fy": want["fy"], "k1": want["k1"], "vy": want["v0"][1]}
+    print({k: (got[k], ref[k]) for k in got})
+    for k in got:
+        assert abs(got[k] - ref[k]) <= 1e-9 * abs(ref[k]), (k, got[k], ref[k])
+    assert got["fy"] != 0.0
+
+    def loss(**dp):
+        m = sc.solids[0]["material"]
+        keep = (m.E, m.fy, m.k1)
+        m.E += dp.get("E", 0.0)
+        m.fy += dp.get("fy", 0.0)
+        m.k1 += dp.get("k1", 0.0)
+        e = Engine(sc.params)
+        sc.load(e)
+        e.step(steps, 0.0)
+        x = e.get_solid(0)["pos"]
+        e.close()
+        m.E, m.fy, m.k1 = keep
+        return float(((x - xstar) ** 2).sum())
+
+    m = sc.solids[0]["material"]
+    for key, h, rtol in (("E", 1e-6 * m.E, 1e-6), ("fy", 1e-6 * m.fy, 1e-6), ("k1", 1e-5 * m.k1, 1e-4)):
+        fd = (loss(**{key: h}) - loss(**{key: -h})) / (2 * h)
+        print(key, got[key], fd)
+        assert abs(got[key] - fd) <= rtol * abs(fd), (key, got[key], fd)
