@@ -67,7 +67,6 @@ struct jump_ctx {
   bool runs_valid = false; // the recorded runs match the particles' current bins
   int* h_cg = nullptr;   // pinned: CgState::all_done and ::iters read back
   int cg_check = 8;      // CG iterations before the first convergence read-back
-  hipGraphExec_t cg_chunk = nullptr;  // kCgChunk CG iterations, captured once
   std::vector<void*> part_allocs;
   std::vector<void*> grid_allocs;
   double* d_corners = nullptr;
@@ -657,43 +656,23 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
 // matrix stencil radius (width - 1).
 constexpr double kMuslTol = 1e-14;  // CG: relative residual per right-hand side
 constexpr int kMuslMaxIt = 2000;
-// The blended mass solve: CG iterations of three launches each, in chunks
-// of kCgChunk iterations captured once into a HIP graph and replayed (an
-// iteration that finds every column converged turns the rest into no-ops).
-// The host reads the convergence state back at the first chunk boundary at
-// or after as many iterations as the last solve needed, then after every
-// chunk, so a step usually waits for it once.
-constexpr int kCgChunk = 4;
-template <int R> static void cg_iterations(jump_ctx* ctx, int n) {
-  const int nblk = (ctx->NN + kCgBlock - 1) / kCgBlock;
-  const int ntile = cg_tiles(ctx->G);
-  for (int k = 0; k < n; k++) {
-    hipLaunchKernelGGL((k_cg_spmv<R>), dim3(ntile), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, ctx->cgm);
-    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(kCgBlock), 0, ctx->stream, ctx->cgm, ntile, kMuslTol);
-    hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kCgBlock), 0, ctx->stream, ctx->NN, ctx->cgm);
-  }
-}
+// The blended mass solve: CG iterations of three launches each; the host
+// reads the convergence state back after as many iterations as the last
+// solve needed (then every 4), so a step usually waits for it once.
 template <int R> static void solve_musl2d(jump_ctx* ctx) {
   const int nn = ctx->NN;
+  const int nblk = (nn + kCgBlock - 1) / kCgBlock;
+  const int ntile = cg_tiles(ctx->G);
   CgArrays& c = ctx->cgm;
   LAUNCH(k_cg_init, nn, nn, ctx->mu, ctx->g.m, c);
-  if (!ctx->cg_chunk) {  // capture one chunk (the arrays and shapes are fixed for the context)
-    hipGraph_t graph = nullptr;
-    bool ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-    if (ok) {
-      cg_iterations<R>(ctx, kCgChunk);
-      ok = hipStreamEndCapture(ctx->stream, &graph) == hipSuccess;
-    }
-    if (ok) ok = hipGraphInstantiate(&ctx->cg_chunk, graph, nullptr, nullptr, 0) == hipSuccess;
-    if (graph) (void)hipGraphDestroy(graph);
-    if (!ok) ctx->cg_chunk = nullptr;  // eager launches below
-  }
-  const int check = (std::max(ctx->cg_check, kCgChunk) + kCgChunk - 1) / kCgChunk * kCgChunk;
+  int check = std::max(ctx->cg_check, 4);
   const char* st_done = reinterpret_cast<const char*>(c.st) + offsetof(CgState, all_done);
   static_assert(offsetof(CgState, iters) == offsetof(CgState, all_done) + sizeof(int), "read-back layout");
-  for (int it = 0; it < kMuslMaxIt; it += kCgChunk) {
-    if (!ctx->cg_chunk || hipGraphLaunch(ctx->cg_chunk, ctx->stream) != hipSuccess) cg_iterations<R>(ctx, kCgChunk);
-    if (it + kCgChunk < check) continue;
+  for (int it = 0; it < kMuslMaxIt; it++) {
+    hipLaunchKernelGGL((k_cg_spmv<R>), dim3(ntile), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, c);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(kCgBlock), 0, ctx->stream, c, ntile, it, kMuslTol);
+    hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kCgBlock), 0, ctx->stream, nn, c);
+    if (it + 1 != check) continue;
     if (hipMemcpyAsync(ctx->h_cg, st_done, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
       break;  // the error stays with the stream and is reported by the step
@@ -701,6 +680,7 @@ template <int R> static void solve_musl2d(jump_ctx* ctx) {
       ctx->cg_check = ctx->h_cg[1] + 1;  // the iteration that found every column converged
       break;
     }
+    check += 4;
   }
   LAUNCH(k_cg_finish, nn, nn, ctx->mu, c, ctx->d_flags);
 }
@@ -899,7 +879,6 @@ int jump_ctx_destroy(jump_ctx* ctx) {
   free_list(ctx->part_allocs);
   free_list(ctx->grid_allocs);
   if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
-  if (ctx->cg_chunk) (void)hipGraphExecDestroy(ctx->cg_chunk);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -1486,8 +1486,6 @@ struct CgState {
   int done[4];
   int all_done;  // every column converged (or had a zero right-hand side)
   int iters;     // iterations that updated at least one column
-  int it;        // iterations run (the scalars kernel's own count, so that
-                 // a graph of several iterations needs no per-launch argument)
 };
 struct CgArrays {
   double* dinv;  // the preconditioner: 1/m, 1 at massless nodes (nn)
@@ -1596,13 +1594,12 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(GridDesc G, Musl2Arrays mu
   }
 }
 
-__global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, double tol) {
+__global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, int it, double tol) {
   constexpr int NW = kCgBlock / 64;
   __shared__ double red[12][NW];
   __shared__ double tot[12];
   CgState& st = *cg.st;
   if (st.all_done) return;
-  const int it = st.it;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double acc[12];
 #pragma unroll
@@ -1651,7 +1648,6 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, 
   }
   if (any) st.iters = it + 1;
   st.all_done = all ? 1 : 0;
-  st.it = it + 1;
 }
 
 __global__ void __launch_bounds__(kCgBlock) k_cg_update(int nn, CgArrays cg) {

@@ -112,8 +112,8 @@ def test_adjoint3d_checkpoint_interval_does_not_change_the_gradient(hip_lib):
 
 
 def test_taylor_bar_gradients_reduced_size(hip_lib):
-    """Config c5 at reduced size (particle spacing x4: 7,056 particles,
-    16 x 47 x 16 nodes, 60 steps, a snapshot every 20): the inverse problem's
+    """Config c5 at reduced size (particle spacing x4: 7,700 particles,
+    11 x 41 x 11 nodes, 60 steps, a snapshot every 20): the inverse problem's
     loss L = sum |x_T - x*|^2 against a run with other (E, fy, hardening);
     its gradient by the reverse mode equals the dual-number run of the CPU
     restatement to 1e-9 and central differences of the GPU forward to 1e-6,
