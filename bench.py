@@ -185,7 +185,7 @@ def batched_leg(n_inst, steps, warmup, draw):
 
 
 def tubes_leg(steps, warmup):
-    """Config c3: lateral compression of an 8 x 8 array of CPDI-Q4 rings
+    """The CPDI tube array: lateral compression of an 8 x 8 array of CPDI-Q4 rings
     (outer radius 5 mm, wall 0.5 mm, 16 x 1200 domains each, corners jittered
     by 0.1 cell, Neo-Hookean E = 1 GPa) between rigid platens closing at 1 m/s,
     2D USL with multi-body contact between the 64 rings and the platens, on a
@@ -327,15 +327,16 @@ def adjoint_leg(steps, K_recompute=10):
 
 
 def taylor_leg(steps, K):
-    """Config c5: the differentiable Taylor bar (Main_TaylorBar.jl:43-130;
+    """The differentiable Taylor bar (Main_TaylorBar.jl:43-130;
     scenes.taylor_bar: copper cylinder r = 3.2 mm, L = 32.4 mm, 190 m/s onto
-    a rigid wall, J2, GIMP, 3D MUSL, 496,871 particles).  A run with hidden
+    a rigid wall, J2, GIMP, 3D MUSL, 499,928 particles).  A run with hidden
     parameters (E, fy, hardening drawn with seed 7) gives the target
     positions x*; the loss is L = sum |x_T - x*|^2 for the nominal
     parameters, and its gradient with respect to E, nu, fy, k1 and the
     initial velocity comes from the recorded run and the backward sweep with
-    a snapshot every K steps (checkpoint and recompute).  Timed against
-    `steps` plain steps."""
+    the state of every step kept (the automatic interval, when HBM holds
+    them) and, second, a snapshot every K steps (checkpoint and recompute);
+    both timed against `steps` plain steps."""
     import numpy as np
     import torch
 
@@ -380,17 +381,25 @@ def taylor_leg(steps, K):
         e.close()
         return t_fwd, t_back, g, loss, pullback.checkpoint_every, rc
 
-    run(K)  # the first run allocates the record and adjoint buffers, kept by the library
-    t_fwd, t_back, g, loss, k_used, rc = run(K)
-    return {"workload": f"taylor_bar: copper cylinder, J2, GIMP, 3D MUSL, {sc.n_particles} particles, "
-                        f"{sc.params.nx}x{sc.params.ny}x{sc.params.nz} nodes",
-            "particles": sc.n_particles, "steps": steps, "checkpoint_every": k_used,
-            "hidden": hidden, "loss": loss,
-            "gradient": {"E": float(g["E"][0]), "nu": float(g["nu"][0]), "fy": float(g["fy"][0]),
-                         "k1": float(g["k1"][0]), "v0": [float(x) for x in g["v0"][0]]},
-            "plain_ms_per_step": t_plain * 1e3 / steps,
-            "recorded_forward_ms_per_step": t_fwd * 1e3 / steps, "backward_ms_per_step": t_back * 1e3 / steps,
-            "adjoint_over_plain": (t_fwd + t_back) / t_plain, "step_flags": rc}
+    def timed(k):
+        run(k)  # the first run allocates the record and adjoint buffers, kept by the library
+        t_fwd, t_back, g, loss, k_used, rc = run(k)
+        return g, loss, {"checkpoint_every": k_used, "recorded_forward_ms_per_step": t_fwd * 1e3 / steps,
+                         "backward_ms_per_step": t_back * 1e3 / steps,
+                         "adjoint_over_plain": (t_fwd + t_back) / t_plain, "step_flags": rc}
+
+    g, loss, stored = timed(0)  # automatic interval: every state kept when HBM holds them
+    g2, _, rec = timed(K)
+    out = {"workload": f"taylor_bar: copper cylinder, J2, GIMP, 3D MUSL, {sc.n_particles} particles, "
+                       f"{sc.params.nx}x{sc.params.ny}x{sc.params.nz} nodes",
+           "particles": sc.n_particles, "steps": steps, "hidden": hidden, "loss": loss,
+           "gradient": {"E": float(g["E"][0]), "nu": float(g["nu"][0]), "fy": float(g["fy"][0]),
+                        "k1": float(g["k1"][0]), "v0": [float(x) for x in g["v0"][0]]},
+           "plain_ms_per_step": t_plain * 1e3 / steps}
+    out.update(stored)
+    out["recompute"] = rec
+    out["gradients_equal"] = all(bool((g[k] == g2[k]).all()) for k in g)
+    return out
 
 
 def main():

@@ -76,8 +76,18 @@ typedef enum {
  * densely and factors it, this library keeps it as a (2W-1)^2-point stencil
  * (W the stencil width) and solves by preconditioned conjugate gradients to
  * a relative residual of 1e-14 (JUMP_FLAG_SOLVER if that takes more than
- * 2000 iterations).  No contact and no rigid solids with it. */
-typedef enum { JUMP_ALG_USL = 0, JUMP_ALG_MUSL = 1 } jump_algorithm;
+ * 2000 iterations).  No contact and no rigid solids with it.
+ * JUMP_ALG_APIC (Algorithm.jl:41, declared by the reference with the
+ * per-particle affine matrix Cmat of Solid.jl:82, but never stepped): the 2D
+ * USL step with the affine particle-in-cell transfers of Jiang et al., "The
+ * affine particle-in-cell method" (ACM TOG 34(4), 2015): P2G carries
+ * m N (v_p + C_p (x_I - x_p)), G2P takes v_p = sum N v_I and
+ * C_p = B_p D_p^-1 (B_p = sum N v_I (x_I - x_p)^T, D_p = sum N (x_I - x_p)
+ * (x_I - x_p)^T; for the linear hat C_p = sum v_I grad N^T).  2D only; the
+ * linear, B-spline and GIMP bases; no CPDI, contact, batches or reverse mode.
+ * C_p starts at zero, as Cmat does (Solid.jl:111); jump_get_affine and
+ * jump_set_affine read and write it. */
+typedef enum { JUMP_ALG_USL = 0, JUMP_ALG_MUSL = 1, JUMP_ALG_APIC = 2 } jump_algorithm;
 
 /* Material kinds: Material.jl:29-31 (Rigid), :37-81 (Elastic/Hooke),
  * :100-122 (NeoHookean), :128-275 (ElastoPlastic J2), :280-441 (JohnsonCook). */
@@ -224,6 +234,14 @@ int jump_get_solid(jump_ctx* ctx, int32_t solid, double* pos, double* vel,
  * column-major matrix in 3D (Material.jl:151).  Any pointer may be NULL. */
 int jump_get_solid_state(jump_ctx* ctx, int32_t solid, double* alpha,
                          double* alphadot, double* vm, double* pstrain);
+
+/* APIC (JUMP_ALG_APIC): the affine matrix C_p of each particle of a solid,
+ * 4 doubles per particle, column-major, as Solid.Cmat holds it (Solid.jl:82,
+ * "APIC BpDp"; MMatrix{2,2}).  jump_set_affine writes it, for a start from
+ * an affine velocity field (C_p = grad v).  Other algorithms:
+ * JUMP_ERR_UNSUPPORTED. */
+int jump_get_affine(jump_ctx* ctx, int32_t solid, double* cmat);
+int jump_set_affine(jump_ctx* ctx, int32_t solid, const double* cmat);
 
 /* Copy the grid arrays of the last step (Grid.jl:78-90, 181-186), AoS
  * d*nodeCount for the vector fields.  Any pointer may be NULL. */

@@ -35,6 +35,7 @@ BASIS_GIMP = 4
 
 ALG_USL = 0
 ALG_MUSL = 1
+ALG_APIC = 2
 
 P2G_SHAPES = 6  # JUMP_P2G_SHAPES
 P2G_SHAPE_NAMES = ["compact/128", "compact/64", "compact/256", "rows/128", "rows/64", "rows/256"]
@@ -120,6 +121,8 @@ def _declare(lib, prefix):
         "flags": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_uint32)]),
         "get_solid": (ctypes.c_int, [H, _i32, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
         "get_solid_state": (ctypes.c_int, [H, _i32, _pd, _pd, _pd, _pd]),
+        "get_affine": (ctypes.c_int, [H, _i32, _pd]),
+        "set_affine": (ctypes.c_int, [H, _i32, _pd]),
         "get_grid": (ctypes.c_int, [H, _pd, _pd, _pd, _pd, _pd]),
         "energies": (ctypes.c_int, [H, _pd, _pd]),
     }
@@ -188,7 +191,8 @@ PHASES = ["sort", "p2g", "rigid", "g2p", "g2p1", "p2g2", "g2p2", "solve"]
 
 EXPORTED = [
     "jump_ctx_create", "jump_ctx_destroy", "jump_last_error", "jump_add_solid", "jump_set_fixed", "jump_step",
-    "jump_flags", "jump_get_solid", "jump_get_solid_state", "jump_get_grid", "jump_energies", "jump_get_bins",
+    "jump_flags", "jump_get_solid", "jump_get_solid_state", "jump_get_affine", "jump_set_affine", "jump_get_grid",
+    "jump_energies", "jump_get_bins",
     "jump_abi_version", "jump_profile", "jump_profile_read",
     "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
     "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies", "jump_batch_destroy",

@@ -48,6 +48,7 @@ struct jump_ctx {
   // authoritative while !on_device.
   int64_t np = 0;
   std::vector<double> hx[3], hv[3], hm, hV, hV0, hF[9], heps[6], hsig[6], halpha, halphadot, hvm, hep[6];
+  std::vector<double> hC[4];  // APIC: affine matrices, column-major
   std::vector<std::vector<double2>> hpairs;  // transfer buffers of the interleaved gather fields
   std::vector<int> hsid, helem[4];
   std::vector<uint8_t> hfixed;
@@ -247,6 +248,8 @@ static int alloc_particles(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &P.sid, n);
   if (ctx->prm.basis == JUMP_BASIS_CPDI_Q4)
     for (int k = 0; k < 4 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.elem[k], n);
+  if (ctx->G.apic)
+    for (int k = 0; k < 4 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.C[k], n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_key, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_rank, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_sorted, n + 1);
@@ -301,6 +304,8 @@ static int upload(jump_ctx* ctx) {
   if (!r) r = h2d(ctx, P.sid, ctx->hsid);
   if (ctx->prm.basis == JUMP_BASIS_CPDI_Q4)
     for (int k = 0; k < 4 && !r; k++) r = h2d(ctx, P.elem[k], ctx->helem[k]);
+  if (ctx->G.apic)
+    for (int k = 0; k < 4 && !r; k++) r = h2d(ctx, P.C[k], ctx->hC[k]);
   std::vector<SolidDev> sd;
   for (auto& s : ctx->solids) sd.push_back(s.dev);
   if (!r) r = h2d(ctx, ctx->d_solids, sd);
@@ -332,6 +337,8 @@ static int download(jump_ctx* ctx) {
   if (!r) r = d2h(ctx, ctx->halphadot, P.alphadot);
   if (!r) r = d2h(ctx, ctx->hvm, P.vm);
   for (int k = 0; k < S && !r; k++) r = d2h(ctx, ctx->hep[k], P.ep[k]);
+  if (ctx->G.apic)
+    for (int k = 0; k < 4 && !r; k++) r = d2h(ctx, ctx->hC[k], P.C[k]);
   if (r) return r;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   for (int f = 0; f < nf; f++) {
@@ -811,9 +818,12 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   if (p->dim == 3 && p->basis == JUMP_BASIS_CPDI_Q4) return JUMP_ERR_UNSUPPORTED;
   // The reference implements USL and MUSL (blended mass matrix) in 2D and
   // MUSL in 3D on this path (3D USL is marked unfinished and broken).
-  if (p->algorithm != JUMP_ALG_USL && p->algorithm != JUMP_ALG_MUSL) return JUMP_ERR_ARG;
+  if (p->algorithm != JUMP_ALG_USL && p->algorithm != JUMP_ALG_MUSL && p->algorithm != JUMP_ALG_APIC)
+    return JUMP_ERR_ARG;
   if (p->dim == 3 && p->algorithm != JUMP_ALG_MUSL) return JUMP_ERR_UNSUPPORTED;
   if (p->dim == 2 && p->algorithm == JUMP_ALG_MUSL && p->contact) return JUMP_ERR_UNSUPPORTED;
+  // APIC: the 2D USL step with affine transfers, for the point bases
+  if (p->algorithm == JUMP_ALG_APIC && (p->contact || p->basis == JUMP_BASIS_CPDI_Q4)) return JUMP_ERR_UNSUPPORTED;
   if (p->nx < 2 || p->ny < 2 || (p->dim == 3 && p->nz < 2)) return JUMP_ERR_ARG;
   const int64_t nz = p->dim == 3 ? p->nz : 1;
   if (p->nx >= 65536 || p->ny >= 65536 || nz >= 65536) return JUMP_ERR_UNSUPPORTED;
@@ -829,6 +839,7 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   ctx->device = device;
   GridDesc& G = ctx->G;
   G.dim = p->dim;
+  G.apic = p->algorithm == JUMP_ALG_APIC ? 1 : 0;
   G.nx = int(p->nx);
   G.ny = int(p->ny);
   G.nz = int(nz);
@@ -964,6 +975,8 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
     ctx->halphadot.push_back(0.0);
     ctx->hvm.push_back(0.0);
     ctx->hsid.push_back(sid);
+    if (ctx->G.apic)
+      for (int k = 0; k < 4; k++) ctx->hC[k].push_back(0.0);  // Solid.jl:111
     if (cpdi)
       for (int q = 0; q < 4; q++) ctx->helem[q].push_back(coff + int(ds->elems[4 * i + q]));
   }
@@ -1047,6 +1060,8 @@ static std::vector<std::vector<double>*> state_arrays(jump_ctx* ctx) {
   a.push_back(&ctx->halphadot);
   a.push_back(&ctx->hvm);
   for (int k = 0; k < S; k++) a.push_back(&ctx->hep[k]);
+  if (ctx->G.apic)
+    for (int k = 0; k < 4; k++) a.push_back(&ctx->hC[k]);
   return a;
 }
 
@@ -1171,6 +1186,36 @@ int jump_get_solid_state(jump_ctx* ctx, int32_t si, double* alpha, double* alpha
       }
     }
   }
+  return JUMP_OK;
+}
+
+int jump_get_affine(jump_ctx* ctx, int32_t si, double* cmat) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (si < 0 || size_t(si) >= ctx->solids.size() || !cmat) return set_err(ctx, JUMP_ERR_ARG, "bad solid or null array");
+  if (!ctx->G.apic) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "affine matrices exist in APIC contexts only");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int r = download(ctx);
+  if (r) return r;
+  const HostSolid& s = ctx->solids[size_t(si)];
+  const size_t o = size_t(s.offset);
+  for (size_t i = 0; i < size_t(s.n); i++)
+    for (int k = 0; k < 4; k++) cmat[4 * i + size_t(k)] = ctx->hC[k][o + i];
+  return JUMP_OK;
+}
+
+int jump_set_affine(jump_ctx* ctx, int32_t si, const double* cmat) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (si < 0 || size_t(si) >= ctx->solids.size() || !cmat) return set_err(ctx, JUMP_ERR_ARG, "bad solid or null array");
+  if (!ctx->G.apic) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "affine matrices exist in APIC contexts only");
+  HIP_TRY(hipSetDevice(ctx->device));
+  int r = download(ctx);  // the other solids' current state
+  if (r) return r;
+  const HostSolid& s = ctx->solids[size_t(si)];
+  const size_t o = size_t(s.offset);
+  for (size_t i = 0; i < size_t(s.n); i++)
+    for (int k = 0; k < 4; k++) ctx->hC[k][o + i] = cmat[4 * i + size_t(k)];
+  ctx->on_device = false;  // the next call that needs the device uploads it
+  ctx->adj.ready = false;
   return JUMP_OK;
 }
 
@@ -1817,28 +1862,44 @@ int jump_batch_tuning(jump_batch* b, int32_t g, int32_t* ngroups, int32_t* shape
 }
 
 // ------------------------------------------------------------------ reverse mode
-// The 2D USL step differentiated by hand (mpm_adjoint.hpp).  The recorded
-// run keeps the particle state every K steps; the backward sweep rebuilds
-// each segment's states from its snapshot (keeping the grid each step used),
-// then walks the segment's steps backwards.  Memory: (nsteps/K + K + 1)
-// snapshots of 136 bytes per particle and K grids of 40 bytes per node.
-// A snapshot is the particle state a step reads: the pair arrays (x, m, V,
-// v, sigma), F and the strain, and in 3D the J2 history (alpha and the
-// plastic strain): 17 doubles per particle in 2D, 36 in 3D.
-static size_t snap_doubles(const jump_ctx* ctx) {
-  return size_t(ctx->np) * (ctx->D == 2 ? 2 * PairLayout<2>::NP + 7 : 2 * PairLayout<3>::NP + 9 + 6 + 1 + 6);
+// The 2D USL and 3D MUSL steps differentiated by hand (mpm_adjoint.hpp,
+// mpm_adjoint3.hpp).  The recorded run keeps the particle state every K
+// steps; the backward sweep rebuilds each segment's states from its snapshot
+// (keeping the grid and bins each step used), then walks the segment's steps
+// backwards.  Memory: (nsteps/K + K + 1) snapshots of 136 (2D) or 288 (3D)
+// bytes per particle, K grids of 40 or 80 bytes per node and K bins.
+
+// Copies the arrays of `segs` (device pointer, bytes) to or from the buffer
+// `buf`, laid out back to back with each array starting 256-byte aligned, in
+// one launch.
+static size_t seg_offset(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+static int copy_segments(jump_ctx* ctx, char* buf, const std::vector<std::pair<void*, size_t>>& segs, bool save) {
+  CopyTable t{};
+  size_t off = 0;
+  long long most = 0;
+  if (segs.size() > size_t(kCopySegs)) return set_err(ctx, JUMP_ERR_STATE, "too many arrays for one copy launch");
+  for (size_t k = 0; k < segs.size(); k++) {
+    unsigned* b = reinterpret_cast<unsigned*>(buf + off);
+    unsigned* a = static_cast<unsigned*>(segs[k].first);
+    t.src[k] = save ? a : b;
+    t.dst[k] = save ? b : a;
+    t.words[k] = (long long)(segs[k].second / 4);
+    most = std::max(most, t.words[k]);
+    off += seg_offset(segs[k].second);
+  }
+  if (segs.empty() || most == 0) return JUMP_OK;
+  const unsigned bx = unsigned(std::min<long long>((most / 4 + kBlock) / kBlock, 1024));
+  hipLaunchKernelGGL(k_copy_table, dim3(bx, unsigned(segs.size())), dim3(kBlock), 0, ctx->stream, t);
+  return JUMP_OK;
 }
 
-static int snap_copy(jump_ctx* ctx, double* buf, bool save) {
+// A snapshot is the particle state a step reads: the pair arrays (x, m, V,
+// v, sigma), F and the strain, and in 3D the J2 history (alpha and the
+// plastic strain): 17 arrays of n doubles in 2D, 36 in 3D (pairs counted
+// twice), each 256-byte aligned in the buffer.
+static std::vector<std::pair<void*, size_t>> snap_segments(const jump_ctx* ctx) {
   const size_t n = size_t(ctx->np);
   const int D = ctx->D;
-  double* q = buf;
-  auto cp = [&](void* dev, size_t bytes) {
-    hipError_t e = save ? hipMemcpyAsync(q, dev, bytes, hipMemcpyDeviceToDevice, ctx->stream)
-                        : hipMemcpyAsync(dev, q, bytes, hipMemcpyDeviceToDevice, ctx->stream);
-    q += bytes / sizeof(double);
-    return e;
-  };
   std::vector<std::pair<void*, size_t>> arrays;
   const int npair = D == 2 ? PairLayout<2>::NP : PairLayout<3>::NP;
   for (int j = 0; j < npair; j++) arrays.push_back({ctx->P.pr[j], n * sizeof(double2)});
@@ -1848,10 +1909,18 @@ static int snap_copy(jump_ctx* ctx, double* buf, bool save) {
     arrays.push_back({ctx->P.alpha, n * sizeof(double)});
     for (int k = 0; k < 6; k++) arrays.push_back({ctx->P.ep[k], n * sizeof(double)});
   }
-  for (auto& a : arrays)
-    if (cp(a.first, a.second) != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "snapshot copy");
+  return arrays;
+}
+static size_t snap_bytes(const jump_ctx* ctx) {
+  size_t b = 0;
+  for (auto& a : snap_segments(ctx)) b += seg_offset(a.second);
+  return b;
+}
+
+static int snap_copy(jump_ctx* ctx, double* buf, bool save) {
+  int r = copy_segments(ctx, reinterpret_cast<char*>(buf), snap_segments(ctx), save);
   if (!save) ctx->runs_valid = false;  // a restored state: the next step merges at every node
-  return JUMP_OK;
+  return r;
 }
 
 static int adjoint_supported(jump_ctx* ctx) {
@@ -1872,20 +1941,17 @@ static int adjoint_supported(jump_ctx* ctx) {
 }
 
 // The bins a step sorted its particles into: start (NN+1) and sorted (np+1).
+static std::vector<std::pair<void*, size_t>> bins_segments(const jump_ctx* ctx) {
+  return {{ctx->d_start, size_t(ctx->NN + 1) * sizeof(int)},
+          {ctx->d_sorted, size_t(ctx->np + 1) * sizeof(unsigned long long)}};
+}
 static size_t bins_bytes(const jump_ctx* ctx) {
-  return size_t(ctx->NN + 1) * sizeof(int) + size_t(ctx->np + 1) * sizeof(unsigned long long);
+  size_t b = 0;
+  for (auto& a : bins_segments(ctx)) b += seg_offset(a.second);
+  return b;
 }
 static int bins_copy(jump_ctx* ctx, void* buf, bool save) {
-  const size_t sb = size_t(ctx->NN + 1) * sizeof(int), qb = size_t(ctx->np + 1) * sizeof(unsigned long long);
-  char* q = static_cast<char*>(buf);
-  hipError_t e1 = save ? hipMemcpyAsync(q, ctx->d_start, sb, hipMemcpyDeviceToDevice, ctx->stream)
-                       : hipMemcpyAsync(ctx->d_start, q, sb, hipMemcpyDeviceToDevice, ctx->stream);
-  // the sorted array is kept 8-byte aligned after the start array
-  char* q2 = q + ((sb + 255) & ~size_t(255));
-  hipError_t e2 = save ? hipMemcpyAsync(q2, ctx->d_sorted, qb, hipMemcpyDeviceToDevice, ctx->stream)
-                       : hipMemcpyAsync(ctx->d_sorted, q2, qb, hipMemcpyDeviceToDevice, ctx->stream);
-  if (e1 != hipSuccess || e2 != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "bins copy");
-  return JUMP_OK;
+  return copy_segments(ctx, static_cast<char*>(buf), bins_segments(ctx), save);
 }
 
 // The grid a step's G2P used: nodal mass and momenta (2D: m, p0, p, 5
@@ -1899,14 +1965,12 @@ static int grid_copy(jump_ctx* ctx, double* buf, bool save) {
                       ctx->g.p[1], ctx->g.p[2], ctx->g.p2[0], ctx->g.p2[1], ctx->g.p2[2]};
     for (int a = 0; a < 10; a++) arrs[a] = a3[a];
   }
-  for (int a = 0; a < grid_arrays(ctx); a++) {
-    hipError_t e = save ? hipMemcpyAsync(buf + a * nn, arrs[a], nn * sizeof(double), hipMemcpyDeviceToDevice,
-                                         ctx->stream)
-                        : hipMemcpyAsync(arrs[a], buf + a * nn, nn * sizeof(double), hipMemcpyDeviceToDevice,
-                                         ctx->stream);
-    if (e != hipSuccess) return set_err(ctx, JUMP_ERR_DEVICE, "grid copy");
-  }
-  return JUMP_OK;
+  std::vector<std::pair<void*, size_t>> segs;
+  for (int a = 0; a < grid_arrays(ctx); a++) segs.push_back({arrs[a], nn * sizeof(double)});
+  return copy_segments(ctx, reinterpret_cast<char*>(buf), segs, save);
+}
+static size_t grid_bytes(const jump_ctx* ctx) {
+  return size_t(grid_arrays(ctx)) * seg_offset(size_t(ctx->NN) * sizeof(double));
 }
 
 template <int B> static void adj_step_back3(jump_ctx* ctx) {
@@ -1937,24 +2001,22 @@ template <int B> static void adj_step_back(jump_ctx* ctx) {
 // Device bytes a recorded run of nsteps steps at interval K holds: nck
 // snapshots, K per-step states, grids and bins, the final state.
 static size_t adjoint_record_bytes(const jump_ctx* ctx, int64_t nsteps, int64_t K) {
-  const size_t snap = snap_doubles(ctx) * sizeof(double);
-  const size_t per_step = snap + size_t(grid_arrays(ctx)) * size_t(ctx->NN) * sizeof(double) + bins_bytes(ctx) + 256;
+  const size_t snap = snap_bytes(ctx);
+  const size_t per_step = snap + grid_bytes(ctx) + bins_bytes(ctx);
   const size_t nck = size_t((nsteps + K - 1) / K);
   return (nck + 1) * snap + size_t(std::min(K, nsteps)) * per_step;
 }
 
 // The largest interval whose record fits in 3/4 of the device memory that is
 // free once this context's previous record is released.  The adjoint arrays
-// of the backward sweep (about 25 doubles per particle) are set aside first.
+// of the backward sweep (2D 25, 3D 54 doubles per particle) are set aside first.
 static int adjoint_auto_interval(jump_ctx* ctx, int64_t nsteps, int64_t* K) {
   size_t freeb = 0, total = 0;
   HIP_TRY(hipMemGetInfo(&freeb, &total));
   const auto& A = ctx->adj;
   if (A.snap_np == ctx->np) {  // the buffers held now are freed or reused
-    const size_t snap = snap_doubles(ctx) * sizeof(double);
-    freeb += (A.ckpt.size() + A.segbuf.size() + (A.final_state ? 1 : 0)) * snap +
-             A.gridbuf.size() * size_t(grid_arrays(ctx)) * size_t(ctx->NN) * sizeof(double) +
-             A.binbuf.size() * (bins_bytes(ctx) + 256);
+    freeb += (A.ckpt.size() + A.segbuf.size() + (A.final_state ? 1 : 0)) * snap_bytes(ctx) +
+             A.gridbuf.size() * grid_bytes(ctx) + A.binbuf.size() * bins_bytes(ctx);
   }
   // the adjoint arrays of the backward sweep: 2D 25 doubles per particle and
   // 6 per node; 3D 54 and 10
@@ -1978,7 +2040,7 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
   auto& A = ctx->adj;
   int64_t K = K_arg;
   if (K == 0 && (r = adjoint_auto_interval(ctx, nsteps, &K))) return r;
-  const size_t sd = snap_doubles(ctx);
+  const size_t sd = snap_bytes(ctx) / sizeof(double);
   const int64_t nck = (nsteps + K - 1) / K;
   const size_t nseg = size_t(std::min<int64_t>(K, nsteps));
   // buffers are kept from run to run while they are large enough
@@ -1997,11 +2059,11 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
       if ((r = alloc(&c, sd))) return r;
     A.gridbuf.resize(nseg);
     for (auto& c : A.gridbuf)
-      if ((r = alloc(&c, size_t(grid_arrays(ctx)) * size_t(ctx->NN)))) return r;
+      if ((r = alloc(&c, grid_bytes(ctx) / sizeof(double)))) return r;
     A.binbuf.resize(nseg);
     for (auto& c : A.binbuf) {
       char* q = nullptr;
-      if ((r = dev_alloc(ctx, A.allocs, &q, bins_bytes(ctx) + 256))) return r;
+      if ((r = dev_alloc(ctx, A.allocs, &q, bins_bytes(ctx)))) return r;
       c = q;
     }
     if ((r = alloc(&A.final_state, sd))) return r;
@@ -2172,15 +2234,18 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
       arr.a[nsum++] = a;
   }
   std::vector<long long> range(2 * ns);
+  long long most = 1;
   for (size_t si = 0; si < ns; si++) {
     range[2 * si] = ctx->solids[si].offset;
     range[2 * si + 1] = ctx->solids[si].offset + ctx->solids[si].n;
+    most = std::max<long long>(most, ctx->solids[si].n);
   }
+  const int nchunk = int((most + kSumChunk - 1) / kSumChunk);
   long long* d_range = nullptr;
-  double* d_sums = nullptr;
+  double *d_sums = nullptr, *d_part = nullptr;
   std::vector<void*> tmp;
-  if ((r = dev_alloc(ctx, tmp, &d_range, range.size()))) return r;
-  if ((r = dev_alloc(ctx, tmp, &d_sums, size_t(nsum) * ns))) {
+  if ((r = dev_alloc(ctx, tmp, &d_range, range.size())) || (r = dev_alloc(ctx, tmp, &d_sums, size_t(nsum) * ns)) ||
+      (r = dev_alloc(ctx, tmp, &d_part, size_t(nsum) * ns * size_t(nchunk)))) {
     free_list(tmp);
     return r;
   }
@@ -2188,7 +2253,11 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
   hipError_t he = hipMemcpyAsync(d_range, range.data(), range.size() * sizeof(long long), hipMemcpyHostToDevice,
                                  ctx->stream);
   if (he == hipSuccess) {
-    hipLaunchKernelGGL(k_adj_sums, dim3(unsigned(ns)), dim3(kBlock), 0, ctx->stream, d_range, arr, nsum, d_sums);
+    for (size_t s0 = 0; s0 < ns; s0 += 65535)
+      hipLaunchKernelGGL(k_adj_sums_part, dim3(unsigned(nchunk), unsigned(std::min<size_t>(ns - s0, 65535))),
+                         dim3(kBlock), 0, ctx->stream, d_range, arr, nsum, int(s0), d_part);
+    hipLaunchKernelGGL(k_adj_sums_final, dim3(unsigned(ns)), dim3(kBlock), 0, ctx->stream, d_part, nchunk, nsum,
+                       d_sums);
     he = hipMemcpyAsync(sums.data(), d_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   }
   if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);

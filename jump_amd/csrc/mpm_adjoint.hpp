@@ -331,6 +331,31 @@ __global__ void __launch_bounds__(kBlock) k_adj_p2g(Particles P, GridDesc G, con
   ad.Vb[i] += Vb;
 }
 
+// Many device-to-device copies in one launch (the state, grid and bins a
+// recorded step keeps): segment blockIdx.y copies words[y] 4-byte words from
+// src[y] to dst[y], 16 bytes per thread, grid-stride over blockIdx.x.  Every
+// array the library allocates starts 256-byte aligned and is a whole number
+// of 4-byte words long.
+constexpr int kCopySegs = 48;
+struct CopyTable {
+  const unsigned* src[kCopySegs];
+  unsigned* dst[kCopySegs];
+  long long words[kCopySegs];
+};
+__global__ void __launch_bounds__(kBlock) k_copy_table(CopyTable t) {
+  const int sg = blockIdx.y;
+  const unsigned* __restrict__ src = t.src[sg];
+  unsigned* __restrict__ dst = t.dst[sg];
+  const long long n = t.words[sg], n4 = n >> 2;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint4* __restrict__ s4 = reinterpret_cast<const uint4*>(src);
+  uint4* __restrict__ d4 = reinterpret_cast<uint4*>(dst);
+  for (long long i = i0; i < n4; i += stride) d4[i] = s4[i];
+  const long long tail = 4 * n4 + i0;  // the last n % 4 words
+  if (tail < n) dst[tail] = src[tail];
+}
+
 __global__ void __launch_bounds__(kBlock) k_fill_f64(double* __restrict__ a, double v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;

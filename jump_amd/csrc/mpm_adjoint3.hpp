@@ -504,27 +504,47 @@ __global__ void __launch_bounds__(kBlock) k_adj3_p2g(Particles P, GridDesc G, co
 }
 
 // Per-solid sums of per-particle adjoint arrays (nsum of them, array k at
-// arr[k]): one block per solid over its particle range.
+// arr[k]), in two passes of fixed order: block (c, solid) sums chunk c of the
+// solid's particle range (kSumChunk particles) into part[solid][c][k]
+// (solid = si0 + blockIdx.y, the grid's y extent being limited); then
+// one block per solid sums its nchunk partials.
 constexpr int kAdjSums = 7;
+constexpr int kSumChunk = 8192;
 struct AdjSumArrays {
   const double* a[kAdjSums];
 };
-__global__ void __launch_bounds__(kBlock) k_adj_sums(const long long* __restrict__ range, AdjSumArrays arr, int nsum,
-                                                     double* __restrict__ out) {
-  __shared__ double sh[kAdjSums][kBlock];
-  const long long b = range[2 * blockIdx.x], e = range[2 * blockIdx.x + 1];
-  double acc[kAdjSums];
-  for (int k = 0; k < kAdjSums; k++) acc[k] = 0.0;
-  for (long long i = b + threadIdx.x; i < e; i += kBlock)
-    for (int k = 0; k < nsum; k++) acc[k] += arr.a[k][i];
+__device__ __forceinline__ void block_sum(double (*sh)[kBlock], double* acc, int nsum) {
   for (int k = 0; k < kAdjSums; k++) sh[k][threadIdx.x] = acc[k];
   __syncthreads();
   for (int off = kBlock / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off)
-      for (int k = 0; k < kAdjSums; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+      for (int k = 0; k < nsum; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x < nsum) out[nsum * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ void __launch_bounds__(kBlock) k_adj_sums_part(const long long* __restrict__ range, AdjSumArrays arr,
+                                                          int nsum, int si0, double* __restrict__ part) {
+  __shared__ double sh[kAdjSums][kBlock];
+  const int c = blockIdx.x, si = si0 + blockIdx.y;
+  const long long e = range[2 * si + 1];
+  const long long b = range[2 * si] + (long long)c * kSumChunk, ce = b + kSumChunk < e ? b + kSumChunk : e;
+  double acc[kAdjSums];
+  for (int k = 0; k < kAdjSums; k++) acc[k] = 0.0;
+  for (long long i = b + threadIdx.x; i < ce; i += kBlock)
+    for (int k = 0; k < nsum; k++) acc[k] += arr.a[k][i];
+  block_sum(sh, acc, nsum);
+  if (threadIdx.x < nsum) part[(size_t(si) * gridDim.x + c) * nsum + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ void __launch_bounds__(kBlock) k_adj_sums_final(const double* __restrict__ part, int nchunk, int nsum,
+                                                           double* __restrict__ out) {
+  __shared__ double sh[kAdjSums][kBlock];
+  const int si = blockIdx.x;
+  double acc[kAdjSums];
+  for (int k = 0; k < kAdjSums; k++) acc[k] = 0.0;
+  for (int c = threadIdx.x; c < nchunk; c += kBlock)
+    for (int k = 0; k < nsum; k++) acc[k] += part[(size_t(si) * nchunk + c) * nsum + k];
+  block_sum(sh, acc, nsum);
+  if (threadIdx.x < nsum) out[nsum * si + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 }  // namespace jh

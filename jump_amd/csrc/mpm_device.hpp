@@ -25,6 +25,7 @@ namespace jh {
 
 struct GridDesc {
   int dim;
+  int apic;  // the step's transfers are APIC (2D): P2G adds m N C_p (x_I - x_p), G2P forms C_p
   int nx, ny, nz, nxy, nn;
   double xmin, ymin, zmin;
   double dx, dy, dz, dxI, dyI, dzI;
@@ -79,6 +80,7 @@ struct Particles {
   int* sid;      // solid index of each particle
   int* elem[4];  // CPDI: global corner index of each domain corner
   double2* pr[7];  // the pair arrays x, m, V, v and sigma live in
+  double* C[4];    // APIC: affine matrix C_p, column-major (2D)
   int n;
 };
 

@@ -366,7 +366,16 @@ __device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G,
     const double* sg = v + 2 * D + 2;
     const double Nim = N * fM;
     am += Nim;
-    for (int k = 0; k < D; k++) ap[k] += Nim * vp[k];
+    if (D == 2 && G.apic) {
+      // APIC: momentum of the affine field v_p + C_p (x_I - x_p) at the node
+      const double e0 = (G.xmin + double(ni) * G.dx) - v[0], e1 = (G.ymin + double(nj) * G.dy) - v[1];
+      const double C0 = P.C[0][p], C1 = P.C[1][p], C2 = P.C[2][p], C3 = P.C[3][p];
+      const double a0 = C0 * e0 + C2 * e1, a1 = C1 * e0 + C3 * e1;
+      ap[0] += Nim * (vp[0] + a0);
+      ap[1] += Nim * (vp[1] + a1);
+    } else {
+      for (int k = 0; k < D; k++) ap[k] += Nim * vp[k];
+    }
     if (D == 2) {
       af[0] += (-fV) * (sg[0] * dN[0] + sg[2] * dN[1]) + (fM * G.body[0]) * N;
       af[1] += (-fV) * (sg[2] * dN[0] + sg[1] * dN[1]) + (fM * G.body[1]) * N;
@@ -1138,6 +1147,66 @@ __device__ __forceinline__ void usl2d_particle_update(const Particles& P, const 
   if (fl) atomicOr(flags, fl);
 }
 
+// APIC G2P (Jiang et al., "The affine particle-in-cell method", ACM TOG
+// 34(4) 2015) in the 2D USL step; the reference declares APIC and the
+// affine matrix Cmat (Algorithm.jl:41, Solid.jl:82) without a step.  Over
+// the nodes with m_I > tol, vI = p_I / m_I: v_p = sum N vI (PIC),
+// B = sum (N vI) (x_I - x_p)^T, x and L as the USL gather; over the whole
+// stencil D = sum (N (x_I - x_p)) (x_I - x_p)^T; C_p = B D^-1 by cofactors,
+// or C_p = L for the linear hat (D singular on grid lines; B D^-1 equals
+// sum vI grad N^T for multilinear weights).
+template <int B>
+__device__ __forceinline__ void g2p_apic2d(const Particles& P, const GridDesc& G, const SolidDev& s,
+                                           const double* corners, const GridArrays& g, double tol, double dt, int i,
+                                           unsigned* flags) {
+  const double xp0 = P.x[0][i], xp1 = P.x[1][i];
+  double v0 = 0.0, v1 = 0.0, x0 = xp0, x1 = xp1;
+  double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  double B0 = 0.0, B1 = 0.0, B2 = 0.0, B3 = 0.0, D0 = 0.0, D1 = 0.0, D3 = 0.0;
+  bool ok = walk_stencil<2, B, false, true>(P, G, s, corners, i, [&](int n, double Ni, const double* dN) {
+    const double e0 = (G.xmin + double(n % G.nx) * G.dx) - xp0, e1 = (G.ymin + double(n / G.nx) * G.dy) - xp1;
+    const double n0 = Ni * e0, n1 = Ni * e1;
+    D0 += n0 * e0;
+    D1 += n0 * e1;
+    D3 += n1 * e1;
+    const double mI = g.m[n];
+    if (mI > tol) {
+      const double invM = 1.0 / mI;
+      const double vI0 = g.p[0][n] * invM, vI1 = g.p[1][n] * invM;
+      const double w0 = Ni * vI0, w1 = Ni * vI1;
+      v0 += w0;
+      v1 += w1;
+      x0 += Ni * vI0 * dt;
+      x1 += Ni * vI1 * dt;
+      L0 += dN[0] * vI0;
+      L1 += dN[0] * vI1;
+      L2 += dN[1] * vI0;
+      L3 += dN[1] * vI1;
+      B0 += w0 * e0;
+      B1 += w1 * e0;
+      B2 += w0 * e1;
+      B3 += w1 * e1;
+    }
+  });
+  if (!ok) {
+    atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
+    return;
+  }
+  double C[4];
+  if (B == JUMP_BASIS_LINEAR) {
+    C[0] = L0; C[1] = L1; C[2] = L2; C[3] = L3;
+  } else {
+    const double det = D0 * D3 - D1 * D1;
+    const double i0 = D3 / det, i1 = (-D1) / det, i3 = D0 / det;
+    C[0] = B0 * i0 + B2 * i1;
+    C[1] = B1 * i0 + B3 * i1;
+    C[2] = B0 * i1 + B2 * i3;
+    C[3] = B1 * i1 + B3 * i3;
+  }
+  for (int k = 0; k < 4; k++) P.C[k][i] = C[k];
+  usl2d_particle_update(P, s, i, dt, v0, v1, x0, x1, L0, L1, L2, L3, flags);
+}
+
 // SolveExplicitSolidMechanics2D.jl:485-534
 template <int B, bool MULTI>
 __global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
@@ -1151,6 +1220,10 @@ __global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, 
   const GridArrays g = MULTI ? grid_view(g0, Bt.node_off[s.inst]) : g0;
   const double tol = MULTI ? Bt.inst[s.inst].tol : tol0;
   const double dt = MULTI ? Bt.inst[s.inst].dt : dt0;
+  if (G.apic) {
+    g2p_apic2d<B>(P, G, s, corners, g, tol, dt, i, flags);
+    return;
+  }
   double v0 = P.v[0][i], v1 = P.v[1][i];
   double x0 = P.x[0][i], x1 = P.x[1][i];
   double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;

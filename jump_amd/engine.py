@@ -162,6 +162,18 @@ class Engine:
                                                              ("alpha", "alphadot", "vm", "pstrain")]))
         return out
 
+    def get_affine(self, s):
+        """APIC: the solid's affine matrices C_p, (n, 4) column-major."""
+        out = np.empty((self.counts[s], 4))
+        self._check(self._f("get_affine")(self.h, s, _ffi.dptr(out)))
+        return out
+
+    def set_affine(self, s, cmat):
+        c = np.ascontiguousarray(cmat, dtype=np.float64)
+        if c.shape != (self.counts[s], 4):
+            raise ValueError(f"cmat must have shape ({self.counts[s]}, 4), got {c.shape}")
+        self._check(self._f("set_affine")(self.h, s, _ffi.dptr(c)))
+
     def get_grid(self):
         d, nn = self.dim, self.nn
         out = dict(mass=np.empty(nn), momentum0=np.empty((nn, d)), momentum=np.empty((nn, d)),

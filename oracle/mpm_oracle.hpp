@@ -94,6 +94,7 @@ template <class T> struct Solid {
   std::vector<T> mass, vol0, vol, pos, vel, F, strain, stress;  // AoS
   // material history, per particle (reference keeps it in the material)
   std::vector<T> alpha, alphadot, vm, pstrain;  // pstrain: 3 (2D) or 9 (3D)
+  std::vector<T> cmat;  // APIC: the affine matrix C_p = B_p D_p^-1, 4 per particle column-major (Solid.jl:82)
   // CPDI-Q4 particle domains: static corners and 0-based connectivity
   std::vector<T> corners;
   std::vector<i64> elems;
@@ -884,6 +885,137 @@ template <class T> void step_usl_2d(Ctx<T>& g, const T& t) {
     }
   }
   // CPDI corner update (:556-572) only moves a local copy: no effect.
+  (void)t;
+}
+
+// APIC in the 2D USL step: the affine particle-in-cell transfers of Jiang,
+// Schroeder, Selle, Teran and Stomakhin, "The affine particle-in-cell
+// method", ACM TOG 34(4) 2015, on the step above.  The reference declares
+// the algorithm (Algorithm.jl:41) and the per-particle affine matrix
+// (Solid.jl:82, "APIC BpDp", zero-initialised at :111) but has no step for
+// it.  Differences from the USL step:
+//   P2G momentum0 += N m (v_p + C_p (x_I - x_p))
+//   G2P over the nodes with m_I > tol, vI = p_I / m_I:
+//     v_p = sum N vI (PIC: replaces the FLIP increment)
+//     B = sum (N vI) (x_I - x_p)^T,  D = sum (N (x_I - x_p)) (x_I - x_p)^T over
+//     the whole stencil,  C_p = B D^-1 (2x2 inverse by cofactors);
+//     for the linear hat D is singular on grid lines and C_p = sum vI grad N^T
+//     (= L), the multilinear identity B D^-1 = sum vI grad N^T of the paper
+//   x, L, F, strain, stress: as USL, from the position before the step.
+// Node coordinates x_I = min + (i-1) h (Grid.jl:149-152).  Floating-point
+// order (the GPU follows it term by term):
+//   d = (xI - xp, yI - yp)
+//   P2G: a = (C0 d0 + C2 d1, C1 d0 + C3 d1); p0 += Nim (v + a)
+//   G2P: w = N vI; B0 += w0 d0; B1 += w1 d0; B2 += w0 d1; B3 += w1 d1
+//        e = N d; D0 += e0 d0; D1 += e0 d1; D3 += e1 d1
+//        det = D0 D3 - D1 D1; (i0, i1, i3) = (D3, -D1, D0) / det
+//        C = (B0 i0 + B2 i1, B1 i0 + B3 i1, B0 i1 + B2 i3, B1 i1 + B3 i3)
+template <class T> void step_apic_2d(Ctx<T>& g, const T& t) {
+  const T dt = g.dt;
+  grid_reset(g, false);
+  for (auto& s : g.solids) {
+    if (s.rigid) continue;
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, true);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T fV = s.vol[ip], fM = s.mass[ip];
+      const T* vp = &s.vel[2 * ip];
+      const T* sg = &s.stress[4 * ip];
+      const T* C = &s.cmat[4 * ip];
+      const T xp0 = s.pos[2 * ip], xp1 = s.pos[2 * ip + 1];
+      T b0 = fM * g.body[0], b1 = fM * g.body[1];
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        T Ni = g.funcs[i];
+        T d1 = g.ders[2 * i], d2 = g.ders[2 * i + 1];
+        T e0 = node_x(g, in % g.nx + 1) - xp0;
+        T e1 = node_y(g, in / g.nx + 1) - xp1;
+        T a0 = C[0] * e0 + C[2] * e1, a1 = C[1] * e0 + C[3] * e1;
+        T Nim = Ni * fM;
+        g.gmass[in] += Nim;
+        g.gp0[2 * in] += Nim * (vp[0] + a0);
+        g.gp0[2 * in + 1] += Nim * (vp[1] + a1);
+        T fx = (-fV) * (sg[0] * d1 + sg[2] * d2) + b0 * Ni;
+        T fy = (-fV) * (sg[1] * d1 + sg[3] * d2) + b1 * Ni;
+        g.gf[2 * in] += fx;
+        g.gf[2 * in + 1] += fy;
+      }
+    }
+  }
+  for (i64 i = 0; i < g.nn; i++) {
+    for (int k = 0; k < 2; k++) g.gp[2 * i + k] = g.gp0[2 * i + k] + g.gf[2 * i + k] * dt;
+    for (int k = 0; k < 2; k++)
+      if (g.fixed[2 * i + k]) { g.gp0[2 * i + k] = 0.0; g.gp[2 * i + k] = 0.0; }
+  }
+  for (auto& s : g.solids) {  // rigid override, as the USL step
+    if (!s.rigid) continue;
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = sup_linear_2d(g, &s.pos[2 * ip]);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      for (int i = 0; i < 4; i++) {
+        i64 id = g.near[i];
+        T mi = g.gmass[id];
+        if (s.mat.fixed) {
+          g.gp[2 * id] = 0.0; g.gp[2 * id + 1] = 0.0;
+          g.gp0[2 * id] = 0.0; g.gp0[2 * id + 1] = 0.0;
+        } else {
+          if (val(s.mat.vx) != 0.0) { g.gp[2 * id] = mi * s.mat.vx; g.gp0[2 * id] = mi * s.mat.vx; }
+          if (val(s.mat.vy) != 0.0) { g.gp[2 * id + 1] = mi * s.mat.vy; g.gp0[2 * id + 1] = mi * s.mat.vy; }
+        }
+      }
+    }
+  }
+  for (auto& s : g.solids) {
+    if (s.rigid) continue;
+    for (i64 ip = 0; ip < s.n; ip++) {
+      int sup = support(g, s, ip, true);
+      if (sup < 0) { g.flags |= JUMP_FLAG_OUT_OF_GRID; continue; }
+      T L[4] = {0.0, 0.0, 0.0, 0.0}, Bm[4] = {0.0, 0.0, 0.0, 0.0}, Dm[4] = {0.0, 0.0, 0.0, 0.0};
+      T v0 = 0.0, v1 = 0.0;
+      const T xp0 = s.pos[2 * ip], xp1 = s.pos[2 * ip + 1];
+      T x0 = xp0, x1 = xp1;
+      for (int i = 0; i < sup; i++) {
+        i64 in = g.near[i];
+        T Ni = g.funcs[i];
+        T d1 = g.ders[2 * i], d2 = g.ders[2 * i + 1];
+        T e0 = node_x(g, in % g.nx + 1) - xp0;
+        T e1 = node_y(g, in / g.nx + 1) - xp1;
+        T n0 = Ni * e0, n1 = Ni * e1;
+        Dm[0] += n0 * e0; Dm[1] += n0 * e1; Dm[3] += n1 * e1;
+        T mI = g.gmass[in];
+        if (val(mI) > val(g.tol)) {
+          T invM = 1.0 / mI;
+          T vI0 = g.gp[2 * in] * invM, vI1 = g.gp[2 * in + 1] * invM;
+          T w0 = Ni * vI0, w1 = Ni * vI1;
+          v0 += w0;
+          v1 += w1;
+          x0 += Ni * vI0 * dt;
+          x1 += Ni * vI1 * dt;
+          L[0] += d1 * vI0; L[1] += d1 * vI1; L[2] += d2 * vI0; L[3] += d2 * vI1;
+          Bm[0] += w0 * e0; Bm[1] += w1 * e0; Bm[2] += w0 * e1; Bm[3] += w1 * e1;
+        }
+      }
+      T* C = &s.cmat[4 * ip];
+      if (g.basis == JUMP_BASIS_LINEAR) {
+        for (int k = 0; k < 4; k++) C[k] = L[k];
+      } else {
+        T det = Dm[0] * Dm[3] - Dm[1] * Dm[1];
+        T i0 = Dm[3] / det, i1 = (-Dm[1]) / det, i3 = Dm[0] / det;
+        C[0] = Bm[0] * i0 + Bm[2] * i1;
+        C[1] = Bm[1] * i0 + Bm[3] * i1;
+        C[2] = Bm[0] * i1 + Bm[2] * i3;
+        C[3] = Bm[1] * i1 + Bm[3] * i3;
+      }
+      usl2d_particle_update(g, s, ip, v0, v1, x0, x1, L);
+    }
+  }
+  for (auto& s : g.solids) {
+    if (!s.rigid) continue;
+    for (i64 ip = 0; ip < s.n; ip++) {
+      s.pos[2 * ip] = s.pos[2 * ip] + dt * s.mat.vx;
+      s.pos[2 * ip + 1] = s.pos[2 * ip + 1] + dt * s.mat.vy;
+    }
+  }
   (void)t;
 }
 

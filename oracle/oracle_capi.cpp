@@ -37,6 +37,9 @@ int orc_ctx_create(const jump_params* p, orc_ctx** out) {
   if (p->dim == 2 && p->algorithm == JUMP_ALG_MUSL && (p->contact || p->nx * p->ny > 4096))
     return JUMP_ERR_UNSUPPORTED;  // the dense 2D MUSL restatement: no contact, up to 4096 nodes
   if (p->dim == 3 && p->algorithm != JUMP_ALG_MUSL) return JUMP_ERR_UNSUPPORTED;
+  if (p->algorithm == JUMP_ALG_APIC && (p->contact || p->basis == JUMP_BASIS_CPDI_Q4))
+    return JUMP_ERR_UNSUPPORTED;  // APIC: the 2D step without contact, point bases
+  if (p->algorithm < JUMP_ALG_USL || p->algorithm > JUMP_ALG_APIC) return JUMP_ERR_ARG;
   orc_ctx* h = new (std::nothrow) orc_ctx();
   if (!h) return JUMP_ERR_NOMEM;
   OCtx& g = h->c;
@@ -122,6 +125,7 @@ int orc_add_solid(orc_ctx* h, const jump_solid_desc* ds, const jump_material* ma
   s.alphadot.assign(n, 0.0);
   s.vm.assign(n, 0.0);
   s.pstrain.assign((d == 2 ? 3 : 9) * n, 0.0);
+  if (g.alg == JUMP_ALG_APIC) s.cmat.assign(4 * n, 0.0);  // Solid.jl:111
   if (g.basis == JUMP_BASIS_CPDI_Q4) {
     s.corners.assign(ds->corners, ds->corners + 2 * size_t(ds->ncorners));
     s.elems.assign(ds->elems, ds->elems + 4 * n);
@@ -153,6 +157,7 @@ int orc_step(orc_ctx* h, int64_t nsteps, double* t) {
   for (int64_t s = 0; s < nsteps; s++) {
     if (g.dim == 2 && g.contact) orc::step_usl_2d_contact(g, *t);
     else if (g.dim == 2 && g.alg == JUMP_ALG_MUSL) orc::step_musl_2d(g);
+    else if (g.dim == 2 && g.alg == JUMP_ALG_APIC) orc::step_apic_2d(g, *t);
     else if (g.dim == 2) orc::step_usl_2d(g, *t);
     else if (g.contact) orc::step_musl_3d_contact(g);
     else orc::step_musl_3d(g);
@@ -198,7 +203,7 @@ int orc_dual_gradient(orc_ctx* h, int32_t solid, int64_t nsteps, const double* p
   if (!h || !pos_bar || !grad || nsteps < 0) return JUMP_ERR_ARG;
   const OCtx& g = h->c;
   if (solid < 0 || size_t(solid) >= g.solids.size()) return fail(h, JUMP_ERR_ARG, "bad solid index");
-  if (g.contact || (g.dim == 2 && g.alg == JUMP_ALG_MUSL)) return fail(h, JUMP_ERR_UNSUPPORTED, "dual run: USL 2D, MUSL 3D");
+  if (g.contact || (g.dim == 2 && g.alg != JUMP_ALG_USL)) return fail(h, JUMP_ERR_UNSUPPORTED, "dual run: USL 2D, MUSL 3D");
   orc::Ctx<D7> q;
   q.dim = g.dim; q.basis = g.basis; q.alg = g.alg; q.ghostcell = g.ghostcell;
   q.tol = g.tol; q.alpha = g.alpha; q.dt = g.dt;
@@ -277,6 +282,21 @@ int orc_get_solid_state(orc_ctx* h, int32_t si, double* alpha, double* alphadot,
   if (!h || si < 0 || size_t(si) >= h->c.solids.size()) return JUMP_ERR_ARG;
   const auto& s = h->c.solids[size_t(si)];
   copy_out(s.alpha, alpha); copy_out(s.alphadot, alphadot); copy_out(s.vm, vm); copy_out(s.pstrain, pstrain);
+  return JUMP_OK;
+}
+
+int orc_get_affine(orc_ctx* h, int32_t si, double* cmat) {
+  if (!h || si < 0 || size_t(si) >= h->c.solids.size() || !cmat) return JUMP_ERR_ARG;
+  if (h->c.alg != JUMP_ALG_APIC) return JUMP_ERR_UNSUPPORTED;
+  copy_out(h->c.solids[size_t(si)].cmat, cmat);
+  return JUMP_OK;
+}
+
+int orc_set_affine(orc_ctx* h, int32_t si, const double* cmat) {
+  if (!h || si < 0 || size_t(si) >= h->c.solids.size() || !cmat) return JUMP_ERR_ARG;
+  if (h->c.alg != JUMP_ALG_APIC) return JUMP_ERR_UNSUPPORTED;
+  auto& s = h->c.solids[size_t(si)];
+  s.cmat.assign(cmat, cmat + 4 * size_t(s.n));
   return JUMP_OK;
 }
 

@@ -112,7 +112,7 @@ def test_adjoint3d_checkpoint_interval_does_not_change_the_gradient(hip_lib):
 
 
 def test_taylor_bar_gradients_reduced_size(hip_lib):
-    """Config c5 at reduced size (particle spacing x4: 7,700 particles,
+    """The Taylor bar at reduced size (particle spacing x4: 7,700 particles,
     11 x 41 x 11 nodes, 60 steps, a snapshot every 20): the inverse problem's
     loss L = sum |x_T - x*|^2 against a run with other (E, fy, hardening);
     its gradient by the reverse mode equals the dual-number run of the CPU

@@ -143,7 +143,7 @@ def test_musl3d_impact_small(hip_lib):
 
 
 def test_two_disks_energy_history(hip_lib):
-    """Config c1 (Main_TwoDisks.jl at 20x20 cells): 3500 steps, energies every
+    """Two disks (Main_TwoDisks.jl at 20x20 cells): 3500 steps, energies every
     10 steps against the committed record and the live restatement."""
     _say("two disks 3500 steps")
     from oracle.pyoracle import OracleEngine
@@ -254,7 +254,7 @@ def test_error_flags_match_restatement(hip_lib):
 
 
 def test_benchmark_size_properties(hip_lib):
-    """Config c2 at full size (4.23M particles, 256x256x64 nodes): mass is
+    """The 3D impact at full size (4.23M particles, 256x256x64 nodes): mass is
     conserved by P2G, the grid update conserves momentum where no node is
     fixed, and the run is bitwise reproducible."""
     _say("impact3d full size")
@@ -278,7 +278,7 @@ def test_benchmark_size_properties(hip_lib):
 
 
 def test_impact_full_size_matches_oracle(hip_lib):
-    """Config c2 at full size (4,229,423 particles, 256x256x64 nodes, J2
+    """The 3D impact at full size (4,229,423 particles, 256x256x64 nodes, J2
     plate, quadratic B-splines, MUSL): 3 steps on the GPU equal the CPU
     restatement bit for bit, particles, material history and grid."""
     _say("impact3d full size vs oracle, 3 steps")
@@ -290,7 +290,7 @@ def test_impact_full_size_matches_oracle(hip_lib):
 
 
 def test_batched_256_instances(hip_lib):
-    """Config c4's 256 seeded two-rings instances (seeds 1000..1255; 5.65M
+    """The 256 seeded two-rings instances (seeds 1000..1255; 5.65M
     particles, grids 65 to 511 nodes wide, linear or GIMP, drawn E and speed)
     in one batch: after 4 steps every member equals its instance stepped alone
     bit for bit, energies included, and three sampled instances (the smallest,
@@ -586,7 +586,7 @@ def test_cpdi_tube_array_contact_matches_oracle(hip_lib):
 
 
 def test_cpdi_tube_array_full_size_properties(hip_lib):
-    """Config c3 at its stated shape: an 8 x 8 array of CPDI rings (64
+    """The CPDI tube array at full shape: an 8 x 8 array of CPDI rings (64
     solids, 1,239,784 particles with the platens) on a 2048 x 2048 grid, rigid
     platens at 1 m/s, contact between rings and platens, corners jittered by
     0.1 cell: no flag is raised, the rings' x momentum stays zero (ring-ring
