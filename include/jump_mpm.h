@@ -306,15 +306,18 @@ const char* jump_batch_last_error(const jump_batch* batch);
  * batch's own particles and the fastest is kept; autotune == 0: every group
  * uses `shape` (JUMP_ERR_ARG outside 0..JUMP_P2G_SHAPES-1).  Shapes: 0 active nodes
  * compacted, 128 threads per workgroup; 1 compacted, 64; 2 compacted, 256;
- * 3 every node in grid-row order, 128; 4 rows, 64; 5 rows, 256.  Every shape
- * sums each node in the same order, so neither option changes any result.
+ * 3 every node in grid-row order, 128; 4 rows, 64; 5 rows, 256; 6, 7, 8 a
+ * persistent kernel of 128, 64 or 256 threads per workgroup, as many
+ * workgroups as stay resident, each taking (instance, tile) items of the
+ * compacted order from a work queue.  Every shape sums each node in the
+ * same order, so neither option changes any result.
  * jump_batch_tuning reports group g's chosen shape and the measured ms per
  * launch of each shape (ms: JUMP_P2G_SHAPES doubles, may be NULL) and the
  * number of groups (members with the same basis and summation order, in
  * order of first appearance); JUMP_ERR_STATE before the group is tuned.  A
  * group of one member steps as a single context and has no batched gather:
  * it is not tuned, and its shape is reported as -1. */
-#define JUMP_P2G_SHAPES 6
+#define JUMP_P2G_SHAPES 9
 int jump_batch_options(jump_batch* batch, int32_t graph, int32_t autotune, int32_t shape);
 int jump_batch_tuning(jump_batch* batch, int32_t group, int32_t* ngroups, int32_t* shape, double* ms);
 

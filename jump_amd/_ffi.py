@@ -37,8 +37,9 @@ ALG_USL = 0
 ALG_MUSL = 1
 ALG_APIC = 2
 
-P2G_SHAPES = 6  # JUMP_P2G_SHAPES
-P2G_SHAPE_NAMES = ["compact/128", "compact/64", "compact/256", "rows/128", "rows/64", "rows/256"]
+P2G_SHAPES = 9  # JUMP_P2G_SHAPES
+P2G_SHAPE_NAMES = ["compact/128", "compact/64", "compact/256", "rows/128", "rows/64", "rows/256",
+                   "queue/128", "queue/64", "queue/256"]
 
 ORDER_REFERENCE = 0
 ORDER_BINNED = 1

@@ -80,11 +80,13 @@ struct jump_ctx {
   unsigned* d_flags = nullptr;
   double* d_red = nullptr;
   int* d_wmax = nullptr;
+  int* d_queue = nullptr;       // work-queue counter of the persistent batched gather
+  int queue_grid[3] = {0, 0, 0};  // its resident workgroups per block size (64, 128, 256), 0: not yet known
   uint8_t* d_fixed = nullptr;
   int cpdi_w = 0;
   bool grid_valid = false;
   int alloc_gen = 0;    // bumped whenever the particle arrays are reallocated (captured graphs hold their addresses)
-  int p2g_shape = 0;    // batched gather launch shape (kP2GShapes), chosen by the autotuner
+  int p2g_shape = 0;    // batched gather launch shape (0..JUMP_P2G_SHAPES-1), chosen by the autotuner
 
   // Batch instances (an internal context of jump_batch holds several 2D
   // problems; an ordinary context is one instance with grid G).  NN: nodes
@@ -371,6 +373,7 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_sums, (nn + 1) / kScanItems + 2);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_flags, 1);
   if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_wmax, 1);
+  if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_queue, 1);
   if (!r && D == 3) {
     for (double2** v : {&g.mq, &g.qq, &g.pxy, &g.p2xy, &g.pz2})
       if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, nn);
@@ -510,15 +513,11 @@ template <int D, int B, bool MULTI> static int bin_and_sort(jump_ctx* ctx, int* 
 
 // Launch shapes of the batched gather (P2G over the nodes of all instances):
 // the node order (active nodes compacted, or every node in grid-row order)
-// and the workgroup size.  Every shape sums each node in the same order, so
-// the choice changes the time only; the autotuner times them on the batch's
-// own particles (tune_batched_p2g).
-struct P2GShape {
-  bool compact;
-  int block;
-};
-constexpr P2GShape kP2GShapes[JUMP_P2G_SHAPES] = {{true, 128}, {true, 64}, {true, 256},
-                                                  {false, 128}, {false, 64}, {false, 256}};
+// and the workgroup size, as one workgroup per tile of nodes (shapes 0-5), or
+// as a persistent kernel whose resident workgroups take (instance, tile)
+// items of the compacted order from a work queue (6-8).  Every shape sums
+// each node in the same order, so the choice changes the time only; the
+// autotuner times them on the batch's own particles (tune_batched_p2g).
 
 template <int B, int ORDER, bool COMPACT, int BLK> static void p2g_batched_as(jump_ctx* ctx) {
   hipLaunchKernelGGL((k_p2g<2, B, 0, ORDER, COMPACT, true, BLK>), dim3((ctx->NN + BLK - 1) / BLK), dim3(BLK), 0,
@@ -526,14 +525,36 @@ template <int B, int ORDER, bool COMPACT, int BLK> static void p2g_batched_as(ju
                      ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, ctx->prm.dt, ctx->bt(), 0.0,
                      RunList{});
 }
+// The persistent gather: as many workgroups as stay resident on the device
+// (occupancy x CUs, looked up once per context and block size), at most one
+// per item; the queue counter is zeroed by a kernel (graph capture records
+// no memset).
+template <int B, int ORDER, int BLK> static void p2g_queue_as(jump_ctx* ctx, int slot) {
+  if (ctx->queue_grid[slot] == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2g_queue<B, ORDER, BLK>, BLK, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
+      per_cu = cus = 0;
+    ctx->queue_grid[slot] = std::max(1, per_cu * cus);
+  }
+  const int nlog = (ctx->NN + BLK - 1) / BLK;
+  const int grid = std::max(1, std::min(nlog, ctx->queue_grid[slot]));
+  LAUNCH(k_fill_i32, 1, ctx->d_queue, 0, 1);
+  hipLaunchKernelGGL((k_p2g_queue<B, ORDER, BLK>), dim3(grid), dim3(BLK), 0, ctx->stream, ctx->P, ctx->G,
+                     ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->d_order,
+                     ctx->d_nodepos, ctx->g, ctx->cpdi_w, ctx->prm.dt, ctx->bt(), ctx->d_queue);
+}
 template <int B, int ORDER> static void p2g_batched(jump_ctx* ctx, int shape) {
-  static_assert(JUMP_P2G_SHAPES == 6, "one case per shape");
+  static_assert(JUMP_P2G_SHAPES == 9, "one case per shape");
   switch (shape) {
     case 1: p2g_batched_as<B, ORDER, true, 64>(ctx); break;
     case 2: p2g_batched_as<B, ORDER, true, 256>(ctx); break;
     case 3: p2g_batched_as<B, ORDER, false, 128>(ctx); break;
     case 4: p2g_batched_as<B, ORDER, false, 64>(ctx); break;
     case 5: p2g_batched_as<B, ORDER, false, 256>(ctx); break;
+    case 6: p2g_queue_as<B, ORDER, 128>(ctx, 1); break;
+    case 7: p2g_queue_as<B, ORDER, 64>(ctx, 0); break;
+    case 8: p2g_queue_as<B, ORDER, 256>(ctx, 2); break;
     default: p2g_batched_as<B, ORDER, true, 128>(ctx); break;
   }
 }

@@ -598,39 +598,19 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
   }
 }
 
-// BLK: threads (nodes) per workgroup; the batched step lets the autotuner
-// choose it (the single-problem steps use kP2GBlock).
-template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI, int BLK = kP2GBlock>
-__global__ void __launch_bounds__(BLK) k_p2g(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
-                                                   const double* __restrict__ corners, const int* __restrict__ start0,
-                                                   const unsigned long long* __restrict__ sorted,
-                                                   const unsigned long long* __restrict__ bxyz,
-                                                   const int* __restrict__ order, const int* __restrict__ nodepos,
-                                                   GridArrays g0, int Wrt, double dt0, BatchDev Bt,
-                                                   double alpha, RunList rl) {
-  constexpr int MW = MaxW<B>::W;
-  constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
-  // Reference order: per window bin, the position in `sorted` of its next
-  // unconsumed particle and that particle (kNoParticle when exhausted).
-  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
-  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
+// The nodes of logical block L (BLK threads, one node each): the body of
+// the gather kernels below.
+template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI, int BLK>
+__device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0, const SolidDev* __restrict__ S,
+                                          const double* __restrict__ corners, const int* __restrict__ start0,
+                                          const unsigned long long* __restrict__ sorted,
+                                          const unsigned long long* __restrict__ bxyz,
+                                          const int* __restrict__ order, const int* __restrict__ nodepos,
+                                          const GridArrays& g0, int Wrt, double dt0, const BatchDev& Bt,
+                                          double alpha, const RunList& rl, int L, int (*s_cur)[BLK],
+                                          int (*s_head)[BLK]) {
   const int t = threadIdx.x;
   const int NN = MULTI ? Bt.NN : G0.nn;
-  // XCD: the 3D re-projection pass deals its (compacted) blocks to the XCDs
-  // in runs of kXcdChunk (1.115 -> 1.066 ms on the 4.2M-particle impact;
-  // the force pass keeps the hardware order, it was slower with any run
-  // length tried, 2.12 -> 2.19-3.47 ms, as with 8x4x4 node tiles, 2.9-3.8,
-  // and again once it replayed runs, 1.47 -> 1.92 ms with runs of 16)
-  constexpr bool XCD = D == 3 && !MULTI && COMPACT;
-  // 3D: blocks in descending node order.  The costliest nodes (the crater,
-  // where windows change and merge again, and the sphere's mixed windows)
-  // sit at the top of the plate, the highest node ids; started last they
-  // set the kernel's tail.  Both grids have exactly nlog (logical) blocks.
-  constexpr bool REV = D == 3 && !MULTI;
-  const int nlog = (NN + BLK - 1) / BLK;
-  const int L0 = XCD ? xcd_block(nlog) : int(blockIdx.x);
-  if (L0 < 0) return;
-  const int L = REV ? nlog - 1 - L0 : L0;
   const int i = L * BLK + t;
   if (i >= NN) return;
   const int ng = COMPACT ? ldu(order, unsigned(i)) : i;  // node in the (concatenated) node arrays
@@ -759,6 +739,74 @@ __global__ void __launch_bounds__(BLK) k_p2g(Particles P, GridDesc G0, const Sol
       g.p2xy[n] = make_double2(ap[0], ap[1]);
       g.pz2[n] = make_double2(g.p[D - 1][n], ap[D - 1]);
     }
+  }
+}
+
+// BLK: threads (nodes) per workgroup; the batched step lets the autotuner
+// choose it (the single-problem steps use kP2GBlock).
+template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI, int BLK = kP2GBlock>
+__global__ void __launch_bounds__(BLK) k_p2g(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
+                                                   const double* __restrict__ corners, const int* __restrict__ start0,
+                                                   const unsigned long long* __restrict__ sorted,
+                                                   const unsigned long long* __restrict__ bxyz,
+                                                   const int* __restrict__ order, const int* __restrict__ nodepos,
+                                                   GridArrays g0, int Wrt, double dt0, BatchDev Bt,
+                                                   double alpha, RunList rl) {
+  constexpr int MW = MaxW<B>::W;
+  constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
+  // Reference order: per window bin, the position in `sorted` of its next
+  // unconsumed particle and that particle (kNoParticle when exhausted).
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
+  const int NN = MULTI ? Bt.NN : G0.nn;
+  // XCD: the 3D re-projection pass deals its (compacted) blocks to the XCDs
+  // in runs of kXcdChunk (1.115 -> 1.066 ms on the 4.2M-particle impact;
+  // the force pass keeps the hardware order, it was slower with any run
+  // length tried, 2.12 -> 2.19-3.47 ms, as with 8x4x4 node tiles, 2.9-3.8,
+  // and again once it replayed runs, 1.47 -> 1.92 ms with runs of 16)
+  constexpr bool XCD = D == 3 && !MULTI && COMPACT;
+  // 3D: blocks in descending node order.  The costliest nodes (the crater,
+  // where windows change and merge again, and the sphere's mixed windows)
+  // sit at the top of the plate, the highest node ids; started last they
+  // set the kernel's tail.  Both grids have exactly nlog (logical) blocks.
+  constexpr bool REV = D == 3 && !MULTI;
+  const int nlog = (NN + BLK - 1) / BLK;
+  const int L0 = XCD ? xcd_block(nlog) : int(blockIdx.x);
+  if (L0 < 0) return;
+  const int L = REV ? nlog - 1 - L0 : L0;
+  p2g_block<D, B, PASS, ORDER, COMPACT, MULTI, BLK>(P, G0, S, corners, start0, sorted, bxyz, order, nodepos, g0,
+                                                     Wrt, dt0, Bt, alpha, rl, L, s_cur, s_head);
+}
+
+// The batched force pass as a persistent kernel over a work queue: a grid
+// sized to the resident workgroups, each taking the next item (BLK
+// consecutive nodes of the compacted active-node order, all instances'
+// nodes in instance order: an (instance, tile) item) from the counter
+// `queue` (zeroed before the launch) until the items run out; every
+// workgroup leaves the loop together when the counter passes the last item.
+// Each node is summed as in k_p2g, so the results are the same bits.
+template <int B, int ORDER, int BLK>
+__global__ void __launch_bounds__(BLK) k_p2g_queue(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
+                                                   const double* __restrict__ corners, const int* __restrict__ start0,
+                                                   const unsigned long long* __restrict__ sorted,
+                                                   const unsigned long long* __restrict__ bxyz,
+                                                   const int* __restrict__ order, const int* __restrict__ nodepos,
+                                                   GridArrays g0, int Wrt, double dt0, BatchDev Bt,
+                                                   int* __restrict__ queue) {
+  constexpr int MW = MaxW<B>::W;
+  constexpr int NB = MW * MW;
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
+  __shared__ int s_item;
+  const int nlog = (Bt.NN + BLK - 1) / BLK;
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int L = s_item;
+    __syncthreads();  // every thread has read the item before the next is taken
+    if (L >= nlog) break;
+    p2g_block<2, B, 0, ORDER, true, true, BLK>(P, G0, S, corners, start0, sorted, bxyz, order, nodepos, g0, Wrt,
+                                               dt0, Bt, 0.0, RunList{}, L, s_cur, s_head);
   }
 }
 

@@ -430,7 +430,8 @@ def test_batch_members_step_as_alone(hip_lib):
 
 def test_batch_graph_replay_and_gather_shapes_equal_eager(hip_lib):
     """HIP-graph replay (one step and chunks of 8) and every launch shape of
-    the batched gather leave each member bitwise equal to the eager batch in
+    the batched gather (the persistent work-queue kernel included) leave each
+    member bitwise equal to the eager batch in
     the default shape, through calls of 3, 8, 1 and 12 steps with energies
     and state reads between them; the autotuner keeps the shape it measured
     fastest."""
@@ -452,7 +453,8 @@ def test_batch_graph_replay_and_gather_shapes_equal_eager(hip_lib):
     ref.options(graph=False, autotune=False, shape=0)
     variants = [("graph+autotune", dict())] + \
         [(f"graph shape {k}", dict(graph=True, autotune=False, shape=k)) for k in range(_ffi.P2G_SHAPES)] + \
-        [("eager shape 5", dict(graph=False, autotune=False, shape=5))]
+        [("eager shape 5", dict(graph=False, autotune=False, shape=5)),
+         ("eager shape 8", dict(graph=False, autotune=False, shape=8))]
     runs = []
     for name, kw in variants:
         members, b = make()
