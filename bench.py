@@ -254,11 +254,14 @@ def musl2d_leg(steps, warmup, cells=256):
 def adjoint_leg(steps, K_recompute=10):
     """Reverse mode against the plain step on the two-rings impact at 1025 x
     513 nodes, 4 particles per cell and direction (1,844,952 particles,
-    2D USL, GIMP, Neo-Hookean E = 2 MPa, 30 mm/s), `steps` steps.  plain:
-    jump_step; adjoint: the recorded run plus the backward sweep, for
-    L = sum |x_T - x_0|^2.  Two checkpoint intervals: automatic (the library
-    keeps every step's state when HBM holds them, so nothing is recomputed),
-    and every K_recompute steps (snapshot and recompute)."""
+    2D USL, GIMP, Neo-Hookean E = 2 MPa, 30 mm/s), `steps` steps, each ring
+    also given a radial breathing velocity of 10 mm/s at its outer radius so
+    that it deforms within the run and the loss depends on E and nu (the
+    rings do not meet in 50 steps).  plain: jump_step; adjoint: the recorded
+    run plus the backward sweep, for L = sum |x_T - x_0|^2.  Two checkpoint
+    intervals: automatic (the library keeps every step's state when HBM
+    holds them, so nothing is recomputed), and every K_recompute steps
+    (snapshot and recompute)."""
     import numpy as np
     import torch
 
@@ -266,6 +269,9 @@ def adjoint_leg(steps, K_recompute=10):
     from jump_amd.engine import Engine
 
     sc = scenes.two_rings(1025, 4, _ffi.BASIS_GIMP, 2.0, 30.0)
+    for s_ in sc.solids:
+        c = s_["pos"].mean(axis=0)
+        s_["vel"] = s_["vel"] + 10.0 * (s_["pos"] - c) / 40.0
     e = Engine(sc.params, device=0)
     sc.load(e)
     e.step(2, 0.0)

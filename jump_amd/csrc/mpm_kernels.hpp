@@ -346,7 +346,9 @@ template <int D, int PASS> struct P2GFields {
 // operation order (SolveExplicitSolidMechanics2D.jl:411-421;
 // SolveExplicitSolidMechanics3D.jl:69-82 and :135-138).
 // v: the particle's fields as P2GFields loads them.
-template <int D, int B, int PASS>
+// AFF: the node's grid may be an APIC step's (single 2D problems; batches
+// refuse APIC), read from G.apic.
+template <int D, int B, int PASS, bool AFF = false>
 __device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G, const SolidDev* S,
                                           const double* corners, int p, const double* v, int ni, int nj, int nk,
                                           double& am, double* ap, double* af) {
@@ -366,7 +368,7 @@ __device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G,
     const double* sg = v + 2 * D + 2;
     const double Nim = N * fM;
     am += Nim;
-    if (D == 2 && G.apic) {
+    if (AFF && D == 2 && G.apic) {
       // APIC: momentum of the affine field v_p + C_p (x_I - x_p) at the node
       const double e0 = (G.xmin + double(ni) * G.dx) - v[0], e1 = (G.ymin + double(nj) * G.dy) - v[1];
       const double C0 = P.C[0][p], C1 = P.C[1][p], C2 = P.C[2][p], C3 = P.C[3][p];
@@ -390,13 +392,13 @@ __device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G,
   }
 }
 
-template <int D, int B, int PASS>
+template <int D, int B, int PASS, bool AFF = false>
 __device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, const SolidDev* S,
                                         const double* corners, int p, int ni, int nj, int nk, double& am,
                                         double* ap, double* af) {
   double v[P2GFields<D, PASS>::NF + 1];
   P2GFields<D, PASS>::load(P, unsigned(p), v);
-  p2g_add_v<D, B, PASS>(P, G, S, corners, p, v, ni, nj, nk, am, ap, af);
+  p2g_add_v<D, B, PASS, AFF>(P, G, S, corners, p, v, ni, nj, nk, am, ap, af);
 }
 
 // PASS 0: mass, momentum0, force, then p = p0 + f dt and the Dirichlet mask
@@ -649,7 +651,8 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
   for (int k = 0; k < D; k++) { ap[k] = 0.0; af[k] = 0.0; }
   constexpr bool DEFER = PASS == 1 || D == 2;
   if (PASS == 0 && still_empty) return;
-  auto visit = [&](int p) { p2g_add<D, B, PASS>(P, G, S, corners, p, ni, nj, nk, am, ap, af); };
+  constexpr bool AFF = D == 2 && PASS == 0 && !MULTI;
+  auto visit = [&](int p) { p2g_add<D, B, PASS, AFF>(P, G, S, corners, p, ni, nj, nk, am, ap, af); };
   // Replay.  The re-projection loads the next particle's fields (and the
   // next run's entry) while the current particle is summed; the sums keep
   // their order (0.683 -> 0.651 ms on the impact).  The force pass loads
@@ -1268,9 +1271,11 @@ __global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, 
   const GridArrays g = MULTI ? grid_view(g0, Bt.node_off[s.inst]) : g0;
   const double tol = MULTI ? Bt.inst[s.inst].tol : tol0;
   const double dt = MULTI ? Bt.inst[s.inst].dt : dt0;
-  if (G.apic) {
-    g2p_apic2d<B>(P, G, s, corners, g, tol, dt, i, flags);
-    return;
+  if constexpr (!MULTI) {
+    if (G.apic) {  // (batches refuse APIC)
+      g2p_apic2d<B>(P, G, s, corners, g, tol, dt, i, flags);
+      return;
+    }
   }
   double v0 = P.v[0][i], v1 = P.v[1][i];
   double x0 = P.x[0][i], x1 = P.x[1][i];
