@@ -235,6 +235,30 @@ int jump_get_solid(jump_ctx* ctx, int32_t solid, double* pos, double* vel,
 int jump_get_solid_state(jump_ctx* ctx, int32_t solid, double* alpha,
                          double* alphadot, double* vm, double* pstrain);
 
+/* Body forces.  The 2D steps evaluate the body force at each particle's
+ * position at the start of the step, bodyforce(body, xx[ip], t)
+ * (SolveExplicitSolidMechanics2D.jl:408 USL, :125 MUSL), and P2G adds
+ * m_p b N to the nodal force; the 3D step applies none.
+ * JUMP_BODY_CONSTANT: params.body (ConstantBodyForce2D, BodyForce.jl:58-60,
+ * :86-88), the default.  JUMP_BODY_VORTEX2D: VortexBodyForce2D(G, T, Ri, Ro,
+ * rho0, mu, xcenter, ycenter) (BodyForce.jl:34-53, :94-121), the
+ * manufactured body force under which an annulus Ri < r < Ro of a
+ * Neo-Hookean solid (shear modulus mu, density rho0) rotates by
+ * alpha(r, t) = G sin(pi t / T) h(r), h = 1 - 8 s^2 + 16 s^4,
+ * s = (r - (Ri + Ro)/2) / (Ri - Ro) (the exact map of Main_Vortex_FEMP.jl);
+ * prm = those 8 values.  JUMP_BODY_VORTEX2D evaluates it at each particle's
+ * current position, as the ULMPM step calls bodyforce; the manufactured
+ * solution is written in material coordinates (the total-Lagrangian FEMP
+ * solver evaluates it at the reference positions), and
+ * JUMP_BODY_VORTEX2D_X0 evaluates it at each particle's position when it
+ * was added, under which the step converges to the exact map.  The time of
+ * a step is the t of jump_step, accumulated by dt per step as the
+ * reference's loop does.  2D only; the reverse mode and batches refuse a
+ * vortex force.  kind JUMP_BODY_CONSTANT takes prm = the 2 components and
+ * replaces params.body. */
+typedef enum { JUMP_BODY_CONSTANT = 0, JUMP_BODY_VORTEX2D = 1, JUMP_BODY_VORTEX2D_X0 = 2 } jump_body_kind;
+int jump_set_body_force(jump_ctx* ctx, int32_t kind, const double* prm, int32_t nprm);
+
 /* APIC (JUMP_ALG_APIC): the affine matrix C_p of each particle of a solid,
  * 4 doubles per particle, column-major, as Solid.Cmat holds it (Solid.jl:82,
  * "APIC BpDp"; MMatrix{2,2}).  jump_set_affine writes it, for a start from

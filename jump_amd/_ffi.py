@@ -37,6 +37,10 @@ ALG_USL = 0
 ALG_MUSL = 1
 ALG_APIC = 2
 
+BODY_CONSTANT = 0  # jump_body_kind
+BODY_VORTEX2D = 1
+BODY_VORTEX2D_X0 = 2
+
 P2G_SHAPES = 9  # JUMP_P2G_SHAPES
 P2G_SHAPE_NAMES = ["compact/128", "compact/64", "compact/256", "rows/128", "rows/64", "rows/256",
                    "queue/128", "queue/64", "queue/256"]
@@ -124,6 +128,7 @@ def _declare(lib, prefix):
         "get_solid_state": (ctypes.c_int, [H, _i32, _pd, _pd, _pd, _pd]),
         "get_affine": (ctypes.c_int, [H, _i32, _pd]),
         "set_affine": (ctypes.c_int, [H, _i32, _pd]),
+        "set_body_force": (ctypes.c_int, [H, _i32, _pd, _i32]),
         "get_grid": (ctypes.c_int, [H, _pd, _pd, _pd, _pd, _pd]),
         "energies": (ctypes.c_int, [H, _pd, _pd]),
     }
@@ -193,7 +198,7 @@ PHASES = ["sort", "p2g", "rigid", "g2p", "g2p1", "p2g2", "g2p2", "solve"]
 EXPORTED = [
     "jump_ctx_create", "jump_ctx_destroy", "jump_last_error", "jump_add_solid", "jump_set_fixed", "jump_step",
     "jump_flags", "jump_get_solid", "jump_get_solid_state", "jump_get_affine", "jump_set_affine", "jump_get_grid",
-    "jump_energies", "jump_get_bins",
+    "jump_energies", "jump_get_bins", "jump_set_body_force",
     "jump_abi_version", "jump_profile", "jump_profile_read",
     "jump_checkpoint_bytes", "jump_checkpoint_save", "jump_checkpoint_load",
     "jump_batch_create", "jump_batch_step", "jump_batch_sync", "jump_batch_energies", "jump_batch_destroy",

@@ -49,6 +49,9 @@ struct jump_ctx {
   int64_t np = 0;
   std::vector<double> hx[3], hv[3], hm, hV, hV0, hF[9], heps[6], hsig[6], halpha, halphadot, hvm, hep[6];
   std::vector<double> hC[4];  // APIC: affine matrices, column-major
+  std::vector<double> hX0[2]; // positions when added (a body force in material coordinates)
+  int body_kind = JUMP_BODY_CONSTANT;  // the body force (jump_set_body_force)
+  double vortex[8] = {};               // JUMP_BODY_VORTEX2D: G, T, Ri, Ro, rho0, mu, xcenter, ycenter
   std::vector<std::vector<double2>> hpairs;  // transfer buffers of the interleaved gather fields
   std::vector<int> hsid, helem[4];
   std::vector<uint8_t> hfixed;
@@ -252,6 +255,10 @@ static int alloc_particles(jump_ctx* ctx) {
     for (int k = 0; k < 4 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.elem[k], n);
   if (ctx->G.apic)
     for (int k = 0; k < 4 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.C[k], n);
+  if (ctx->G.bodyfield)
+    for (int k = 0; k < 2 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.bf[k], n);
+  if (ctx->body_kind == JUMP_BODY_VORTEX2D_X0)
+    for (int k = 0; k < 2 && !r; k++) r = dev_alloc(ctx, ctx->part_allocs, &P.X0[k], n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_key, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_rank, n);
   if (!r) r = dev_alloc(ctx, ctx->part_allocs, &ctx->d_sorted, n + 1);
@@ -308,6 +315,8 @@ static int upload(jump_ctx* ctx) {
     for (int k = 0; k < 4 && !r; k++) r = h2d(ctx, P.elem[k], ctx->helem[k]);
   if (ctx->G.apic)
     for (int k = 0; k < 4 && !r; k++) r = h2d(ctx, P.C[k], ctx->hC[k]);
+  if (ctx->body_kind == JUMP_BODY_VORTEX2D_X0)
+    for (int k = 0; k < 2 && !r; k++) r = h2d(ctx, P.X0[k], ctx->hX0[k]);
   std::vector<SolidDev> sd;
   for (auto& s : ctx->solids) sd.push_back(s.dev);
   if (!r) r = h2d(ctx, ctx->d_solids, sd);
@@ -756,9 +765,19 @@ static int prepare_cpdi(jump_ctx* ctx) {
   return JUMP_OK;
 }
 
-static int run_steps(jump_ctx* ctx, int64_t nsteps) {
+// t0: the time of the first step (the reference's accumulated t), which a
+// position-dependent body force reads.
+static int run_steps(jump_ctx* ctx, int64_t nsteps, double t0 = 0.0) {
   const bool ref = ctx->prm.order == JUMP_ORDER_REFERENCE;
+  double t = t0;
   for (int64_t s = 0; s < nsteps; s++) {
+    if (ctx->G.bodyfield) {  // bodyforce(body, xx[ip], t) at the start of the step
+      VortexDev vd;
+      for (int k = 0; k < 8; k++) vd.v[k] = ctx->vortex[k];
+      if (ctx->body_kind == JUMP_BODY_VORTEX2D_X0) LAUNCH(k_body_vortex2d<true>, int(ctx->np), ctx->P, vd, t);
+      else LAUNCH(k_body_vortex2d<false>, int(ctx->np), ctx->P, vd, t);
+    }
+    t += ctx->prm.dt;
     if (ctx->D == 2 && ctx->prm.algorithm == JUMP_ALG_MUSL) {
       switch (ctx->prm.basis) {
         case JUMP_BASIS_LINEAR:
@@ -996,6 +1015,8 @@ int jump_add_solid(jump_ctx* ctx, const jump_solid_desc* ds, const jump_material
     ctx->halphadot.push_back(0.0);
     ctx->hvm.push_back(0.0);
     ctx->hsid.push_back(sid);
+    if (D == 2)
+      for (int k = 0; k < 2; k++) ctx->hX0[k].push_back(ds->pos[2 * i + k]);
     if (ctx->G.apic)
       for (int k = 0; k < 4; k++) ctx->hC[k].push_back(0.0);  // Solid.jl:111
     if (cpdi)
@@ -1038,7 +1059,7 @@ int jump_step(jump_ctx* ctx, int64_t nsteps, double* t_inout) {
   }
   r = prepare_cpdi(ctx);
   if (r) return r;
-  r = run_steps(ctx, nsteps);
+  r = run_steps(ctx, nsteps, *t_inout);
   if (r) return r;
   unsigned fl = 0;
   HIP_TRY(hipMemcpyAsync(&fl, ctx->d_flags, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
@@ -1236,6 +1257,34 @@ int jump_set_affine(jump_ctx* ctx, int32_t si, const double* cmat) {
   for (size_t i = 0; i < size_t(s.n); i++)
     for (int k = 0; k < 4; k++) ctx->hC[k][o + i] = cmat[4 * i + size_t(k)];
   ctx->on_device = false;  // the next call that needs the device uploads it
+  ctx->adj.ready = false;
+  return JUMP_OK;
+}
+
+int jump_set_body_force(jump_ctx* ctx, int32_t kind, const double* prm, int32_t nprm) {
+  if (!ctx) return JUMP_ERR_ARG;
+  if (!prm && nprm > 0) return set_err(ctx, JUMP_ERR_ARG, "null parameters");
+  if (kind == JUMP_BODY_CONSTANT) {
+    if (nprm != ctx->D) return set_err(ctx, JUMP_ERR_ARG, "a constant body force takes dim values");
+    for (int k = 0; k < ctx->D; k++) ctx->G.body[k] = ctx->prm.body[k] = prm[k];
+  } else if (kind == JUMP_BODY_VORTEX2D || kind == JUMP_BODY_VORTEX2D_X0) {
+    if (nprm != 8) return set_err(ctx, JUMP_ERR_ARG, "VortexBodyForce2D takes 8 values");
+    if (ctx->D != 2) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the vortex body force is 2D");
+    if (!(prm[1] != 0.0 && prm[2] != prm[3] && prm[4] > 0.0))
+      return set_err(ctx, JUMP_ERR_ARG, "bad vortex parameters (T != 0, Ri != Ro, rho0 > 0)");
+    for (int k = 0; k < 8; k++) ctx->vortex[k] = prm[k];
+  } else {
+    return set_err(ctx, JUMP_ERR_ARG, "bad body force kind");
+  }
+  const int field = kind == JUMP_BODY_CONSTANT ? 0 : 1;
+  if (field != ctx->G.bodyfield || kind != ctx->body_kind) {  // the particle arrays change
+    HIP_TRY(hipSetDevice(ctx->device));
+    int r = download(ctx);
+    if (r) return r;
+    ctx->on_device = false;
+    ctx->G.bodyfield = field;
+  }
+  ctx->body_kind = kind;
   ctx->adj.ready = false;
   return JUMP_OK;
 }
@@ -1653,6 +1702,7 @@ int jump_batch_create(jump_ctx* const* ctxs, int32_t n, int device, jump_batch**
     const jump_ctx* c = ctxs[i];
     if (!c) return JUMP_ERR_ARG;
     if (c->D != 2 || c->prm.algorithm != JUMP_ALG_USL) return JUMP_ERR_UNSUPPORTED;  // batches are 2D USL
+    if (c->G.bodyfield) return JUMP_ERR_UNSUPPORTED;  // the graphs replay fixed times
     if (c->prm.basis == JUMP_BASIS_CPDI_Q4 || c->prm.contact) return JUMP_ERR_UNSUPPORTED;
     if (c->device != device) return JUMP_ERR_ARG;
   }
@@ -1950,6 +2000,8 @@ static int adjoint_supported(jump_ctx* ctx) {
   if (!(usl2d || musl3d) || ctx->ninst != 1)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the 2D USL and 3D MUSL steps");
   if (ctx->prm.contact) return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint does not cover contact");
+  if (ctx->G.bodyfield)
+    return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers a constant body force only");
   if (ctx->prm.basis != JUMP_BASIS_LINEAR && ctx->prm.basis != JUMP_BASIS_GIMP)
     return set_err(ctx, JUMP_ERR_UNSUPPORTED, "the adjoint covers the linear and GIMP bases");
   for (auto& s : ctx->solids) {

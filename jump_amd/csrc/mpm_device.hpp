@@ -26,6 +26,7 @@ namespace jh {
 struct GridDesc {
   int dim;
   int apic;  // the step's transfers are APIC (2D): P2G adds m N C_p (x_I - x_p), G2P forms C_p
+  int bodyfield;  // the body force is per particle (Particles::bf), not the constant body
   int nx, ny, nz, nxy, nn;
   double xmin, ymin, zmin;
   double dx, dy, dz, dxI, dyI, dzI;
@@ -81,6 +82,8 @@ struct Particles {
   int* elem[4];  // CPDI: global corner index of each domain corner
   double2* pr[7];  // the pair arrays x, m, V, v and sigma live in
   double* C[4];    // APIC: affine matrix C_p, column-major (2D)
+  double* bf[2];   // per-particle body force of the step (2D, a position-dependent body force)
+  double* X0[2];   // positions when added (a body force in material coordinates)
   int n;
 };
 

@@ -379,8 +379,10 @@ __device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G,
       for (int k = 0; k < D; k++) ap[k] += Nim * vp[k];
     }
     if (D == 2) {
-      af[0] += (-fV) * (sg[0] * dN[0] + sg[2] * dN[1]) + (fM * G.body[0]) * N;
-      af[1] += (-fV) * (sg[2] * dN[0] + sg[1] * dN[1]) + (fM * G.body[1]) * N;
+      const double b0 = AFF && G.bodyfield ? P.bf[0][p] : G.body[0];
+      const double b1 = AFF && G.bodyfield ? P.bf[1][p] : G.body[1];
+      af[0] += (-fV) * (sg[0] * dN[0] + sg[2] * dN[1]) + (fM * b0) * N;
+      af[1] += (-fV) * (sg[2] * dN[0] + sg[1] * dN[1]) + (fM * b1) * N;
     } else {
       af[0] -= fV * (sg[0] * dN[0] + sg[3] * dN[1] + sg[4] * dN[2]);
       af[1] -= fV * (sg[3] * dN[0] + sg[1] * dN[1] + sg[5] * dN[2]);
@@ -877,7 +879,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
   const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
   ContactSlot s0, s1;
   bool overflow = false;
-  auto add = [&](ContactSlot& s, int sid, double N, const double* dN, const double* v) {
+  auto add = [&](ContactSlot& s, int sid, double N, const double* dN, const double* v, const double* bp) {
     const double fM = v[2], fV = v[3];
     const double* vp = v + 4;
     const double* sg = v + 6;
@@ -886,8 +888,8 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
     s.m += Nim;
     s.p0[0] += Nim * vp[0];
     s.p0[1] += Nim * vp[1];
-    s.f[0] += (-fV) * (sg[0] * dN[0] + sg[2] * dN[1]) + (fM * G.body[0]) * N;
-    s.f[1] += (-fV) * (sg[2] * dN[0] + sg[1] * dN[1]) + (fM * G.body[1]) * N;
+    s.f[0] += (-fV) * (sg[0] * dN[0] + sg[2] * dN[1]) + (fM * bp[0]) * N;
+    s.f[1] += (-fV) * (sg[2] * dN[0] + sg[1] * dN[1]) + (fM * bp[1]) * N;
     s.nr[0] += fM * dN[0];
     s.nr[1] += fM * dN[1];
   };
@@ -917,8 +919,9 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
     bool member;
     node_weight<2, B, false, true>(P, G, lp, corners, p, v, ni + 1, nj + 1, 1, N, dN, member);
     if (!member) return;
-    if (s0.body < 0 || s0.body == sid) add(s0, sid, N, dN, v);
-    else if (s1.body < 0 || s1.body == sid) add(s1, sid, N, dN, v);
+    const double bp[2] = {G.bodyfield ? P.bf[0][p] : G.body[0], G.bodyfield ? P.bf[1][p] : G.body[1]};
+    if (s0.body < 0 || s0.body == sid) add(s0, sid, N, dN, v, bp);
+    else if (s1.body < 0 || s1.body == sid) add(s1, sid, N, dN, v, bp);
     else overflow = true;
   });
   if (overflow) atomicOr(flags, JUMP_FLAG_CONTACT_OVERFLOW);
@@ -1196,6 +1199,46 @@ __device__ __forceinline__ void usl2d_particle_update(const Particles& P, const 
   P.sig[0][i] = sg[0]; P.sig[1][i] = sg[1]; P.sig[2][i] = sg[2];
   if (!(isfinite(x0) && isfinite(x1) && isfinite(v0) && isfinite(v1))) fl |= JUMP_FLAG_NONFINITE;
   if (fl) atomicOr(flags, fl);
+}
+
+// VortexBodyForce2D (BodyForce.jl:94-121) at every particle's position at
+// the start of the step, time t: bodyforce(body, xx[ip], t)
+// (SolveExplicitSolidMechanics2D.jl:408, :125), in the reference's
+// operation order (x^2 = x*x, x^3 = x*x*x, x^4 = (x*x)*(x*x)).  X0: at the
+// particle's position when it was added instead (material coordinates, in
+// which the manufactured solution is written).  v: G, T, Ri, Ro, rho0, mu,
+// xcenter, ycenter.
+struct VortexDev {
+  double v[8];
+};
+template <bool X0>
+__global__ void __launch_bounds__(kBlock) k_body_vortex2d(Particles P, VortexDev vd, double t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  auto p2 = [](double x) { return x * x; };
+  auto p3 = [](double x) { return x * x * x; };
+  auto p4 = [](double x) { return (x * x) * (x * x); };
+  const double G = vd.v[0], Tp = vd.v[1], Ri = vd.v[2], Ro = vd.v[3];
+  const double rho0 = vd.v[4], mu = vd.v[5], xc = vd.v[6], yc = vd.v[7];
+  const double x = (X0 ? P.X0[0][i] : P.x[0][i]) - xc, y = (X0 ? P.X0[1][i] : P.x[1][i]) - yc;
+  const double r = sqrt(x * x + y * y);
+  const double theta = atan2(y, x);
+  const double R = (Ri + Ro) / 2;
+  const double h = 1 - 8 * p2((r - R) / (Ri - Ro)) + 16 * p4((r - R) / (Ri - Ro));
+  const double hp = (-16) * (r - R) / p2(Ri - Ro) + 16 * 4 * p3(r - R) / p4(Ri - Ro);
+  const double hpp = -16 / p2(Ri - Ro) + 16 * 4 * 3 * p2(r - R) / p4(Ri - Ro);
+  const double pi = M_PI;
+  const double gg = G * sin(pi * t / Tp);
+  const double gp = G * pi / Tp * cos(pi * t / Tp);
+  const double gpp = -p2(pi) / p2(Tp) * gg;
+  const double alpha = gg * h;
+  const double mdr = mu / rho0;
+  const double br = (mdr * (3 * gg * hp + r * gg * hpp) - r * gpp * h) * sin(alpha) +
+                    (mdr * r * p2(gg * hp) - r * p2(gp * h)) * cos(alpha);
+  const double bt = (-mdr * (3 * gg * hp + r * gg * hpp) + r * gpp * h) * cos(alpha) +
+                    (mdr * r * p2(gg * hp) + r * p2(gp * h)) * sin(alpha);
+  P.bf[0][i] = br * cos(theta) - bt * sin(theta);
+  P.bf[1][i] = br * sin(theta) + bt * cos(theta);
 }
 
 // APIC G2P (Jiang et al., "The affine particle-in-cell method", ACM TOG
@@ -1528,8 +1571,9 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_musl2d(Particles P, GridDesc 
     am += Nim;
     ap[0] += Nim * v[4];
     ap[1] += Nim * v[5];
-    af[0] += (-fV) * (v[6] * dN[0] + v[8] * dN[1]) + (fM * G.body[0]) * N;
-    af[1] += (-fV) * (v[8] * dN[0] + v[7] * dN[1]) + (fM * G.body[1]) * N;
+    const double b0 = G.bodyfield ? P.bf[0][p] : G.body[0], b1 = G.bodyfield ? P.bf[1][p] : G.body[1];
+    af[0] += (-fV) * (v[6] * dN[0] + v[8] * dN[1]) + (fM * b0) * N;
+    af[1] += (-fV) * (v[8] * dN[0] + v[7] * dN[1]) + (fM * b1) * N;
     if (B == JUMP_BASIS_CPDI_Q4) {
       CpdiW c;
       cpdi_setup(corners, P.elem[0][p], P.elem[1][p], P.elem[2][p], P.elem[3][p], G, c);

@@ -162,6 +162,12 @@ class Engine:
                                                              ("alpha", "alphadot", "vm", "pstrain")]))
         return out
 
+    def set_body_force(self, kind, prm):
+        """jump_set_body_force: _ffi.BODY_CONSTANT with the dim components, or
+        _ffi.BODY_VORTEX2D with (G, T, Ri, Ro, rho0, mu, xcenter, ycenter)."""
+        q = np.ascontiguousarray(prm, dtype=np.float64).ravel()
+        self._check(self._f("set_body_force")(self.h, int(kind), _ffi.dptr(q), int(q.size)))
+
     def get_affine(self, s):
         """APIC: the solid's affine matrices C_p, (n, 4) column-major."""
         out = np.empty((self.counts[s], 4))

@@ -21,6 +21,7 @@ import numpy as np
 from . import _ffi
 from .algorithm import MUSL, USL
 from .basis import GIMPBasis
+from .bodyforce import VortexBodyForce2D
 from .dirichlet import fix_Dirichlet_grid
 from .engine import Engine, make_params
 from .fix import EnergiesFix, closeFile, compute
@@ -40,9 +41,10 @@ def build_engine(grid, solids, basis, alg, data, device=0, engine=None):
     HIP library's Engine on `device`)."""
     dim = grid.dim
     dt = float(data["dt"])
+    bf = None
     if dim == 2:
         bf = data["bodyforce"]  # required by the 2D USL solver (:340)
-        body = tuple(np.asarray(bf.g, dtype=np.float64).tolist()) + (0.0,)
+        body = tuple(np.asarray(bf.g, dtype=np.float64).tolist()) + (0.0,) if hasattr(bf, "g") else (0.0, 0.0, 0.0)
     else:
         body = (0.0, 0.0, 0.0)  # the 3D MUSL step applies no body force
     ghost = "ghostcell" in data
@@ -50,6 +52,8 @@ def build_engine(grid, solids, basis, alg, data, device=0, engine=None):
                          tolerance=getattr(alg, "tolerance", 0.0), alpha=getattr(alg, "alpha", 1.0), body=body,
                          ghostcell=ghost)
     eng = Engine(params, device=device) if engine is None else engine(params)
+    if isinstance(bf, VortexBodyForce2D):  # evaluated per particle and step on the device
+        eng.set_body_force(_ffi.BODY_VORTEX2D, bf.params())
     for s in solids:
         lp = s.lp
         if isinstance(basis, GIMPBasis) and lp is None and not s.rigid:

@@ -452,6 +452,42 @@ def impact3d(nodes=(256, 256, 64), plate_mm=(100.0, 100.0, 20.0), sphere_r_mm=5.
     return Scene("impact3d", p, solids, fixed=fixed, steps=steps)
 
 
+def vortex_annulus(cells=40, k=2, basis=_ffi.BASIS_LINEAR, G=0.5, T=1.0, t_end=0.1, cfl=0.1, E=1.0e6):
+    """The vortex manufactured solution (Main_Vortex_FEMP.jl:41-109 set-up,
+    BodyForce.jl:94-121): an annulus 0.75 < r < 1.25 of Neo-Hookean solid
+    (E = 1 MPa, nu = 0.3, rho = 1050) centred in the grid [0, 2.8]^2 (the
+    driver's B-spline grid: the reference's B-spline support assumes a grid
+    starting at 0),
+    here as particles (k per cell and direction, buildParticleForRing) on the
+    2D USL step with VortexBodyForce2D(G, T, Ri, Ro, rho, mu, centre).  The
+    exact solution rotates each point by alpha = G sin(pi t/T) h(r); the
+    particles start with its velocity at t = 0, alpha'(0) (-y, x) with
+    alpha'(0) = G pi/T h(r).  The driver steps its FEMP solver at 0.02
+    dx/c; the MPM step here at cfl dx/c, to t_end (meta: steps, the vortex
+    force)."""
+    from .bodyforce import VortexBodyForce2D
+    nu, rho = 0.3, 1050.0
+    Ri, Ro, l = 0.75, 1.25, 1.4
+    mu = E / (2 * (1 + nu))
+    dx = 2 * l / cells
+    f = dx / k
+    pts = buildParticleForRing([l, l], Ri, Ro, f)
+    vf = VortexBodyForce2D(G, T, Ri, Ro, rho, mu, l, l)
+    rel = pts - l
+    r = np.sqrt((rel ** 2).sum(axis=1))
+    s_ = (r - vf.R) / (Ri - Ro)
+    ad = G * math.pi / T * (1 - 8 * s_ ** 2 + 16 * s_ ** 4)
+    lp = 0.5 * f if basis == _ffi.BASIS_GIMP else None
+    sol = _solid(neohookean(E, nu, rho), pts, (0.0, 0.0), f * f, rho, lp=lp)
+    sol["vel"] = np.ascontiguousarray(np.stack([-ad * rel[:, 1], ad * rel[:, 0]], axis=1))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    c = math.sqrt((lam + 2 * mu) / rho)
+    dt = cfl * dx / c
+    steps = int(math.ceil(t_end / dt))
+    p = make_params(2, basis, _ffi.ALG_USL, dt, (0.0, 0.0), (2 * l, 2 * l), (cells + 1, cells + 1))
+    return Scene("vortex_annulus", p, [sol], steps=steps, meta={"vortex": vf, "steps": steps, "dt": dt})
+
+
 def taylor_bar(scale=1.0, steps=400, basis=_ffi.BASIS_GIMP, material="j2", E=117e9, fy=400e6, H=100e6, v0=190.0,
                alpha=0.95):
     """Taylor bar (Main_TaylorBar.jl:43-130): copper cylinder r=3.2 mm,

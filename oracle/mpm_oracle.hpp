@@ -92,6 +92,7 @@ template <class T> struct Solid {
   bool rigid = false;
   MatConst<T> mat;
   std::vector<T> mass, vol0, vol, pos, vel, F, strain, stress;  // AoS
+  std::vector<T> pos0;  // positions when added (the material coordinates of a body force)
   // material history, per particle (reference keeps it in the material)
   std::vector<T> alpha, alphadot, vm, pstrain;  // pstrain: 3 (2D) or 9 (3D)
   std::vector<T> cmat;  // APIC: the affine matrix C_p = B_p D_p^-1, 4 per particle column-major (Solid.jl:82)
@@ -114,6 +115,8 @@ template <class T> struct Ctx {
   std::vector<Solid<T>> solids;
   uint32_t flags = 0;
   bool contact = false;  // multi-body contact (2D USL, 3D MUSL)
+  int body_kind = 0;     // JUMP_BODY_CONSTANT (body), JUMP_BODY_VORTEX2D(_X0) (vortex)
+  double vortex[8] = {};  // G, T, Ri, Ro, rho0, mu, xcenter, ycenter
   T fric;
   // contact: per node two solid slots; per-slot vectors are indexed
   // (2 node + slot) dim + component
@@ -130,6 +133,50 @@ template <class T> struct Ctx {
 // node coordinate along an axis: xmin + (i-1)*dx with 1-based i (Grid.jl:149-152)
 template <class T> inline T node_x(const Ctx<T>& g, i64 i1) { return g.xmin + double(i1 - 1) * g.dx; }
 template <class T> inline T node_y(const Ctx<T>& g, i64 j1) { return g.ymin + double(j1 - 1) * g.dy; }
+
+// The body force at a particle, bodyforce(body, xx[ip], t)
+// (SolveExplicitSolidMechanics2D.jl:408, :125): the constant g
+// (BodyForce.jl:86-88), or VortexBodyForce2D (BodyForce.jl:94-121) at the
+// particle's current position (the ULMPM call) or at its position when it
+// was added (_X0: material coordinates, as the manufactured solution and the
+// FEMP solver take them), written
+// in the reference's operation order; Julia's x^2 and x^3 are the products
+// x*x and x*x*x, x^4 is taken as (x*x)*(x*x) (parity unpinned for that
+// rounding: the reference's ^ is Julia's power by squaring).  The vortex is
+// evaluated on the values (the dual-number run refuses it).
+inline double p2(double x) { return x * x; }
+inline double p3(double x) { return x * x * x; }
+inline double p4(double x) { return (x * x) * (x * x); }
+template <class T> void body_at(const Ctx<T>& g, const Solid<T>& s, i64 ip, const T& t, T* b) {
+  if (g.body_kind != JUMP_BODY_VORTEX2D && g.body_kind != JUMP_BODY_VORTEX2D_X0) {
+    b[0] = g.body[0];
+    b[1] = g.body[1];
+    return;
+  }
+  const double G = g.vortex[0], Tp = g.vortex[1], Ri = g.vortex[2], Ro = g.vortex[3];
+  const double rho0 = g.vortex[4], mu = g.vortex[5], xc = g.vortex[6], yc = g.vortex[7];
+  const double tt = val(t);
+  const T* xp = g.body_kind == JUMP_BODY_VORTEX2D_X0 ? &s.pos0[2 * ip] : &s.pos[2 * ip];
+  const double x = val(xp[0]) - xc, y = val(xp[1]) - yc;
+  const double r = std::sqrt(x * x + y * y);
+  const double theta = std::atan2(y, x);
+  const double R = (Ri + Ro) / 2;
+  const double h = 1 - 8 * p2((r - R) / (Ri - Ro)) + 16 * p4((r - R) / (Ri - Ro));
+  const double hp = (-16) * (r - R) / p2(Ri - Ro) + 16 * 4 * p3(r - R) / p4(Ri - Ro);
+  const double hpp = -16 / p2(Ri - Ro) + 16 * 4 * 3 * p2(r - R) / p4(Ri - Ro);
+  const double pi = M_PI;
+  const double gg = G * std::sin(pi * tt / Tp);
+  const double gp = G * pi / Tp * std::cos(pi * tt / Tp);
+  const double gpp = -p2(pi) / p2(Tp) * gg;
+  const double alpha = gg * h;
+  const double mdr = mu / rho0;
+  const double br = (mdr * (3 * gg * hp + r * gg * hpp) - r * gpp * h) * std::sin(alpha) +
+                    (mdr * r * p2(gg * hp) - r * p2(gp * h)) * std::cos(alpha);
+  const double bt = (-mdr * (3 * gg * hp + r * gg * hpp) + r * gpp * h) * std::cos(alpha) +
+                    (mdr * r * p2(gg * hp) + r * p2(gp * h)) * std::sin(alpha);
+  b[0] = br * std::cos(theta) - bt * std::sin(theta);
+  b[1] = br * std::sin(theta) + bt * std::cos(theta);
+}
 template <class T> inline T node_z(const Ctx<T>& g, i64 k1) { return g.zmin + double(k1 - 1) * g.dz; }
 
 // ---------------------------------------------------------------- B-splines
@@ -807,7 +854,9 @@ template <class T> void step_usl_2d(Ctx<T>& g, const T& t) {
       T fV = s.vol[ip], fM = s.mass[ip];
       const T* vp = &s.vel[2 * ip];
       const T* sg = &s.stress[4 * ip];
-      T b0 = fM * g.body[0], b1 = fM * g.body[1];
+      T bb[2];
+      body_at(g, s, ip, t, bb);
+      T b0 = fM * bb[0], b1 = fM * bb[1];
       for (int i = 0; i < sup; i++) {
         i64 in = g.near[i];
         T Ni = g.funcs[i];
@@ -923,7 +972,9 @@ template <class T> void step_apic_2d(Ctx<T>& g, const T& t) {
       const T* sg = &s.stress[4 * ip];
       const T* C = &s.cmat[4 * ip];
       const T xp0 = s.pos[2 * ip], xp1 = s.pos[2 * ip + 1];
-      T b0 = fM * g.body[0], b1 = fM * g.body[1];
+      T bb[2];
+      body_at(g, s, ip, t, bb);
+      T b0 = fM * bb[0], b1 = fM * bb[1];
       for (int i = 0; i < sup; i++) {
         i64 in = g.near[i];
         T Ni = g.funcs[i];
@@ -1073,7 +1124,9 @@ template <class T> void step_usl_2d_contact(Ctx<T>& g, const T& t) {
       T fV = s.vol[ip], fM = s.mass[ip];
       const T* vp = &s.vel[2 * ip];
       const T* sg = &s.stress[4 * ip];
-      T b0 = fM * g.body[0], b1 = fM * g.body[1];
+      T bb[2];
+      body_at(g, s, ip, t, bb);
+      T b0 = fM * bb[0], b1 = fM * bb[1];
       for (int i = 0; i < sup; i++) {
         i64 in = g.near[i];
         int k;
@@ -1380,7 +1433,7 @@ template <class T> bool lu_solve(std::vector<T>& A, i64 n, std::vector<T>& B, in
 // v_I, then the particle update.  The CPDI corner update reads the
 // re-projected momentum, which this step never fills, so corners stay put
 // (:289-309).  Dense, as the reference: up to 4096 nodes.
-template <class T> bool step_musl_2d(Ctx<T>& g) {
+template <class T> bool step_musl_2d(Ctx<T>& g, const T& t) {
   const T dt = g.dt, alpha = g.alpha;
   const i64 nn = g.nn;
   if (nn > 4096) return false;
@@ -1393,7 +1446,9 @@ template <class T> bool step_musl_2d(Ctx<T>& g) {
       T fV = s.vol[ip], fM = s.mass[ip];
       const T* vp = &s.vel[2 * ip];
       const T* sg = &s.stress[4 * ip];
-      T b0 = fM * g.body[0], b1 = fM * g.body[1];
+      T bb[2];
+      body_at(g, s, ip, t, bb);
+      T b0 = fM * bb[0], b1 = fM * bb[1];
       for (int i = 0; i < sup; i++) {
         i64 id = g.near[i];
         T Ni = g.funcs[i];

@@ -113,6 +113,7 @@ int orc_add_solid(orc_ctx* h, const jump_solid_desc* ds, const jump_material* ma
   s.vol0.assign(ds->volume0, ds->volume0 + n);
   s.pos.assign(ds->pos, ds->pos + d * n);
   s.vel.assign(ds->vel, ds->vel + d * n);
+  s.pos0 = s.pos;
   s.F.assign(dd * n, 0.0);
   s.strain.assign(dd * n, 0.0);
   s.stress.assign(dd * n, 0.0);
@@ -156,7 +157,7 @@ int orc_step(orc_ctx* h, int64_t nsteps, double* t) {
   OCtx& g = h->c;
   for (int64_t s = 0; s < nsteps; s++) {
     if (g.dim == 2 && g.contact) orc::step_usl_2d_contact(g, *t);
-    else if (g.dim == 2 && g.alg == JUMP_ALG_MUSL) orc::step_musl_2d(g);
+    else if (g.dim == 2 && g.alg == JUMP_ALG_MUSL) orc::step_musl_2d(g, *t);
     else if (g.dim == 2 && g.alg == JUMP_ALG_APIC) orc::step_apic_2d(g, *t);
     else if (g.dim == 2) orc::step_usl_2d(g, *t);
     else if (g.contact) orc::step_musl_3d_contact(g);
@@ -204,6 +205,7 @@ int orc_dual_gradient(orc_ctx* h, int32_t solid, int64_t nsteps, const double* p
   const OCtx& g = h->c;
   if (solid < 0 || size_t(solid) >= g.solids.size()) return fail(h, JUMP_ERR_ARG, "bad solid index");
   if (g.contact || (g.dim == 2 && g.alg != JUMP_ALG_USL)) return fail(h, JUMP_ERR_UNSUPPORTED, "dual run: USL 2D, MUSL 3D");
+  if (g.body_kind != JUMP_BODY_CONSTANT) return fail(h, JUMP_ERR_UNSUPPORTED, "dual run: constant body force only");
   orc::Ctx<D7> q;
   q.dim = g.dim; q.basis = g.basis; q.alg = g.alg; q.ghostcell = g.ghostcell;
   q.tol = g.tol; q.alpha = g.alpha; q.dt = g.dt;
@@ -282,6 +284,24 @@ int orc_get_solid_state(orc_ctx* h, int32_t si, double* alpha, double* alphadot,
   if (!h || si < 0 || size_t(si) >= h->c.solids.size()) return JUMP_ERR_ARG;
   const auto& s = h->c.solids[size_t(si)];
   copy_out(s.alpha, alpha); copy_out(s.alphadot, alphadot); copy_out(s.vm, vm); copy_out(s.pstrain, pstrain);
+  return JUMP_OK;
+}
+
+int orc_set_body_force(orc_ctx* h, int32_t kind, const double* prm, int32_t nprm) {
+  if (!h || (!prm && nprm > 0)) return JUMP_ERR_ARG;
+  OCtx& g = h->c;
+  if (kind == JUMP_BODY_CONSTANT) {
+    if (nprm != g.dim) return fail(h, JUMP_ERR_ARG, "a constant body force takes dim values");
+    for (int k = 0; k < g.dim; k++) g.body[k] = prm[k];
+    g.body_kind = JUMP_BODY_CONSTANT;
+    return JUMP_OK;
+  }
+  if ((kind != JUMP_BODY_VORTEX2D && kind != JUMP_BODY_VORTEX2D_X0) || nprm != 8)
+    return fail(h, JUMP_ERR_ARG, "bad body force");
+  if (g.dim != 2) return fail(h, JUMP_ERR_UNSUPPORTED, "the vortex body force is 2D");
+  if (!(prm[1] != 0.0 && prm[2] != prm[3] && prm[4] > 0.0)) return fail(h, JUMP_ERR_ARG, "bad vortex parameters");
+  for (int k = 0; k < 8; k++) g.vortex[k] = prm[k];
+  g.body_kind = kind;
   return JUMP_OK;
 }
 
