@@ -608,8 +608,12 @@ template <int B, int ORDER> static void step_usl2d_contact(jump_ctx* ctx) {
   bin_and_sort<2, B, false>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g_contact<B, ORDER>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
-             ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags);
+  if (ctx->G.bodyfield)
+    LAUNCH_P2G((k_p2g_contact<B, ORDER, true>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags);
+  else
+    LAUNCH_P2G((k_p2g_contact<B, ORDER, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p_usl2d_contact<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->cg, ctx->prm.tolerance, dt,
@@ -728,8 +732,12 @@ template <int B, int ORDER, int R> static void step_musl2d(jump_ctx* ctx) {
   bin_and_sort<2, B, false>(ctx);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g_musl2d<B, ORDER, R>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
-             ctx->d_bxyz, ctx->g, ctx->mu, ctx->cpdi_w, dt);
+  if (ctx->G.bodyfield)
+    LAUNCH_P2G((k_p2g_musl2d<B, ORDER, R, true>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->mu, ctx->cpdi_w, dt);
+  else
+    LAUNCH_P2G((k_p2g_musl2d<B, ORDER, R, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->mu, ctx->cpdi_w, dt);
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   solve_musl2d<R>(ctx);

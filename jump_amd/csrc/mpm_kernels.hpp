@@ -861,7 +861,8 @@ __device__ __forceinline__ bool hat_node(const GridDesc& G, const double* x, int
   return true;
 }
 
-template <int B, int ORDER>
+// BF: the body force is per particle (GridDesc::bodyfield), not the constant.
+template <int B, int ORDER, bool BF = false>
 __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                            const double* __restrict__ corners,
                                                            const int* __restrict__ start,
@@ -919,7 +920,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
     bool member;
     node_weight<2, B, false, true>(P, G, lp, corners, p, v, ni + 1, nj + 1, 1, N, dN, member);
     if (!member) return;
-    const double bp[2] = {G.bodyfield ? P.bf[0][p] : G.body[0], G.bodyfield ? P.bf[1][p] : G.body[1]};
+    const double bp[2] = {BF ? P.bf[0][p] : G.body[0], BF ? P.bf[1][p] : G.body[1]};
     if (s0.body < 0 || s0.body == sid) add(s0, sid, N, dN, v, bp);
     else if (s1.body < 0 || s1.body == sid) add(s1, sid, N, dN, v, bp);
     else overflow = true;
@@ -1532,7 +1533,8 @@ constexpr double kMuslEps = 0.1;  // the lumped share of the blend (:150)
 // P2G: mass, momentum0, force as the USL pass, and the row of M, each node
 // in the reference's particle order (the row entries accumulate
 // (N_I m) N_J as massMatrix[id,idj] += Nim * Nj, :120-125).
-template <int B, int ORDER, int R>
+// BF: the body force is per particle (GridDesc::bodyfield), not the constant.
+template <int B, int ORDER, int R, bool BF = false>
 __global__ void __launch_bounds__(kP2GBlock) k_p2g_musl2d(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                           const double* __restrict__ corners,
                                                           const int* __restrict__ start,
@@ -1571,7 +1573,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_musl2d(Particles P, GridDesc 
     am += Nim;
     ap[0] += Nim * v[4];
     ap[1] += Nim * v[5];
-    const double b0 = G.bodyfield ? P.bf[0][p] : G.body[0], b1 = G.bodyfield ? P.bf[1][p] : G.body[1];
+    const double b0 = BF ? P.bf[0][p] : G.body[0], b1 = BF ? P.bf[1][p] : G.body[1];
     af[0] += (-fV) * (v[6] * dN[0] + v[8] * dN[1]) + (fM * b0) * N;
     af[1] += (-fV) * (v[8] * dN[0] + v[7] * dN[1]) + (fM * b1) * N;
     if (B == JUMP_BASIS_CPDI_Q4) {
