@@ -125,6 +125,7 @@ struct jump_ctx {
     AdjNodes3 an3{};
     AdjSources3 src3{};
     std::vector<double> dfy, dk1;  // per solid, from the last backward sweep
+    int lanes = 1;  // lanes per node of the backward sweep's node gathers (1, 2, 4)
   } adj;
   uint32_t flags = 0;
 
@@ -2054,29 +2055,61 @@ static size_t grid_bytes(const jump_ctx* ctx) {
   return size_t(grid_arrays(ctx)) * seg_offset(size_t(ctx->NN) * sizeof(double));
 }
 
-template <int B> static void adj_step_back3(jump_ctx* ctx) {
+// Lanes per node of the backward sweep's node gathers (k_adj_nodes,
+// k_adj3_nodes1/2): the particles per cell and direction of the run's
+// particles, (cell volume / mean initial particle volume)^(1/D), as 1, 2 or
+// 4.  It is fixed when the run is recorded, so every backward sweep of the
+// run sums in the same order whatever the checkpoint interval.  Measured on
+// the 2D two-rings adjoint (4 per cell and direction) and the Taylor bar (2).
+static int adjoint_lanes(const jump_ctx* ctx) {
+  double v = 0.0;
+  for (double x : ctx->hV0) v += x;
+  if (ctx->hV0.empty() || !(v > 0.0)) return 1;
+  v /= double(ctx->hV0.size());
+  const double cell = ctx->G.dx * ctx->G.dy * (ctx->D == 3 ? ctx->G.dz : 1.0);
+  const double r = std::pow(cell / v, 1.0 / double(ctx->D));
+  return r >= 3.5 ? 4 : (r >= 1.5 ? 2 : 1);
+}
+
+template <int B, int S> static void adj_step_back3_s(jump_ctx* ctx) {
   const int np = int(ctx->np), nn = ctx->NN;
+  const int64_t nl = int64_t(nn) * S;
   const double dt = ctx->prm.dt, alpha = ctx->prm.alpha;
   auto& A = ctx->adj;
   LAUNCH((k_adj3_g2p2<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, dt, A.ap3, A.src3);
-  LAUNCH((k_adj3_nodes2<B>), nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g, dt, A.src3, A.an3);
-  LAUNCH((k_adj3_g2p1<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, alpha, A.an3, A.ap3, A.src3);
-  LAUNCH((k_adj3_nodes1<B>), nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g, alpha, A.src3,
+  LAUNCH((k_adj3_nodes2<B, S>), nl, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g, dt, A.src3,
          A.an3);
+  LAUNCH((k_adj3_g2p1<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, alpha, A.an3, A.ap3, A.src3);
+  LAUNCH((k_adj3_nodes1<B, S>), nl, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g, alpha,
+         A.src3, A.an3);
   LAUNCH(k_adj3_grid, nn, nn, ctx->d_fixed, dt, A.an3);
   LAUNCH((k_adj3_p2g<B>), np, ctx->P, ctx->G, ctx->d_solids, A.an3, A.ap3);
 }
+template <int B> static void adj_step_back3(jump_ctx* ctx) {
+  switch (ctx->adj.lanes) {
+    case 4: adj_step_back3_s<B, 4>(ctx); break;
+    case 2: adj_step_back3_s<B, 2>(ctx); break;
+    default: adj_step_back3_s<B, 1>(ctx); break;
+  }
+}
 
-template <int B> static void adj_step_back(jump_ctx* ctx) {
+template <int B, int S> static void adj_step_back_s(jump_ctx* ctx) {
   const int np = int(ctx->np), nn = ctx->NN;
   const double dt = ctx->prm.dt;
   AdjNodes& an = ctx->adj.an;
   LAUNCH((k_adj_g2p<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->g, ctx->prm.tolerance, dt, ctx->adj.ap,
          ctx->adj.src);
-  LAUNCH((k_adj_nodes<B>), nn, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g,
+  LAUNCH((k_adj_nodes<B, S>), int64_t(nn) * S, ctx->P, ctx->G, ctx->d_solids, ctx->d_start, ctx->d_sorted, ctx->g,
          ctx->prm.tolerance, dt, ctx->adj.src, an);
   LAUNCH(k_adj_grid, nn, nn, ctx->d_fixed, dt, an);
   LAUNCH((k_adj_p2g<B>), np, ctx->P, ctx->G, ctx->d_solids, an, ctx->adj.ap);
+}
+template <int B> static void adj_step_back(jump_ctx* ctx) {
+  switch (ctx->adj.lanes) {
+    case 4: adj_step_back_s<B, 4>(ctx); break;
+    case 2: adj_step_back_s<B, 2>(ctx); break;
+    default: adj_step_back_s<B, 1>(ctx); break;
+  }
 }
 
 // Device bytes a recorded run of nsteps steps at interval K holds: nck
@@ -2153,6 +2186,7 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
   A.ready = false;
   A.dfy.clear();
   A.dk1.clear();
+  A.lanes = adjoint_lanes(ctx);
   unsigned fl = 0;
   const int64_t last0 = (nck - 1) * K;  // the last segment's states and grids are kept as they go by
   for (int64_t s = 0; s < nsteps; s++) {

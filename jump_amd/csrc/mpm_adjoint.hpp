@@ -208,19 +208,39 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
   ad.mu[i] += mub;
 }
 
+// Lanes per node of the node gathers below.  With one lane per node, the
+// lanes of a wave sit on neighbouring nodes of a grid row, whose windows are
+// one cell apart: at each visit lane i reads a particle one cell's worth of
+// particles after lane i-1's (4 apart at 4 particles per cell and
+// direction), so every 16-byte wave-load touches 4x the cache lines it
+// uses.  With S lanes per node, lane s of a node takes entries s, s+S,
+// s+2S, ... of each window bin; on a lattice of S particles per cell and
+// direction (x fastest) the S lanes read S consecutive particles, and the
+// wave's loads are contiguous.  The partial sums are combined in a fixed
+// order, so the gathers stay deterministic.
+template <int S> __device__ __forceinline__ double lanes_sum(double v) {
+  // ((l0 + l1) + (l2 + l3)) in every lane of the group: addition commutes
+  // exactly, so all S lanes hold the same bits
+  if (S >= 2) v += __shfl_xor(v, 1);
+  if (S >= 4) v += __shfl_xor(v, 2);
+  return v;
+}
+
 // Node adjoints of the G2P sums, gathered per node over the particles of
-// its window (the step's bins, any order: the sum is not bit-matched to a
-// reference).  u = P/m, a = (P - P0)/m:
+// its window (the step's bins, any fixed order: the sum is not bit-matched
+// to a reference).  u = P/m, a = (P - P0)/m:
 //   Pb  += (ub + ab)/m    P0b -= ab/m    mb -= (u.ub + a.ab)/m
 //   ub = N dt xb' + Lb g (L[a][b] = u_a g_b),   ab = N vb'
-template <int B>
-__global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+// S lanes per node (lanes_sum).
+template <int B, int S>
+__global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, const SolidDev* __restrict__ S_,
                                                       const int* __restrict__ start,
                                                       const unsigned long long* __restrict__ sorted, GridArrays g,
                                                       double tol, double dt, AdjSources src, AdjNodes an) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= G.nn) return;
-  const double mI = g.m[n];
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int n = int(tid / S), sl = int(tid % S);
+  const bool live = n < G.nn;  // every lane reaches the sums below
+  const double mI = live ? g.m[n] : 0.0;
   double Pb0 = 0.0, Pb1 = 0.0, P0b0 = 0.0, P0b1 = 0.0, mb = 0.0;
   if (mI > tol) {
     constexpr int W = Width<B>::W;
@@ -233,26 +253,25 @@ __global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, c
       for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
         const int bin = bi + G.nx * bj;
         const int e = start[bin + 1];
-        int c = start[bin];
+        int c = start[bin] + sl;
         if (c >= e) continue;
         // the next particle's index, solid and position are loaded while
         // the current one is summed (the order of the sums is unchanged)
         int q = int(unsigned(sorted[c]));
         int sq = P.sid[q];
-        double xq = P.x[0][q], yq = P.x[1][q];
-        for (; c < e; c++) {
+        double2 xy = ldu(P.pr[0], unsigned(q));
+        for (; c < e; c += S) {
           int qn = q, sn = sq;
-          double xn = xq, yn = yq;
-          if (c + 1 < e) {
-            qn = int(unsigned(sorted[c + 1]));
+          double2 xyn = xy;
+          if (c + S < e) {
+            qn = int(unsigned(sorted[c + S]));
             sn = P.sid[qn];
-            xn = P.x[0][qn];
-            yn = P.x[1][qn];
+            xyn = ldu(P.pr[0], unsigned(qn));
           }
-          const SolidDev& s = S[sq];
+          const SolidDev& s = S_[sq];
           double wx, d1x, d2x, wy, d1y, d2y;
-          axis_w2<B>(xq, X, s.lp[0], ni + 1, wx, d1x, d2x);
-          axis_w2<B>(yq, Y, s.lp[1], nj + 1, wy, d1y, d2y);
+          axis_w2<B>(xy.x, X, s.lp[0], ni + 1, wx, d1x, d2x);
+          axis_w2<B>(xy.y, Y, s.lp[1], nj + 1, wy, d1y, d2y);
           const double N = wx * wy, gx = d1x * wy, gy = wx * d1y;
           const double2 xb2 = src.xb[q], vb2 = src.vb[q], L01 = src.Lb[0][q], L23 = src.Lb[1][q];
           const double xb[2] = {xb2.x, xb2.y}, vb[2] = {vb2.x, vb2.y};
@@ -267,11 +286,16 @@ __global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, c
           mb -= (u[0] * ub0 + u[1] * ub1 + a[0] * ab0 + a[1] * ab1) * iM;
           q = qn;
           sq = sn;
-          xq = xn;
-          yq = yn;
+          xy = xyn;
         }
       }
   }
+  Pb0 = lanes_sum<S>(Pb0);
+  Pb1 = lanes_sum<S>(Pb1);
+  P0b0 = lanes_sum<S>(P0b0);
+  P0b1 = lanes_sum<S>(P0b1);
+  mb = lanes_sum<S>(mb);
+  if (!live || sl != 0) return;
   an.mb[n] = mb;
   an.Pb[0][n] = Pb0;
   an.Pb[1][n] = Pb1;

@@ -315,10 +315,13 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p2(Particles P, GridDesc G, c
 }
 
 // Node gather over the particles of the node's window (the step's bins).
-// fn(particle, N, g[3]) for every particle whose stencil covers node n.
-template <int B, class Fn>
-__device__ __forceinline__ void window_gather3(const Particles& P, const GridDesc& G, const SolidDev* S,
-                                               const int* start, const unsigned long long* sorted, int n, Fn&& fn) {
+// fn(particle, N, g[3]) for every particle whose stencil covers node n that
+// lane sl of the node's S lanes takes: entries sl, sl+S, ... of each bin
+// (lanes_sum).
+template <int B, int S, class Fn>
+__device__ __forceinline__ void window_gather3(const Particles& P, const GridDesc& G, const SolidDev* SD,
+                                               const int* start, const unsigned long long* sorted, int n, int sl,
+                                               Fn&& fn) {
   constexpr int W = Width<B>::W;
   const int ni = n % G.nx, nj = (n / G.nx) % G.ny, nk = n / G.nxy;
   const Axis X = axis_of(G, 0), Y = axis_of(G, 1), Z = axis_of(G, 2);
@@ -327,12 +330,13 @@ __device__ __forceinline__ void window_gather3(const Particles& P, const GridDes
       for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
         const int bin = bi + G.nx * (bj + G.ny * bk);
         const int e = start[bin + 1];
-        for (int c = start[bin]; c < e; c++) {
+        for (int c = start[bin] + sl; c < e; c += S) {
           const int q = int(unsigned(sorted[c]));
-          const SolidDev& s = S[P.sid[q]];
+          const SolidDev& s = SD[P.sid[q]];
+          const double2 xy = ldu(P.pr[0], unsigned(q));
           double wx, d1x, d2x, wy, d1y, d2y, wz, d1z, d2z;
-          axis_w2<B>(P.x[0][q], X, s.lp[0], ni + 1, wx, d1x, d2x);
-          axis_w2<B>(P.x[1][q], Y, s.lp[1], nj + 1, wy, d1y, d2y);
+          axis_w2<B>(xy.x, X, s.lp[0], ni + 1, wx, d1x, d2x);
+          axis_w2<B>(xy.y, Y, s.lp[1], nj + 1, wy, d1y, d2y);
           axis_w2<B>(P.x[2][q], Z, s.lp[2], nk + 1, wz, d1z, d2z);
           const double gr[3] = {d1x * wy * wz, wx * d1y * wz, wx * wy * d1z};
           fn(q, wx * wy * wz, gr);
@@ -342,20 +346,22 @@ __device__ __forceinline__ void window_gather3(const Particles& P, const GridDes
 
 //   P_b  += N dt x'_b / m      P2_b[b] += sum_a g_a L_b[a,b] / m
 //   m_b  -= (N dt P.x'_b + sum_ab g_a P2_b L_b[a,b]) / m^2
-template <int B>
-__global__ void __launch_bounds__(kBlock) k_adj3_nodes2(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+// S lanes per node (lanes_sum).
+template <int B, int S>
+__global__ void __launch_bounds__(kBlock) k_adj3_nodes2(Particles P, GridDesc G, const SolidDev* __restrict__ SD,
                                                         const int* __restrict__ start,
                                                         const unsigned long long* __restrict__ sorted, GridArrays g,
                                                         double dt, AdjSources3 src, AdjNodes3 an) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= G.nn) return;
-  const double m = g.m[n];
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int n = int(tid / S), sl = int(tid % S);
+  const bool live = n < G.nn;  // every lane reaches the sums below
+  const double m = live ? g.m[n] : 0.0;
   double Pb[3] = {0.0, 0.0, 0.0}, P2b[3] = {0.0, 0.0, 0.0}, mb = 0.0;
   if (m > 0.0) {
     const double iM = 1.0 / m;
     const double u[3] = {g.p[0][n] * iM, g.p[1][n] * iM, g.p[2][n] * iM};
     const double u2[3] = {g.p2[0][n] * iM, g.p2[1][n] * iM, g.p2[2][n] * iM};
-    window_gather3<B>(P, G, S, start, sorted, n, [&](int q, double N, const double* gr) {
+    window_gather3<B, S>(P, G, SD, start, sorted, n, sl, [&](int q, double N, const double* gr) {
       const double xb[3] = {src.xb[0][q], src.xb[1][q], src.xb[2][q]};
       double ub[3], u2b[3];
       for (int b = 0; b < 3; b++) {
@@ -367,6 +373,12 @@ __global__ void __launch_bounds__(kBlock) k_adj3_nodes2(Particles P, GridDesc G,
       mb -= (u[0] * ub[0] + u[1] * ub[1] + u[2] * ub[2] + u2[0] * u2b[0] + u2[1] * u2b[1] + u2[2] * u2b[2]) * iM;
     });
   }
+  for (int k = 0; k < 3; k++) {
+    Pb[k] = lanes_sum<S>(Pb[k]);
+    P2b[k] = lanes_sum<S>(P2b[k]);
+  }
+  mb = lanes_sum<S>(mb);
+  if (!live || sl != 0) return;
   an.mb[n] = mb;
   for (int k = 0; k < 3; k++) {
     an.Pb[k][n] = Pb[k];
@@ -416,26 +428,32 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p1(Particles P, GridDesc G, c
 }
 
 //   P_b += N v'_b / m    P0_b -= alpha N v'_b / m    m_b -= N (P - alpha P0).v'_b / m^2
-template <int B>
-__global__ void __launch_bounds__(kBlock) k_adj3_nodes1(Particles P, GridDesc G, const SolidDev* __restrict__ S,
+// S lanes per node (lanes_sum).
+template <int B, int S>
+__global__ void __launch_bounds__(kBlock) k_adj3_nodes1(Particles P, GridDesc G, const SolidDev* __restrict__ SD,
                                                         const int* __restrict__ start,
                                                         const unsigned long long* __restrict__ sorted, GridArrays g,
                                                         double alpha, AdjSources3 src, AdjNodes3 an) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= G.nn) return;
-  const double m = g.m[n];
-  if (!(m > 0.0)) return;
-  const double iM = 1.0 / m;
-  const double q[3] = {(g.p[0][n] - alpha * g.p0[0][n]) * iM, (g.p[1][n] - alpha * g.p0[1][n]) * iM,
-                       (g.p[2][n] - alpha * g.p0[2][n]) * iM};
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int n = int(tid / S), sl = int(tid % S);
+  const bool live = n < G.nn;  // every lane reaches the sums below
+  const double m = live ? g.m[n] : 0.0;
   double Pb[3] = {0.0, 0.0, 0.0}, mb = 0.0;
-  window_gather3<B>(P, G, S, start, sorted, n, [&](int p, double N, const double*) {
-    for (int k = 0; k < 3; k++) {
-      const double b = N * src.vb[k][p];
-      Pb[k] += b * iM;
-      mb -= q[k] * b * iM;
-    }
-  });
+  if (m > 0.0) {
+    const double iM = 1.0 / m;
+    const double q[3] = {(g.p[0][n] - alpha * g.p0[0][n]) * iM, (g.p[1][n] - alpha * g.p0[1][n]) * iM,
+                         (g.p[2][n] - alpha * g.p0[2][n]) * iM};
+    window_gather3<B, S>(P, G, SD, start, sorted, n, sl, [&](int p, double N, const double*) {
+      for (int k = 0; k < 3; k++) {
+        const double b = N * src.vb[k][p];
+        Pb[k] += b * iM;
+        mb -= q[k] * b * iM;
+      }
+    });
+  }
+  for (int k = 0; k < 3; k++) Pb[k] = lanes_sum<S>(Pb[k]);
+  mb = lanes_sum<S>(mb);
+  if (!(m > 0.0) || sl != 0) return;
   an.mb[n] += mb;
   for (int k = 0; k < 3; k++) {
     an.Pb[k][n] += Pb[k];

@@ -531,12 +531,13 @@ def taylor_bar(scale=1.0, steps=400, basis=_ffi.BASIS_GIMP, material="j2", E=117
 
 
 def block3d(nodes=(12, 12, 12), basis=_ffi.BASIS_QUAD_BSPLINE, material="elastic", steps=10, v=(1.0, -2.0, 0.5),
-            seed=0):
+            seed=0, ppc=2):
     """Small 3D block with a random velocity field and random initial stress,
-    for parity tests of one material/basis pair."""
+    for parity tests of one material/basis pair; ppc particles per cell and
+    direction."""
     rng = np.random.default_rng(seed)
     dx = 1.0 / (nodes[0] - 1)
-    f = dx / 2
+    f = dx / ppc
     ctr = [0.5 + 0.13 * dx, 0.5 + 0.07 * dx, 0.5 + 0.11 * dx]
     pts = buildParticleForBlock(ctr, 0.45, 0.45, 0.45, f)
     rho, E, nu = 1000.0, 1.0e6, 0.3
@@ -561,12 +562,13 @@ def block3d(nodes=(12, 12, 12), basis=_ffi.BASIS_QUAD_BSPLINE, material="elastic
 
 
 def block2d(cells=16, basis=_ffi.BASIS_LINEAR, material="elastic", steps=10, seed=0, algorithm=_ffi.ALG_USL,
-            alpha=1.0):
+            alpha=1.0, ppc=2):
     """Small 2D block with random velocity and stress (parity tests); USL, or
-    MUSL with the blended mass matrix and FLIP fraction `alpha`."""
+    MUSL with the blended mass matrix and FLIP fraction `alpha`; ppc
+    particles per cell and direction."""
     rng = np.random.default_rng(seed)
     dx = 1.0 / cells
-    f = dx / 2
+    f = dx / ppc
     pts = buildParticleForRectangle([0.5 + 0.13 * dx, 0.5 + 0.07 * dx], 0.45, 0.45, f)
     rho, E, nu = 1000.0, 1.0e6, 0.3
     mat = {"elastic": elastic(E, nu, rho), "neohookean": neohookean(E, nu, rho),

@@ -118,3 +118,36 @@ def test_adjoint2d_matches_dual_numbers(hip_lib, basis, material):
     print({k: (got[k], ref[k]) for k in got})
     for k in got:
         assert abs(got[k] - ref[k]) <= 1e-9 * abs(ref[k]), (k, got[k], ref[k])
+
+
+@pytest.mark.parametrize("ppc", [1, 4])
+def test_adjoint2d_lanes_per_node(hip_lib, ppc):
+    """The node gathers of the backward sweep split each node over as many
+    lanes as the run has particles per cell and direction (1, 2 or 4, fixed
+    when the run is recorded); the 2 of the blocks above, and here 1 and 4,
+    each against the dual-number restatement at relative 1e-9, and the
+    gradient independent of the checkpoint interval."""
+    from oracle.pyoracle import OracleEngine
+    sc = scenes.block2d(cells=12, basis=_ffi.BASIS_GIMP, material="neohookean", steps=NSTEPS, seed=7, ppc=ppc)
+    rng = np.random.default_rng(5)
+    n = sc.n_particles
+    pb, vb = rng.normal(size=(n, 2)), rng.normal(size=(n, 2))
+    res = []
+    for K in (7, NSTEPS):
+        e = Engine(sc.params)
+        sc.load(e)
+        (pos, vel, t, rc), pullback = rrule(e, NSTEPS, checkpoint_every=K)
+        assert rc == 0
+        res.append(pullback(pb, vb))
+        e.close()
+    for k in ("E", "nu", "v0"):
+        assert np.array_equal(res[0][k], res[1][k]), k
+    o = OracleEngine(sc.params)
+    sc.load(o)
+    want = o.dual_gradient(0, NSTEPS, pb, vb)
+    g = res[0]
+    got = {"E": g["E"][0], "nu": g["nu"][0], "v0": g["v0"][0, 0], "v1": g["v0"][0, 1]}
+    ref = {"E": want["E"], "nu": want["nu"], "v0": want["v0"][0], "v1": want["v0"][1]}
+    print(ppc, {k: (got[k], ref[k]) for k in got})
+    for k in got:
+        assert abs(got[k] - ref[k]) <= 1e-9 * abs(ref[k]), (k, got[k], ref[k])

@@ -102,6 +102,33 @@ def test_adjoint3d_matches_central_differences(hip_lib):
         assert abs(g["v0"][0, k] - fd) <= 1e-6 * abs(fd), (k, g["v0"][0, k], fd)
 
 
+@pytest.mark.parametrize("ppc", [1, 4])
+def test_adjoint3d_lanes_per_node(hip_lib, ppc):
+    """The backward sweep's node gathers at 1 and 4 lanes per node (1 and 4
+    particles per cell and direction; the blocks above run 2): against the
+    dual-number restatement at relative 1e-9, and independent of the
+    checkpoint interval."""
+    print(f"adjoint3d lanes ppc={ppc}", flush=True)
+    sc = scenes.block3d(nodes=(9, 9, 9), basis=_ffi.BASIS_GIMP, material="j2", steps=NSTEPS, seed=6, ppc=ppc)
+    pb, vb = _bars(sc)
+    g = _gpu_gradient(sc, pb, vb, 3)
+    g2 = _gpu_gradient(sc, pb, vb, NSTEPS)
+    for k in ("E", "nu", "fy", "k1", "v0"):
+        assert np.array_equal(g[k], g2[k]), k
+    o = OracleEngine(sc.params)
+    sc.load(o)
+    want = o.dual_gradient(0, NSTEPS, pb, vb)
+    o.close()
+    got = {"E": g["E"][0], "nu": g["nu"][0], "fy": g["fy"][0], "k1": g["k1"][0]}
+    v0 = want.pop("v0")
+    for k in range(3):
+        got[f"vel{k}"] = g["v0"][0, k]
+        want[f"vel{k}"] = v0[k]
+    print({k: (got[k], want[k]) for k in got})
+    for k in got:
+        assert abs(got[k] - want[k]) <= 1e-9 * max(abs(want[k]), 1e-300) + 1e-300, (k, got[k], want[k])
+
+
 def test_adjoint3d_checkpoint_interval_does_not_change_the_gradient(hip_lib):
     sc = _scene(_ffi.BASIS_GIMP, "j2")
     pb, vb = _bars(sc)
