@@ -669,9 +669,12 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   bin_and_sort<3, B, false>(ctx, ORDER == JUMP_ORDER_REFERENCE ? ctx->d_dirty : nullptr, rl.gen);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
-  LAUNCH_P2G((k_p2g<3, B, 0, ORDER, false, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-             ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha,
-             rl);
+  constexpr int LN0 = ORDER == JUMP_ORDER_REFERENCE ? kP2GLanes0 : 1;  // lanes per node
+  constexpr int LN1 = ORDER == JUMP_ORDER_REFERENCE ? kP2GLanes1 : 1;
+  hipLaunchKernelGGL((k_p2g<3, B, 0, ORDER, false, false, kP2GBlock, LN0>),
+                     dim3((ctx->NN + kP2GBlock / LN0 - 1) / (kP2GBlock / LN0)), dim3(kP2GBlock), 0, ctx->stream,
+                     ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz,
+                     ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha, rl);
   if (ORDER == JUMP_ORDER_REFERENCE) ctx->runs_valid = true;
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
@@ -679,8 +682,8 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_G2P1);
   prof_begin(ctx);
   {
-    const int rows = (ctx->NN + kP2GBlock - 1) / kP2GBlock;
-    hipLaunchKernelGGL((k_p2g<3, B, 1, ORDER, true, false>), dim3(xcd_grid(rows)), dim3(kP2GBlock),
+    const int rows = (ctx->NN + kP2GBlock / LN1 - 1) / (kP2GBlock / LN1);
+    hipLaunchKernelGGL((k_p2g<3, B, 1, ORDER, true, false, kP2GBlock, LN1>), dim3(xcd_grid(rows)), dim3(kP2GBlock),
                        0, ctx->stream, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
                        ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha,
                        rl);

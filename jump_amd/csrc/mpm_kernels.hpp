@@ -429,6 +429,10 @@ __device__ __forceinline__ void p2g_add(const Particles& P, const GridDesc& G, c
 // particles in index order: deterministic, no merge, a different rounding
 // than the reference's fold.
 constexpr int kP2GBlock = 128;
+// Lanes per node of the 3D single-problem gathers in reference order
+// (p2g_block's LPN): the force pass and the re-projection.
+constexpr int kP2GLanes0 = 1;
+constexpr int kP2GLanes1 = 2;
 // Work order of the P2G threads.  Only nodes whose window holds a particle
 // ("active") have work.  With waves on 64-node grid rows, lanes past the
 // solid's edge idle: about half the lane slots on the 4.2M-particle impact.
@@ -515,10 +519,11 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
                                                      const unsigned long long* bxyz, int off, int inst, int W,
                                                      int ni, int nj, int nk, int (*s_cur)[BLK],
                                                      int (*s_head)[BLK], Visit&& visit,
-                                                     OnRun&& on_run = OnRun{}) {
+                                                     OnRun&& on_run = OnRun{}, int slot = -1) {
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
-  const int t = threadIdx.x;
+  // the thread's column of s_cur/s_head (lanes that share a node share it)
+  const int t = slot < 0 ? int(threadIdx.x) : slot;
   if constexpr (ORDER == JUMP_ORDER_BINNED) {
     const int k0 = (D == 3) ? max(nk - W + 1, 0) : 0;
     for (int bk = k0; bk <= nk; bk++)
@@ -602,20 +607,101 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
   }
 }
 
-// The nodes of logical block L (BLK threads, one node each): the body of
-// the gather kernels below.
-template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI, int BLK>
+// The terms one particle adds to a 3D node's accumulators, each formed by
+// the expression p2g_add_v adds, so that folding the terms in particle order
+// gives the same bits: PASS 0 t = (N m, N m v, force term (subtracted)),
+// PASS 1 t = N m v.
+template <int B, int PASS> struct Terms3 { static constexpr int NT = PASS == 0 ? 7 : 3; };
+template <int B, int PASS>
+__device__ __forceinline__ void p2g_terms3(const Particles& P, const GridDesc& G, const SolidDev* S,
+                                           const double* corners, int p, const double* v, int ni, int nj, int nk,
+                                           double* t) {
+  double lp[3] = {0.0, 0.0, 0.0};
+  if (B == JUMP_BASIS_GIMP) {
+    const SolidDev& s = S[P.sid[p]];
+    for (int k = 0; k < 3; k++) lp[k] = s.lp[k];
+  }
+  double N, dN[3];
+  bool member;
+  node_weight<3, B, PASS == 1, PASS == 0>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
+  const double fM = v[3];
+  if (PASS == 0) {
+    const double fV = v[4];
+    const double* vp = v + 5;
+    const double* sg = v + 8;
+    const double Nim = N * fM;
+    t[0] = Nim;
+    for (int k = 0; k < 3; k++) t[1 + k] = Nim * vp[k];
+    t[4] = fV * (sg[0] * dN[0] + sg[3] * dN[1] + sg[4] * dN[2]);
+    t[5] = fV * (sg[3] * dN[0] + sg[1] * dN[1] + sg[5] * dN[2]);
+    t[6] = fV * (sg[4] * dN[0] + sg[5] * dN[1] + sg[2] * dN[2]);
+  } else {
+    const double Nm = N * fM;
+    for (int k = 0; k < 3; k++) t[k] = Nm * v[5 + k];
+  }
+}
+template <int PASS>
+__device__ __forceinline__ void p2g_fold3(const double* t, double& am, double* ap, double* af) {
+  if (PASS == 0) {
+    am += t[0];
+    for (int k = 0; k < 3; k++) ap[k] += t[1 + k];
+    for (int k = 0; k < 3; k++) af[k] -= t[4 + k];
+  } else {
+    for (int k = 0; k < 3; k++) ap[k] += t[k];
+  }
+}
+// A double from lane 2j (SEL 0) or 2j+1 (SEL 1) of this lane's pair, by DPP
+// within the quad (quad_perm [0,0,2,2] or [1,1,3,3]).
+template <int SEL> __device__ __forceinline__ double pair_lane(double x) {
+  constexpr int ctrl = SEL == 0 ? 0xA0 : 0xF5;
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+  const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(b)), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), ctrl, 0xF, 0xF, false);
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(unsigned(hi)) << 32) |
+                                                     unsigned(lo)));
+}
+// Two lanes per node: lane 0 of the pair holds the terms of the earlier
+// particle, lane 1 those of the next (if `two`); both lanes fold both, in
+// that order, so both hold the node's sums.
+template <int PASS>
+__device__ __forceinline__ void p2g_fold_pair(const double* t, bool two, double& am, double* ap, double* af) {
+  constexpr int NT = PASS == 0 ? 7 : 3;
+  double a[NT], b[NT];
+#pragma unroll
+  for (int j = 0; j < NT; j++) {
+    a[j] = pair_lane<0>(t[j]);
+    b[j] = pair_lane<1>(t[j]);
+  }
+  p2g_fold3<PASS>(a, am, ap, af);
+  if (two) p2g_fold3<PASS>(b, am, ap, af);
+}
+
+// The nodes of logical block L (BLK threads, one node each, or LPN = 2: two
+// lanes per node): the body of the gather kernels below.
+//
+// LPN = 2 (3D, reference order).  A wave whose lanes sit on neighbouring
+// nodes of a row, one lane each, reads at every visit particles a cell's
+// worth of particles apart (2 on the impact's lattice), so each 16-byte
+// field load touches twice the cache lines it uses.  With two lanes per
+// node, lane h of the pair computes the terms of the node's particles
+// h, h+2, h+4, ... of its sequence; the pair swaps them (DPP) and both lanes
+// fold them in the sequence's order: every sum keeps the reference's order
+// and bits, and a wave's loads read consecutive particles.
+template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI, int BLK, int LPN = 1>
 __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0, const SolidDev* __restrict__ S,
                                           const double* __restrict__ corners, const int* __restrict__ start0,
                                           const unsigned long long* __restrict__ sorted,
                                           const unsigned long long* __restrict__ bxyz,
                                           const int* __restrict__ order, const int* __restrict__ nodepos,
                                           const GridArrays& g0, int Wrt, double dt0, const BatchDev& Bt,
-                                          double alpha, const RunList& rl, int L, int (*s_cur)[BLK],
-                                          int (*s_head)[BLK]) {
+                                          double alpha, const RunList& rl, int L, int (*s_cur)[BLK / LPN],
+                                          int (*s_head)[BLK / LPN]) {
+  static_assert(LPN == 1 || (LPN == 2 && D == 3 && !MULTI && ORDER == JUMP_ORDER_REFERENCE),
+                "two lanes per node: the 3D single-problem gathers in reference order");
   const int t = threadIdx.x;
+  const int slot = t / LPN, h = t % LPN;  // node of the block, lane within the node
   const int NN = MULTI ? Bt.NN : G0.nn;
-  const int i = L * BLK + t;
+  const int i = L * (BLK / LPN) + slot;
   if (i >= NN) return;
   const int ng = COMPACT ? ldu(order, unsigned(i)) : i;  // node in the (concatenated) node arrays
   constexpr bool RUNS = D == 3 && !MULTI && ORDER == JUMP_ORDER_REFERENCE;
@@ -654,7 +740,36 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
   constexpr bool DEFER = PASS == 1 || D == 2;
   if (PASS == 0 && still_empty) return;
   constexpr bool AFF = D == 2 && PASS == 0 && !MULTI;
-  auto visit = [&](int p) { p2g_add<D, B, PASS, AFF>(P, G, S, corners, p, ni, nj, nk, am, ap, af); };
+  constexpr int NF = P2GFields<D, PASS>::NF;
+  int pend = -1;  // LPN 2: the first particle of a pair not summed yet
+  auto visit = [&](int p) {
+    if constexpr (LPN == 1) {
+      p2g_add<D, B, PASS, AFF>(P, G, S, corners, p, ni, nj, nk, am, ap, af);
+    } else {
+      if (pend < 0) {
+        pend = p;
+        return;
+      }
+      const int mine = h ? p : pend;
+      double v[NF + 1], tt[Terms3<B, PASS>::NT];
+      P2GFields<D, PASS>::load(P, unsigned(mine), v);
+      p2g_terms3<B, PASS>(P, G, S, corners, mine, v, ni, nj, nk, tt);
+      p2g_fold_pair<PASS>(tt, true, am, ap, af);
+      pend = -1;
+    }
+  };
+  // the pair's last particle when the sequence has odd length (both lanes)
+  auto flush = [&]() {
+    if constexpr (LPN == 2) {
+      if (pend >= 0) {
+        double v[NF + 1], tt[Terms3<B, PASS>::NT];
+        P2GFields<D, PASS>::load(P, unsigned(pend), v);
+        p2g_terms3<B, PASS>(P, G, S, corners, pend, v, ni, nj, nk, tt);
+        p2g_fold3<PASS>(tt, am, ap, af);
+        pend = -1;
+      }
+    }
+  };
   // Replay.  The re-projection loads the next particle's fields (and the
   // next run's entry) while the current particle is summed; the sums keep
   // their order (0.683 -> 0.651 ms on the impact).  The force pass loads
@@ -662,37 +777,97 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
   // on (1.463 -> 1.378 ms; all 14 fields ahead: 1.46 -> 1.53 ms, 117 -> 140
   // VGPRs).
   auto replay = [&](int nr) {
-    if constexpr (PASS == 0) {
+    if constexpr (LPN == 2 && PASS == 1) {
+      // this lane's particles of each run, the next one's fields (and the
+      // next run's entry) loaded while the current one is summed
+      unsigned e = nr > 0 ? rl.runs[n] : 0u;
       for (int r = 0; r < nr; r++) {
-        const unsigned e = rl.runs[size_t(r) * NN + n];
         const int p0 = int(e >> 3), len = int(e & 7u) + 1;
-        double2 a = ldu(P.pr[0], unsigned(p0)), b = ldu(P.pr[1], unsigned(p0));  // (x, y) (z, m)
-        for (int k = 0; k < len; k++) {
-          double v[P2GFields<D, PASS>::NF + 1];
-          v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
-          P2GFields<D, PASS>::load(P, unsigned(p0 + k), v, 2);
-          if (k + 1 < len) {
-            a = ldu(P.pr[0], unsigned(p0 + k + 1));
-            b = ldu(P.pr[1], unsigned(p0 + k + 1));
+        if (r + 1 < nr) e = rl.runs[size_t(r + 1) * NN + n];
+        double v[NF + 1], w[NF + 1];
+        if (h < len) P2GFields<D, PASS>::load(P, unsigned(p0 + h), v);
+        for (int k = 0; k < len; k += 2) {
+          const int mine = k + h;
+          double tt[Terms3<B, PASS>::NT];
+          if (mine < len) {
+            if (mine + 2 < len) P2GFields<D, PASS>::load(P, unsigned(p0 + mine + 2), w);
+            p2g_terms3<B, PASS>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt);
+            if (mine + 2 < len)
+#pragma unroll
+              for (int q = 0; q < NF + 1; q++) v[q] = w[q];
+          } else {
+#pragma unroll
+            for (int j = 0; j < Terms3<B, PASS>::NT; j++) tt[j] = 0.0;
           }
-          p2g_add_v<D, B, PASS>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+          p2g_fold_pair<PASS>(tt, k + 1 < len, am, ap, af);
         }
       }
       return;
-    }
-    constexpr int NF = P2GFields<D, PASS>::NF;
-    unsigned e = nr > 0 ? rl.runs[n] : 0u;
-    for (int r = 0; r < nr; r++) {
-      const int p0 = int(e >> 3), len = int(e & 7u) + 1;
-      if (r + 1 < nr) e = rl.runs[size_t(r + 1) * NN + n];
-      double v[NF + 1], w[NF + 1];
-      P2GFields<D, PASS>::load(P, unsigned(p0), v);
-      for (int k = 0; k < len; k++) {
-        if (k + 1 < len) P2GFields<D, PASS>::load(P, unsigned(p0 + k + 1), w);
-        p2g_add_v<D, B, PASS>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
-        if (k + 1 < len)
+    } else if constexpr (LPN == 2) {
+      for (int r = 0; r < nr; r++) {
+        const unsigned e = rl.runs[size_t(r) * NN + n];
+        const int p0 = int(e >> 3), len = int(e & 7u) + 1;
+        double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);  // (x, y) (z, m) of this lane's next
+        if (PASS == 0 && h < len) {
+          a = ldu(P.pr[0], unsigned(p0 + h));
+          b = ldu(P.pr[1], unsigned(p0 + h));
+        }
+        for (int k = 0; k < len; k += 2) {
+          const int mine = k + h;
+          double tt[Terms3<B, PASS>::NT];
+          if (mine < len) {
+            double v[NF + 1];
+            if (PASS == 0) {
+              v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+              P2GFields<D, PASS>::load(P, unsigned(p0 + mine), v, 2);
+              if (mine + 2 < len) {
+                a = ldu(P.pr[0], unsigned(p0 + mine + 2));
+                b = ldu(P.pr[1], unsigned(p0 + mine + 2));
+              }
+            } else {
+              P2GFields<D, PASS>::load(P, unsigned(p0 + mine), v);
+            }
+            p2g_terms3<B, PASS>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt);
+          } else {
 #pragma unroll
-          for (int q = 0; q < NF + 1; q++) v[q] = w[q];
+            for (int j = 0; j < Terms3<B, PASS>::NT; j++) tt[j] = 0.0;
+          }
+          p2g_fold_pair<PASS>(tt, k + 1 < len, am, ap, af);
+        }
+      }
+      return;
+    } else {
+      if constexpr (PASS == 0) {
+        for (int r = 0; r < nr; r++) {
+          const unsigned e = rl.runs[size_t(r) * NN + n];
+          const int p0 = int(e >> 3), len = int(e & 7u) + 1;
+          double2 a = ldu(P.pr[0], unsigned(p0)), b = ldu(P.pr[1], unsigned(p0));  // (x, y) (z, m)
+          for (int k = 0; k < len; k++) {
+            double v[NF + 1];
+            v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+            P2GFields<D, PASS>::load(P, unsigned(p0 + k), v, 2);
+            if (k + 1 < len) {
+              a = ldu(P.pr[0], unsigned(p0 + k + 1));
+              b = ldu(P.pr[1], unsigned(p0 + k + 1));
+            }
+            p2g_add_v<D, B, PASS>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+          }
+        }
+        return;
+      }
+      unsigned e = nr > 0 ? rl.runs[n] : 0u;
+      for (int r = 0; r < nr; r++) {
+        const int p0 = int(e >> 3), len = int(e & 7u) + 1;
+        if (r + 1 < nr) e = rl.runs[size_t(r + 1) * NN + n];
+        double v[NF + 1], w[NF + 1];
+        P2GFields<D, PASS>::load(P, unsigned(p0), v);
+        for (int k = 0; k < len; k++) {
+          if (k + 1 < len) P2GFields<D, PASS>::load(P, unsigned(p0 + k + 1), w);
+          p2g_add_v<D, B, PASS>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+          if (k + 1 < len)
+#pragma unroll
+            for (int q = 0; q < NF + 1; q++) v[q] = w[q];
+        }
       }
     }
   };
@@ -702,23 +877,28 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
     replay(rl.nruns[n]);  // no particle entered or left the window since they were recorded
   } else if (RUNS && PASS == 0) {
     // merge, recording the runs (none for a node whose window is empty)
+    // (the two lanes of a node record the same entries; lane 0 stores them)
     int nr = 0;
     for_window_particles<D, B, ORDER, DEFER>(P, G, start, sorted, bxyz, off, inst, W, ni, nj, nk, s_cur, s_head, visit,
                                              [&](int p0, int len) {
                                                while (len > 0) {
                                                  const int c = min(len, 8);
-                                                 if (nr < kRunCap)
+                                                 if (nr < kRunCap && h == 0)
                                                    rl.runs[size_t(nr) * NN + n] = (unsigned(p0) << 3) | unsigned(c - 1);
                                                  nr++;
                                                  p0 += c;
                                                  len -= c;
                                                }
-                                             });
-    rl.nruns[n] = nr;
+                                             },
+                                             slot);
+    flush();
+    if (h == 0) rl.nruns[n] = nr;
   } else {
     for_window_particles<D, B, ORDER, DEFER>(P, G, start, sorted, bxyz, off, inst, W, ni, nj, nk, s_cur, s_head,
-                                             visit);
+                                             visit, NoRuns{}, slot);
+    flush();
   }
+  if (LPN == 2 && h != 0) return;  // both lanes hold the node's sums
   if (PASS == 0) {
     const uint8_t fx = g.fixed[n];
     g.m[n] = am;
@@ -749,7 +929,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
 
 // BLK: threads (nodes) per workgroup; the batched step lets the autotuner
 // choose it (the single-problem steps use kP2GBlock).
-template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI, int BLK = kP2GBlock>
+template <int D, int B, int PASS, int ORDER, bool COMPACT, bool MULTI, int BLK = kP2GBlock, int LPN = 1>
 __global__ void __launch_bounds__(BLK) k_p2g(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
                                                    const double* __restrict__ corners, const int* __restrict__ start0,
                                                    const unsigned long long* __restrict__ sorted,
@@ -761,8 +941,8 @@ __global__ void __launch_bounds__(BLK) k_p2g(Particles P, GridDesc G0, const Sol
   constexpr int NB = (D == 3) ? MW * MW * MW : MW * MW;
   // Reference order: per window bin, the position in `sorted` of its next
   // unconsumed particle and that particle (kNoParticle when exhausted).
-  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
-  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK];
+  __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK / LPN];
+  __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][BLK / LPN];
   const int NN = MULTI ? Bt.NN : G0.nn;
   // XCD: the 3D re-projection pass deals its (compacted) blocks to the XCDs
   // in runs of kXcdChunk (1.115 -> 1.066 ms on the 4.2M-particle impact;
@@ -775,12 +955,12 @@ __global__ void __launch_bounds__(BLK) k_p2g(Particles P, GridDesc G0, const Sol
   // sit at the top of the plate, the highest node ids; started last they
   // set the kernel's tail.  Both grids have exactly nlog (logical) blocks.
   constexpr bool REV = D == 3 && !MULTI;
-  const int nlog = (NN + BLK - 1) / BLK;
+  const int nlog = (NN + BLK / LPN - 1) / (BLK / LPN);
   const int L0 = XCD ? xcd_block(nlog) : int(blockIdx.x);
   if (L0 < 0) return;
   const int L = REV ? nlog - 1 - L0 : L0;
-  p2g_block<D, B, PASS, ORDER, COMPACT, MULTI, BLK>(P, G0, S, corners, start0, sorted, bxyz, order, nodepos, g0,
-                                                     Wrt, dt0, Bt, alpha, rl, L, s_cur, s_head);
+  p2g_block<D, B, PASS, ORDER, COMPACT, MULTI, BLK, LPN>(P, G0, S, corners, start0, sorted, bxyz, order, nodepos,
+                                                          g0, Wrt, dt0, Bt, alpha, rl, L, s_cur, s_head);
 }
 
 // The batched force pass as a persistent kernel over a work queue: a grid

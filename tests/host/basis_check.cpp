@@ -45,6 +45,12 @@ int main() {
     }
     for (int i = -300000; i <= 300000; i++)
       if (!check(i * 1e-5)) return 1;
+    // random doubles in [-2, 2] (a fixed linear congruential sequence)
+    uint64_t z = 88172645463325252ull;
+    for (int i = 0; i < 1000000; i++) {
+      z = z * 6364136223846793005ull + 1442695040888963407ull;
+      if (!check(-2.0 + 4.0 * double(z >> 11) * (1.0 / 9007199254740992.0))) return 1;
+    }
     if (!check(NAN) || !check(INFINITY) || !check(-INFINITY) || !check(-0.0)) return 1;
   }
   printf("%ld\n", checked);
