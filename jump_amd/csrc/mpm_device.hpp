@@ -258,7 +258,8 @@ __device__ __forceinline__ bool first_node(double x, int k, const Axis& A, doubl
 // -N_other*sign*hI); for the B-splines it is dN/dr; for GIMP dN/dx.
 //   r on the x axis uses the node position (with xmin); on the y/z axes the
 //   B-spline bases use (id-1)*h without the origin (Basis.jl:442,555,567).
-template <int B, bool QUIRK, bool SPLIT = false>
+// INTERIOR: the caller knows the node is interior on this axis (type 3).
+template <int B, bool QUIRK, bool SPLIT = false, bool INTERIOR = false>
 __device__ __forceinline__ void axis_weight(double x, int k, const Axis& A, double lp, int id, int sel4, double& w,
                                             double& dw) {
   if (B == JUMP_BASIS_LINEAR) {
@@ -267,7 +268,8 @@ __device__ __forceinline__ void axis_weight(double x, int k, const Axis& A, doub
     dw = jsign(d);
   } else if (B == JUMP_BASIS_QUAD_BSPLINE) {
     double r = (k == 0) ? (x - (A.mn + double(id - 1) * A.h)) * A.hI : (x - double(id - 1) * A.h) * A.hI;
-    if (SPLIT) qbs_eval_split(qbs_type(id, A.n, QUIRK), r, w, dw);
+    if (INTERIOR) qbs_eval3(r, w, dw);
+    else if (SPLIT) qbs_eval_split(qbs_type(id, A.n, QUIRK), r, w, dw);
     else qbs_eval(qbs_type(id, A.n, QUIRK), r, w, dw);
   } else if (B == JUMP_BASIS_CUBIC_BSPLINE) {
     double r = (k == 0) ? (x - (A.mn + double(id - 1) * A.h)) * A.hI : (x - double(id - 1) * A.h) * A.hI;

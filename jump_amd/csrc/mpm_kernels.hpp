@@ -286,7 +286,8 @@ __global__ void __launch_bounds__(kBlock) k_binsort(const int* __restrict__ star
 // Weight N and gradient of node (ii,jj,kk) (1-based) for a particle at xp.
 // QUIRK selects the getShapeFunctions variant (used by the MUSL
 // re-projection): QBS boundary typing, cubic z-axis type-4 test on jd.
-template <int D, int B, bool QUIRK, bool GRAD>
+// INTERIOR: quadratic B-splines at a node interior on every axis (type 3).
+template <int D, int B, bool QUIRK, bool GRAD, bool INTERIOR = false>
 __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& G, const double* lp,
                                             const double* corners, int p, const double* xp, int ii, int jj, int kk,
                                             double& N, double* dN, bool& member) {
@@ -300,8 +301,8 @@ __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& 
   }
   Axis X = axis_of(G, 0), Y = axis_of(G, 1);
   double wx, dwx, wy, dwy;
-  axis_weight<B, QUIRK, true>(xp[0], 0, X, lp[0], ii, ii, wx, dwx);
-  axis_weight<B, QUIRK, true>(xp[1], 1, Y, lp[1], jj, jj, wy, dwy);
+  axis_weight<B, QUIRK, true, INTERIOR>(xp[0], 0, X, lp[0], ii, ii, wx, dwx);
+  axis_weight<B, QUIRK, true, INTERIOR>(xp[1], 1, Y, lp[1], jj, jj, wy, dwy);
   if (D == 2) {
     if (GRAD) combine2<B>(X, Y, wx, dwx, wy, dwy, N, dN);
     else N = wx * wy;
@@ -309,7 +310,7 @@ __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& 
     Axis Z = axis_of(G, 2);
     double wz, dwz;
     int sel4 = (QUIRK && B == JUMP_BASIS_CUBIC_BSPLINE) ? jj : kk;
-    axis_weight<B, QUIRK, true>(xp[2], 2, Z, lp[2], kk, sel4, wz, dwz);
+    axis_weight<B, QUIRK, true, INTERIOR>(xp[2], 2, Z, lp[2], kk, sel4, wz, dwz);
     if (GRAD) combine3<B>(X, Y, Z, wx, dwx, wy, dwy, wz, dwz, N, dN);
     else N = wx * wy * wz;
   }
@@ -348,7 +349,7 @@ template <int D, int PASS> struct P2GFields {
 // v: the particle's fields as P2GFields loads them.
 // AFF: the node's grid may be an APIC step's (single 2D problems; batches
 // refuse APIC), read from G.apic.
-template <int D, int B, int PASS, bool AFF = false>
+template <int D, int B, int PASS, bool AFF = false, bool INTERIOR = false>
 __device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G, const SolidDev* S,
                                           const double* corners, int p, const double* v, int ni, int nj, int nk,
                                           double& am, double* ap, double* af) {
@@ -359,7 +360,7 @@ __device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G,
   }
   double N, dN[3];
   bool member;
-  node_weight<D, B, PASS == 1, PASS == 0>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
+  node_weight<D, B, PASS == 1, PASS == 0, INTERIOR>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
   if (!member) return;
   const double fM = v[D];
   if (PASS == 0) {
@@ -607,12 +608,14 @@ __device__ __forceinline__ void for_window_particles(const Particles& P, const G
   }
 }
 
+template <bool V> struct BoolTag { static constexpr bool value = V; };
+
 // The terms one particle adds to a 3D node's accumulators, each formed by
 // the expression p2g_add_v adds, so that folding the terms in particle order
 // gives the same bits: PASS 0 t = (N m, N m v, force term (subtracted)),
 // PASS 1 t = N m v.
 template <int B, int PASS> struct Terms3 { static constexpr int NT = PASS == 0 ? 7 : 3; };
-template <int B, int PASS>
+template <int B, int PASS, bool INTERIOR = false>
 __device__ __forceinline__ void p2g_terms3(const Particles& P, const GridDesc& G, const SolidDev* S,
                                            const double* corners, int p, const double* v, int ni, int nj, int nk,
                                            double* t) {
@@ -623,7 +626,7 @@ __device__ __forceinline__ void p2g_terms3(const Particles& P, const GridDesc& G
   }
   double N, dN[3];
   bool member;
-  node_weight<3, B, PASS == 1, PASS == 0>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
+  node_weight<3, B, PASS == 1, PASS == 0, INTERIOR>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
   const double fM = v[3];
   if (PASS == 0) {
     const double fV = v[4];
@@ -776,7 +779,8 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
   // only the next particle's (x, y) (z, m) ahead, which its weights wait
   // on (1.463 -> 1.378 ms; all 14 fields ahead: 1.46 -> 1.53 ms, 117 -> 140
   // VGPRs).
-  auto replay = [&](int nr) {
+  auto replay_as = [&](auto inner, int nr) {
+    constexpr bool INT = decltype(inner)::value;
     if constexpr (LPN == 2 && PASS == 1) {
       // this lane's particles of each run, the next one's fields (and the
       // next run's entry) loaded while the current one is summed
@@ -791,7 +795,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
           double tt[Terms3<B, PASS>::NT];
           if (mine < len) {
             if (mine + 2 < len) P2GFields<D, PASS>::load(P, unsigned(p0 + mine + 2), w);
-            p2g_terms3<B, PASS>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt);
+            p2g_terms3<B, PASS, INT>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt);
             if (mine + 2 < len)
 #pragma unroll
               for (int q = 0; q < NF + 1; q++) v[q] = w[q];
@@ -827,7 +831,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
             } else {
               P2GFields<D, PASS>::load(P, unsigned(p0 + mine), v);
             }
-            p2g_terms3<B, PASS>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt);
+            p2g_terms3<B, PASS, INT>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt);
           } else {
 #pragma unroll
             for (int j = 0; j < Terms3<B, PASS>::NT; j++) tt[j] = 0.0;
@@ -850,7 +854,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
               a = ldu(P.pr[0], unsigned(p0 + k + 1));
               b = ldu(P.pr[1], unsigned(p0 + k + 1));
             }
-            p2g_add_v<D, B, PASS>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+            p2g_add_v<D, B, PASS, false, INT>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
           }
         }
         return;
@@ -863,13 +867,26 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
         P2GFields<D, PASS>::load(P, unsigned(p0), v);
         for (int k = 0; k < len; k++) {
           if (k + 1 < len) P2GFields<D, PASS>::load(P, unsigned(p0 + k + 1), w);
-          p2g_add_v<D, B, PASS>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+          p2g_add_v<D, B, PASS, false, INT>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
           if (k + 1 < len)
 #pragma unroll
             for (int q = 0; q < NF + 1; q++) v[q] = w[q];
         }
       }
     }
+  };
+  // Nodes interior on every axis (quadratic B-splines, 3D: all but the two
+  // outermost node layers of each face) replay with the interior weights
+  // alone, with no test of the node type in the loop.
+  auto replay = [&](int nr) {
+    if constexpr (D == 3 && B == JUMP_BASIS_QUAD_BSPLINE) {
+      if (qbs_type(ni + 1, G.nx, PASS == 1) == 3 && qbs_type(nj + 1, G.ny, PASS == 1) == 3 &&
+          qbs_type(nk + 1, G.nz, PASS == 1) == 3) {
+        replay_as(BoolTag<true>{}, nr);
+        return;
+      }
+    }
+    replay_as(BoolTag<false>{}, nr);
   };
   if (RUNS && PASS == 1 && rl.nruns[n] <= kRunCap) {
     replay(rl.nruns[n]);  // the runs this step's force pass left
