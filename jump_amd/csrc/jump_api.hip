@@ -387,6 +387,10 @@ static int alloc_grid(jump_ctx* ctx) {
   if (!r && D == 3) {
     for (double2** v : {&g.mq, &g.qq, &g.pxy, &g.p2xy, &g.pz2})
       if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, nn);
+  }
+  // run lists of the reference-order gathers: 3D, and single 2D problems of
+  // up to kRunNodes2D nodes (kRunCap entries of 4 bytes per node)
+  if (!r && (D == 3 || (ctx->ninst == 1 && nn <= size_t(kRunNodes2D)))) {
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->runs.runs, size_t(kRunCap) * nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->runs.nruns, nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &ctx->d_dirty, nn);
@@ -503,18 +507,18 @@ static void scan_nodes(jump_ctx* ctx, const int* in, int* out) {
   LAUNCH(k_scan_add, n, out, ctx->d_sums, n);
 }
 
-// dirty, gen: 3D run replay (k_bin stamps nodes whose window changed)
+// dirty, gen: run replay (k_bin stamps nodes whose window changed)
 template <int D, int B, bool MULTI> static int bin_and_sort(jump_ctx* ctx, int* dirty = nullptr, int gen = 0) {
   const int nn = ctx->NN;
   const int np = int(ctx->np);
+  const int W = (B == JUMP_BASIS_CPDI_Q4) ? ctx->cpdi_w : Width<B>::W;
   LAUNCH(k_fill_i32, nn + 1, ctx->d_cnt, 0, nn + 1);
   LAUNCH((k_bin<D, B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_key, ctx->d_rank,
-         ctx->d_bxyz, ctx->d_cnt, ctx->d_flags, ctx->bt(), dirty, gen, ctx->prm.contact);
+         ctx->d_bxyz, ctx->d_cnt, ctx->d_flags, ctx->bt(), dirty, gen, ctx->prm.contact, W);
   scan_nodes(ctx, ctx->d_cnt, ctx->d_start);
   LAUNCH(k_scatter, np, ctx->d_key, ctx->d_start, ctx->d_rank, ctx->d_sorted, np, nn);
   LAUNCH(k_binsort, nn, ctx->d_start, ctx->d_sorted, nn);
   // P2G work order: active nodes first (d_cnt is free again as the flags)
-  const int W = (B == JUMP_BASIS_CPDI_Q4) ? ctx->cpdi_w : Width<B>::W;
   LAUNCH((k_node_flags<D, MULTI>), nn + 1, ctx->G, ctx->d_start, W, ctx->d_cnt, ctx->bt());
   scan_nodes(ctx, ctx->d_cnt, ctx->d_nodepos);
   LAUNCH(k_node_order, nn, ctx->d_nodepos, nn, ctx->d_order);
@@ -569,14 +573,32 @@ template <int B, int ORDER> static void p2g_batched(jump_ctx* ctx, int shape) {
   }
 }
 
+// The run lists for this step's reference-order gather: this step's stamp
+// (after an upload every node merges and records).  None (runs null) in the
+// binned order or where the context has no lists.
+template <int ORDER> static RunList begin_runs(jump_ctx* ctx) {
+  RunList rl = ctx->runs;
+  if (ORDER != JUMP_ORDER_REFERENCE || !rl.runs) return RunList{};
+  if (ctx->runs_gen == INT_MAX) {  // stamps wrapped: start over
+    ctx->runs_gen = 0;
+    ctx->runs_valid = false;
+    LAUNCH(k_fill_i32, ctx->NN, ctx->d_dirty, 0, ctx->NN);
+  }
+  rl.dirty = ctx->d_dirty;
+  rl.gen = ++ctx->runs_gen;
+  rl.all = ctx->runs_valid ? 0 : 1;
+  return rl;
+}
+
 // MULTI (a batch of 2D instances): the gather runs over all instances' nodes
 // in the launch shape the autotuner chose (by default the compacted active
 // nodes, since the instances' grids are mostly larger than their solids).
 template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
+  const RunList rl = MULTI ? RunList{} : begin_runs<ORDER>(ctx);
   prof_begin(ctx);
-  bin_and_sort<2, B, MULTI>(ctx);
+  bin_and_sort<2, B, MULTI>(ctx, rl.runs ? ctx->d_dirty : nullptr, rl.gen);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   if constexpr (MULTI && B != JUMP_BASIS_CPDI_Q4) {  // (batches refuse CPDI)
@@ -584,8 +606,9 @@ template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   } else {
     LAUNCH_P2G((k_p2g<2, B, 0, ORDER, MULTI, MULTI>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
                ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt(), 0.0,
-               RunList{});
+               rl);
   }
+  if (rl.runs) ctx->runs_valid = true;
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   for (int r = 0; r <= ctx->max_rigid_rank; r++)
@@ -605,16 +628,18 @@ template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
 template <int B, int ORDER> static void step_usl2d_contact(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
+  const RunList rl = begin_runs<ORDER>(ctx);
   prof_begin(ctx);
-  bin_and_sort<2, B, false>(ctx);
+  bin_and_sort<2, B, false>(ctx, rl.runs ? ctx->d_dirty : nullptr, rl.gen);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   if (ctx->G.bodyfield)
     LAUNCH_P2G((k_p2g_contact<B, ORDER, true>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags);
+               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags, rl);
   else
     LAUNCH_P2G((k_p2g_contact<B, ORDER, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags);
+               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags, rl);
+  if (rl.runs) ctx->runs_valid = true;
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p_usl2d_contact<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->cg, ctx->prm.tolerance, dt,
@@ -652,21 +677,10 @@ template <int B, int ORDER> static void step_musl3d_contact(jump_ctx* ctx) {
 template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
-  // run replay (reference order): this step's stamp; after an upload every
-  // node merges and records
-  RunList rl = ctx->runs;
-  if (ORDER == JUMP_ORDER_REFERENCE) {
-    if (ctx->runs_gen == INT_MAX) {  // stamps wrapped: start over
-      ctx->runs_gen = 0;
-      ctx->runs_valid = false;
-      LAUNCH(k_fill_i32, ctx->NN, ctx->d_dirty, 0, ctx->NN);
-    }
-    rl.dirty = ctx->d_dirty;
-    rl.gen = ++ctx->runs_gen;
-    rl.all = ctx->runs_valid ? 0 : 1;
-  }
+  // run replay (reference order)
+  const RunList rl = begin_runs<ORDER>(ctx);
   prof_begin(ctx);
-  bin_and_sort<3, B, false>(ctx, ORDER == JUMP_ORDER_REFERENCE ? ctx->d_dirty : nullptr, rl.gen);
+  bin_and_sort<3, B, false>(ctx, rl.runs ? ctx->d_dirty : nullptr, rl.gen);
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   constexpr int LN0 = ORDER == JUMP_ORDER_REFERENCE ? kP2GLanes0 : 1;  // lanes per node
@@ -675,7 +689,7 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
                      dim3((ctx->NN + kP2GBlock / LN0 - 1) / (kP2GBlock / LN0)), dim3(kP2GBlock), 0, ctx->stream,
                      ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted, ctx->d_bxyz,
                      ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha, rl);
-  if (ORDER == JUMP_ORDER_REFERENCE) ctx->runs_valid = true;
+  if (rl.runs) ctx->runs_valid = true;
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
   LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);

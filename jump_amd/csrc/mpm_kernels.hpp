@@ -76,21 +76,21 @@ __global__ void __launch_bounds__(kBlock) k_bin(Particles P, GridDesc G0, const 
                                                 const double* __restrict__ corners, int* __restrict__ key,
                                                 int* __restrict__ rank, unsigned long long* __restrict__ bxyz,
                                                 int* __restrict__ cnt, unsigned* __restrict__ flags, BatchDev Bt,
-                                                int* __restrict__ dirty, int gen, int rigid_hat) {
+                                                int* __restrict__ dirty, int gen, int rigid_hat, int W) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  // 3D: a particle whose bin changed since the last binning changes the
-  // windows of the nodes that contain its old or new bin; stamp them
-  const int old = (D == 3 && !MULTI && dirty) ? key[i] : -1;
+  // Run replay (single problems): a particle whose bin changed since the
+  // last binning changes the windows of the nodes that contain its old or
+  // new bin (W per axis); stamp them
+  const int old = (!MULTI && dirty) ? key[i] : -1;
   auto mark = [&](int b) {
-    if (D != 3 || MULTI || !dirty || b < 0) return;
-    constexpr int W = Width<B>::W;
-    const int bx = b % G0.nx, by = (b / G0.nx) % G0.ny, bz = b / G0.nxy;
-    for (int c = 0; c < W; c++)
+    if (MULTI || !dirty || b < 0) return;
+    const int bx = b % G0.nx, by = (b / G0.nx) % G0.ny, bz = D == 3 ? b / G0.nxy : 0;
+    for (int c = 0; c < (D == 3 ? W : 1); c++)
       for (int y = 0; y < W; y++)
         for (int a = 0; a < W; a++)
-          if (bx + a < G0.nx && by + y < G0.ny && bz + c < G0.nz)
-            dirty[(bx + a) + G0.nx * (by + y) + G0.nxy * (bz + c)] = gen;
+          if (bx + a < G0.nx && by + y < G0.ny && (D == 2 || bz + c < G0.nz))
+            dirty[(bx + a) + G0.nx * (by + y) + (D == 3 ? G0.nxy * (bz + c) : 0)] = gen;
   };
   const SolidDev& s = S[P.sid[i]];
   const GridDesc& G = MULTI ? Bt.inst[s.inst].G : G0;
@@ -507,6 +507,8 @@ struct NoRuns {
 // the crater reached 57 by step 400.  At 48 such nodes merged again in the
 // re-projection and lengthened its tail (0.70 -> 0.93 ms after step 160).
 constexpr int kRunCap = 64;
+// 2D problems keep run lists up to this many nodes (8.6 GB of lists).
+constexpr long long kRunNodes2D = 1ll << 25;
 struct RunList {
   unsigned* runs;    // kRunCap planes of NN entries, by node
   int* nruns;        // per node: entries recorded (> kRunCap: overflow)
@@ -707,13 +709,16 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
   const int i = L * (BLK / LPN) + slot;
   if (i >= NN) return;
   const int ng = COMPACT ? ldu(order, unsigned(i)) : i;  // node in the (concatenated) node arrays
-  constexpr bool RUNS = D == 3 && !MULTI && ORDER == JUMP_ORDER_REFERENCE;
+  // run replay: single problems in reference order, where the lists exist
+  constexpr bool RUNS = !MULTI && ORDER == JUMP_ORDER_REFERENCE;
+  const bool runs_on = RUNS && rl.runs != nullptr;
   // A node whose window held no particle when its runs were recorded, and
   // which no particle has entered since, still holds the zeros that both
   // passes wrote in the step that emptied it (or in the first step after an
   // upload, when every node merges and writes): skip the writes.  3.6M of
   // the impact's 4.2M nodes, 0.46 GB of writes per force pass.
-  const bool still_empty = RUNS && !rl.all && rl.nruns[ng] == 0 && rl.dirty[ng] != rl.gen;
+  // (3D only: in 2D the rigid override writes grid nodes after the gather)
+  const bool still_empty = D == 3 && runs_on && !rl.all && rl.nruns[ng] == 0 && rl.dirty[ng] != rl.gen;
   if (COMPACT && i >= ldu(nodepos, unsigned(NN))) {  // inactive node: no particle reaches it
     if (PASS == 1 && still_empty) return;
     if (PASS == 0) {
@@ -854,7 +859,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
               a = ldu(P.pr[0], unsigned(p0 + k + 1));
               b = ldu(P.pr[1], unsigned(p0 + k + 1));
             }
-            p2g_add_v<D, B, PASS, false, INT>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+            p2g_add_v<D, B, PASS, AFF, INT>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
           }
         }
         return;
@@ -867,7 +872,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
         P2GFields<D, PASS>::load(P, unsigned(p0), v);
         for (int k = 0; k < len; k++) {
           if (k + 1 < len) P2GFields<D, PASS>::load(P, unsigned(p0 + k + 1), w);
-          p2g_add_v<D, B, PASS, false, INT>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+          p2g_add_v<D, B, PASS, AFF, INT>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
           if (k + 1 < len)
 #pragma unroll
             for (int q = 0; q < NF + 1; q++) v[q] = w[q];
@@ -888,11 +893,11 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
     }
     replay_as(BoolTag<false>{}, nr);
   };
-  if (RUNS && PASS == 1 && rl.nruns[n] <= kRunCap) {
+  if (runs_on && PASS == 1 && rl.nruns[n] <= kRunCap) {
     replay(rl.nruns[n]);  // the runs this step's force pass left
-  } else if (RUNS && PASS == 0 && !rl.all && rl.dirty[n] != rl.gen && rl.nruns[n] <= kRunCap) {
+  } else if (runs_on && PASS == 0 && !rl.all && rl.dirty[n] != rl.gen && rl.nruns[n] <= kRunCap) {
     replay(rl.nruns[n]);  // no particle entered or left the window since they were recorded
-  } else if (RUNS && PASS == 0) {
+  } else if (runs_on && PASS == 0) {
     // merge, recording the runs (none for a node whose window is empty)
     // (the two lanes of a node record the same entries; lane 0 stores them)
     int nr = 0;
@@ -1066,7 +1071,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
                                                            const unsigned long long* __restrict__ sorted,
                                                            const unsigned long long* __restrict__ bxyz, GridArrays g,
                                                            ContactArrays c, int Wrt, double dt, double fric,
-                                                           unsigned* __restrict__ flags) {
+                                                           unsigned* __restrict__ flags, RunList rl) {
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = MW * MW;
   __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
@@ -1091,7 +1096,7 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
     s.nr[0] += fM * dN[0];
     s.nr[1] += fM * dN[1];
   };
-  for_window_particles<2, B, ORDER, true>(P, G, start, sorted, bxyz, 0, 0, W, ni, nj, 0, s_cur, s_head, [&](int p) {
+  auto visit = [&](int p) {
     double v[PairLayout<2>::NF + 1];
     P2GFields<2, 0>::load(P, unsigned(p), v);
     double lp[3] = {0.0, 0.0, 0.0};
@@ -1121,7 +1126,36 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
     if (s0.body < 0 || s0.body == sid) add(s0, sid, N, dN, v, bp);
     else if (s1.body < 0 || s1.body == sid) add(s1, sid, N, dN, v, bp);
     else overflow = true;
-  });
+  };
+  // Run replay as in p2g_block: a node whose window no particle entered or
+  // left since the last step visits the runs of particle indices it
+  // recorded then, in the same order (CPDI bins follow the static corners,
+  // so only the platens' particles ever change them).
+  const bool runs_on = ORDER == JUMP_ORDER_REFERENCE && rl.runs != nullptr;
+  if (runs_on && !rl.all && rl.dirty[n] != rl.gen && rl.nruns[n] <= kRunCap) {
+    const int nr = rl.nruns[n];
+    for (int r = 0; r < nr; r++) {
+      const unsigned e = rl.runs[size_t(r) * G.nn + n];
+      const int p0 = int(e >> 3), len = int(e & 7u) + 1;
+      for (int k = 0; k < len; k++) visit(p0 + k);
+    }
+  } else if (runs_on) {
+    int nr = 0;
+    for_window_particles<2, B, ORDER, true>(P, G, start, sorted, bxyz, 0, 0, W, ni, nj, 0, s_cur, s_head, visit,
+                                            [&](int p0, int len) {
+                                              while (len > 0) {
+                                                const int cc = min(len, 8);
+                                                if (nr < kRunCap)
+                                                  rl.runs[size_t(nr) * G.nn + n] = (unsigned(p0) << 3) | unsigned(cc - 1);
+                                                nr++;
+                                                p0 += cc;
+                                                len -= cc;
+                                              }
+                                            });
+    rl.nruns[n] = nr;
+  } else {
+    for_window_particles<2, B, ORDER, true>(P, G, start, sorted, bxyz, 0, 0, W, ni, nj, 0, s_cur, s_head, visit);
+  }
   if (overflow) atomicOr(flags, JUMP_FLAG_CONTACT_OVERFLOW);
   const uint8_t fx = g.fixed[n];
   double pk[2][2], v[2][2];
