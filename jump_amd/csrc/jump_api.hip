@@ -418,11 +418,13 @@ static int alloc_grid(jump_ctx* ctx) {
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &m.rhs, 4 * nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &m.mv, 4 * nn);
     CgArrays& c = ctx->cgm;
-    for (double** v : {&c.x, &c.r, &c.p, &c.s, &c.w})
+    for (double** v : {&c.x, &c.p})
       if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, 4 * nn);
+    for (double** v : {&c.r, &c.s, &c.w})  // two slots each
+      if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, 8 * nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.dinv, nn);
-    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.part, 12 * size_t(cg_tiles(ctx->G)));
-    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.st, 1);
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.part, 2 * 12 * size_t(cg_tiles(ctx->G)));
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.st, 2);
     if (!r && hipHostMalloc(reinterpret_cast<void**>(&ctx->h_cg), 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
       r = set_err(ctx, JUMP_ERR_NOMEM, "hipHostMalloc failed");
   } else if (ctx->prm.contact) {
@@ -715,9 +717,11 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
 // matrix stencil radius (width - 1).
 constexpr double kMuslTol = 1e-14;  // CG: relative residual per right-hand side
 constexpr int kMuslMaxIt = 2000;
-// The blended mass solve: CG iterations of three launches each; the host
-// reads the convergence state back after as many iterations as the last
-// solve needed (then every 4), so a step usually waits for it once.
+// The blended mass solve: CG iterations of one launch each (k_cg_iter) up
+// to kCgFusedTiles node tiles, else of three (k_cg_spmv, k_cg_scalars,
+// k_cg_update); the same iterates either way.  The host reads the
+// convergence state back after as many iterations as the last solve needed
+// (then every 4), so a step usually waits for it once.
 template <int R> static void solve_musl2d(jump_ctx* ctx) {
   const int nn = ctx->NN;
   const int nblk = (nn + kCgBlock - 1) / kCgBlock;
@@ -725,23 +729,43 @@ template <int R> static void solve_musl2d(jump_ctx* ctx) {
   CgArrays& c = ctx->cgm;
   LAUNCH(k_cg_init, nn, nn, ctx->mu, ctx->g.m, c);
   int check = std::max(ctx->cg_check, 4);
-  const char* st_done = reinterpret_cast<const char*>(c.st) + offsetof(CgState, all_done);
   static_assert(offsetof(CgState, iters) == offsetof(CgState, all_done) + sizeof(int), "read-back layout");
-  for (int it = 0; it < kMuslMaxIt; it++) {
-    hipLaunchKernelGGL((k_cg_spmv<R>), dim3(ntile), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, c);
-    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(kCgBlock), 0, ctx->stream, c, ntile, it, kMuslTol);
-    hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kCgBlock), 0, ctx->stream, nn, c);
-    if (it + 1 != check) continue;
-    if (hipMemcpyAsync(ctx->h_cg, st_done, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
-      break;  // the error stays with the stream and is reported by the step
-    if (ctx->h_cg[0]) {
-      ctx->cg_check = ctx->h_cg[1] + 1;  // the iteration that found every column converged
-      break;
+  auto read_back = [&](const CgState* st) {
+    const char* p = reinterpret_cast<const char*>(st) + offsetof(CgState, all_done);
+    return hipMemcpyAsync(ctx->h_cg, p, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+           hipStreamSynchronize(ctx->stream) == hipSuccess;
+  };
+  const CgState* last = c.st;
+  if (ntile <= kCgFusedTiles) {
+    // launch m forms iteration m-1's scalars: the state after iteration it
+    // is the one launch it+1 leaves, in slot (it+1) & 1
+    for (int m = 0; m <= kMuslMaxIt; m++) {
+      hipLaunchKernelGGL((k_cg_iter<R>), dim3(ntile), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, c, m, ntile,
+                         kMuslTol);
+      last = c.st + (m & 1);
+      if (m < check) continue;
+      if (!read_back(last)) break;  // the error stays with the stream and is reported by the step
+      if (ctx->h_cg[0]) {
+        ctx->cg_check = ctx->h_cg[1] + 1;  // the launch that found every column converged
+        break;
+      }
+      check = m + 4;
     }
-    check += 4;
+  } else {
+    for (int it = 0; it < kMuslMaxIt; it++) {
+      hipLaunchKernelGGL((k_cg_spmv<R>), dim3(ntile), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, c);
+      hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(kCgBlock), 0, ctx->stream, c, ntile, it, kMuslTol);
+      hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kCgBlock), 0, ctx->stream, nn, c);
+      if (it + 1 != check) continue;
+      if (!read_back(c.st)) break;
+      if (ctx->h_cg[0]) {
+        ctx->cg_check = ctx->h_cg[1] + 1;  // the iteration that found every column converged
+        break;
+      }
+      check += 4;
+    }
   }
-  LAUNCH(k_cg_finish, nn, nn, ctx->mu, c, ctx->d_flags);
+  LAUNCH(k_cg_finish, nn, nn, ctx->mu, c, last, ctx->d_flags);
 }
 template <int B, int ORDER, int R> static void step_musl2d(jump_ctx* ctx) {
   const int np = int(ctx->np);

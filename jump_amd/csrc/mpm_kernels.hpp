@@ -1890,15 +1890,17 @@ struct CgState {
   int all_done;  // every column converged (or had a zero right-hand side)
   int iters;     // iterations that updated at least one column
 };
+// r, s, w, part and st hold two slots each (the fused iteration, k_cg_iter,
+// reads one and writes the other); the three-launch iteration uses slot 0.
 struct CgArrays {
   double* dinv;  // the preconditioner: 1/m, 1 at massless nodes (nn)
   double* x;     // four columns of nn each, likewise below
-  double* r;
+  double* r;     // slot k at r + 4 nn k
   double* p;
-  double* s;
-  double* w;
-  double* part;  // [block][12]: (r,z) x4, (w,z) x4, (r,r) x4
-  CgState* st;
+  double* s;     // slot k at s + 4 nn k
+  double* w;     // slot k at w + 4 nn k
+  double* part;  // [slot][block][12]: (r,z) x4, (w,z) x4, (r,r) x4
+  CgState* st;   // [slot]
 };
 constexpr int kCgBlock = 256;
 // The SpMV works on 32 x 8 node tiles, staging z = D^-1 r over the tile and
@@ -1916,16 +1918,23 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nn, Musl2Arrays mu, cons
     CgState z{};
     *cg.st = z;
   }
+  if (i == 1) {
+    CgState z{};
+    cg.st[1] = z;
+  }
   if (i >= nn) return;
   const double mi = gm[i];
   cg.dinv[i] = mi > 0.0 ? 1.0 / mi : 1.0;
+  const size_t n4 = 4 * size_t(nn);
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const size_t k = size_t(c) * nn + i;
     cg.x[k] = 0.0;
     cg.r[k] = mu.rhs[k];
+    cg.r[n4 + k] = mu.rhs[k];
     cg.p[k] = 0.0;
     cg.s[k] = 0.0;
+    cg.s[n4 + k] = 0.0;
   }
 }
 
@@ -1997,19 +2006,19 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_spmv(GridDesc G, Musl2Arrays mu
   }
 }
 
-__global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, int it, double tol) {
+// The twelve totals of the blocks' partial sums, in a fixed order (every
+// workgroup that calls it gets the same bits): per thread a strided sum, per
+// wave a shuffle tree, then the waves in order.  Ends with a barrier.
+__device__ __forceinline__ void cg_block_totals(const double* __restrict__ part, int nblk,
+                                                double (*red)[kCgBlock / 64], double* tot) {
   constexpr int NW = kCgBlock / 64;
-  __shared__ double red[12][NW];
-  __shared__ double tot[12];
-  CgState& st = *cg.st;
-  if (st.all_done) return;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double acc[12];
 #pragma unroll
   for (int c = 0; c < 12; c++) acc[c] = 0.0;
   for (int b = t; b < nblk; b += kCgBlock)
 #pragma unroll
-    for (int c = 0; c < 12; c++) acc[c] += cg.part[size_t(b) * 12 + c];
+    for (int c = 0; c < 12; c++) acc[c] += part[size_t(b) * 12 + c];
 #pragma unroll
   for (int c = 0; c < 12; c++) {
     double v = acc[c];
@@ -2025,7 +2034,10 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, 
     tot[t] = v;
   }
   __syncthreads();
-  if (t != 0) return;
+}
+// Iteration it's scalars from the totals: the stopping test per column,
+// alpha and beta (Chronopoulos-Gear), the history in st.
+__device__ __forceinline__ void cg_scalars_apply(const double* tot, int it, double tol, CgState& st) {
   bool all = true, any = false;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
@@ -2053,6 +2065,16 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, 
   st.all_done = all ? 1 : 0;
 }
 
+__global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, int it, double tol) {
+  constexpr int NW = kCgBlock / 64;
+  __shared__ double red[12][NW];
+  __shared__ double tot[12];
+  CgState& st = *cg.st;
+  if (st.all_done) return;
+  cg_block_totals(cg.part, nblk, red, tot);
+  if (threadIdx.x == 0) cg_scalars_apply(tot, it, tol, st);
+}
+
 __global__ void __launch_bounds__(kCgBlock) k_cg_update(int nn, CgArrays cg) {
   const CgState& st = *cg.st;
   if (st.all_done) return;
@@ -2074,12 +2096,155 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_update(int nn, CgArrays cg) {
   }
 }
 
+// One CG iteration in one launch: iteration m-1's scalars (every workgroup
+// reduces the partials of iteration m-1's SpMV in the same order and forms
+// the same alpha and beta; workgroup 0 stores the state), iteration m-1's
+// update of its own nodes (p and x in place, r and s into the other slot),
+// and iteration m's SpMV on the updated residual, whose halo values each
+// workgroup forms itself with the owner's expression.  r, s, w, the
+// partials and the state alternate between two slots by the parity of m,
+// so no workgroup reads what another writes in the same launch.  Every
+// value is formed as k_cg_spmv, k_cg_scalars and k_cg_update form it: the
+// iterates are bitwise those of the three-launch iteration.  Each workgroup
+// reads all nblk partials, so this pays while nblk is small (kCgFusedTiles).
+constexpr int kCgFusedTiles = 1024;
+template <int R>
+__global__ void __launch_bounds__(kCgBlock) k_cg_iter(GridDesc G, Musl2Arrays mu, CgArrays cg, int m, int nblk,
+                                                      double tol) {
+  constexpr int K1 = 2 * R + 1;
+  constexpr int NW = kCgBlock / 64;
+  constexpr int LX = kCgTX + 2 * R, LY = kCgTY + 2 * R;
+  __shared__ double zs[4][LY][LX];
+  __shared__ double red[12][NW];
+  __shared__ double tot[12];
+  __shared__ CgState ns;
+  const int a = m & 1, b = a ^ 1;  // the slots written and read
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const CgState* so = cg.st + b;
+  if (so->all_done) {  // converged earlier: carry the state to this slot
+    if (blockIdx.x == 0 && t == 0) cg.st[a] = *so;
+    return;
+  }
+  const int nn = G.nn;
+  const size_t n4 = 4 * size_t(nn);
+  const double* r_o = cg.r + b * n4;
+  const double* s_o = cg.s + b * n4;
+  const double* w_o = cg.w + b * n4;
+  double* r_n = cg.r + a * n4;
+  double* s_n = cg.s + a * n4;
+  double* w_n = cg.w + a * n4;
+  const bool upd = m > 0;
+  if (upd) {
+    cg_block_totals(cg.part + size_t(b) * nblk * 12, nblk, red, tot);
+    if (t == 0) {
+      ns = *so;
+      cg_scalars_apply(tot, m - 1, tol, ns);
+      if (blockIdx.x == 0) cg.st[a] = ns;
+    }
+  } else if (t == 0) {
+    ns = *so;
+    if (blockIdx.x == 0) cg.st[a] = ns;
+  }
+  __syncthreads();
+  if (ns.all_done) return;  // the same in every workgroup
+  const int ntx = (G.nx + kCgTX - 1) / kCgTX;
+  const int x0 = int(blockIdx.x % unsigned(ntx)) * kCgTX, y0 = int(blockIdx.x / unsigned(ntx)) * kCgTY;
+  const int tx = t % kCgTX, ty = t / kCgTX;
+  const int ix = x0 + tx, iy = y0 + ty;
+  const bool own = ix < G.nx && iy < G.ny;
+  const int i = own ? ix + G.nx * iy : 0;
+  // iteration m-1's update of this workgroup's nodes (k_cg_update)
+  double rnew[4] = {0.0, 0.0, 0.0, 0.0};
+  if (own) {
+    const double di = cg.dinv[i];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const size_t k = size_t(c) * nn + i;
+      const double ri = r_o[k];
+      if (upd && !ns.done[c]) {
+        const double al = ns.alpha[c], be = ns.beta[c];
+        const double pk = ri * di + be * cg.p[k];
+        const double sk = w_o[k] + be * s_o[k];
+        cg.p[k] = pk;
+        s_n[k] = sk;
+        cg.x[k] += al * pk;
+        rnew[c] = ri - al * sk;
+      } else {
+        s_n[k] = s_o[k];
+        rnew[c] = ri;
+      }
+      r_n[k] = rnew[c];
+    }
+  }
+  // z = D^-1 r over the tile and its halo, r updated as its owner updates it
+  for (int e = t; e < LX * LY; e += kCgBlock) {
+    const int lx = e % LX, ly = e / LX;
+    const int gx = x0 + lx - R, gy = y0 + ly - R;
+    const bool in = gx >= 0 && gx < G.nx && gy >= 0 && gy < G.ny;
+    const int j = in ? gx + G.nx * gy : 0;
+    const double dj = in ? cg.dinv[j] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      double rj = 0.0;
+      if (in) {
+        const size_t k = size_t(c) * nn + j;
+        rj = r_o[k];
+        if (upd && !ns.done[c]) rj = rj - ns.alpha[c] * (w_o[k] + ns.beta[c] * s_o[k]);
+      }
+      zs[c][ly][lx] = in ? rj * dj : 0.0;
+    }
+  }
+  __syncthreads();
+  // iteration m's SpMV and partial sums (k_cg_spmv)
+  double acc[12];
+#pragma unroll
+  for (int c = 0; c < 12; c++) acc[c] = 0.0;
+  if (own) {
+    double y[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int bb = -R; bb <= R; bb++) {
+      if (iy + bb < 0 || iy + bb >= G.ny) continue;
+#pragma unroll
+      for (int aa = -R; aa <= R; aa++) {
+        if (ix + aa < 0 || ix + aa >= G.nx) continue;
+        const double cf = mu.brow[size_t((aa + R) + K1 * (bb + R)) * nn + i];
+#pragma unroll
+        for (int c = 0; c < 4; c++) y[c] += cf * zs[c][ty + bb + R][tx + aa + R];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const double ri = rnew[c];
+      const double zi = zs[c][ty + R][tx + R];
+      w_n[size_t(c) * nn + i] = y[c];
+      acc[c] = ri * zi;
+      acc[4 + c] = y[c] * zi;
+      acc[8 + c] = ri * ri;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    double v = acc[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[c][wv] = v;
+  }
+  __syncthreads();
+  if (t < 12) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) v += red[t][w];
+    cg.part[(size_t(a) * nblk + blockIdx.x) * 12 + t] = v;
+  }
+}
+
 // mv = (x0, x1, x0 + x2, x1 + x3); the solver flag if a column did not
 // converge.
-__global__ void __launch_bounds__(kBlock) k_cg_finish(int nn, Musl2Arrays mu, CgArrays cg,
+// st: the final state's slot.
+__global__ void __launch_bounds__(kBlock) k_cg_finish(int nn, Musl2Arrays mu, CgArrays cg, const CgState* st,
                                                       unsigned* __restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0 && cg.st->all_done != 1) atomicOr(flags, JUMP_FLAG_SOLVER);
+  if (i == 0 && st->all_done != 1) atomicOr(flags, JUMP_FLAG_SOLVER);
   if (i >= nn) return;
   const double x0 = cg.x[i], x1 = cg.x[size_t(nn) + i];
   mu.mv[i] = x0;

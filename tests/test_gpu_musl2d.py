@@ -184,13 +184,15 @@ def test_musl2d_cpdi_tubes_stretching(hip_lib):
     _compare(g, o, 1)
 
 
+@pytest.mark.parametrize("cells", [300, 600])
 @pytest.mark.parametrize("basis", [_ffi.BASIS_LINEAR, _ffi.BASIS_QUAD_BSPLINE, _ffi.BASIS_CUBIC_BSPLINE])
-def test_musl2d_large_grid_closed_forms(hip_lib, basis):
-    """A 301 x 301 grid (90601 nodes, beyond the dense restatement): a
-    stress-free block translating under gravity gains exactly dt g per step
-    at every particle, whatever the FLIP fraction."""
-    _say(f"musl2d large grid basis={basis}")
-    cells = 300
+def test_musl2d_large_grid_closed_forms(hip_lib, basis, cells):
+    """A 301 x 301 grid (90601 nodes, beyond the dense restatement) and a
+    601 x 601 grid (1444 node tiles: the solve takes three launches per
+    iteration there, one below 1024 tiles): a stress-free block translating
+    under gravity gains exactly dt g per step at every particle, whatever
+    the FLIP fraction."""
+    _say(f"musl2d large grid basis={basis} cells={cells}")
     dx = 1.0 / cells
     f = dx / 2
     xs = np.arange(0.2 + f / 2, 0.8, f)
