@@ -121,6 +121,7 @@ struct jump_ctx {
     AdjParticles ap{};
     AdjNodes an{};
     AdjSources src{};
+    NodeVel2 nv{};        // 2D: the nodal velocities of the recorded step
     AdjParticles3 ap3{};  // 3D
     AdjNodes3 an3{};
     AdjSources3 src3{};
@@ -606,7 +607,8 @@ template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
   if constexpr (MULTI && B != JUMP_BASIS_CPDI_Q4) {  // (batches refuse CPDI)
     p2g_batched<B, ORDER>(ctx, ctx->p2g_shape);
   } else {
-    LAUNCH_P2G((k_p2g<2, B, 0, ORDER, MULTI, MULTI>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
+    LAUNCH_P2G((k_p2g<2, B, 0, ORDER, MULTI || kUsl2dCompact, MULTI>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners,
+               ctx->d_start,
                ctx->d_sorted, ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, ctx->cpdi_w, dt, ctx->bt(), 0.0,
                rl);
   }
@@ -636,11 +638,13 @@ template <int B, int ORDER> static void step_usl2d_contact(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_SORT);
   prof_begin(ctx);
   if (ctx->G.bodyfield)
-    LAUNCH_P2G((k_p2g_contact<B, ORDER, true>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags, rl);
+    LAUNCH_P2G((k_p2g_contact<B, ORDER, true, kContactCompact>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners,
+               ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction,
+               ctx->d_flags, rl, ctx->d_order, ctx->d_nodepos);
   else
-    LAUNCH_P2G((k_p2g_contact<B, ORDER, false>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start,
-               ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction, ctx->d_flags, rl);
+    LAUNCH_P2G((k_p2g_contact<B, ORDER, false, kContactCompact>), ctx->P, ctx->G, ctx->d_solids, ctx->d_corners,
+               ctx->d_start, ctx->d_sorted, ctx->d_bxyz, ctx->g, ctx->cg, ctx->cpdi_w, dt, ctx->prm.friction,
+               ctx->d_flags, rl, ctx->d_order, ctx->d_nodepos);
   if (rl.runs) ctx->runs_valid = true;
   prof_end(ctx, JUMP_PHASE_P2G);
   prof_begin(ctx);
@@ -2064,9 +2068,13 @@ static int adjoint_supported(jump_ctx* ctx) {
 }
 
 // The bins a step sorted its particles into: start (NN+1) and sorted (np+1).
+// The bins record also keeps the step's active-node order (d_order,
+// d_nodepos), which the backward sweep's node gathers walk.
 static std::vector<std::pair<void*, size_t>> bins_segments(const jump_ctx* ctx) {
   return {{ctx->d_start, size_t(ctx->NN + 1) * sizeof(int)},
-          {ctx->d_sorted, size_t(ctx->np + 1) * sizeof(unsigned long long)}};
+          {ctx->d_sorted, size_t(ctx->np + 1) * sizeof(unsigned long long)},
+          {ctx->d_order, size_t(ctx->NN) * sizeof(int)},
+          {ctx->d_nodepos, size_t(ctx->NN + 1) * sizeof(int)}};
 }
 static size_t bins_bytes(const jump_ctx* ctx) {
   size_t b = 0;
@@ -2111,6 +2119,8 @@ struct AdjStep {
   GridArrays g;
   const int* start;
   const unsigned long long* sorted;
+  const int* order;    // the step's nodes, those with a particle in their window first
+  const int* nodepos;  // nodepos[NN]: the number of those
 };
 template <class T>
 static void seg_redirect(const std::vector<std::pair<void*, size_t>>& segs, void* buf, T*& p) {
@@ -2124,7 +2134,7 @@ static void seg_redirect(const std::vector<std::pair<void*, size_t>>& segs, void
   }
 }
 static AdjStep adj_step_view(const jump_ctx* ctx, double* snap, double* grid, void* bins) {
-  AdjStep st{ctx->P, ctx->g, ctx->d_start, ctx->d_sorted};
+  AdjStep st{ctx->P, ctx->g, ctx->d_start, ctx->d_sorted, ctx->d_order, ctx->d_nodepos};
   const auto ss = snap_segments(ctx);
   for (auto& q : st.P.pr) seg_redirect(ss, snap, q);
   for (auto& q : st.P.F) seg_redirect(ss, snap, q);
@@ -2143,6 +2153,8 @@ static AdjStep adj_step_view(const jump_ctx* ctx, double* snap, double* grid, vo
   const auto bs = bins_segments(ctx);
   seg_redirect(bs, bins, st.start);
   seg_redirect(bs, bins, st.sorted);
+  seg_redirect(bs, bins, st.order);
+  seg_redirect(bs, bins, st.nodepos);
   return st;
 }
 
@@ -2168,9 +2180,11 @@ template <int B, int S> static void adj_step_back3_s(jump_ctx* ctx, const AdjSte
   const double dt = ctx->prm.dt, alpha = ctx->prm.alpha;
   auto& A = ctx->adj;
   LAUNCH((k_adj3_g2p2<B>), np, st.P, ctx->G, ctx->d_solids, st.g, dt, A.ap3, A.src3);
-  LAUNCH((k_adj3_nodes2<B, S>), nl, st.P, ctx->G, ctx->d_solids, st.start, st.sorted, st.g, dt, A.src3, A.an3);
+  LAUNCH((k_adj3_nodes2<B, S>), nl, st.P, ctx->G, ctx->d_solids, st.start, st.sorted, st.g, dt, A.src3, A.an3,
+         st.order, st.nodepos);
   LAUNCH((k_adj3_g2p1<B>), np, st.P, ctx->G, ctx->d_solids, st.g, alpha, A.an3, A.ap3, A.src3);
-  LAUNCH((k_adj3_nodes1<B, S>), nl, st.P, ctx->G, ctx->d_solids, st.start, st.sorted, st.g, alpha, A.src3, A.an3);
+  LAUNCH((k_adj3_nodes1<B, S>), nl, st.P, ctx->G, ctx->d_solids, st.start, st.sorted, st.g, alpha, A.src3, A.an3,
+         st.order, st.nodepos);
   LAUNCH(k_adj3_grid, nn, nn, ctx->d_fixed, dt, A.an3);
   LAUNCH((k_adj3_p2g<B>), np, st.P, ctx->G, ctx->d_solids, A.an3, A.ap3);
 }
@@ -2186,9 +2200,11 @@ template <int B, int S> static void adj_step_back_s(jump_ctx* ctx, const AdjStep
   const int np = int(ctx->np), nn = ctx->NN;
   const double dt = ctx->prm.dt;
   AdjNodes& an = ctx->adj.an;
-  LAUNCH((k_adj_g2p<B>), np, st.P, ctx->G, ctx->d_solids, st.g, ctx->prm.tolerance, dt, ctx->adj.ap, ctx->adj.src);
+  LAUNCH(k_adj_nodevel, nn, nn, st.g, ctx->prm.tolerance, ctx->adj.nv);
+  LAUNCH((k_adj_g2p<B>), np, st.P, ctx->G, ctx->d_solids, st.g, ctx->prm.tolerance, dt, ctx->adj.ap, ctx->adj.src,
+         ctx->adj.nv);
   LAUNCH((k_adj_nodes<B, S>), int64_t(nn) * S, st.P, ctx->G, ctx->d_solids, st.start, st.sorted, st.g,
-         ctx->prm.tolerance, dt, ctx->adj.src, an);
+         ctx->prm.tolerance, dt, ctx->adj.src, an, st.order, st.nodepos);
   LAUNCH(k_adj_grid, nn, nn, ctx->d_fixed, dt, an);
   LAUNCH((k_adj_p2g<B>), np, st.P, ctx->G, ctx->d_solids, an, ctx->adj.ap);
 }
@@ -2321,6 +2337,7 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
     A.ap = AdjParticles{};
     A.an = AdjNodes{};
     A.src = AdjSources{};
+    A.nv = NodeVel2{};
     A.ap3 = AdjParticles3{};
     A.an3 = AdjNodes3{};
     A.src3 = AdjSources3{};
@@ -2340,6 +2357,8 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
       AdjSources& src = A.src;
       for (double2** v : {&src.xb, &src.vb, &src.Lb[0], &src.Lb[1]})
         if (!r) r = dev_alloc(ctx, A.adj_allocs, v, n);
+      for (double2** v : {&A.nv.uL, &A.nv.u, &A.nv.a})
+        if (!r) r = dev_alloc(ctx, A.adj_allocs, v, nn);
       if (!r) r = alloc(&an.mb, nn);
       for (int k = 0; k < 2 && !r; k++) r = alloc(&an.Pb[k], nn);
       for (int k = 0; k < 2 && !r; k++) r = alloc(&an.P0b[k], nn);

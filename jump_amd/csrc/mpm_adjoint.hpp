@@ -72,48 +72,89 @@ __device__ __forceinline__ void axis_w2(double x, const Axis& A, double lp, int 
   }
 }
 
+// A particle's stencil: first node and the per-axis weights with their
+// first and second derivatives, formed once; each() calls
+// fn(node, N, g[2], H[3] = {Nxx, Nxy, Nyy}, k) for the support, k the node's
+// place in the walk (y outer, x inner).
+template <int B> struct Stencil2 {
+  static constexpr int W = Width<B>::W;
+  int f[2];
+  double w[2][W], d1[2][W], d2[2][W];
+  __device__ __forceinline__ bool init(const Particles& P, const GridDesc& G, const SolidDev& s, int p) {
+    const Axis A[2] = {axis_of(G, 0), axis_of(G, 1)};
+    for (int a = 0; a < 2; a++)
+      if (!first_node<B, 2>(P.x[a][p], a, A[a], s.lp[a], f[a])) return false;
+    for (int a = 0; a < 2; a++)
+      for (int o = 0; o < W; o++) axis_w2<B>(P.x[a][p], A[a], s.lp[a], f[a] + o, w[a][o], d1[a][o], d2[a][o]);
+    return true;
+  }
+  template <class Fn> __device__ __forceinline__ void each(const GridDesc& G, Fn&& fn) const {
+#pragma unroll
+    for (int b = 0; b < W; b++)
+#pragma unroll
+      for (int a = 0; a < W; a++) {
+        const double N = w[0][a] * w[1][b];
+        const double g[2] = {d1[0][a] * w[1][b], w[0][a] * d1[1][b]};
+        const double H[3] = {d2[0][a] * w[1][b], d1[0][a] * d1[1][b], w[0][a] * d2[1][b]};
+        fn((f[0] + a - 1) + G.nx * (f[1] + b - 1), N, g, H, a + W * b);
+      }
+  }
+};
 // Calls fn(node, N, g[2], H[3] = {Nxx, Nxy, Nyy}) for the support of p.
 template <int B, class Fn>
 __device__ __forceinline__ bool walk_stencil_h(const Particles& P, const GridDesc& G, const SolidDev& s, int p,
                                                Fn&& fn) {
-  constexpr int W = Width<B>::W;
-  const Axis A[2] = {axis_of(G, 0), axis_of(G, 1)};
-  int f[2];
-  for (int a = 0; a < 2; a++)
-    if (!first_node<B, 2>(P.x[a][p], a, A[a], s.lp[a], f[a])) return false;
-  double w[2][W], d1[2][W], d2[2][W];
-  for (int a = 0; a < 2; a++)
-    for (int o = 0; o < W; o++) axis_w2<B>(P.x[a][p], A[a], s.lp[a], f[a] + o, w[a][o], d1[a][o], d2[a][o]);
-  for (int b = 0; b < W; b++)
-    for (int a = 0; a < W; a++) {
-      const double N = w[0][a] * w[1][b];
-      const double g[2] = {d1[0][a] * w[1][b], w[0][a] * d1[1][b]};
-      const double H[3] = {d2[0][a] * w[1][b], d1[0][a] * d1[1][b], w[0][a] * d2[1][b]};
-      fn((f[0] + a - 1) + G.nx * (f[1] + b - 1), N, g, H);
-    }
+  Stencil2<B> st;
+  if (!st.init(P, G, s, p)) return false;
+  st.each(G, [&](int n, double N, const double* g, const double* H, int) { fn(n, N, g, H); });
   return true;
+}
+
+// Per node, the velocities k_adj_g2p reads (nodes with mass above the
+// tolerance only): uL = p/m as the forward's G2P forms it (so the velocity
+// gradient it recomputes is the forward's), u = p (1/m) and a = (p - p0) (1/m)
+// (k_adj_nodevel).
+struct NodeVel2 {
+  double2* uL;
+  double2* u;
+  double2* a;
+};
+__global__ void __launch_bounds__(kBlock) k_adj_nodevel(int nn, GridArrays g, double tol, NodeVel2 nv) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const double mI = g.m[n];
+  if (!(mI > tol)) return;
+  const double p0 = g.p[0][n], p1 = g.p[1][n];
+  nv.uL[n] = make_double2(p0 / mI, p1 / mI);
+  const double iM = 1.0 / mI;
+  nv.u[n] = make_double2(p0 * iM, p1 * iM);
+  nv.a[n] = make_double2((p0 - g.p0[0][n]) * iM, (p1 - g.p0[1][n]) * iM);
 }
 
 template <int B>
 __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                     GridArrays g, double tol, double dt, AdjParticles ad,
-                                                    AdjSources src) {
+                                                    AdjSources src, NodeVel2 nv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
   if (s.rigid) return;
   // forward recomputation of L, eps', F', J
   double L0 = 0.0, L1 = 0.0, L2 = 0.0, L3 = 0.0;
-  const bool ok = walk_stencil_h<B>(P, G, s, i, [&](int n, double, const double* gr, const double*) {
-    const double mI = g.m[n];
-    if (mI > tol) {
-      const double u0 = g.p[0][n] / mI, u1 = g.p[1][n] / mI;
-      L0 += gr[0] * u0;
-      L1 += gr[0] * u1;
-      L2 += gr[1] * u0;
-      L3 += gr[1] * u1;
-    }
-  });
+  Stencil2<B> stc;
+  unsigned mask = 0;  // the stencil's nodes with mass above the tolerance
+  const bool ok = stc.init(P, G, s, i);
+  if (ok)
+    stc.each(G, [&](int n, double, const double* gr, const double*, int k) {
+      if (g.m[n] > tol) {
+        mask |= 1u << k;
+        const double2 uL = nv.uL[n];
+        L0 += gr[0] * uL.x;
+        L1 += gr[0] * uL.y;
+        L2 += gr[1] * uL.x;
+        L3 += gr[1] * uL.y;
+      }
+    });
   if (!ok) {  // the forward left this particle unchanged: identity adjoint, no node sources
     src.xb[i] = make_double2(0.0, 0.0);
     src.vb[i] = make_double2(0.0, 0.0);
@@ -184,12 +225,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
   src.Lb[0][i] = make_double2(Lb0, Lb1);
   src.Lb[1][i] = make_double2(Lb2, Lb3);
   double xb[2] = {xb1[0], xb1[1]};
-  walk_stencil_h<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
-    const double mI = g.m[n];
-    if (!(mI > tol)) return;
-    const double iM = 1.0 / mI;
-    const double u[2] = {g.p[0][n] * iM, g.p[1][n] * iM};
-    const double a[2] = {(g.p[0][n] - g.p0[0][n]) * iM, (g.p[1][n] - g.p0[1][n]) * iM};
+  stc.each(G, [&](int n, double N, const double* gr, const double* H, int k) {
+    if (!((mask >> k) & 1u)) return;
+    const double2 u2 = nv.u[n], a2 = nv.a[n];
+    const double u[2] = {u2.x, u2.y};
+    const double a[2] = {a2.x, a2.y};
     const double Nb = dt * (u[0] * xb1[0] + u[1] * xb1[1]) + a[0] * vb1[0] + a[1] * vb1[1];
     const double gb[2] = {Lb0 * u[0] + Lb1 * u[1], Lb2 * u[0] + Lb3 * u[1]};
     xb[0] += Nb * gr[0] + H[0] * gb[0] + H[1] * gb[1];
@@ -231,16 +271,21 @@ template <int S> __device__ __forceinline__ double lanes_sum(double v) {
 // to a reference).  u = P/m, a = (P - P0)/m:
 //   Pb  += (ub + ab)/m    P0b -= ab/m    mb -= (u.ub + a.ab)/m
 //   ub = N dt xb' + Lb g (L[a][b] = u_a g_b),   ab = N vb'
-// S lanes per node (lanes_sum).
+// S lanes per node (lanes_sum).  The nodes are taken in the step's order,
+// those with a particle in their window first (order, nodepos); the rest
+// have no mass and get zeros.
 template <int B, int S>
 __global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, const SolidDev* __restrict__ S_,
                                                       const int* __restrict__ start,
                                                       const unsigned long long* __restrict__ sorted, GridArrays g,
-                                                      double tol, double dt, AdjSources src, AdjNodes an) {
+                                                      double tol, double dt, AdjSources src, AdjNodes an,
+                                                      const int* __restrict__ order,
+                                                      const int* __restrict__ nodepos) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int n = int(tid / S), sl = int(tid % S);
-  const bool live = n < G.nn;  // every lane reaches the sums below
-  const double mI = live ? g.m[n] : 0.0;
+  const int q = int(tid / S), sl = int(tid % S);
+  const bool live = q < G.nn;  // every lane reaches the sums below
+  const int n = live ? order[q] : 0;
+  const double mI = live && q < nodepos[G.nn] ? g.m[n] : 0.0;
   double Pb0 = 0.0, Pb1 = 0.0, P0b0 = 0.0, P0b1 = 0.0, mb = 0.0;
   if (mI > tol) {
     constexpr int W = Width<B>::W;

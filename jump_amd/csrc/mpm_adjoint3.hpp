@@ -346,16 +346,20 @@ __device__ __forceinline__ void window_gather3(const Particles& P, const GridDes
 
 //   P_b  += N dt x'_b / m      P2_b[b] += sum_a g_a L_b[a,b] / m
 //   m_b  -= (N dt P.x'_b + sum_ab g_a P2_b L_b[a,b]) / m^2
-// S lanes per node (lanes_sum).
+// S lanes per node (lanes_sum); the nodes in the step's order, those with a
+// particle in their window first (order, nodepos).
 template <int B, int S>
 __global__ void __launch_bounds__(kBlock) k_adj3_nodes2(Particles P, GridDesc G, const SolidDev* __restrict__ SD,
                                                         const int* __restrict__ start,
                                                         const unsigned long long* __restrict__ sorted, GridArrays g,
-                                                        double dt, AdjSources3 src, AdjNodes3 an) {
+                                                        double dt, AdjSources3 src, AdjNodes3 an,
+                                                        const int* __restrict__ order,
+                                                        const int* __restrict__ nodepos) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int n = int(tid / S), sl = int(tid % S);
-  const bool live = n < G.nn;  // every lane reaches the sums below
-  const double m = live ? g.m[n] : 0.0;
+  const int q = int(tid / S), sl = int(tid % S);
+  const bool live = q < G.nn;  // every lane reaches the sums below
+  const int n = live ? order[q] : 0;
+  const double m = live && q < nodepos[G.nn] ? g.m[n] : 0.0;
   double Pb[3] = {0.0, 0.0, 0.0}, P2b[3] = {0.0, 0.0, 0.0}, mb = 0.0;
   if (m > 0.0) {
     const double iM = 1.0 / m;
@@ -428,16 +432,20 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p1(Particles P, GridDesc G, c
 }
 
 //   P_b += N v'_b / m    P0_b -= alpha N v'_b / m    m_b -= N (P - alpha P0).v'_b / m^2
-// S lanes per node (lanes_sum).
+// S lanes per node (lanes_sum); the nodes in the step's order (order,
+// nodepos).
 template <int B, int S>
 __global__ void __launch_bounds__(kBlock) k_adj3_nodes1(Particles P, GridDesc G, const SolidDev* __restrict__ SD,
                                                         const int* __restrict__ start,
                                                         const unsigned long long* __restrict__ sorted, GridArrays g,
-                                                        double alpha, AdjSources3 src, AdjNodes3 an) {
+                                                        double alpha, AdjSources3 src, AdjNodes3 an,
+                                                        const int* __restrict__ order,
+                                                        const int* __restrict__ nodepos) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int n = int(tid / S), sl = int(tid % S);
-  const bool live = n < G.nn;  // every lane reaches the sums below
-  const double m = live ? g.m[n] : 0.0;
+  const int q = int(tid / S), sl = int(tid % S);
+  const bool live = q < G.nn;  // every lane reaches the sums below
+  const int n = live ? order[q] : 0;
+  const double m = live && q < nodepos[G.nn] ? g.m[n] : 0.0;
   double Pb[3] = {0.0, 0.0, 0.0}, mb = 0.0;
   if (m > 0.0) {
     const double iM = 1.0 / m;

@@ -434,6 +434,14 @@ constexpr int kP2GBlock = 128;
 // (p2g_block's LPN): the force pass and the re-projection.
 constexpr int kP2GLanes0 = 1;
 constexpr int kP2GLanes1 = 2;
+// Node order of the single 2D gathers: the active nodes compacted (true) or
+// grid rows, for the contact step and the plain USL step.
+// Measured: the two-rings USL gather (1025 x 513 nodes, thick rings) 0.609
+// ms in rows, 0.415 compacted; the CPDI tube array's contact gather (thin
+// rings on 2048 x 2048 nodes) 1.878 ms in rows, 2.150 compacted (a wave of
+// compacted nodes spans several ring walls and touches more cache lines).
+constexpr bool kContactCompact = false;
+constexpr bool kUsl2dCompact = true;
 // Work order of the P2G threads.  Only nodes whose window holds a particle
 // ("active") have work.  With waves on 64-node grid rows, lanes past the
 // solid's edge idle: about half the lane slots on the 4.2M-particle impact.
@@ -1064,20 +1072,27 @@ __device__ __forceinline__ bool hat_node(const GridDesc& G, const double* x, int
 }
 
 // BF: the body force is per particle (GridDesc::bodyfield), not the constant.
-template <int B, int ORDER, bool BF = false>
+// COMPACT: threads take the nodes in the order active nodes first (order,
+// nodepos: k_node_order); a node with an empty window skips the walk and
+// writes what an empty walk leaves.
+template <int B, int ORDER, bool BF = false, bool COMPACT = false>
 __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                            const double* __restrict__ corners,
                                                            const int* __restrict__ start,
                                                            const unsigned long long* __restrict__ sorted,
                                                            const unsigned long long* __restrict__ bxyz, GridArrays g,
                                                            ContactArrays c, int Wrt, double dt, double fric,
-                                                           unsigned* __restrict__ flags, RunList rl) {
+                                                           unsigned* __restrict__ flags, RunList rl,
+                                                           const int* __restrict__ order,
+                                                           const int* __restrict__ nodepos) {
   constexpr int MW = MaxW<B>::W;
   constexpr int NB = MW * MW;
   __shared__ int s_cur[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
   __shared__ int s_head[ORDER == JUMP_ORDER_REFERENCE ? NB : 1][kP2GBlock];
-  const int n = blockIdx.x * kP2GBlock + threadIdx.x;
-  if (n >= G.nn) return;
+  const int i = blockIdx.x * kP2GBlock + threadIdx.x;
+  if (i >= G.nn) return;
+  const int n = COMPACT ? ldu(order, unsigned(i)) : i;
+  const bool active = !COMPACT || i < ldu(nodepos, unsigned(G.nn));
   const int ni = n % G.nx, nj = n / G.nx;
   const int W = (B == JUMP_BASIS_CPDI_Q4) ? Wrt : Width<B>::W;
   ContactSlot s0, s1;
@@ -1132,7 +1147,10 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_contact(Particles P, GridDesc
   // recorded then, in the same order (CPDI bins follow the static corners,
   // so only the platens' particles ever change them).
   const bool runs_on = ORDER == JUMP_ORDER_REFERENCE && rl.runs != nullptr;
-  if (runs_on && !rl.all && rl.dirty[n] != rl.gen && rl.nruns[n] <= kRunCap) {
+  if (!active) {
+    // empty window: no particle to visit (its run list, if stale, is
+    // recorded again when a particle enters and stamps the node)
+  } else if (runs_on && !rl.all && rl.dirty[n] != rl.gen && rl.nruns[n] <= kRunCap) {
     const int nr = rl.nruns[n];
     for (int r = 0; r < nr; r++) {
       const unsigned e = rl.runs[size_t(r) * G.nn + n];
