@@ -125,6 +125,7 @@ struct jump_ctx {
     AdjParticles3 ap3{};  // 3D
     AdjNodes3 an3{};
     AdjSources3 src3{};
+    NodeVel3 nv3{};  // 3D: the nodal factors of the recorded step
     std::vector<double> dfy, dk1;  // per solid, from the last backward sweep
     int lanes = 1;  // lanes per node of the backward sweep's node gathers (1, 2, 4)
   } adj;
@@ -2179,10 +2180,11 @@ template <int B, int S> static void adj_step_back3_s(jump_ctx* ctx, const AdjSte
   const int64_t nl = int64_t(nn) * S;
   const double dt = ctx->prm.dt, alpha = ctx->prm.alpha;
   auto& A = ctx->adj;
-  LAUNCH((k_adj3_g2p2<B>), np, st.P, ctx->G, ctx->d_solids, st.g, dt, A.ap3, A.src3);
+  LAUNCH(k_adj3_nodevel, nn, nn, st.g, alpha, A.nv3);
+  LAUNCH((k_adj3_g2p2<B>), np, st.P, ctx->G, ctx->d_solids, st.g, dt, A.ap3, A.src3, A.nv3);
   LAUNCH((k_adj3_nodes2<B, S>), nl, st.P, ctx->G, ctx->d_solids, st.start, st.sorted, st.g, dt, A.src3, A.an3,
          st.order, st.nodepos);
-  LAUNCH((k_adj3_g2p1<B>), np, st.P, ctx->G, ctx->d_solids, st.g, alpha, A.an3, A.ap3, A.src3);
+  LAUNCH((k_adj3_g2p1<B>), np, st.P, ctx->G, ctx->d_solids, st.g, alpha, A.an3, A.ap3, A.src3, A.nv3);
   LAUNCH((k_adj3_nodes1<B, S>), nl, st.P, ctx->G, ctx->d_solids, st.start, st.sorted, st.g, alpha, A.src3, A.an3,
          st.order, st.nodepos);
   LAUNCH(k_adj3_grid, nn, nn, ctx->d_fixed, dt, A.an3);
@@ -2341,6 +2343,7 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
     A.ap3 = AdjParticles3{};
     A.an3 = AdjNodes3{};
     A.src3 = AdjSources3{};
+    A.nv3 = NodeVel3{};
     A.adj_np = ctx->np;
     auto alloc = [&](double** p, size_t count) { return dev_alloc(ctx, A.adj_allocs, p, count); };
     if (D == 2) {
@@ -2376,6 +2379,12 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
       for (int k = 0; k < 3 && !r; k++) r = alloc(&src.xb[k], n);
       for (int k = 0; k < 9 && !r; k++) r = alloc(&src.Lb[k], n);
       for (int k = 0; k < 3 && !r; k++) r = alloc(&src.vb[k], n);
+      NodeVel3& nv = A.nv3;
+      if (!r) r = alloc(&nv.mii, nn);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&nv.q[k], nn);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&nv.qm[k], nn);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&nv.v[k], nn);
+      for (int k = 0; k < 3 && !r; k++) r = alloc(&nv.v2[k], nn);
       AdjNodes3& an = A.an3;
       if (!r) r = alloc(&an.mb, nn);
       for (int k = 0; k < 3 && !r; k++) r = alloc(&an.Pb[k], nn);

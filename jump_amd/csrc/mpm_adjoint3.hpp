@@ -63,6 +63,65 @@ struct AdjSources3 {
   double* vb[3];  // v'_b, total (G2P-1)
 };
 
+// A particle's 3D stencil formed once (as Stencil2): each() calls
+// fn(node, N, g[3], H[6] = {xx, xy, xz, yy, yz, zz}) in walk_stencil_h3's
+// order.
+template <int B> struct Stencil3 {
+  static constexpr int W = Width<B>::W;
+  int f[3];
+  double w[3][W], d1[3][W], d2[3][W];
+  __device__ __forceinline__ bool init(const Particles& P, const GridDesc& G, const SolidDev& s, int p) {
+    const Axis A[3] = {axis_of(G, 0), axis_of(G, 1), axis_of(G, 2)};
+    for (int a = 0; a < 3; a++)
+      if (!first_node<B, 3>(P.x[a][p], a, A[a], s.lp[a], f[a])) return false;
+    for (int a = 0; a < 3; a++)
+      for (int o = 0; o < W; o++) axis_w2<B>(P.x[a][p], A[a], s.lp[a], f[a] + o, w[a][o], d1[a][o], d2[a][o]);
+    return true;
+  }
+  template <class Fn> __device__ __forceinline__ void each(const GridDesc& G, Fn&& fn) const {
+    for (int c = 0; c < W; c++)
+      for (int b = 0; b < W; b++)
+        for (int a = 0; a < W; a++) {
+          const double N = w[0][a] * w[1][b] * w[2][c];
+          const double g[3] = {d1[0][a] * w[1][b] * w[2][c], w[0][a] * d1[1][b] * w[2][c],
+                               w[0][a] * w[1][b] * d1[2][c]};
+          const double H[6] = {d2[0][a] * w[1][b] * w[2][c], d1[0][a] * d1[1][b] * w[2][c],
+                               d1[0][a] * w[1][b] * d1[2][c], w[0][a] * d2[1][b] * w[2][c],
+                               w[0][a] * d1[1][b] * d1[2][c], w[0][a] * w[1][b] * d2[2][c]};
+          fn((f[0] + a - 1) + G.nx * ((f[1] + b - 1) + G.ny * (f[2] + c - 1)), N, g, H);
+        }
+  }
+};
+
+// Per node, the factors the 3D G2P adjoints read, formed as the forward's
+// G2P passes form them (k_g2p1_musl, k_g2p2_musl: 1/m, then products with
+// it), so that the velocity gradient k_adj3_g2p2 recomputes is the
+// forward's and the J2 branch it differentiates is the forward's:
+// mii = 1/m (0 without mass), q = p - alpha p0, qm = q mii, v = p mii,
+// v2 = p2 mii (k_adj3_nodevel).
+struct NodeVel3 {
+  double* mii;
+  double* q[3];
+  double* qm[3];
+  double* v[3];
+  double* v2[3];
+};
+__global__ void __launch_bounds__(kBlock) k_adj3_nodevel(int nn, GridArrays g, double alpha, NodeVel3 nv) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const double m = g.m[n];
+  const double mii = m > 0.0 ? 1.0 / m : 0.0;
+  nv.mii[n] = mii;
+  if (!(m > 0.0)) return;
+  for (int k = 0; k < 3; k++) {
+    const double q = g.p[k][n] - alpha * g.p0[k][n];
+    nv.q[k][n] = q;
+    nv.qm[k][n] = q * mii;
+    nv.v[k][n] = g.p[k][n] * mii;
+    nv.v2[k][n] = g.p2[k][n] * mii;
+  }
+}
+
 // Calls fn(node, N, g[3], H[6] = {xx, xy, xz, yy, yz, zz}) for the support of p.
 template <int B, class Fn>
 __device__ __forceinline__ bool walk_stencil_h3(const Particles& P, const GridDesc& G, const SolidDev& s, int p,
@@ -160,20 +219,24 @@ __device__ __forceinline__ void j2_3d_reverse(const SolidDev& M, const double* e
 
 template <int B>
 __global__ void __launch_bounds__(kBlock) k_adj3_g2p2(Particles P, GridDesc G, const SolidDev* __restrict__ S,
-                                                      GridArrays g, double dt, AdjParticles3 ad, AdjSources3 src) {
+                                                      GridArrays g, double dt, AdjParticles3 ad, AdjSources3 src,
+                                                      NodeVel3 nv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
-  // forward recomputation of L (column-major a + 3b), eps', F', J
+  // forward recomputation of L (column-major a + 3b), eps', F', J: the
+  // forward's L[a,b] = g_a v2_b with v2 = p2 (1/m)
   double L[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const bool ok = walk_stencil_h3<B>(P, G, s, i, [&](int n, double, const double* gr, const double*) {
-    const double m = g.m[n];
-    if (m > 0.0) {
-      const double u[3] = {g.p2[0][n] / m, g.p2[1][n] / m, g.p2[2][n] / m};
-      for (int b = 0; b < 3; b++)
-        for (int a = 0; a < 3; a++) L[a + 3 * b] += gr[a] * u[b];
-    }
-  });
+  Stencil3<B> stc;
+  const bool ok = stc.init(P, G, s, i);
+  if (ok)
+    stc.each(G, [&](int n, double, const double* gr, const double*) {
+      if (nv.mii[n] > 0.0) {
+        const double u[3] = {nv.v2[0][n], nv.v2[1][n], nv.v2[2][n]};
+        for (int b = 0; b < 3; b++)
+          for (int a = 0; a < 3; a++) L[a + 3 * b] += gr[a] * u[b];
+      }
+    });
   if (!ok) {  // the forward left this particle unchanged: identity adjoint, no sources
     for (int k = 0; k < 3; k++) { src.xb[k][i] = 0.0; src.vb[k][i] = 0.0; }
     for (int k = 0; k < 9; k++) src.Lb[k][i] = 0.0;
@@ -288,12 +351,10 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p2(Particles P, GridDesc G, c
   for (int k = 0; k < 9; k++) src.Lb[k][i] = Lb[k];
   // x' = x + sum N dt P/m and L through N(x), g(x)
   double xb[3] = {xb1[0], xb1[1], xb1[2]};
-  walk_stencil_h3<B>(P, G, s, i, [&](int n, double, const double* gr, const double* H) {
-    const double m = g.m[n];
-    if (!(m > 0.0)) return;
-    const double iM = 1.0 / m;
-    const double u[3] = {g.p[0][n] * iM, g.p[1][n] * iM, g.p[2][n] * iM};
-    const double u2[3] = {g.p2[0][n] * iM, g.p2[1][n] * iM, g.p2[2][n] * iM};
+  stc.each(G, [&](int n, double, const double* gr, const double* H) {
+    if (!(nv.mii[n] > 0.0)) return;
+    const double u[3] = {nv.v[0][n], nv.v[1][n], nv.v[2][n]};
+    const double u2[3] = {nv.v2[0][n], nv.v2[1][n], nv.v2[2][n]};
     const double Nb = dt * (u[0] * xb1[0] + u[1] * xb1[1] + u[2] * xb1[2]);
     double gb[3];  // adjoint of g_a: sum_b Lb[a,b] u2_b
     for (int a = 0; a < 3; a++) gb[a] = Lb[a] * u2[0] + Lb[a + 3] * u2[1] + Lb[a + 6] * u2[2];
@@ -395,7 +456,7 @@ __global__ void __launch_bounds__(kBlock) k_adj3_nodes2(Particles P, GridDesc G,
 template <int B>
 __global__ void __launch_bounds__(kBlock) k_adj3_g2p1(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                       GridArrays g, double alpha, AdjNodes3 an, AdjParticles3 ad,
-                                                      AdjSources3 src) {
+                                                      AdjSources3 src, NodeVel3 nv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
@@ -403,25 +464,25 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p1(Particles P, GridDesc G, c
   // forward recomputation of v', and the re-projection's part of v'_b
   double vn[3] = {alpha * P.v[0][i], alpha * P.v[1][i], alpha * P.v[2][i]};
   double vx[3] = {0.0, 0.0, 0.0}, vb[3] = {ad.vb[0][i], ad.vb[1][i], ad.vb[2][i]};
-  const bool ok = walk_stencil_h3<B>(P, G, s, i, [&](int n, double N, const double*, const double*) {
-    const double m = g.m[n];
-    if (m > 0.0)
-      for (int k = 0; k < 3; k++) vx[k] += (g.p[k][n] - alpha * g.p0[k][n]) * ((1.0 / m) * N);
-    for (int k = 0; k < 3; k++) vb[k] += N * mp * an.P2b[k][n];
-  });
+  Stencil3<B> stc;
+  const bool ok = stc.init(P, G, s, i);
+  if (ok)
+    stc.each(G, [&](int n, double N, const double*, const double*) {
+      const double mii = nv.mii[n];
+      if (mii > 0.0)
+        for (int k = 0; k < 3; k++) vx[k] += nv.q[k][n] * (mii * N);
+      for (int k = 0; k < 3; k++) vb[k] += N * mp * an.P2b[k][n];
+    });
   if (!ok) {
     for (int k = 0; k < 3; k++) src.vb[k][i] = 0.0;
     return;
   }
   for (int k = 0; k < 3; k++) vn[k] += vx[k];
   double xb[3] = {0.0, 0.0, 0.0};
-  walk_stencil_h3<B>(P, G, s, i, [&](int n, double, const double* gr, const double*) {
+  stc.each(G, [&](int n, double, const double* gr, const double*) {
     double Nb = mp * (vn[0] * an.P2b[0][n] + vn[1] * an.P2b[1][n] + vn[2] * an.P2b[2][n]);
-    const double m = g.m[n];
-    if (m > 0.0) {
-      const double iM = 1.0 / m;
-      for (int k = 0; k < 3; k++) Nb += (g.p[k][n] - alpha * g.p0[k][n]) * iM * vb[k];
-    }
+    if (nv.mii[n] > 0.0)
+      for (int k = 0; k < 3; k++) Nb += nv.qm[k][n] * vb[k];
     for (int c = 0; c < 3; c++) xb[c] += Nb * gr[c];
   });
   for (int k = 0; k < 3; k++) {
