@@ -1657,7 +1657,12 @@ __global__ void __launch_bounds__(kBlock) k_g2p1_musl(Particles P, GridDesc G, c
     atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
     return;
   }
-  for (int k = 0; k < 3; k++) P.v[k][i] = alpha * P.v[k][i] + vx[k];
+  double vn[3];
+  for (int k = 0; k < 3; k++) {
+    vn[k] = alpha * P.v[k][i] + vx[k];
+    P.v[k][i] = vn[k];
+  }
+  if (!finite3(vn[0], vn[1], vn[2])) atomicOr(flags, JUMP_FLAG_NONFINITE);  // final for the step
 }
 
 // The particle update of G2P-2 (SolveExplicitSolidMechanics3D.jl:186-211)
@@ -1721,7 +1726,8 @@ __device__ __forceinline__ void musl3d_particle_update(const Particles& P, const
     default: break;
   }
   for (int k = 0; k < 6; k++) P.sig[k][i] = sg[k];
-  if (!(finite3(xx[0], xx[1], xx[2]) && finite3(P.v[0][i], P.v[1][i], P.v[2][i]))) fl |= JUMP_FLAG_NONFINITE;
+  // (the velocity, final after G2P-1, is checked there)
+  if (!finite3(xx[0], xx[1], xx[2])) fl |= JUMP_FLAG_NONFINITE;
   if (fl) atomicOr(flags, fl);
 }
 
@@ -2470,7 +2476,12 @@ __global__ void __launch_bounds__(kBlock) k_g2p1_musl_contact(Particles P, GridD
     atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
     return;
   }
-  for (int a = 0; a < 3; a++) P.v[a][i] = alpha * P.v[a][i] + vx[a];
+  double vn[3];
+  for (int a = 0; a < 3; a++) {
+    vn[a] = alpha * P.v[a][i] + vx[a];
+    P.v[a][i] = vn[a];
+  }
+  if (!finite3(vn[0], vn[1], vn[2])) atomicOr(flags, JUMP_FLAG_NONFINITE);  // final for the step
 }
 
 // Re-projection per slot (SolveExplicitSolidMechanics3D.jl:135-138): the
