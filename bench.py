@@ -56,7 +56,9 @@ def algorithmic_bytes(phase, n_plate, n_sphere, n_active):
     if phase == "p2g2":
         return 56 * n + 56 * n_active  # read x,m,v; write p2 and the pairs (p2x,p2y) (pz,p2z)
     if phase == "g2p2":
-        plate = 208 + 264  # read x,V0,F,eps,alpha,ep; write x,V,F,eps,sig,alpha,ep,vm
+        # read x,V0,F,eps,alpha,ep; write x,V,F,eps,sig,vm (alpha and ep are
+        # written only where the step is plastic: 56 B more per such particle)
+        plate = 208 + 208
         sphere = 200 + 208  # read x,V0,F,eps,sig; write x,V,F,eps,sig,vm
         return plate * n_plate + sphere * n_sphere + 64 * n_active  # + read 4 pairs (minv,p,p2)
     return 0

@@ -457,7 +457,8 @@ __device__ __forceinline__ void j2_2d(const SolidDev& M, const double* eps, doub
   vm = sqrt(d0 * d0 + d1 * d1 + 2.0 * (d2 * d2));
 }
 // J2 3D, Material.jl:228-275 (returns sigma)
-__device__ __forceinline__ void j2_3d(const SolidDev& M, const double* e, double* s, double* ep, double& alpha,
+// Returns whether the return map changed ep and alpha (the plastic branch).
+__device__ __forceinline__ bool j2_3d(const SolidDev& M, const double* e, double* s, double* ep, double& alpha,
                                       double& vm) {
   double tr = e[0] + e[1] + e[2];
   double c = 0.333333333 * tr;
@@ -492,6 +493,7 @@ __device__ __forceinline__ void j2_3d(const SolidDev& M, const double* e, double
   double cs = 0.333333333 * (r[0] + r[1] + r[2]);
   double d0 = r[0] - cs, d1 = r[1] - cs, d2 = r[2] - cs;
   vm = sqrt(d0 * d0 + d1 * d1 + d2 * d2 + 2.0 * (r[3] * r[3] + r[4] * r[4] + r[5] * r[5]));
+  return !(f <= 0.0);
 }
 // Johnson-Cook, Material.jl:386-441
 __device__ __forceinline__ void johnson_cook(const SolidDev& M, const double* e, const double* de, double dtime,

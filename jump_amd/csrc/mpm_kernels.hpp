@@ -1709,9 +1709,11 @@ __device__ __forceinline__ void musl3d_particle_update(const Particles& P, const
       double ep[6];
       for (int k = 0; k < 6; k++) ep[k] = P.ep[k][i];
       double al = P.alpha[i], vm;
-      j2_3d(s, e, sg, ep, al, vm);
-      for (int k = 0; k < 6; k++) P.ep[k][i] = ep[k];
-      P.alpha[i] = al;
+      // an elastic step leaves ep and alpha as they are: no write-back
+      if (j2_3d(s, e, sg, ep, al, vm)) {
+        for (int k = 0; k < 6; k++) P.ep[k][i] = ep[k];
+        P.alpha[i] = al;
+      }
       P.vm[i] = vm;
       break;
     }
