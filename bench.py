@@ -224,7 +224,7 @@ def musl2d_leg(steps, warmup, cells=256):
     (SolveExplicitSolidMechanics2D.jl:25-329) on a 257 x 257 grid: a
     stress-free block of 94,249 particles, quadratic B-splines (scenes.
     musl_block2d).  Per-phase times from HIP events; the solve is the
-    conjugate-gradient solve of the blended mass matrix on the device."""
+    Chebyshev solve of the blended mass matrix on the device."""
     import torch
 
     from jump_amd import scenes
@@ -246,7 +246,7 @@ def musl2d_leg(steps, warmup, cells=256):
     e.close()
     n = sc.n_particles
     return {"workload": f"musl_block2d: stress-free block, quadratic B-spline, MUSL(0.99), {cells + 1}x{cells + 1} "
-                        "nodes, blended mass solved by device CG",
+                        "nodes, blended mass solved by a device Chebyshev iteration",
             "particles": n, "grid_nodes": [cells + 1, cells + 1], "steps": steps,
             "value": n * steps / dt, "unit": "particle-steps/s", "ms_per_step": dt * 1e3 / steps,
             "phases_ms": {k: round(ms / c, 4) for k, (ms, c) in prof.items() if c},

@@ -64,13 +64,11 @@ struct jump_ctx {
   ContactArrays cg{};    // contact mode only, 2D
   ContactArrays3 cg3{};  // contact mode only, 3D
   Musl2Arrays mu{};      // 2D MUSL only: matrix rows, right-hand sides, velocities
-  CgArrays cgm{};        // 2D MUSL only: the conjugate-gradient vectors and state
+  ChebArrays ch{};       // 2D MUSL only: the Chebyshev iteration's vectors
   RunList runs{};        // 3D: the force pass's particle runs per node, replayed by the re-projection
   int* d_dirty = nullptr;  // 3D: per node, the stamp of the step its window last changed
   int runs_gen = 0;        // stamp of the last step
   bool runs_valid = false; // the recorded runs match the particles' current bins
-  int* h_cg = nullptr;   // pinned: CgState::all_done and ::iters read back
-  int cg_check = 8;      // CG iterations before the first convergence read-back
   std::vector<void*> part_allocs;
   std::vector<void*> grid_allocs;
   double* d_corners = nullptr;
@@ -114,7 +112,7 @@ struct jump_ctx {
     std::vector<void*> allocs;      // snapshots and grids of the recorded run
     std::vector<void*> adj_allocs;  // adjoint arrays
     std::vector<double*> ckpt;    // state at steps 0, K, 2K, ...
-    std::vector<double*> segbuf;  // the states of one segment, recomputed
+    std::vector<double*> segbuf;  // the states of one segment after its first (that one is its checkpoint; [0] unused)
     std::vector<double*> gridbuf; // and their grids (m, p0, p)
     std::vector<void*> binbuf;    // and their bins (start, sorted)
     double* final_state = nullptr;
@@ -419,16 +417,11 @@ static int alloc_grid(jump_ctx* ctx) {
     r = dev_alloc(ctx, ctx->grid_allocs, &m.brow, K1 * K1 * nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &m.rhs, 4 * nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &m.mv, 4 * nn);
-    CgArrays& c = ctx->cgm;
-    for (double** v : {&c.x, &c.p})
-      if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, 4 * nn);
-    for (double** v : {&c.r, &c.s, &c.w})  // two slots each
+    ChebArrays& c = ctx->ch;
+    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.x, 4 * nn);
+    for (double** v : {&c.r, &c.d})  // two slots each
       if (!r) r = dev_alloc(ctx, ctx->grid_allocs, v, 8 * nn);
     if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.dinv, nn);
-    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.part, 2 * 12 * size_t(cg_tiles(ctx->G)));
-    if (!r) r = dev_alloc(ctx, ctx->grid_allocs, &c.st, 2);
-    if (!r && hipHostMalloc(reinterpret_cast<void**>(&ctx->h_cg), 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
-      r = set_err(ctx, JUMP_ERR_NOMEM, "hipHostMalloc failed");
   } else if (ctx->prm.contact) {
     ContactArrays& c = ctx->cg;
     for (int k = 0; k < 2 && !r; k++) {
@@ -720,57 +713,37 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
 // 2D MUSL with the blended mass matrix: P2G with the matrix rows and the
 // grid fields, the solve for the nodal velocities, G2P-1, G2P-2.  R: the
 // matrix stencil radius (width - 1).
-constexpr double kMuslTol = 1e-14;  // CG: relative residual per right-hand side
+constexpr double kMuslTol = 1e-14;  // bound on the error's B-norm, relative to the solution's
 constexpr int kMuslMaxIt = 2000;
-// The blended mass solve: CG iterations of one launch each (k_cg_iter) up
-// to kCgFusedTiles node tiles, else of three (k_cg_spmv, k_cg_scalars,
-// k_cg_update); the same iterates either way.  The host reads the
-// convergence state back after as many iterations as the last solve needed
-// (then every 4), so a step usually waits for it once.
+// The Chebyshev iteration count for kMuslTol: the least k with
+// 2 q^k / (1 + q^2k) <= kMuslTol, q = (sqrt 10 - 1) / (sqrt 10 + 1).
+static int cheb_iterations() {
+  const double q = (std::sqrt(10.0) - 1.0) / (std::sqrt(10.0) + 1.0);
+  double qk = 1.0;
+  for (int k = 1; k < kMuslMaxIt; k++) {
+    qk *= q;
+    if (2.0 * qk / (1.0 + qk * qk) <= kMuslTol) return k;
+  }
+  return kMuslMaxIt;
+}
+// The blended mass solve: K Chebyshev iterations of one launch each; the
+// last writes the nodal velocities mv.  The launches do not depend on the
+// data, so the host never waits.
 template <int R> static void solve_musl2d(jump_ctx* ctx) {
   const int nn = ctx->NN;
-  const int nblk = (nn + kCgBlock - 1) / kCgBlock;
-  const int ntile = cg_tiles(ctx->G);
-  CgArrays& c = ctx->cgm;
-  LAUNCH(k_cg_init, nn, nn, ctx->mu, ctx->g.m, c);
-  int check = std::max(ctx->cg_check, 4);
-  static_assert(offsetof(CgState, iters) == offsetof(CgState, all_done) + sizeof(int), "read-back layout");
-  auto read_back = [&](const CgState* st) {
-    const char* p = reinterpret_cast<const char*>(st) + offsetof(CgState, all_done);
-    return hipMemcpyAsync(ctx->h_cg, p, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-           hipStreamSynchronize(ctx->stream) == hipSuccess;
-  };
-  const CgState* last = c.st;
-  if (ntile <= kCgFusedTiles) {
-    // launch m forms iteration m-1's scalars: the state after iteration it
-    // is the one launch it+1 leaves, in slot (it+1) & 1
-    for (int m = 0; m <= kMuslMaxIt; m++) {
-      hipLaunchKernelGGL((k_cg_iter<R>), dim3(ntile), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, c, m, ntile,
-                         kMuslTol);
-      last = c.st + (m & 1);
-      if (m < check) continue;
-      if (!read_back(last)) break;  // the error stays with the stream and is reported by the step
-      if (ctx->h_cg[0]) {
-        ctx->cg_check = ctx->h_cg[1] + 1;  // the launch that found every column converged
-        break;
-      }
-      check = m + 4;
-    }
-  } else {
-    for (int it = 0; it < kMuslMaxIt; it++) {
-      hipLaunchKernelGGL((k_cg_spmv<R>), dim3(ntile), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->mu, c);
-      hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(kCgBlock), 0, ctx->stream, c, ntile, it, kMuslTol);
-      hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kCgBlock), 0, ctx->stream, nn, c);
-      if (it + 1 != check) continue;
-      if (!read_back(c.st)) break;
-      if (ctx->h_cg[0]) {
-        ctx->cg_check = ctx->h_cg[1] + 1;  // the iteration that found every column converged
-        break;
-      }
-      check += 4;
-    }
+  const int ntile = cheb_tiles(ctx->G);
+  const ChebArrays& ch = ctx->ch;
+  const double theta = 0.55, delta = 0.45, sigma = theta / delta;
+  LAUNCH(k_cheb_init, nn, nn, ctx->mu, ctx->g.m, ch, 1.0 / theta);
+  static const int K = cheb_iterations();
+  // iteration k forms d_k+1 = c1 d_k + c2 D^-1 r_k+1
+  double rho = 1.0 / sigma;
+  for (int k = 0; k < K; k++) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    hipLaunchKernelGGL((k_cheb_iter<R>), dim3(ntile), dim3(kChebBlock), 0, ctx->stream, ctx->G, ctx->mu, ch, k & 1,
+                       rn * rho, 2.0 * rn / delta, k + 1 == K ? 1 : 0);
+    rho = rn;
   }
-  LAUNCH(k_cg_finish, nn, nn, ctx->mu, c, last, ctx->d_flags);
 }
 template <int B, int ORDER, int R> static void step_musl2d(jump_ctx* ctx) {
   const int np = int(ctx->np);
@@ -961,7 +934,6 @@ int jump_ctx_create(const jump_params* p, int device, jump_ctx** out) {
   int r = alloc_grid(ctx);
   if (r) {
     free_list(ctx->grid_allocs);
-    if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return r;
@@ -984,7 +956,6 @@ int jump_ctx_destroy(jump_ctx* ctx) {
   free_list(ctx->adj.adj_allocs);
   free_list(ctx->part_allocs);
   free_list(ctx->grid_allocs);
-  if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1992,7 +1963,7 @@ int jump_batch_tuning(jump_batch* b, int32_t g, int32_t* ngroups, int32_t* shape
 // mpm_adjoint3.hpp).  The recorded run keeps the particle state every K
 // steps; the backward sweep rebuilds each segment's states from its snapshot
 // (keeping the grid and bins each step used), then walks the segment's steps
-// backwards.  Memory: (nsteps/K + K + 1) snapshots of 136 (2D) or 288 (3D)
+// backwards.  Memory: (nsteps/K + K) snapshots of 136 (2D) or 288 (3D)
 // bytes per particle, K grids of 40 or 80 bytes per node and K bins.
 
 // Copies the arrays of `segs` (device pointer, bytes) to or from the buffer
@@ -2219,12 +2190,13 @@ template <int B> static void adj_step_back(jump_ctx* ctx, const AdjStep& st) {
 }
 
 // Device bytes a recorded run of nsteps steps at interval K holds: nck
-// snapshots, K per-step states, grids and bins, the final state.
+// snapshots, K grids and bins, K - 1 per-step states (a segment's first
+// state is its snapshot), the final state.
 static size_t adjoint_record_bytes(const jump_ctx* ctx, int64_t nsteps, int64_t K) {
   const size_t snap = snap_bytes(ctx);
-  const size_t per_step = snap + grid_bytes(ctx) + bins_bytes(ctx);
+  const size_t kk = size_t(std::min(K, nsteps));
   const size_t nck = size_t((nsteps + K - 1) / K);
-  return (nck + 1) * snap + size_t(std::min(K, nsteps)) * per_step;
+  return (nck + kk) * snap + kk * (grid_bytes(ctx) + bins_bytes(ctx));
 }
 
 // The largest interval whose record fits in 3/4 of the device memory that is
@@ -2235,8 +2207,8 @@ static int adjoint_auto_interval(jump_ctx* ctx, int64_t nsteps, int64_t* K) {
   HIP_TRY(hipMemGetInfo(&freeb, &total));
   const auto& A = ctx->adj;
   if (A.snap_np == ctx->np) {  // the buffers held now are freed or reused
-    freeb += (A.ckpt.size() + A.segbuf.size() + (A.final_state ? 1 : 0)) * snap_bytes(ctx) +
-             A.gridbuf.size() * grid_bytes(ctx) + A.binbuf.size() * bins_bytes(ctx);
+    const size_t states = A.ckpt.size() + (A.segbuf.empty() ? 0 : A.segbuf.size() - 1) + (A.final_state ? 1 : 0);
+    freeb += states * snap_bytes(ctx) + A.gridbuf.size() * grid_bytes(ctx) + A.binbuf.size() * bins_bytes(ctx);
   }
   // the adjoint arrays of the backward sweep: 2D 25 doubles per particle and
   // 6 per node; 3D 54 and 10
@@ -2274,9 +2246,9 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
     A.ckpt.resize(size_t(nck));
     for (auto& c : A.ckpt)
       if ((r = alloc(&c, sd))) return r;
-    A.segbuf.resize(nseg);
-    for (auto& c : A.segbuf)
-      if ((r = alloc(&c, sd))) return r;
+    A.segbuf.assign(nseg, nullptr);
+    for (size_t q = 1; q < nseg; q++)
+      if ((r = alloc(&A.segbuf[q], sd))) return r;
     A.gridbuf.resize(nseg);
     for (auto& c : A.gridbuf)
       if ((r = alloc(&c, grid_bytes(ctx) / sizeof(double)))) return r;
@@ -2297,7 +2269,7 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
   const int64_t last0 = (nck - 1) * K;  // the last segment's states and grids are kept as they go by
   for (int64_t s = 0; s < nsteps; s++) {
     if (s % K == 0 && (r = snap_copy(ctx, A.ckpt[size_t(s / K)], true))) return r;
-    if (s >= last0 && (r = snap_copy(ctx, A.segbuf[size_t(s - last0)], true))) return r;
+    if (s > last0 && (r = snap_copy(ctx, A.segbuf[size_t(s - last0)], true))) return r;
     if ((r = run_steps(ctx, 1))) return r;
     if (s >= last0 && (r = grid_copy(ctx, A.gridbuf[size_t(s - last0)], true))) return r;
     if (s >= last0 && (r = bins_copy(ctx, A.binbuf[size_t(s - last0)], true))) return r;
@@ -2422,10 +2394,12 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
   const int64_t nseg = (A.nsteps + A.K - 1) / A.K;
   for (int64_t seg = nseg - 1; seg >= 0 && !r; seg--) {
     const int64_t s0 = seg * A.K, s1 = std::min<int64_t>(s0 + A.K, A.nsteps);
+    // the state step s0 + q read: the segment's checkpoint for its first step
+    auto seg_state = [&](size_t q) { return q == 0 ? A.ckpt[size_t(seg)] : A.segbuf[q]; };
     if (seg != A.cached_seg) {
       if ((r = snap_copy(ctx, A.ckpt[size_t(seg)], false))) break;
       for (int64_t j = s0; j < s1 && !r; j++) {  // rebuild the segment: states and the grids their steps used
-        if ((r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
+        if (j > s0 && (r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
         if ((r = run_steps(ctx, 1))) break;
         if ((r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], true))) break;
         r = bins_copy(ctx, A.binbuf[size_t(j - s0)], true);
@@ -2434,7 +2408,7 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
     }
     for (int64_t j = s1 - 1; j >= s0 && !r; j--) {
       const size_t q = size_t(j - s0);
-      const AdjStep st = adj_step_view(ctx, A.segbuf[q], A.gridbuf[q], A.binbuf[q]);
+      const AdjStep st = adj_step_view(ctx, seg_state(q), A.gridbuf[q], A.binbuf[q]);
       if (D == 2) {
         if (ctx->prm.basis == JUMP_BASIS_LINEAR) adj_step_back<JUMP_BASIS_LINEAR>(ctx, st);
         else adj_step_back<JUMP_BASIS_GIMP>(ctx, st);

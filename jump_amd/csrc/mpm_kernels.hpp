@@ -1894,389 +1894,141 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_musl2d(Particles P, GridDesc 
   }
 }
 
-// Conjugate gradients for B X = RHS preconditioned with the lumped mass
-// diag(m) (1 at massless nodes, whose rows are the identity), four
-// right-hand sides at once.  With non-negative shape functions
-// 0.1 diag(m) <= B <= diag(m) (x.Mx = sum_p m_p (N_p.x)^2 <= sum_I m_I x_I^2
-// since the N_p sum to one), so the preconditioned condition number is at
-// most 10 whatever the particle layout: about 40-50 iterations to 1e-14.
-// One iteration is three launches over the whole device:
-//   k_cg_spmv     z = D^-1 r, w = B z, per-block partial sums of (r, z),
-//                 (w, z), (r, r) for each column
-//   k_cg_scalars  one workgroup: the partials in a fixed order, the
-//                 convergence test |r| <= tol |b| per column, alpha, beta
-//   k_cg_update   p = z + beta p, s = w + beta s, x += alpha p, r -= alpha s
-// (the single-reduction form of Chronopoulos and Gear, 1989: s = B p is
-// carried, not recomputed).  Every sum has a fixed order, so the solve is
-// bitwise reproducible.  Once every column has converged the kernels return
-// at once; the host stops launching when it sees that.
-struct CgState {
-  double alpha[4], beta[4], gamma[4], bb[4];
-  int done[4];
-  int all_done;  // every column converged (or had a zero right-hand side)
-  int iters;     // iterations that updated at least one column
-};
-// r, s, w, part and st hold two slots each (the fused iteration, k_cg_iter,
-// reads one and writes the other); the three-launch iteration uses slot 0.
-struct CgArrays {
-  double* dinv;  // the preconditioner: 1/m, 1 at massless nodes (nn)
-  double* x;     // four columns of nn each, likewise below
-  double* r;     // slot k at r + 4 nn k
-  double* p;
-  double* s;     // slot k at s + 4 nn k
-  double* w;     // slot k at w + 4 nn k
-  double* part;  // [slot][block][12]: (r,z) x4, (w,z) x4, (r,r) x4
-  CgState* st;   // [slot]
-};
-constexpr int kCgBlock = 256;
-// The SpMV works on 32 x 8 node tiles, staging z = D^-1 r over the tile and
-// its R-wide halo in LDS (each z then feeds up to (2R+1)^2 rows).
-constexpr int kCgTX = 32, kCgTY = 8;
-static_assert(kCgTX * kCgTY == kCgBlock, "one node per thread");
-__host__ __device__ inline int cg_tiles(const GridDesc& G) {
-  return ((G.nx + kCgTX - 1) / kCgTX) * ((G.ny + kCgTY - 1) / kCgTY);
+// The blended mass matrix B X = RHS, four right-hand sides at once, is
+// solved preconditioned with the lumped mass D = diag(m) (1 at massless
+// nodes, whose rows are the identity).  With non-negative shape functions
+// that sum to at most one, 0.1 D <= B <= D: x.Mx = sum_p m_p (N_p.x)^2
+// <= sum_p m_p sum_I N_I x_I^2 = sum_I m_I x_I^2 (Cauchy-Schwarz) and M is
+// positive semi-definite, so the spectrum of D^-1 B lies in [0.1, 1]
+// whatever the particle layout.
+constexpr int kChebBlock = 256;
+// The iteration works on 32 x 8 node tiles, staging d over the tile and its
+// halo in LDS (each d then feeds up to (2R+1)^2 rows).
+constexpr int kChebTX = 32, kChebTY = 8;
+static_assert(kChebTX * kChebTY == kChebBlock, "one node per thread");
+__host__ __device__ inline int cheb_tiles(const GridDesc& G) {
+  return ((G.nx + kChebTX - 1) / kChebTX) * ((G.ny + kChebTY - 1) / kChebTY);
 }
 
-__global__ void __launch_bounds__(kBlock) k_cg_init(int nn, Musl2Arrays mu, const double* __restrict__ gm,
-                                                    CgArrays cg) {
+// Chebyshev iteration (Saad, Iterative Methods for
+// Sparse Linear Systems, 2nd ed., Algorithm 12.1, with the preconditioner
+// D = diag(m)).  The spectrum of D^-1 B lies in [0.1, 1] (above), so with
+// theta = 0.55, delta = 0.45, sigma = theta / delta:
+//   x0 = 0, r0 = b, d0 = D^-1 r0 / theta, rho0 = 1 / sigma
+//   x_k+1 = x_k + d_k,  r_k+1 = r_k - B d_k,  rho_k+1 = 1 / (2 sigma - rho_k)
+//   d_k+1 = rho_k+1 rho_k d_k + (2 rho_k+1 / delta) D^-1 r_k+1
+// and the B-norm of the error after k iterations is at most
+// 2 q^k / (1 + q^2k) of the first, q = (sqrt 10 - 1) / (sqrt 10 + 1), for
+// any particle layout.  The iteration count follows from that bound and
+// the coefficients are formed on the host, so an iteration needs no inner
+// product and no read-back: it is one launch with no reduction.  r and d
+// alternate between two slots per launch, so no workgroup reads what
+// another writes in the same launch; x is read and written at its owner's
+// nodes only.
+struct ChebArrays {
+  double* dinv;  // 1/m, 1 at massless nodes (nn)
+  double* x;     // four columns of nn each
+  double* r;     // slot k at r + 4 nn k, likewise d
+  double* d;
+};
+__global__ void __launch_bounds__(kBlock) k_cheb_init(int nn, Musl2Arrays mu, const double* __restrict__ gm,
+                                                      ChebArrays ch, double itheta) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    CgState z{};
-    *cg.st = z;
-  }
-  if (i == 1) {
-    CgState z{};
-    cg.st[1] = z;
-  }
   if (i >= nn) return;
   const double mi = gm[i];
-  cg.dinv[i] = mi > 0.0 ? 1.0 / mi : 1.0;
-  const size_t n4 = 4 * size_t(nn);
+  const double di = mi > 0.0 ? 1.0 / mi : 1.0;
+  ch.dinv[i] = di;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const size_t k = size_t(c) * nn + i;
-    cg.x[k] = 0.0;
-    cg.r[k] = mu.rhs[k];
-    cg.r[n4 + k] = mu.rhs[k];
-    cg.p[k] = 0.0;
-    cg.s[k] = 0.0;
-    cg.s[n4 + k] = 0.0;
+    const double b = mu.rhs[k];
+    ch.x[k] = 0.0;
+    ch.r[k] = b;  // slot 0
+    ch.d[k] = b * di * itheta;
   }
 }
 
-template <int R>
-__global__ void __launch_bounds__(kCgBlock) k_cg_spmv(GridDesc G, Musl2Arrays mu, CgArrays cg) {
+// (B v)_i for the four columns of v at node (ix, iy), v held in LDS with
+// the node at (lx, ly).  The row's (2R+1)^2 coefficients are all loaded
+// first (every row is stored whole), so their loads are in flight together;
+// the sum then takes them in a fixed order (b outer, a inner), skipping
+// neighbours outside the grid.
+template <int R, int LX, int LY>
+__device__ __forceinline__ void cheb_spmv(const GridDesc& G, const Musl2Arrays& mu, const double (*v)[LY][LX], int ix,
+                                          int iy, int lx, int ly, double* y) {
   constexpr int K1 = 2 * R + 1;
-  constexpr int NW = kCgBlock / 64;
-  constexpr int LX = kCgTX + 2 * R, LY = kCgTY + 2 * R;
-  __shared__ double zs[4][LY][LX];
-  __shared__ double red[12][NW];
-  if (cg.st->all_done) return;  // uniform: every workgroup returns
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nn = G.nn, i = ix + G.nx * iy;
+  double cf[K1 * K1];
+#pragma unroll
+  for (int k = 0; k < K1 * K1; k++) cf[k] = mu.brow[size_t(k) * nn + i];
+#pragma unroll
+  for (int c = 0; c < 4; c++) y[c] = 0.0;
+#pragma unroll
+  for (int b = -R; b <= R; b++) {
+    if (iy + b < 0 || iy + b >= G.ny) continue;
+#pragma unroll
+    for (int a = -R; a <= R; a++) {
+      if (ix + a < 0 || ix + a >= G.nx) continue;
+      const double w = cf[(a + R) + K1 * (b + R)];
+#pragma unroll
+      for (int c = 0; c < 4; c++) y[c] += w * v[c][ly + b][lx + a];
+    }
+  }
+}
+
+// One iteration on the workgroup's tile: B d_k with d_k staged over the tile
+// and its R-wide halo, then x, r_k+1 and d_k+1 = c1 d_k + c2 D^-1 r_k+1.
+// Reads slot `slot` of r and d, writes the other.  The last iteration writes
+// the nodal velocities instead: mv = (x0, x1, x0 + x2, x1 + x3).
+template <int R>
+__global__ void __launch_bounds__(kChebBlock) k_cheb_iter(GridDesc G, Musl2Arrays mu, ChebArrays ch, int slot,
+                                                        double c1, double c2, int last) {
+  constexpr int LX = kChebTX + 2 * R, LY = kChebTY + 2 * R;
+  __shared__ double ds[4][LY][LX];
+  const int t = threadIdx.x;
   const int nn = G.nn;
-  const int ntx = (G.nx + kCgTX - 1) / kCgTX;
-  const int x0 = int(blockIdx.x % unsigned(ntx)) * kCgTX, y0 = int(blockIdx.x / unsigned(ntx)) * kCgTY;
-  // z over the tile and its halo (zero outside the grid; never read there)
-  for (int e = t; e < LX * LY; e += kCgBlock) {
+  const size_t n4 = 4 * size_t(nn);
+  const double* d_o = ch.d + size_t(slot) * n4;
+  const double* r_o = ch.r + size_t(slot) * n4;
+  double* d_n = ch.d + size_t(slot ^ 1) * n4;
+  double* r_n = ch.r + size_t(slot ^ 1) * n4;
+  const int ntx = (G.nx + kChebTX - 1) / kChebTX;
+  const int x0 = int(blockIdx.x % unsigned(ntx)) * kChebTX, y0 = int(blockIdx.x / unsigned(ntx)) * kChebTY;
+  for (int e = t; e < LX * LY; e += kChebBlock) {
     const int lx = e % LX, ly = e / LX;
     const int gx = x0 + lx - R, gy = y0 + ly - R;
     const bool in = gx >= 0 && gx < G.nx && gy >= 0 && gy < G.ny;
     const int j = in ? gx + G.nx * gy : 0;
-    const double dj = in ? cg.dinv[j] : 0.0;
 #pragma unroll
-    for (int c = 0; c < 4; c++) zs[c][ly][lx] = in ? cg.r[size_t(c) * nn + j] * dj : 0.0;
+    for (int c = 0; c < 4; c++) ds[c][ly][lx] = in ? d_o[size_t(c) * nn + j] : 0.0;
   }
   __syncthreads();
-  const int tx = t % kCgTX, ty = t / kCgTX;
+  const int tx = t % kChebTX, ty = t / kChebTX;
   const int ix = x0 + tx, iy = y0 + ty;
-  double acc[12];
-#pragma unroll
-  for (int c = 0; c < 12; c++) acc[c] = 0.0;
-  if (ix < G.nx && iy < G.ny) {
-    const int i = ix + G.nx * iy;
-    double y[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int b = -R; b <= R; b++) {
-      if (iy + b < 0 || iy + b >= G.ny) continue;
-#pragma unroll
-      for (int a = -R; a <= R; a++) {
-        if (ix + a < 0 || ix + a >= G.nx) continue;
-        const double cf = mu.brow[size_t((a + R) + K1 * (b + R)) * nn + i];
-#pragma unroll
-        for (int c = 0; c < 4; c++) y[c] += cf * zs[c][ty + b + R][tx + a + R];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const double ri = cg.r[size_t(c) * nn + i];
-      const double zi = zs[c][ty + R][tx + R];
-      cg.w[size_t(c) * nn + i] = y[c];
-      acc[c] = ri * zi;
-      acc[4 + c] = y[c] * zi;
-      acc[8 + c] = ri * ri;
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 12; c++) {
-    double v = acc[c];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if (lane == 0) red[c][wv] = v;
-  }
-  __syncthreads();
-  if (t < 12) {
-    double v = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; w++) v += red[t][w];
-    cg.part[size_t(blockIdx.x) * 12 + t] = v;
-  }
-}
-
-// The twelve totals of the blocks' partial sums, in a fixed order (every
-// workgroup that calls it gets the same bits): per thread a strided sum, per
-// wave a shuffle tree, then the waves in order.  Ends with a barrier.
-__device__ __forceinline__ void cg_block_totals(const double* __restrict__ part, int nblk,
-                                                double (*red)[kCgBlock / 64], double* tot) {
-  constexpr int NW = kCgBlock / 64;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  double acc[12];
-#pragma unroll
-  for (int c = 0; c < 12; c++) acc[c] = 0.0;
-  for (int b = t; b < nblk; b += kCgBlock)
-#pragma unroll
-    for (int c = 0; c < 12; c++) acc[c] += part[size_t(b) * 12 + c];
-#pragma unroll
-  for (int c = 0; c < 12; c++) {
-    double v = acc[c];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if (lane == 0) red[c][wv] = v;
-  }
-  __syncthreads();
-  if (t < 12) {
-    double v = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; w++) v += red[t][w];
-    tot[t] = v;
-  }
-  __syncthreads();
-}
-// Iteration it's scalars from the totals: the stopping test per column,
-// alpha and beta (Chronopoulos-Gear), the history in st.
-__device__ __forceinline__ void cg_scalars_apply(const double* tot, int it, double tol, CgState& st) {
-  bool all = true, any = false;
+  if (ix >= G.nx || iy >= G.ny) return;
+  const int i = ix + G.nx * iy;
+  double y[4];
+  cheb_spmv<R, LX, LY>(G, mu, ds, ix, iy, tx + R, ty + R, y);
+  const double di = ch.dinv[i];
+  double xn[4];
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    const double gam = tot[c], del = tot[4 + c], rr = tot[8 + c];
-    if (it == 0) {
-      st.bb[c] = rr;
-      st.done[c] = rr == 0.0;
-    } else if (!st.done[c]) {
-      st.done[c] = rr <= tol * tol * st.bb[c];
-    }
-    if (st.done[c]) {
-      st.alpha[c] = 0.0;
-      st.beta[c] = 0.0;
-      continue;
-    }
-    all = false;
-    any = true;
-    const double be = it == 0 ? 0.0 : gam / st.gamma[c];
-    const double al = it == 0 ? gam / del : gam / (del - be * gam / st.alpha[c]);
-    st.beta[c] = be;
-    st.alpha[c] = al;
-    st.gamma[c] = gam;
-  }
-  if (any) st.iters = it + 1;
-  st.all_done = all ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kCgBlock) k_cg_scalars(CgArrays cg, int nblk, int it, double tol) {
-  constexpr int NW = kCgBlock / 64;
-  __shared__ double red[12][NW];
-  __shared__ double tot[12];
-  CgState& st = *cg.st;
-  if (st.all_done) return;
-  cg_block_totals(cg.part, nblk, red, tot);
-  if (threadIdx.x == 0) cg_scalars_apply(tot, it, tol, st);
-}
-
-__global__ void __launch_bounds__(kCgBlock) k_cg_update(int nn, CgArrays cg) {
-  const CgState& st = *cg.st;
-  if (st.all_done) return;
-  const int i = blockIdx.x * kCgBlock + threadIdx.x;
-  if (i >= nn) return;
-  const double di = cg.dinv[i];
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    if (st.done[c]) continue;
-    const size_t k = size_t(c) * nn + i;
-    const double al = st.alpha[c], be = st.beta[c];
-    const double ri = cg.r[k];
-    const double pk = ri * di + be * cg.p[k];
-    const double sk = cg.w[k] + be * cg.s[k];
-    cg.p[k] = pk;
-    cg.s[k] = sk;
-    cg.x[k] += al * pk;
-    cg.r[k] = ri - al * sk;
-  }
-}
-
-// One CG iteration in one launch: iteration m-1's scalars (every workgroup
-// reduces the partials of iteration m-1's SpMV in the same order and forms
-// the same alpha and beta; workgroup 0 stores the state), iteration m-1's
-// update of its own nodes (p and x in place, r and s into the other slot),
-// and iteration m's SpMV on the updated residual, whose halo values each
-// workgroup forms itself with the owner's expression.  r, s, w, the
-// partials and the state alternate between two slots by the parity of m,
-// so no workgroup reads what another writes in the same launch.  Every
-// value is formed as k_cg_spmv, k_cg_scalars and k_cg_update form it: the
-// iterates are bitwise those of the three-launch iteration.  Each workgroup
-// reads all nblk partials, so this pays while nblk is small (kCgFusedTiles).
-constexpr int kCgFusedTiles = 1024;
-template <int R>
-__global__ void __launch_bounds__(kCgBlock) k_cg_iter(GridDesc G, Musl2Arrays mu, CgArrays cg, int m, int nblk,
-                                                      double tol) {
-  constexpr int K1 = 2 * R + 1;
-  constexpr int NW = kCgBlock / 64;
-  constexpr int LX = kCgTX + 2 * R, LY = kCgTY + 2 * R;
-  __shared__ double zs[4][LY][LX];
-  __shared__ double red[12][NW];
-  __shared__ double tot[12];
-  __shared__ CgState ns;
-  const int a = m & 1, b = a ^ 1;  // the slots written and read
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const CgState* so = cg.st + b;
-  if (so->all_done) {  // converged earlier: carry the state to this slot
-    if (blockIdx.x == 0 && t == 0) cg.st[a] = *so;
-    return;
-  }
-  const int nn = G.nn;
-  const size_t n4 = 4 * size_t(nn);
-  const double* r_o = cg.r + b * n4;
-  const double* s_o = cg.s + b * n4;
-  const double* w_o = cg.w + b * n4;
-  double* r_n = cg.r + a * n4;
-  double* s_n = cg.s + a * n4;
-  double* w_n = cg.w + a * n4;
-  const bool upd = m > 0;
-  if (upd) {
-    cg_block_totals(cg.part + size_t(b) * nblk * 12, nblk, red, tot);
-    if (t == 0) {
-      ns = *so;
-      cg_scalars_apply(tot, m - 1, tol, ns);
-      if (blockIdx.x == 0) cg.st[a] = ns;
-    }
-  } else if (t == 0) {
-    ns = *so;
-    if (blockIdx.x == 0) cg.st[a] = ns;
-  }
-  __syncthreads();
-  if (ns.all_done) return;  // the same in every workgroup
-  const int ntx = (G.nx + kCgTX - 1) / kCgTX;
-  const int x0 = int(blockIdx.x % unsigned(ntx)) * kCgTX, y0 = int(blockIdx.x / unsigned(ntx)) * kCgTY;
-  const int tx = t % kCgTX, ty = t / kCgTX;
-  const int ix = x0 + tx, iy = y0 + ty;
-  const bool own = ix < G.nx && iy < G.ny;
-  const int i = own ? ix + G.nx * iy : 0;
-  // iteration m-1's update of this workgroup's nodes (k_cg_update)
-  double rnew[4] = {0.0, 0.0, 0.0, 0.0};
-  if (own) {
-    const double di = cg.dinv[i];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const size_t k = size_t(c) * nn + i;
-      const double ri = r_o[k];
-      if (upd && !ns.done[c]) {
-        const double al = ns.alpha[c], be = ns.beta[c];
-        const double pk = ri * di + be * cg.p[k];
-        const double sk = w_o[k] + be * s_o[k];
-        cg.p[k] = pk;
-        s_n[k] = sk;
-        cg.x[k] += al * pk;
-        rnew[c] = ri - al * sk;
-      } else {
-        s_n[k] = s_o[k];
-        rnew[c] = ri;
-      }
-      r_n[k] = rnew[c];
+    const size_t q = size_t(c) * nn + i;
+    const double dk = ds[c][ty + R][tx + R];
+    xn[c] = ch.x[q] + dk;
+    const double rn = r_o[q] - y[c];
+    if (!last) {
+      ch.x[q] = xn[c];
+      r_n[q] = rn;
+      d_n[q] = c1 * dk + c2 * (rn * di);
     }
   }
-  // z = D^-1 r over the tile and its halo, r updated as its owner updates it
-  for (int e = t; e < LX * LY; e += kCgBlock) {
-    const int lx = e % LX, ly = e / LX;
-    const int gx = x0 + lx - R, gy = y0 + ly - R;
-    const bool in = gx >= 0 && gx < G.nx && gy >= 0 && gy < G.ny;
-    const int j = in ? gx + G.nx * gy : 0;
-    const double dj = in ? cg.dinv[j] : 0.0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      double rj = 0.0;
-      if (in) {
-        const size_t k = size_t(c) * nn + j;
-        rj = r_o[k];
-        if (upd && !ns.done[c]) rj = rj - ns.alpha[c] * (w_o[k] + ns.beta[c] * s_o[k]);
-      }
-      zs[c][ly][lx] = in ? rj * dj : 0.0;
-    }
+  if (last) {
+    mu.mv[i] = xn[0];
+    mu.mv[size_t(nn) + i] = xn[1];
+    mu.mv[2 * size_t(nn) + i] = xn[0] + xn[2];
+    mu.mv[3 * size_t(nn) + i] = xn[1] + xn[3];
   }
-  __syncthreads();
-  // iteration m's SpMV and partial sums (k_cg_spmv)
-  double acc[12];
-#pragma unroll
-  for (int c = 0; c < 12; c++) acc[c] = 0.0;
-  if (own) {
-    double y[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int bb = -R; bb <= R; bb++) {
-      if (iy + bb < 0 || iy + bb >= G.ny) continue;
-#pragma unroll
-      for (int aa = -R; aa <= R; aa++) {
-        if (ix + aa < 0 || ix + aa >= G.nx) continue;
-        const double cf = mu.brow[size_t((aa + R) + K1 * (bb + R)) * nn + i];
-#pragma unroll
-        for (int c = 0; c < 4; c++) y[c] += cf * zs[c][ty + bb + R][tx + aa + R];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const double ri = rnew[c];
-      const double zi = zs[c][ty + R][tx + R];
-      w_n[size_t(c) * nn + i] = y[c];
-      acc[c] = ri * zi;
-      acc[4 + c] = y[c] * zi;
-      acc[8 + c] = ri * ri;
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 12; c++) {
-    double v = acc[c];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if (lane == 0) red[c][wv] = v;
-  }
-  __syncthreads();
-  if (t < 12) {
-    double v = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; w++) v += red[t][w];
-    cg.part[(size_t(a) * nblk + blockIdx.x) * 12 + t] = v;
-  }
-}
-
-// mv = (x0, x1, x0 + x2, x1 + x3); the solver flag if a column did not
-// converge.
-// st: the final state's slot.
-__global__ void __launch_bounds__(kBlock) k_cg_finish(int nn, Musl2Arrays mu, CgArrays cg, const CgState* st,
-                                                      unsigned* __restrict__ flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0 && st->all_done != 1) atomicOr(flags, JUMP_FLAG_SOLVER);
-  if (i >= nn) return;
-  const double x0 = cg.x[i], x1 = cg.x[size_t(nn) + i];
-  mu.mv[i] = x0;
-  mu.mv[size_t(nn) + i] = x1;
-  mu.mv[2 * size_t(nn) + i] = x0 + cg.x[2 * size_t(nn) + i];
-  mu.mv[3 * size_t(nn) + i] = x1 + cg.x[3 * size_t(nn) + i];
 }
 
 // G2P-1 (getShapeFunctions, :203-226): v = alpha v + sum N (v_I - alpha v0_I)

@@ -2,9 +2,9 @@
 (SolveExplicitSolidMechanics2D.jl:25-329) against the oracle.
 
 The oracle solves B v = p with a dense LU as the reference does; the device
-solves the same system with conjugate gradients to a relative residual of
-1e-14, so the comparison is to a relative 1e-10 of each field's scale
-(not bitwise).  Beyond the oracle's 4096-node limit the checks are the
+solves the same system by the Chebyshev iteration with its error bounded by
+1e-14 of the solution, so the comparison is to a relative 1e-10 of each
+field's scale (not bitwise).  Beyond the oracle's 4096-node limit the checks are the
 size-independent closed forms: a uniformly translating stress-free body
 keeps its velocity and moves by dt v, and a uniform body force adds dt g."""
 import os
@@ -188,10 +188,10 @@ def test_musl2d_cpdi_tubes_stretching(hip_lib):
 @pytest.mark.parametrize("basis", [_ffi.BASIS_LINEAR, _ffi.BASIS_QUAD_BSPLINE, _ffi.BASIS_CUBIC_BSPLINE])
 def test_musl2d_large_grid_closed_forms(hip_lib, basis, cells):
     """A 301 x 301 grid (90601 nodes, beyond the dense restatement) and a
-    601 x 601 grid (1444 node tiles: the solve takes three launches per
-    iteration there, one below 1024 tiles): a stress-free block translating
-    under gravity gains exactly dt g per step at every particle, whatever
-    the FLIP fraction."""
+    601 x 601 grid (1444 node tiles, so tiles of every kind: interior,
+    right and top edges): a stress-free block translating under gravity
+    gains exactly dt g per step at every particle, whatever the FLIP
+    fraction."""
     _say(f"musl2d large grid basis={basis} cells={cells}")
     dx = 1.0 / cells
     f = dx / 2
