@@ -316,6 +316,35 @@ __device__ __forceinline__ void node_weight(const Particles& P, const GridDesc& 
   }
 }
 
+// The y and z axis weights of the last particle a lane weighed against its
+// node (3D force pass replay).  A replayed run is consecutive particles of
+// one lattice row, whose y and z are equal, and the lanes of a wave (nodes
+// of a grid row) walk their runs in step.
+struct YZCache {
+  double y = __builtin_nan(""), z = __builtin_nan("");  // NaN: nothing cached
+  double wy = 0.0, dwy = 0.0, wz = 0.0, dwz = 0.0;
+};
+// node_weight for D = 3 (not CPDI, not GIMP: lp is not compared), reusing
+// the cached y and z weights when every active lane of the wave has the
+// particle y and z it cached; otherwise every lane forms them again.
+// Either way each value is the one node_weight forms from the same inputs.
+template <int B, bool QUIRK, bool INTERIOR>
+__device__ __forceinline__ void node_weight_yz(const GridDesc& G, const double* lp, const double* xp, int ii, int jj,
+                                               int kk, double& N, double* dN, YZCache& c) {
+  static_assert(B != JUMP_BASIS_CPDI_Q4 && B != JUMP_BASIS_GIMP, "weights that depend only on the position");
+  const Axis X = axis_of(G, 0), Y = axis_of(G, 1), Z = axis_of(G, 2);
+  double wx, dwx;
+  axis_weight<B, QUIRK, true, INTERIOR>(xp[0], 0, X, lp[0], ii, ii, wx, dwx);
+  if (!__all(xp[1] == c.y && xp[2] == c.z)) {
+    const int sel4 = (QUIRK && B == JUMP_BASIS_CUBIC_BSPLINE) ? jj : kk;
+    axis_weight<B, QUIRK, true, INTERIOR>(xp[1], 1, Y, lp[1], jj, jj, c.wy, c.dwy);
+    axis_weight<B, QUIRK, true, INTERIOR>(xp[2], 2, Z, lp[2], kk, sel4, c.wz, c.dwz);
+    c.y = xp[1];
+    c.z = xp[2];
+  }
+  combine3<B>(X, Y, Z, wx, dwx, c.wy, c.dwy, c.wz, c.dwz, N, dN);
+}
+
 // Largest stencil window per axis: the basis width, or for CPDI the widest
 // corner bounding box (checked on the host).
 template <int B> struct MaxW { static constexpr int W = Width<B>::W; };
@@ -349,18 +378,21 @@ template <int D, int PASS> struct P2GFields {
 // v: the particle's fields as P2GFields loads them.
 // AFF: the node's grid may be an APIC step's (single 2D problems; batches
 // refuse APIC), read from G.apic.
-template <int D, int B, int PASS, bool AFF = false, bool INTERIOR = false>
+// yz: the lane's cached y and z weights (node_weight_yz), or null.
+template <int D, int B, int PASS, bool AFF = false, bool INTERIOR = false, bool YZ = false>
 __device__ __forceinline__ void p2g_add_v(const Particles& P, const GridDesc& G, const SolidDev* S,
                                           const double* corners, int p, const double* v, int ni, int nj, int nk,
-                                          double& am, double* ap, double* af) {
+                                          double& am, double* ap, double* af, YZCache* yz = nullptr) {
   double lp[3] = {0.0, 0.0, 0.0};
   if (B == JUMP_BASIS_GIMP) {
     const SolidDev& s = S[P.sid[p]];
     for (int k = 0; k < D; k++) lp[k] = s.lp[k];
   }
   double N, dN[3];
-  bool member;
-  node_weight<D, B, PASS == 1, PASS == 0, INTERIOR>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
+  bool member = true;
+  if constexpr (YZ) node_weight_yz<B, PASS == 1, INTERIOR>(G, lp, v, ni + 1, nj + 1, nk + 1, N, dN, *yz);
+  else node_weight<D, B, PASS == 1, PASS == 0, INTERIOR>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN,
+                                                         member);
   if (!member) return;
   const double fM = v[D];
   if (PASS == 0) {
@@ -434,6 +466,9 @@ constexpr int kP2GBlock = 128;
 // (p2g_block's LPN): the force pass and the re-projection.
 constexpr int kP2GLanes0 = 1;
 constexpr int kP2GLanes1 = 2;
+// The 3D replays (the force pass and the re-projection) reuse a lane's y
+// and z weights across a run (YZCache, node_weight_yz).
+constexpr bool kYZCache = true;
 // Node order of the single 2D gathers: the active nodes compacted (true) or
 // grid rows, for the contact step and the plain USL step.
 // Measured: the two-rings USL gather (1025 x 513 nodes, thick rings) 0.609
@@ -625,10 +660,10 @@ template <bool V> struct BoolTag { static constexpr bool value = V; };
 // gives the same bits: PASS 0 t = (N m, N m v, force term (subtracted)),
 // PASS 1 t = N m v.
 template <int B, int PASS> struct Terms3 { static constexpr int NT = PASS == 0 ? 7 : 3; };
-template <int B, int PASS, bool INTERIOR = false>
+template <int B, int PASS, bool INTERIOR = false, bool YZ = false>
 __device__ __forceinline__ void p2g_terms3(const Particles& P, const GridDesc& G, const SolidDev* S,
                                            const double* corners, int p, const double* v, int ni, int nj, int nk,
-                                           double* t) {
+                                           double* t, YZCache* yz = nullptr) {
   double lp[3] = {0.0, 0.0, 0.0};
   if (B == JUMP_BASIS_GIMP) {
     const SolidDev& s = S[P.sid[p]];
@@ -636,7 +671,9 @@ __device__ __forceinline__ void p2g_terms3(const Particles& P, const GridDesc& G
   }
   double N, dN[3];
   bool member;
-  node_weight<3, B, PASS == 1, PASS == 0, INTERIOR>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN, member);
+  if constexpr (YZ) node_weight_yz<B, PASS == 1, INTERIOR>(G, lp, v, ni + 1, nj + 1, nk + 1, N, dN, *yz);
+  else node_weight<3, B, PASS == 1, PASS == 0, INTERIOR>(P, G, lp, corners, p, v, ni + 1, nj + 1, nk + 1, N, dN,
+                                                         member);
   const double fM = v[3];
   if (PASS == 0) {
     const double fV = v[4];
@@ -796,7 +833,10 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
     constexpr bool INT = decltype(inner)::value;
     if constexpr (LPN == 2 && PASS == 1) {
       // this lane's particles of each run, the next one's fields (and the
-      // next run's entry) loaded while the current one is summed
+      // next run's entry) loaded while the current one is summed; the y and
+      // z weights of a run's row formed once (kYZCache)
+      constexpr bool YZ = kYZCache && B != JUMP_BASIS_GIMP && B != JUMP_BASIS_CPDI_Q4;
+      YZCache yz;
       unsigned e = nr > 0 ? rl.runs[n] : 0u;
       for (int r = 0; r < nr; r++) {
         const int p0 = int(e >> 3), len = int(e & 7u) + 1;
@@ -808,7 +848,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
           double tt[Terms3<B, PASS>::NT];
           if (mine < len) {
             if (mine + 2 < len) P2GFields<D, PASS>::load(P, unsigned(p0 + mine + 2), w);
-            p2g_terms3<B, PASS, INT>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt);
+            p2g_terms3<B, PASS, INT, YZ>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt, &yz);
             if (mine + 2 < len)
 #pragma unroll
               for (int q = 0; q < NF + 1; q++) v[q] = w[q];
@@ -855,6 +895,10 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
       return;
     } else {
       if constexpr (PASS == 0) {
+        // 3D, weights of the position alone: the y and z weights of a run's
+        // row are formed once (kYZCache)
+        constexpr bool YZ = kYZCache && D == 3 && B != JUMP_BASIS_GIMP && B != JUMP_BASIS_CPDI_Q4;
+        YZCache yz;
         for (int r = 0; r < nr; r++) {
           const unsigned e = rl.runs[size_t(r) * NN + n];
           const int p0 = int(e >> 3), len = int(e & 7u) + 1;
@@ -867,7 +911,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
               a = ldu(P.pr[0], unsigned(p0 + k + 1));
               b = ldu(P.pr[1], unsigned(p0 + k + 1));
             }
-            p2g_add_v<D, B, PASS, AFF, INT>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af);
+            p2g_add_v<D, B, PASS, AFF, INT, YZ>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af, &yz);
           }
         }
         return;
