@@ -1986,7 +1986,8 @@ static int copy_segments(jump_ctx* ctx, char* buf, const std::vector<std::pair<v
   }
   if (segs.empty() || most == 0) return JUMP_OK;
   const unsigned bx = unsigned(std::min<long long>((most / 4 + kBlock) / kBlock, 1024));
-  hipLaunchKernelGGL(k_copy_table, dim3(bx, unsigned(segs.size())), dim3(kBlock), 0, ctx->stream, t);
+  if (save) hipLaunchKernelGGL(k_copy_table<true>, dim3(bx, unsigned(segs.size())), dim3(kBlock), 0, ctx->stream, t);
+  else hipLaunchKernelGGL(k_copy_table<false>, dim3(bx, unsigned(segs.size())), dim3(kBlock), 0, ctx->stream, t);
   return JUMP_OK;
 }
 

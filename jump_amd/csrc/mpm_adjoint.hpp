@@ -411,16 +411,25 @@ struct CopyTable {
   unsigned* dst[kCopySegs];
   long long words[kCopySegs];
 };
+// SAVE: the copy goes into a record, which is read back only in a later
+// backward sweep, so it is written with non-temporal stores and leaves the
+// caches to the particle state the next step reads.
+constexpr bool kCopyNT = true;
+template <bool SAVE>
 __global__ void __launch_bounds__(kBlock) k_copy_table(CopyTable t) {
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
   const int sg = blockIdx.y;
   const unsigned* __restrict__ src = t.src[sg];
   unsigned* __restrict__ dst = t.dst[sg];
   const long long n = t.words[sg], n4 = n >> 2;
   const long long stride = (long long)gridDim.x * blockDim.x;
   const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint4* __restrict__ s4 = reinterpret_cast<const uint4*>(src);
-  uint4* __restrict__ d4 = reinterpret_cast<uint4*>(dst);
-  for (long long i = i0; i < n4; i += stride) d4[i] = s4[i];
+  const u32x4* __restrict__ s4 = reinterpret_cast<const u32x4*>(src);
+  u32x4* __restrict__ d4 = reinterpret_cast<u32x4*>(dst);
+  for (long long i = i0; i < n4; i += stride) {
+    if (SAVE && kCopyNT) __builtin_nontemporal_store(s4[i], &d4[i]);
+    else d4[i] = s4[i];
+  }
   const long long tail = 4 * n4 + i0;  // the last n % 4 words
   if (tail < n) dst[tail] = src[tail];
 }
