@@ -66,6 +66,7 @@ struct jump_ctx {
   Musl2Arrays mu{};      // 2D MUSL only: matrix rows, right-hand sides, velocities
   ChebArrays ch{};       // 2D MUSL only: the Chebyshev iteration's vectors
   RunList runs{};        // 3D: the force pass's particle runs per node, replayed by the re-projection
+  const Particles* Pout = nullptr;  // 2D USL: where the step writes the state after it (null: in place)
   int* d_dirty = nullptr;  // 3D: per node, the stamp of the step its window last changed
   int runs_gen = 0;        // stamp of the last step
   bool runs_valid = false; // the recorded runs match the particles' current bins
@@ -613,11 +614,17 @@ template <int B, int ORDER, bool MULTI> static void step_usl2d(jump_ctx* ctx) {
     LAUNCH((k_rigid_override<MULTI>), np, ctx->P, ctx->G, ctx->d_solids, r, ctx->g, ctx->d_flags, ctx->bt());
   prof_end(ctx, JUMP_PHASE_RIGID);
   prof_begin(ctx);
-  LAUNCH((k_g2p_usl2d<B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.tolerance, dt,
-         ctx->d_flags, ctx->bt());
+  const bool oop = !MULTI && ctx->Pout;  // the reverse mode's records (single problems)
+  const Particles& Po = oop ? *ctx->Pout : ctx->P;
+  if (oop)
+    LAUNCH((k_g2p_usl2d<B, false, true>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g,
+           ctx->prm.tolerance, dt, ctx->d_flags, ctx->bt(), Po);
+  else
+    LAUNCH((k_g2p_usl2d<B, MULTI>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.tolerance,
+           dt, ctx->d_flags, ctx->bt(), Po);
   prof_end(ctx, JUMP_PHASE_G2P);
   prof_begin(ctx);
-  if (ctx->max_rigid_rank >= 0) LAUNCH((k_rigid_move<MULTI>), np, ctx->P, ctx->d_solids, dt, ctx->bt());
+  if (ctx->max_rigid_rank >= 0) LAUNCH((k_rigid_move<MULTI>), np, Po, ctx->d_solids, dt, ctx->bt());
   prof_end(ctx, JUMP_PHASE_RIGID);
 }
 
@@ -2106,16 +2113,21 @@ static void seg_redirect(const std::vector<std::pair<void*, size_t>>& segs, void
     off += seg_offset(sg.second);
   }
 }
-static AdjStep adj_step_view(const jump_ctx* ctx, double* snap, double* grid, void* bins) {
-  AdjStep st{ctx->P, ctx->g, ctx->d_start, ctx->d_sorted, ctx->d_order, ctx->d_nodepos};
+// The particle arrays with those a snapshot holds pointing into it.
+static Particles snap_view(const jump_ctx* ctx, double* snap) {
+  Particles P = ctx->P;
   const auto ss = snap_segments(ctx);
-  for (auto& q : st.P.pr) seg_redirect(ss, snap, q);
-  for (auto& q : st.P.F) seg_redirect(ss, snap, q);
-  for (auto& q : st.P.eps) seg_redirect(ss, snap, q);
-  seg_redirect(ss, snap, st.P.alpha);
-  for (auto& q : st.P.ep) seg_redirect(ss, snap, q);
+  for (auto& q : P.pr) seg_redirect(ss, snap, q);
+  for (auto& q : P.F) seg_redirect(ss, snap, q);
+  for (auto& q : P.eps) seg_redirect(ss, snap, q);
+  seg_redirect(ss, snap, P.alpha);
+  for (auto& q : P.ep) seg_redirect(ss, snap, q);
   for (int f = 0; f < (ctx->D == 3 ? PairLayout<3>::NF : PairLayout<2>::NF); f++)
-    *gather_field(st.P, ctx->D, f) = pair_field(st.P, f);
+    *gather_field(P, ctx->D, f) = pair_field(P, f);
+  return P;
+}
+static AdjStep adj_step_view(const jump_ctx* ctx, double* snap, double* grid, void* bins) {
+  AdjStep st{snap_view(ctx, snap), ctx->g, ctx->d_start, ctx->d_sorted, ctx->d_order, ctx->d_nodepos};
   const auto gs = grid_segments(ctx);
   seg_redirect(gs, grid, st.g.m);
   for (int k = 0; k < 3; k++) {
@@ -2129,6 +2141,19 @@ static AdjStep adj_step_view(const jump_ctx* ctx, double* snap, double* grid, vo
   seg_redirect(bs, bins, st.order);
   seg_redirect(bs, bins, st.nodepos);
   return st;
+}
+
+// One 2D step from the state in `in` to the state in `out` (null `in`: the
+// live arrays, read and written in place unless `out` is given): the
+// reverse mode steps from one record into the next without copying.
+static int step_between(jump_ctx* ctx, const Particles* in, const Particles* out) {
+  const Particles live = ctx->P;
+  if (in) ctx->P = *in;
+  ctx->Pout = out;
+  const int r = run_steps(ctx, 1);
+  ctx->P = live;
+  ctx->Pout = nullptr;
+  return r;
 }
 
 // Lanes per node of the backward sweep's node gathers (k_adj_nodes,
@@ -2268,10 +2293,21 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
   A.lanes = adjoint_lanes(ctx);
   unsigned fl = 0;
   const int64_t last0 = (nck - 1) * K;  // the last segment's states and grids are kept as they go by
+  const int64_t nlast = nsteps - last0;
   for (int64_t s = 0; s < nsteps; s++) {
     if (s % K == 0 && (r = snap_copy(ctx, A.ckpt[size_t(s / K)], true))) return r;
-    if (s > last0 && (r = snap_copy(ctx, A.segbuf[size_t(s - last0)], true))) return r;
-    if ((r = run_steps(ctx, 1))) return r;
+    if (ctx->D == 2 && s >= last0 && nlast > 1) {
+      // 2D: the step reads its state where it is kept and writes the next
+      // one into its record (the last into the live arrays)
+      const size_t q = size_t(s - last0);
+      const Particles in = q == 0 ? ctx->P : snap_view(ctx, A.segbuf[q]);
+      const Particles out = int64_t(q) + 1 < nlast ? snap_view(ctx, A.segbuf[q + 1]) : ctx->P;
+      r = step_between(ctx, q == 0 ? nullptr : &in, &out);
+    } else {
+      if (s > last0 && (r = snap_copy(ctx, A.segbuf[size_t(s - last0)], true))) return r;
+      r = run_steps(ctx, 1);
+    }
+    if (r) return r;
     if (s >= last0 && (r = grid_copy(ctx, A.gridbuf[size_t(s - last0)], true))) return r;
     if (s >= last0 && (r = bins_copy(ctx, A.binbuf[size_t(s - last0)], true))) return r;
   }
@@ -2398,12 +2434,28 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
     // the state step s0 + q read: the segment's checkpoint for its first step
     auto seg_state = [&](size_t q) { return q == 0 ? A.ckpt[size_t(seg)] : A.segbuf[q]; };
     if (seg != A.cached_seg) {
-      if ((r = snap_copy(ctx, A.ckpt[size_t(seg)], false))) break;
-      for (int64_t j = s0; j < s1 && !r; j++) {  // rebuild the segment: states and the grids their steps used
-        if (j > s0 && (r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
-        if ((r = run_steps(ctx, 1))) break;
-        if ((r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], true))) break;
-        r = bins_copy(ctx, A.binbuf[size_t(j - s0)], true);
+      // rebuild the segment: states and the grids and bins their steps used
+      if (D == 2) {
+        // from record to record: each step reads its state from the
+        // snapshot or the record the previous step wrote, and writes the
+        // next record (the segment's last state into the live arrays)
+        ctx->runs_valid = false;  // the steps start from the snapshot: merge at every node
+        const int64_t n = s1 - s0;
+        for (int64_t q = 0; q < n && !r; q++) {
+          const Particles in = snap_view(ctx, seg_state(size_t(q)));
+          const Particles out = q + 1 < n ? snap_view(ctx, A.segbuf[size_t(q + 1)]) : ctx->P;
+          if ((r = step_between(ctx, &in, &out))) break;
+          if ((r = grid_copy(ctx, A.gridbuf[size_t(q)], true))) break;
+          r = bins_copy(ctx, A.binbuf[size_t(q)], true);
+        }
+      } else {
+        if ((r = snap_copy(ctx, A.ckpt[size_t(seg)], false))) break;
+        for (int64_t j = s0; j < s1 && !r; j++) {
+          if (j > s0 && (r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
+          if ((r = run_steps(ctx, 1))) break;
+          if ((r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], true))) break;
+          r = bins_copy(ctx, A.binbuf[size_t(j - s0)], true);
+        }
       }
       A.cached_seg = seg;
     }

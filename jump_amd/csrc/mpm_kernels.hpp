@@ -1454,15 +1454,21 @@ __device__ __forceinline__ bool finite3(double a, double b, double c) { return i
 // The particle update after the USL 2D gather (SolveExplicitSolidMechanics2D.jl:
 // 506-532): new v and x, strain += dt D, F = F (I + L dt), V = det(F) V0,
 // stress.  L = (L0, L1, L2, L3) = (dN_x v_x, dN_x v_y, dN_y v_x, dN_y v_y).
-__device__ __forceinline__ void usl2d_particle_update(const Particles& P, const SolidDev& s, int i, double dt,
-                                                      double v0, double v1, double x0, double x1, double L0,
-                                                      double L1, double L2, double L3, unsigned* flags) {
-  P.v[0][i] = v0; P.v[1][i] = v1;
-  P.x[0][i] = x0; P.x[1][i] = x1;
+// The state after the step goes to Po: P itself, or with OOP another set of
+// state arrays (the reverse mode's records), in which case the mass, which
+// shares its pair with V, is carried over too.
+template <bool OOP>
+__device__ __forceinline__ void usl2d_particle_update_to(const Particles& P, const Particles& Po, const SolidDev& s,
+                                                         int i, double dt, double v0, double v1, double x0, double x1,
+                                                         double L0, double L1, double L2, double L3,
+                                                         unsigned* flags) {
+  Po.v[0][i] = v0; Po.v[1][i] = v1;
+  Po.x[0][i] = x0; Po.x[1][i] = x1;
+  if (OOP) Po.m[i] = P.m[i];
   // D = (L + L')/2, strain += dt D, F *= (I + L dt), J = det F, V = J V0
   const double D11 = 0.5 * (L0 + L0), D22 = 0.5 * (L3 + L3), D12 = 0.5 * (L2 + L1);
   double e[3] = {P.eps[0][i] + dt * D11, P.eps[1][i] + dt * D22, P.eps[2][i] + dt * D12};
-  P.eps[0][i] = e[0]; P.eps[1][i] = e[1]; P.eps[2][i] = e[2];
+  Po.eps[0][i] = e[0]; Po.eps[1][i] = e[1]; Po.eps[2][i] = e[2];
   const double A0 = 1.0 + L0 * dt, A1 = L1 * dt, A2 = L2 * dt, A3 = 1.0 + L3 * dt;
   const double F0 = P.F[0][i], F1 = P.F[1][i], F2 = P.F[2][i], F3 = P.F[3][i];
   double Fn[4];
@@ -1470,9 +1476,9 @@ __device__ __forceinline__ void usl2d_particle_update(const Particles& P, const 
   Fn[1] = F1 * A0 + F3 * A1;
   Fn[2] = F0 * A2 + F2 * A3;
   Fn[3] = F1 * A2 + F3 * A3;
-  for (int k = 0; k < 4; k++) P.F[k][i] = Fn[k];
+  for (int k = 0; k < 4; k++) Po.F[k][i] = Fn[k];
   const double J = Fn[0] * Fn[3] - Fn[2] * Fn[1];
-  P.V[i] = J * P.V0[i];
+  Po.V[i] = J * P.V0[i];
   unsigned fl = 0;
   if (J < 0.0) fl |= JUMP_FLAG_NEGATIVE_J;
   double sg[3] = {P.sig[0][i], P.sig[1][i], P.sig[2][i]};
@@ -1483,16 +1489,33 @@ __device__ __forceinline__ void usl2d_particle_update(const Particles& P, const 
       double ep[3] = {P.ep[0][i], P.ep[1][i], P.ep[2][i]};
       double al = P.alpha[i], vm;
       j2_2d(s, e, sg, ep, al, vm);
-      P.ep[0][i] = ep[0]; P.ep[1][i] = ep[1]; P.ep[2][i] = ep[2];
-      P.alpha[i] = al;
-      P.vm[i] = vm;
+      Po.ep[0][i] = ep[0]; Po.ep[1][i] = ep[1]; Po.ep[2][i] = ep[2];
+      Po.alpha[i] = al;
+      Po.vm[i] = vm;
       break;
     }
     default: break;
   }
-  P.sig[0][i] = sg[0]; P.sig[1][i] = sg[1]; P.sig[2][i] = sg[2];
+  Po.sig[0][i] = sg[0]; Po.sig[1][i] = sg[1]; Po.sig[2][i] = sg[2];
   if (!(isfinite(x0) && isfinite(x1) && isfinite(v0) && isfinite(v1))) fl |= JUMP_FLAG_NONFINITE;
   if (fl) atomicOr(flags, fl);
+}
+
+__device__ __forceinline__ void usl2d_particle_update(const Particles& P, const SolidDev& s, int i, double dt,
+                                                      double v0, double v1, double x0, double x1, double L0,
+                                                      double L1, double L2, double L3, unsigned* flags) {
+  usl2d_particle_update_to<false>(P, P, s, i, dt, v0, v1, x0, x1, L0, L1, L2, L3, flags);
+}
+// The recorded state of a 2D particle the step leaves as it is (a rigid
+// particle before k_rigid_move, one whose stencil leaves the grid), carried
+// from P to Po.
+__device__ __forceinline__ void carry_state2d(const Particles& P, const Particles& Po, int i) {
+#pragma unroll
+  for (int j = 0; j < PairLayout<2>::NP; j++) Po.pr[j][i] = P.pr[j][i];
+#pragma unroll
+  for (int k = 0; k < 4; k++) Po.F[k][i] = P.F[k][i];
+#pragma unroll
+  for (int k = 0; k < 3; k++) Po.eps[k][i] = P.eps[k][i];
 }
 
 // VortexBodyForce2D (BodyForce.jl:94-121) at every particle's position at
@@ -1595,20 +1618,27 @@ __device__ __forceinline__ void g2p_apic2d(const Particles& P, const GridDesc& G
   usl2d_particle_update(P, s, i, dt, v0, v1, x0, x1, L0, L1, L2, L3, flags);
 }
 
-// SolveExplicitSolidMechanics2D.jl:485-534
-template <int B, bool MULTI>
+// SolveExplicitSolidMechanics2D.jl:485-534.  OOP (single problems, not
+// APIC): the state after the step goes to Po instead of P (the reverse
+// mode's rebuilt records), every particle's recorded state included.
+template <int B, bool MULTI, bool OOP = false>
 __global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, const SolidDev* __restrict__ S,
                                                       const double* __restrict__ corners, GridArrays g0, double tol0,
-                                                      double dt0, unsigned* __restrict__ flags, BatchDev Bt) {
+                                                      double dt0, unsigned* __restrict__ flags, BatchDev Bt,
+                                                      Particles Po) {
+  static_assert(!(OOP && MULTI), "single problems only");
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
-  if (s.rigid) return;
+  if (s.rigid) {
+    if (OOP) carry_state2d(P, Po, i);  // then moved by k_rigid_move on Po
+    return;
+  }
   const GridDesc& G = MULTI ? Bt.inst[s.inst].G : G0;
   const GridArrays g = MULTI ? grid_view(g0, Bt.node_off[s.inst]) : g0;
   const double tol = MULTI ? Bt.inst[s.inst].tol : tol0;
   const double dt = MULTI ? Bt.inst[s.inst].dt : dt0;
-  if constexpr (!MULTI) {
+  if constexpr (!MULTI && !OOP) {
     if (G.apic) {  // (batches refuse APIC)
       g2p_apic2d<B>(P, G, s, corners, g, tol, dt, i, flags);
       return;
@@ -1634,10 +1664,11 @@ __global__ void __launch_bounds__(kBlock) k_g2p_usl2d(Particles P, GridDesc G0, 
     }
   });
   if (!ok) {
+    if (OOP) carry_state2d(P, Po, i);
     atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
     return;
   }
-  usl2d_particle_update(P, s, i, dt, v0, v1, x0, x1, L0, L1, L2, L3, flags);
+  usl2d_particle_update_to<OOP>(P, OOP ? Po : P, s, i, dt, v0, v1, x0, x1, L0, L1, L2, L3, flags);
 }
 
 // G2P with contact: each particle reads the slot of its own solid at every
