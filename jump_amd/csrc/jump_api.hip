@@ -66,7 +66,7 @@ struct jump_ctx {
   Musl2Arrays mu{};      // 2D MUSL only: matrix rows, right-hand sides, velocities
   ChebArrays ch{};       // 2D MUSL only: the Chebyshev iteration's vectors
   RunList runs{};        // 3D: the force pass's particle runs per node, replayed by the re-projection
-  const Particles* Pout = nullptr;  // 2D USL: where the step writes the state after it (null: in place)
+  const Particles* Pout = nullptr;  // 2D USL, 3D MUSL: where the step writes the state after it (null: in place)
   int* d_dirty = nullptr;  // 3D: per node, the stamp of the step its window last changed
   int runs_gen = 0;        // stamp of the last step
   bool runs_valid = false; // the recorded runs match the particles' current bins
@@ -681,6 +681,15 @@ template <int B, int ORDER> static void step_musl3d_contact(jump_ctx* ctx) {
   prof_end(ctx, JUMP_PHASE_G2P2);
 }
 
+// 3D: the particle arrays of P with the pairs that hold the velocity, (V, vx)
+// and (vy, vz), taken from Q (the re-projection reads only vx of the first).
+static Particles with_velocity_of(const Particles& P, const Particles& Q) {
+  Particles R = P;
+  R.pr[2] = Q.pr[2];
+  R.pr[3] = Q.pr[3];
+  for (int f = 0; f < PairLayout<3>::NF; f++) *gather_field(R, 3, f) = pair_field(R, f);
+  return R;
+}
 template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   const int np = int(ctx->np);
   const double dt = ctx->prm.dt;
@@ -698,14 +707,24 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
                      ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha, rl);
   if (rl.runs) ctx->runs_valid = true;
   prof_end(ctx, JUMP_PHASE_P2G);
+  // the reverse mode's records: the state before the step in ctx->P, after
+  // it in *ctx->Pout; the re-projection reads the new velocities there
+  const bool oop = ctx->Pout != nullptr;
+  const Particles& Po = oop ? *ctx->Pout : ctx->P;
+  const Particles Pv = oop ? with_velocity_of(ctx->P, Po) : ctx->P;
   prof_begin(ctx);
-  LAUNCH((k_g2p1_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha, ctx->d_flags);
+  if (oop)
+    LAUNCH((k_g2p1_musl<B, true>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha,
+           ctx->d_flags, Po);
+  else
+    LAUNCH((k_g2p1_musl<B, false>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, ctx->prm.alpha,
+           ctx->d_flags, Po);
   prof_end(ctx, JUMP_PHASE_G2P1);
   prof_begin(ctx);
   {
     const int rows = (ctx->NN + kP2GBlock / LN1 - 1) / (kP2GBlock / LN1);
     hipLaunchKernelGGL((k_p2g<3, B, 1, ORDER, true, false, kP2GBlock, LN1>), dim3(xcd_grid(rows)), dim3(kP2GBlock),
-                       0, ctx->stream, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
+                       0, ctx->stream, Pv, ctx->G, ctx->d_solids, ctx->d_corners, ctx->d_start, ctx->d_sorted,
                        ctx->d_bxyz, ctx->d_order, ctx->d_nodepos, ctx->g, 0, dt, ctx->bt(), ctx->prm.alpha,
                        rl);
   }
@@ -713,7 +732,10 @@ template <int B, int ORDER> static void step_musl3d(jump_ctx* ctx) {
   // The second Dirichlet pass (SolveExplicitSolidMechanics3D.jl:142-156)
   // zeroes components of momentum0/momentum already zeroed by the grid update.
   prof_begin(ctx);
-  LAUNCH((k_g2p2_musl<B>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, dt, ctx->d_flags);
+  if (oop)
+    LAUNCH((k_g2p2_musl<B, true>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, dt, ctx->d_flags, Po);
+  else
+    LAUNCH((k_g2p2_musl<B, false>), np, ctx->P, ctx->G, ctx->d_solids, ctx->d_corners, ctx->g, dt, ctx->d_flags, Po);
   prof_end(ctx, JUMP_PHASE_G2P2);
 }
 
@@ -2143,9 +2165,10 @@ static AdjStep adj_step_view(const jump_ctx* ctx, double* snap, double* grid, vo
   return st;
 }
 
-// One 2D step from the state in `in` to the state in `out` (null `in`: the
-// live arrays, read and written in place unless `out` is given): the
-// reverse mode steps from one record into the next without copying.
+// One step from the state in `in` to the state in `out` (null `in`: the
+// live arrays, read and written in place unless `out` is given), for the
+// steps the reverse mode records (2D USL, 3D MUSL): it steps from one
+// record into the next without copying.
 static int step_between(jump_ctx* ctx, const Particles* in, const Particles* out) {
   const Particles live = ctx->P;
   if (in) ctx->P = *in;
@@ -2296,9 +2319,9 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
   const int64_t nlast = nsteps - last0;
   for (int64_t s = 0; s < nsteps; s++) {
     if (s % K == 0 && (r = snap_copy(ctx, A.ckpt[size_t(s / K)], true))) return r;
-    if (ctx->D == 2 && s >= last0 && nlast > 1) {
-      // 2D: the step reads its state where it is kept and writes the next
-      // one into its record (the last into the live arrays)
+    if (s >= last0 && nlast > 1) {
+      // the step reads its state where it is kept and writes the next one
+      // into its record (the last into the live arrays)
       const size_t q = size_t(s - last0);
       const Particles in = q == 0 ? ctx->P : snap_view(ctx, A.segbuf[q]);
       const Particles out = int64_t(q) + 1 < nlast ? snap_view(ctx, A.segbuf[q + 1]) : ctx->P;
@@ -2434,28 +2457,18 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
     // the state step s0 + q read: the segment's checkpoint for its first step
     auto seg_state = [&](size_t q) { return q == 0 ? A.ckpt[size_t(seg)] : A.segbuf[q]; };
     if (seg != A.cached_seg) {
-      // rebuild the segment: states and the grids and bins their steps used
-      if (D == 2) {
-        // from record to record: each step reads its state from the
-        // snapshot or the record the previous step wrote, and writes the
-        // next record (the segment's last state into the live arrays)
-        ctx->runs_valid = false;  // the steps start from the snapshot: merge at every node
-        const int64_t n = s1 - s0;
-        for (int64_t q = 0; q < n && !r; q++) {
-          const Particles in = snap_view(ctx, seg_state(size_t(q)));
-          const Particles out = q + 1 < n ? snap_view(ctx, A.segbuf[size_t(q + 1)]) : ctx->P;
-          if ((r = step_between(ctx, &in, &out))) break;
-          if ((r = grid_copy(ctx, A.gridbuf[size_t(q)], true))) break;
-          r = bins_copy(ctx, A.binbuf[size_t(q)], true);
-        }
-      } else {
-        if ((r = snap_copy(ctx, A.ckpt[size_t(seg)], false))) break;
-        for (int64_t j = s0; j < s1 && !r; j++) {
-          if (j > s0 && (r = snap_copy(ctx, A.segbuf[size_t(j - s0)], true))) break;
-          if ((r = run_steps(ctx, 1))) break;
-          if ((r = grid_copy(ctx, A.gridbuf[size_t(j - s0)], true))) break;
-          r = bins_copy(ctx, A.binbuf[size_t(j - s0)], true);
-        }
+      // rebuild the segment: states and the grids and bins their steps used,
+      // from record to record: each step reads its state from the snapshot
+      // or the record the previous step wrote, and writes the next record
+      // (the segment's last state into the live arrays)
+      ctx->runs_valid = false;  // the steps start from the snapshot: merge at every node
+      const int64_t n = s1 - s0;
+      for (int64_t q = 0; q < n && !r; q++) {
+        const Particles in = snap_view(ctx, seg_state(size_t(q)));
+        const Particles out = q + 1 < n ? snap_view(ctx, A.segbuf[size_t(q + 1)]) : ctx->P;
+        if ((r = step_between(ctx, &in, &out))) break;
+        if ((r = grid_copy(ctx, A.gridbuf[size_t(q)], true))) break;
+        r = bins_copy(ctx, A.binbuf[size_t(q)], true);
       }
       A.cached_seg = seg;
     }

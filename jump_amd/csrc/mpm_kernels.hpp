@@ -1710,10 +1710,12 @@ __global__ void __launch_bounds__(kBlock) k_g2p_usl2d_contact(Particles P, GridD
 
 // ------------------------------------------------------------------ G2P, MUSL 3D
 // G2P-1, SolveExplicitSolidMechanics3D.jl:109-134 (uses getShapeFunctions).
-template <int B>
+// OOP: the new velocity goes to Po (the reverse mode's records); a particle
+// whose stencil leaves the grid is carried over by k_g2p2_musl.
+template <int B, bool OOP>
 __global__ void __launch_bounds__(kBlock) k_g2p1_musl(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                       const double* __restrict__ corners, GridArrays g, double alpha,
-                                                      unsigned* __restrict__ flags) {
+                                                      unsigned* __restrict__ flags, Particles Po) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
@@ -1735,16 +1737,23 @@ __global__ void __launch_bounds__(kBlock) k_g2p1_musl(Particles P, GridDesc G, c
   double vn[3];
   for (int k = 0; k < 3; k++) {
     vn[k] = alpha * P.v[k][i] + vx[k];
-    P.v[k][i] = vn[k];
+    (OOP ? Po : P).v[k][i] = vn[k];
   }
   if (!finite3(vn[0], vn[1], vn[2])) atomicOr(flags, JUMP_FLAG_NONFINITE);  // final for the step
 }
 
 // The particle update of G2P-2 (SolveExplicitSolidMechanics3D.jl:186-211)
 // from the gathered position xx and velocity gradient L[a,b] = dN_a v_b.
-__device__ __forceinline__ void musl3d_particle_update(const Particles& P, const SolidDev& s, int i, double dt,
-                                                       const double* xx, const double* L, unsigned* flags) {
-  for (int k = 0; k < 3; k++) P.x[k][i] = xx[k];
+// The state after the step goes to Po: P itself, or with OOP another set of
+// state arrays (the reverse mode's records), which then also take the mass
+// (it shares its pair with z) and alpha and the plastic strain after every
+// step, not only after a plastic one.
+template <bool OOP>
+__device__ __forceinline__ void musl3d_particle_update_to(const Particles& P, const Particles& Po, const SolidDev& s,
+                                                          int i, double dt, const double* xx, const double* L,
+                                                          unsigned* flags) {
+  for (int k = 0; k < 3; k++) Po.x[k][i] = xx[k];
+  if (OOP) Po.m[i] = P.m[i];
   // symmetric part: D[a,b] = 0.5 (L[a,b] + L[b,a])
   const double D[6] = {0.5 * (L[0] + L[0]), 0.5 * (L[4] + L[4]), 0.5 * (L[8] + L[8]),
                        0.5 * (L[3] + L[1]), 0.5 * (L[6] + L[2]), 0.5 * (L[7] + L[5])};
@@ -1752,7 +1761,7 @@ __device__ __forceinline__ void musl3d_particle_update(const Particles& P, const
   for (int k = 0; k < 6; k++) {
     dD[k] = dt * D[k];
     e[k] = P.eps[k][i] + dD[k];
-    P.eps[k][i] = e[k];
+    Po.eps[k][i] = e[k];
   }
   double A[9];
   for (int k = 0; k < 9; k++) A[k] = L[k] * dt;
@@ -1761,13 +1770,13 @@ __device__ __forceinline__ void musl3d_particle_update(const Particles& P, const
   for (int k = 0; k < 9; k++) F[k] = P.F[k][i];
   for (int a = 0; a < 3; a++)
     for (int b = 0; b < 3; b++) Fn[a + 3 * b] = F[a] * A[3 * b] + F[a + 3] * A[1 + 3 * b] + F[a + 6] * A[2 + 3 * b];
-  for (int k = 0; k < 9; k++) P.F[k][i] = Fn[k];
+  for (int k = 0; k < 9; k++) Po.F[k][i] = Fn[k];
   // det = dot(col1, cross(col2, col3))
   const double c0 = Fn[4] * Fn[8] - Fn[5] * Fn[7];
   const double c1 = Fn[5] * Fn[6] - Fn[3] * Fn[8];
   const double c2 = Fn[3] * Fn[7] - Fn[4] * Fn[6];
   const double J = Fn[0] * c0 + Fn[1] * c1 + Fn[2] * c2;
-  P.V[i] = J * P.V0[i];
+  Po.V[i] = J * P.V0[i];
   unsigned fl = 0;
   if (J < 0.0) fl |= JUMP_FLAG_NEGATIVE_J;
   double sg[6];
@@ -1776,7 +1785,7 @@ __device__ __forceinline__ void musl3d_particle_update(const Particles& P, const
     case JUMP_MAT_ELASTIC: {
       double vm;
       hooke3(s, e, sg, vm);
-      P.vm[i] = vm;
+      Po.vm[i] = vm;
       break;
     }
     case JUMP_MAT_NEOHOOKEAN: neohooke<3>(s, Fn, J, sg); break;
@@ -1784,35 +1793,57 @@ __device__ __forceinline__ void musl3d_particle_update(const Particles& P, const
       double ep[6];
       for (int k = 0; k < 6; k++) ep[k] = P.ep[k][i];
       double al = P.alpha[i], vm;
-      // an elastic step leaves ep and alpha as they are: no write-back
-      if (j2_3d(s, e, sg, ep, al, vm)) {
-        for (int k = 0; k < 6; k++) P.ep[k][i] = ep[k];
-        P.alpha[i] = al;
+      // an elastic step leaves ep and alpha as they are: no write-back in place
+      const bool plastic = j2_3d(s, e, sg, ep, al, vm);
+      if (OOP || plastic) {
+        for (int k = 0; k < 6; k++) Po.ep[k][i] = ep[k];
+        Po.alpha[i] = al;
       }
-      P.vm[i] = vm;
+      Po.vm[i] = vm;
       break;
     }
     case JUMP_MAT_JOHNSONCOOK: {
       double al = P.alpha[i], ad = P.alphadot[i], vm;
       johnson_cook(s, e, dD, dt, sg, al, ad, vm);
-      P.alpha[i] = al;
-      P.alphadot[i] = ad;
-      P.vm[i] = vm;
+      Po.alpha[i] = al;
+      Po.alphadot[i] = ad;
+      Po.vm[i] = vm;
       break;
     }
     default: break;
   }
-  for (int k = 0; k < 6; k++) P.sig[k][i] = sg[k];
+  for (int k = 0; k < 6; k++) Po.sig[k][i] = sg[k];
   // (the velocity, final after G2P-1, is checked there)
   if (!finite3(xx[0], xx[1], xx[2])) fl |= JUMP_FLAG_NONFINITE;
   if (fl) atomicOr(flags, fl);
 }
 
-// G2P-2, SolveExplicitSolidMechanics3D.jl:161-211
-template <int B>
+__device__ __forceinline__ void musl3d_particle_update(const Particles& P, const SolidDev& s, int i, double dt,
+                                                       const double* xx, const double* L, unsigned* flags) {
+  musl3d_particle_update_to<false>(P, P, s, i, dt, xx, L, flags);
+}
+// The recorded state of a 3D particle the step leaves as it is (its stencil
+// leaves the grid), carried from P to Po.
+__device__ __forceinline__ void carry_state3d(const Particles& P, const Particles& Po, int i) {
+#pragma unroll
+  for (int j = 0; j < PairLayout<3>::NP; j++) Po.pr[j][i] = P.pr[j][i];
+#pragma unroll
+  for (int k = 0; k < 9; k++) Po.F[k][i] = P.F[k][i];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    Po.eps[k][i] = P.eps[k][i];
+    Po.ep[k][i] = P.ep[k][i];
+  }
+  Po.alpha[i] = P.alpha[i];
+}
+
+// G2P-2, SolveExplicitSolidMechanics3D.jl:161-211.  OOP: P holds the state
+// before the step and Po the state after it (Po's velocity from
+// k_g2p1_musl); a particle whose stencil leaves the grid is carried over.
+template <int B, bool OOP>
 __global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, const SolidDev* __restrict__ S,
                                                       const double* __restrict__ corners, GridArrays g, double dt,
-                                                      unsigned* __restrict__ flags) {
+                                                      unsigned* __restrict__ flags, Particles Po) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const SolidDev& s = S[P.sid[i]];
@@ -1835,10 +1866,11 @@ __global__ void __launch_bounds__(kBlock) k_g2p2_musl(Particles P, GridDesc G, c
     }
   });
   if (!ok) {
+    if (OOP) carry_state3d(P, Po, i);
     atomicOr(flags, JUMP_FLAG_OUT_OF_GRID);
     return;
   }
-  musl3d_particle_update(P, s, i, dt, xx, L, flags);
+  musl3d_particle_update_to<OOP>(P, OOP ? Po : P, s, i, dt, xx, L, flags);
 }
 
 // ------------------------------------------------------------------ 2D MUSL
