@@ -2250,7 +2250,7 @@ static size_t adjoint_record_bytes(const jump_ctx* ctx, int64_t nsteps, int64_t 
 
 // The largest interval whose record fits in 3/4 of the device memory that is
 // free once this context's previous record is released.  The adjoint arrays
-// of the backward sweep (2D 25, 3D 54 doubles per particle) are set aside first.
+// of the backward sweep (2D 25, 3D 55 doubles per particle) are set aside first.
 static int adjoint_auto_interval(jump_ctx* ctx, int64_t nsteps, int64_t* K) {
   size_t freeb = 0, total = 0;
   HIP_TRY(hipMemGetInfo(&freeb, &total));
@@ -2260,8 +2260,8 @@ static int adjoint_auto_interval(jump_ctx* ctx, int64_t nsteps, int64_t* K) {
     freeb += states * snap_bytes(ctx) + A.gridbuf.size() * grid_bytes(ctx) + A.binbuf.size() * bins_bytes(ctx);
   }
   // the adjoint arrays of the backward sweep: 2D 25 doubles per particle and
-  // 6 per node; 3D 54 and 10
-  const size_t reserve = (ctx->D == 2 ? 25 : 54) * size_t(ctx->np) * sizeof(double) +
+  // 6 per node; 3D 55 and 10
+  const size_t reserve = (ctx->D == 2 ? 25 : 55) * size_t(ctx->np) * sizeof(double) +
                          (ctx->D == 2 ? 6 : 10) * size_t(ctx->NN) * sizeof(double);
   const size_t budget = freeb > reserve ? (freeb - reserve) / 4 * 3 : 0;
   int64_t k = std::min<int64_t>(nsteps, INT32_MAX);
@@ -2408,9 +2408,8 @@ int jump_adjoint_backward(jump_ctx* ctx, const double* pos_bar, const double* ve
       for (double** v : {&ap.Vb, &ap.ab, &ap.lam, &ap.mu, &ap.fy, &ap.k1})
         if (!r) r = alloc(v, n);
       AdjSources3& src = A.src3;
-      for (int k = 0; k < 3 && !r; k++) r = alloc(&src.xb[k], n);
-      for (int k = 0; k < 9 && !r; k++) r = alloc(&src.Lb[k], n);
-      for (int k = 0; k < 3 && !r; k++) r = alloc(&src.vb[k], n);
+      for (int k = 0; k < 6 && !r; k++) r = dev_alloc(ctx, A.adj_allocs, &src.xL[k], n);
+      for (int k = 0; k < 2 && !r; k++) r = dev_alloc(ctx, A.adj_allocs, &src.vb[k], n);
       NodeVel3& nv = A.nv3;
       if (!r) r = alloc(&nv.mii, nn);
       for (int k = 0; k < 3 && !r; k++) r = alloc(&nv.q[k], nn);

@@ -56,11 +56,35 @@ struct AdjNodes3 {
   double* P0b[3];
   double* P2b[3];
 };
-// Per-particle sources of the node adjoints, gathered per node.
+// Per-particle sources of the node adjoints, gathered per node, in 16-byte
+// pairs that the node gathers load whole: xL holds x'_b then L_b (G2P-2),
+// (x0, x1) (x2, L0) (L1, L2) (L3, L4) (L5, L6) (L7, L8); vb holds v'_b, total
+// (G2P-1), as (v0, v1) (v2, 0).
 struct AdjSources3 {
-  double* xb[3];  // x'_b (G2P-2)
-  double* Lb[9];  // L_b (G2P-2)
-  double* vb[3];  // v'_b, total (G2P-1)
+  double2* xL[6];
+  double2* vb[2];
+  __device__ __forceinline__ void put_xL(int i, const double* xb, const double* Lb) const {
+    const double v[12] = {xb[0], xb[1], xb[2], Lb[0], Lb[1], Lb[2], Lb[3], Lb[4], Lb[5], Lb[6], Lb[7], Lb[8]};
+#pragma unroll
+    for (int j = 0; j < 6; j++) xL[j][i] = make_double2(v[2 * j], v[2 * j + 1]);
+  }
+  __device__ __forceinline__ void get_xL(int i, double* xb, double* Lb) const {
+    double v[12];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      const double2 w = xL[j][i];
+      v[2 * j] = w.x;
+      v[2 * j + 1] = w.y;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) xb[k] = v[k];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Lb[k] = v[3 + k];
+  }
+  __device__ __forceinline__ void put_vb(int i, const double* v) const {
+    vb[0][i] = make_double2(v[0], v[1]);
+    vb[1][i] = make_double2(v[2], 0.0);
+  }
 };
 
 // A particle's 3D stencil formed once (as Stencil2): each() calls
@@ -238,8 +262,9 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p2(Particles P, GridDesc G, c
       }
     });
   if (!ok) {  // the forward left this particle unchanged: identity adjoint, no sources
-    for (int k = 0; k < 3; k++) { src.xb[k][i] = 0.0; src.vb[k][i] = 0.0; }
-    for (int k = 0; k < 9; k++) src.Lb[k][i] = 0.0;
+    const double z[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    src.put_xL(i, z, z);
+    src.put_vb(i, z);
     return;
   }
   const int sa[6] = {0, 1, 2, 0, 0, 1}, sbb[6] = {0, 1, 2, 1, 2, 2};
@@ -347,8 +372,7 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p2(Particles P, GridDesc G, c
     }
   }
   const double xb1[3] = {ad.xb[0][i], ad.xb[1][i], ad.xb[2][i]};
-  for (int k = 0; k < 3; k++) src.xb[k][i] = xb1[k];
-  for (int k = 0; k < 9; k++) src.Lb[k][i] = Lb[k];
+  src.put_xL(i, xb1, Lb);
   // x' = x + sum N dt P/m and L through N(x), g(x)
   double xb[3] = {xb1[0], xb1[1], xb1[2]};
   stc.each(G, [&](int n, double, const double* gr, const double* H) {
@@ -427,11 +451,12 @@ __global__ void __launch_bounds__(kBlock) k_adj3_nodes2(Particles P, GridDesc G,
     const double u[3] = {g.p[0][n] * iM, g.p[1][n] * iM, g.p[2][n] * iM};
     const double u2[3] = {g.p2[0][n] * iM, g.p2[1][n] * iM, g.p2[2][n] * iM};
     window_gather3<B, S>(P, G, SD, start, sorted, n, sl, [&](int q, double N, const double* gr) {
-      const double xb[3] = {src.xb[0][q], src.xb[1][q], src.xb[2][q]};
+      double xb[3], Lq[9];
+      src.get_xL(q, xb, Lq);
       double ub[3], u2b[3];
       for (int b = 0; b < 3; b++) {
         ub[b] = N * dt * xb[b];
-        u2b[b] = gr[0] * src.Lb[3 * b][q] + gr[1] * src.Lb[1 + 3 * b][q] + gr[2] * src.Lb[2 + 3 * b][q];
+        u2b[b] = gr[0] * Lq[3 * b] + gr[1] * Lq[1 + 3 * b] + gr[2] * Lq[2 + 3 * b];
         Pb[b] += ub[b] * iM;
         P2b[b] += u2b[b] * iM;
       }
@@ -474,7 +499,8 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p1(Particles P, GridDesc G, c
       for (int k = 0; k < 3; k++) vb[k] += N * mp * an.P2b[k][n];
     });
   if (!ok) {
-    for (int k = 0; k < 3; k++) src.vb[k][i] = 0.0;
+    const double z[3] = {0.0, 0.0, 0.0};
+    src.put_vb(i, z);
     return;
   }
   for (int k = 0; k < 3; k++) vn[k] += vx[k];
@@ -485,8 +511,8 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p1(Particles P, GridDesc G, c
       for (int k = 0; k < 3; k++) Nb += nv.qm[k][n] * vb[k];
     for (int c = 0; c < 3; c++) xb[c] += Nb * gr[c];
   });
+  src.put_vb(i, vb);
   for (int k = 0; k < 3; k++) {
-    src.vb[k][i] = vb[k];
     ad.xb[k][i] += xb[k];
     ad.vb[k][i] = alpha * vb[k];
   }
@@ -513,8 +539,10 @@ __global__ void __launch_bounds__(kBlock) k_adj3_nodes1(Particles P, GridDesc G,
     const double q[3] = {(g.p[0][n] - alpha * g.p0[0][n]) * iM, (g.p[1][n] - alpha * g.p0[1][n]) * iM,
                          (g.p[2][n] - alpha * g.p0[2][n]) * iM};
     window_gather3<B, S>(P, G, SD, start, sorted, n, sl, [&](int p, double N, const double*) {
+      const double2 v01 = src.vb[0][p], v2 = src.vb[1][p];
+      const double vp[3] = {v01.x, v01.y, v2.x};
       for (int k = 0; k < 3; k++) {
-        const double b = N * src.vb[k][p];
+        const double b = N * vp[k];
         Pb[k] += b * iM;
         mb -= q[k] * b * iM;
       }
