@@ -258,6 +258,12 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
 // direction (x fastest) the S lanes read S consecutive particles, and the
 // wave's loads are contiguous.  The partial sums are combined in a fixed
 // order, so the gathers stay deterministic.
+// The x weight and its derivative of the last particle a lane weighed
+// against its node in a node gather (k_adj_nodes, window_gather3).
+struct XWeightCache {
+  double x = __builtin_nan(""), lp = __builtin_nan("");  // NaN: nothing cached
+  double w = 0.0, d1 = 0.0;
+};
 template <int S> __device__ __forceinline__ double lanes_sum(double v) {
   // ((l0 + l1) + (l2 + l3)) in every lane of the group: addition commutes
   // exactly, so all S lanes hold the same bits
@@ -294,6 +300,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, c
     const double u[2] = {g.p[0][n] * iM, g.p[1][n] * iM};
     const double a[2] = {(g.p[0][n] - g.p0[0][n]) * iM, (g.p[1][n] - g.p0[1][n]) * iM};
     const Axis X = axis_of(G, 0), Y = axis_of(G, 1);
+    XWeightCache xc;
     for (int bj = max(nj - W + 1, 0); bj <= nj; bj++)
       for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
         const int bin = bi + G.nx * bj;
@@ -314,8 +321,16 @@ __global__ void __launch_bounds__(kBlock) k_adj_nodes(Particles P, GridDesc G, c
             xyn = ldu(P.pr[0], unsigned(qn));
           }
           const SolidDev& s = S_[sq];
-          double wx, d1x, d2x, wy, d1y, d2y;
-          axis_w2<B>(xy.x, X, s.lp[0], ni + 1, wx, d1x, d2x);
+          double d2x, wy, d1y, d2y;
+          // a lane's particles of one bin share x on a lattice of S per cell
+          // and direction: its x weight is formed again only when some
+          // active lane of the wave has another x (or GIMP half-width)
+          if (!__all(xy.x == xc.x && s.lp[0] == xc.lp)) {
+            axis_w2<B>(xy.x, X, s.lp[0], ni + 1, xc.w, xc.d1, d2x);
+            xc.x = xy.x;
+            xc.lp = s.lp[0];
+          }
+          const double wx = xc.w, d1x = xc.d1;
           axis_w2<B>(xy.y, Y, s.lp[1], nj + 1, wy, d1y, d2y);
           const double N = wx * wy, gx = d1x * wy, gy = wx * d1y;
           const double2 xb2 = src.xb[q], vb2 = src.vb[q], L01 = src.Lb[0][q], L23 = src.Lb[1][q];

@@ -410,6 +410,7 @@ __device__ __forceinline__ void window_gather3(const Particles& P, const GridDes
   constexpr int W = Width<B>::W;
   const int ni = n % G.nx, nj = (n / G.nx) % G.ny, nk = n / G.nxy;
   const Axis X = axis_of(G, 0), Y = axis_of(G, 1), Z = axis_of(G, 2);
+  XWeightCache xc;
   for (int bk = max(nk - W + 1, 0); bk <= nk; bk++)
     for (int bj = max(nj - W + 1, 0); bj <= nj; bj++)
       for (int bi = max(ni - W + 1, 0); bi <= ni; bi++) {
@@ -419,8 +420,16 @@ __device__ __forceinline__ void window_gather3(const Particles& P, const GridDes
           const int q = int(unsigned(sorted[c]));
           const SolidDev& s = SD[P.sid[q]];
           const double2 xy = ldu(P.pr[0], unsigned(q));
-          double wx, d1x, d2x, wy, d1y, d2y, wz, d1z, d2z;
-          axis_w2<B>(xy.x, X, s.lp[0], ni + 1, wx, d1x, d2x);
+          double d2x, wy, d1y, d2y, wz, d1z, d2z;
+          // a lane's particles of one bin share x on a lattice of S per cell
+          // and direction: its x weight is formed again only when some
+          // active lane of the wave has another x (or GIMP half-width)
+          if (!__all(xy.x == xc.x && s.lp[0] == xc.lp)) {
+            axis_w2<B>(xy.x, X, s.lp[0], ni + 1, xc.w, xc.d1, d2x);
+            xc.x = xy.x;
+            xc.lp = s.lp[0];
+          }
+          const double wx = xc.w, d1x = xc.d1;
           axis_w2<B>(xy.y, Y, s.lp[1], nj + 1, wy, d1y, d2y);
           axis_w2<B>(P.x[2][q], Z, s.lp[2], nk + 1, wz, d1z, d2z);
           const double gr[3] = {d1x * wy * wz, wx * d1y * wz, wx * wy * d1z};
