@@ -2326,8 +2326,7 @@ int jump_adjoint_forward(jump_ctx* ctx, int64_t nsteps, int32_t K_arg, double* t
       const Particles in = q == 0 ? ctx->P : snap_view(ctx, A.segbuf[q]);
       const Particles out = int64_t(q) + 1 < nlast ? snap_view(ctx, A.segbuf[q + 1]) : ctx->P;
       r = step_between(ctx, q == 0 ? nullptr : &in, &out);
-    } else {
-      if (s > last0 && (r = snap_copy(ctx, A.segbuf[size_t(s - last0)], true))) return r;
+    } else {  // before the last segment, or a last segment of one step: in place
       r = run_steps(ctx, 1);
     }
     if (r) return r;
