@@ -760,7 +760,8 @@ static int cheb_iterations() {
 // data, so the host never waits.
 template <int R> static void solve_musl2d(jump_ctx* ctx) {
   const int nn = ctx->NN;
-  const int ntile = cheb_tiles(ctx->G);
+  const bool wide = nn > kChebWideNodes;
+  const int ntile = wide ? cheb_tiles<64, 4>(ctx->G) : cheb_tiles<32, 8>(ctx->G);
   const ChebArrays& ch = ctx->ch;
   const double theta = 0.55, delta = 0.45, sigma = theta / delta;
   LAUNCH(k_cheb_init, nn, nn, ctx->mu, ctx->g.m, ch, 1.0 / theta);
@@ -769,8 +770,12 @@ template <int R> static void solve_musl2d(jump_ctx* ctx) {
   double rho = 1.0 / sigma;
   for (int k = 0; k < K; k++) {
     const double rn = 1.0 / (2.0 * sigma - rho);
-    hipLaunchKernelGGL((k_cheb_iter<R>), dim3(ntile), dim3(kChebBlock), 0, ctx->stream, ctx->G, ctx->mu, ch, k & 1,
-                       rn * rho, 2.0 * rn / delta, k + 1 == K ? 1 : 0);
+    if (wide)
+      hipLaunchKernelGGL((k_cheb_iter<R, 64, 4>), dim3(ntile), dim3(kChebBlock), 0, ctx->stream, ctx->G, ctx->mu, ch,
+                         k & 1, rn * rho, 2.0 * rn / delta, k + 1 == K ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_cheb_iter<R, 32, 8>), dim3(ntile), dim3(kChebBlock), 0, ctx->stream, ctx->G, ctx->mu, ch,
+                         k & 1, rn * rho, 2.0 * rn / delta, k + 1 == K ? 1 : 0);
     rho = rn;
   }
 }

@@ -2009,12 +2009,15 @@ __global__ void __launch_bounds__(kP2GBlock) k_p2g_musl2d(Particles P, GridDesc 
 // positive semi-definite, so the spectrum of D^-1 B lies in [0.1, 1]
 // whatever the particle layout.
 constexpr int kChebBlock = 256;
-// The iteration works on 32 x 8 node tiles, staging d over the tile and its
-// halo in LDS (each d then feeds up to (2R+1)^2 rows).
-constexpr int kChebTX = 32, kChebTY = 8;
-static_assert(kChebTX * kChebTY == kChebBlock, "one node per thread");
-__host__ __device__ inline int cheb_tiles(const GridDesc& G) {
-  return ((G.nx + kChebTX - 1) / kChebTX) * ((G.ny + kChebTY - 1) / kChebTY);
+// The iteration works on TX x TY node tiles (one node per thread), staging d
+// over the tile and its halo in LDS (each d then feeds up to (2R+1)^2 rows):
+// 32 x 8, or 64 x 4 above kChebWideNodes nodes, whose rows read whole
+// wave-wide segments.  Measured (solve ms at 4k / 66k / 263k nodes): 32 x 8
+// 0.41 / 0.54 / 1.26, 64 x 4 0.44 / 0.55 / 1.10.
+constexpr int kChebWideNodes = 150000;
+template <int TX, int TY> __host__ __device__ inline int cheb_tiles(const GridDesc& G) {
+  static_assert(TX * TY == kChebBlock, "one node per thread");
+  return ((G.nx + TX - 1) / TX) * ((G.ny + TY - 1) / TY);
 }
 
 // Chebyshev iteration (Saad, Iterative Methods for
@@ -2087,10 +2090,10 @@ __device__ __forceinline__ void cheb_spmv(const GridDesc& G, const Musl2Arrays& 
 // and its R-wide halo, then x, r_k+1 and d_k+1 = c1 d_k + c2 D^-1 r_k+1.
 // Reads slot `slot` of r and d, writes the other.  The last iteration writes
 // the nodal velocities instead: mv = (x0, x1, x0 + x2, x1 + x3).
-template <int R>
+template <int R, int TX, int TY>
 __global__ void __launch_bounds__(kChebBlock) k_cheb_iter(GridDesc G, Musl2Arrays mu, ChebArrays ch, int slot,
                                                         double c1, double c2, int last) {
-  constexpr int LX = kChebTX + 2 * R, LY = kChebTY + 2 * R;
+  constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
   __shared__ double ds[4][LY][LX];
   const int t = threadIdx.x;
   const int nn = G.nn;
@@ -2099,8 +2102,8 @@ __global__ void __launch_bounds__(kChebBlock) k_cheb_iter(GridDesc G, Musl2Array
   const double* r_o = ch.r + size_t(slot) * n4;
   double* d_n = ch.d + size_t(slot ^ 1) * n4;
   double* r_n = ch.r + size_t(slot ^ 1) * n4;
-  const int ntx = (G.nx + kChebTX - 1) / kChebTX;
-  const int x0 = int(blockIdx.x % unsigned(ntx)) * kChebTX, y0 = int(blockIdx.x / unsigned(ntx)) * kChebTY;
+  const int ntx = (G.nx + TX - 1) / TX;
+  const int x0 = int(blockIdx.x % unsigned(ntx)) * TX, y0 = int(blockIdx.x / unsigned(ntx)) * TY;
   for (int e = t; e < LX * LY; e += kChebBlock) {
     const int lx = e % LX, ly = e / LX;
     const int gx = x0 + lx - R, gy = y0 + ly - R;
@@ -2110,7 +2113,7 @@ __global__ void __launch_bounds__(kChebBlock) k_cheb_iter(GridDesc G, Musl2Array
     for (int c = 0; c < 4; c++) ds[c][ly][lx] = in ? d_o[size_t(c) * nn + j] : 0.0;
   }
   __syncthreads();
-  const int tx = t % kChebTX, ty = t / kChebTX;
+  const int tx = t % TX, ty = t / TX;
   const int ix = x0 + tx, iy = y0 + ty;
   if (ix >= G.nx || iy >= G.ny) return;
   const int i = ix + G.nx * iy;
