@@ -861,6 +861,8 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
       }
       return;
     } else if constexpr (LPN == 2) {
+      constexpr bool YZ = kYZCache && B != JUMP_BASIS_GIMP && B != JUMP_BASIS_CPDI_Q4;
+      YZCache yz;
       for (int r = 0; r < nr; r++) {
         const unsigned e = rl.runs[size_t(r) * NN + n];
         const int p0 = int(e >> 3), len = int(e & 7u) + 1;
@@ -884,7 +886,7 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
             } else {
               P2GFields<D, PASS>::load(P, unsigned(p0 + mine), v);
             }
-            p2g_terms3<B, PASS, INT>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt);
+            p2g_terms3<B, PASS, INT, YZ>(P, G, S, corners, p0 + mine, v, ni, nj, nk, tt, &yz);
           } else {
 #pragma unroll
             for (int j = 0; j < Terms3<B, PASS>::NT; j++) tt[j] = 0.0;
