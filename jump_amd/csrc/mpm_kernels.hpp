@@ -469,6 +469,10 @@ constexpr int kP2GLanes1 = 2;
 // The 3D replays (the force pass and the re-projection) reuse a lane's y
 // and z weights across a run (YZCache, node_weight_yz).
 constexpr bool kYZCache = true;
+// Pairs the 3D force pass's replay loads a particle ahead: (x, y) (z, m)
+// (V, vx) ...  Measured (ms per launch on the impact): 2 1.334, 3 1.327,
+// 4 1.333, 5 1.345; all seven (before the weight cache) 1.46 -> 1.53.
+constexpr int kPrefetchPairs = 3;
 // Node order of the single 2D gathers: the active nodes compacted (true) or
 // grid rows, for the contact step and the plain USL step.
 // Measured: the two-rings USL gather (1025 x 513 nodes, thick rings) 0.609
@@ -826,9 +830,9 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
   // Replay.  The re-projection loads the next particle's fields (and the
   // next run's entry) while the current particle is summed; the sums keep
   // their order (0.683 -> 0.651 ms on the impact).  The force pass loads
-  // only the next particle's (x, y) (z, m) ahead, which its weights wait
-  // on (1.463 -> 1.378 ms; all 14 fields ahead: 1.46 -> 1.53 ms, 117 -> 140
-  // VGPRs).
+  // the next particle's first kPrefetchPairs pairs ahead, (x, y) (z, m),
+  // which its weights wait on, and (V, vx) (1.463 -> 1.378 ms for the first
+  // two; all 14 fields ahead: 1.46 -> 1.53 ms, 117 -> 140 VGPRs).
   auto replay_as = [&](auto inner, int nr) {
     constexpr bool INT = decltype(inner)::value;
     if constexpr (LPN == 2 && PASS == 1) {
@@ -901,17 +905,24 @@ __device__ __forceinline__ void p2g_block(const Particles& P, const GridDesc& G0
         // row are formed once (kYZCache)
         constexpr bool YZ = kYZCache && D == 3 && B != JUMP_BASIS_GIMP && B != JUMP_BASIS_CPDI_Q4;
         YZCache yz;
+        constexpr int NPF = D == 3 ? kPrefetchPairs : 2;  // pairs loaded a particle ahead
         for (int r = 0; r < nr; r++) {
           const unsigned e = rl.runs[size_t(r) * NN + n];
           const int p0 = int(e >> 3), len = int(e & 7u) + 1;
-          double2 a = ldu(P.pr[0], unsigned(p0)), b = ldu(P.pr[1], unsigned(p0));  // (x, y) (z, m)
+          double2 pf[NPF];  // (x, y) (z, m) ...
+#pragma unroll
+          for (int j = 0; j < NPF; j++) pf[j] = ldu(P.pr[j], unsigned(p0));
           for (int k = 0; k < len; k++) {
             double v[NF + 1];
-            v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
-            P2GFields<D, PASS>::load(P, unsigned(p0 + k), v, 2);
+#pragma unroll
+            for (int j = 0; j < NPF; j++) {
+              v[2 * j] = pf[j].x;
+              if (2 * j + 1 < NF) v[2 * j + 1] = pf[j].y;
+            }
+            P2GFields<D, PASS>::load(P, unsigned(p0 + k), v, NPF);
             if (k + 1 < len) {
-              a = ldu(P.pr[0], unsigned(p0 + k + 1));
-              b = ldu(P.pr[1], unsigned(p0 + k + 1));
+#pragma unroll
+              for (int j = 0; j < NPF; j++) pf[j] = ldu(P.pr[j], unsigned(p0 + k + 1));
             }
             p2g_add_v<D, B, PASS, AFF, INT, YZ>(P, G, S, corners, p0 + k, v, ni, nj, nk, am, ap, af, &yz);
           }
