@@ -239,11 +239,9 @@ __global__ void __launch_bounds__(kBlock) k_adj_g2p(Particles P, GridDesc G, con
   ad.xb[1][i] = xb[1];
   // v' = v + ...: identity part of v stays in ad.vb
   for (int k = 0; k < 4; k++) ad.Fb[k][i] = Fb[k];
-  for (int k = 0; k < 3; k++) {
-    ad.eb[k][i] = eb[k];
-    ad.sb[k][i] = 0.0;  // sigma is overwritten by the update; its P2G use is added by k_adj_p2g
-  }
-  ad.Vb[i] = 0.0;  // likewise V
+  // sigma and V are overwritten by the update: their adjoints before the
+  // step are their P2G uses alone, which k_adj_p2g stores
+  for (int k = 0; k < 3; k++) ad.eb[k][i] = eb[k];
   ad.lam[i] += lamb;
   ad.mu[i] += mub;
 }
@@ -392,7 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_p2g(Particles P, GridDesc G, con
   const double v[2] = {P.v[0][i], P.v[1][i]};
   const double sg[3] = {P.sig[0][i], P.sig[1][i], P.sig[2][i]};
   double xb[2] = {0.0, 0.0}, vb[2] = {0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0}, Vb = 0.0;
-  walk_stencil_h<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
+  const bool ok = walk_stencil_h<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
     const double mb = an.mb[n];
     const double p0b[2] = {an.P0b[0][n], an.P0b[1][n]};
     const double fb[2] = {an.Pb[0][n], an.Pb[1][n]};
@@ -411,8 +409,17 @@ __global__ void __launch_bounds__(kBlock) k_adj_p2g(Particles P, GridDesc G, con
   ad.xb[1][i] += xb[1];
   ad.vb[0][i] += vb[0];
   ad.vb[1][i] += vb[1];
-  for (int k = 0; k < 3; k++) ad.sb[k][i] += sb[k];
-  ad.Vb[i] += Vb;
+  // Where the step updated the particle (ok), k_adj_g2p consumed the
+  // adjoints of the updated sigma and V, and these sums are stored; they
+  // start at +0 and are never -0, so they equal 0 + the sum bit for bit.
+  // Elsewhere the adjoints pass through, plus the (zero) sums.
+  if (ok) {
+    for (int k = 0; k < 3; k++) ad.sb[k][i] = sb[k];
+    ad.Vb[i] = Vb;
+  } else {
+    for (int k = 0; k < 3; k++) ad.sb[k][i] += sb[k];
+    ad.Vb[i] += Vb;
+  }
 }
 
 // Many device-to-device copies in one launch (the state, grid and bins a

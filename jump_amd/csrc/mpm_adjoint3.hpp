@@ -386,12 +386,12 @@ __global__ void __launch_bounds__(kBlock) k_adj3_g2p2(Particles P, GridDesc G, c
   });
   for (int k = 0; k < 3; k++) ad.xb[k][i] = xb[k];
   for (int k = 0; k < 9; k++) ad.Fb[k][i] = Fb[k];
+  // sigma and V are overwritten by the update: their adjoints before the
+  // step are their P2G uses alone, which k_adj3_p2g stores
   for (int c = 0; c < 6; c++) {
     ad.eb[c][i] = eb[c];
-    ad.sb[c][i] = 0.0;  // sigma and V are overwritten by the update; their P2G use is added by k_adj3_p2g
     ad.epb[c][i] = epb_in[c];
   }
-  ad.Vb[i] = 0.0;
   ad.ab[i] = ab_in;
   ad.lam[i] += lamb;
   ad.mu[i] += mub;
@@ -598,7 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_adj3_p2g(Particles P, GridDesc G, co
   for (int a = 0; a < 3; a++)
     for (int b = 0; b < 3; b++) sg[a][b] = P.sig[sym<3>(a, b)][i];
   double xb[3] = {0.0, 0.0, 0.0}, vb[3] = {0.0, 0.0, 0.0}, sb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, Vb = 0.0;
-  walk_stencil_h3<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
+  const bool ok = walk_stencil_h3<B>(P, G, s, i, [&](int n, double N, const double* gr, const double* H) {
     const double mb = an.mb[n];
     const double p0b[3] = {an.P0b[0][n], an.P0b[1][n], an.P0b[2][n]};
     const double fb[3] = {an.Pb[0][n], an.Pb[1][n], an.Pb[2][n]};
@@ -623,8 +623,16 @@ __global__ void __launch_bounds__(kBlock) k_adj3_p2g(Particles P, GridDesc G, co
     ad.xb[k][i] += xb[k];
     ad.vb[k][i] += vb[k];
   }
-  for (int c = 0; c < 6; c++) ad.sb[c][i] += sb[c];
-  ad.Vb[i] += Vb;
+  // stored where the step updated the particle (k_adj3_g2p2 consumed the
+  // adjoints of the updated sigma and V; the sums start at +0 and are never
+  // -0, so they equal 0 + the sum bit for bit), passed through elsewhere
+  if (ok) {
+    for (int c = 0; c < 6; c++) ad.sb[c][i] = sb[c];
+    ad.Vb[i] = Vb;
+  } else {
+    for (int c = 0; c < 6; c++) ad.sb[c][i] += sb[c];
+    ad.Vb[i] += Vb;
+  }
 }
 
 // Per-solid sums of per-particle adjoint arrays (nsum of them, array k at
